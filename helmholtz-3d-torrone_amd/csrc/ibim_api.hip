@@ -1,0 +1,492 @@
+// ibim_api.hip -- the device-resident handle API of libibim.so (include/ibim.h).
+#include <cstdarg>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "ibim_internal.hpp"
+
+namespace ibim {
+
+namespace {
+thread_local char g_err[512] = "";
+}
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+}
+
+int DevBuf::ensure(size_t want) {
+    if (want <= bytes && ptr) return 0;
+    release();
+    if (want == 0) want = 16;
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        set_error("hipMalloc(%zu bytes) failed: %s", want, hipGetErrorString(e));
+        return 1;
+    }
+    bytes = want;
+    return 0;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+KernelTimer* timer_for(Context* ctx, const char* name) {
+    for (auto& t : ctx->timers)
+        if (t.name == name) return &t;
+    ctx->timers.push_back(KernelTimer{});
+    ctx->timers.back().name = name;
+    return &ctx->timers.back();
+}
+
+void timer_begin(Context* ctx, KernelTimer* t) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, ctx->stream);
+    t->begin.push_back(e);
+}
+
+void timer_end(Context* ctx, KernelTimer* t) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return;
+    (void)hipEventRecord(e, ctx->stream);
+    t->end.push_back(e);
+}
+
+// Fold recorded event pairs into the totals (waits for the stream).
+static void collect_timers(Context* ctx) {
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& t : ctx->timers) {
+        size_t n = t.begin.size() < t.end.size() ? t.begin.size() : t.end.size();
+        for (size_t i = 0; i < n; ++i) {
+            float ms = 0.f;
+            if (hipEventElapsedTime(&ms, t.begin[i], t.end[i]) == hipSuccess) t.ms_total += ms;
+            (void)hipEventDestroy(t.begin[i]);
+            (void)hipEventDestroy(t.end[i]);
+        }
+        t.launches += (long)n;
+        t.begin.clear();
+        t.end.clear();
+    }
+}
+
+}  // namespace ibim
+
+using namespace ibim;
+
+namespace {
+
+int check_ctx(ibim_ctx* ctx) {
+    if (!ctx) {
+        set_error("null handle");
+        return IBIM_ERR_ARG;
+    }
+    return (int)hipSetDevice(ctx->device) == (int)hipSuccess ? IBIM_OK : IBIM_ERR_HIP;
+}
+
+#define CHECK(ctx)                  \
+    do {                            \
+        int rc_ = check_ctx(ctx);   \
+        if (rc_) return rc_;        \
+    } while (0)
+
+}  // namespace
+
+extern "C" {
+
+int ibim_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+const char* ibim_last_error(void) { return g_err; }
+
+int ibim_create(ibim_ctx** out) {
+    if (!out) return IBIM_ERR_ARG;
+    *out = nullptr;
+    if (ibim_device_count() <= 0) {
+        set_error("no HIP device");
+        return IBIM_ERR_NODEVICE;
+    }
+    ibim_ctx* ctx = new (std::nothrow) ibim_ctx();
+    if (!ctx) return IBIM_ERR_NOMEM;
+    IBIM_HIP(hipGetDevice(&ctx->device));
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        delete ctx;
+        return IBIM_ERR_HIP;
+    }
+    *out = ctx;
+    return IBIM_OK;
+}
+
+void ibim_destroy(ibim_ctx* ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    collect_timers(ctx);
+    for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
+                      &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
+                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine})
+        b->release();
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+int ibim_sync(ibim_ctx* ctx) {
+    CHECK(ctx);
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    return IBIM_OK;
+}
+
+static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
+    if (n < 4 || h <= 0.0) {
+        set_error("grid needs n >= 4 and h > 0");
+        return IBIM_ERR_ARG;
+    }
+    const size_t nodes = (size_t)(n + 1) * (n + 1) * (n + 1);
+    for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc})
+        if (b->ensure(sizeof(double) * nodes)) return IBIM_ERR_NOMEM;
+    ctx->n = n;
+    ctx->lo = lo;
+    ctx->h = h;
+    ctx->have_geometry = false;
+    ctx->count = 0;
+    ctx->mat_precision = -1;
+    return IBIM_OK;
+}
+
+int ibim_set_level_host(ibim_ctx* ctx, int n, double lo, double h, const double* level) {
+    CHECK(ctx);
+    int rc = set_grid(ctx, n, lo, h);
+    if (rc) return rc;
+    const size_t nodes = (size_t)(n + 1) * (n + 1) * (n + 1);
+    IBIM_HIP(hipMemcpyAsync(ctx->level.ptr, level, sizeof(double) * nodes, hipMemcpyHostToDevice, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    return IBIM_OK;
+}
+
+int ibim_set_level_device(ibim_ctx* ctx, int n, double lo, double h, const double* level_dev) {
+    CHECK(ctx);
+    int rc = set_grid(ctx, n, lo, h);
+    if (rc) return rc;
+    const size_t nodes = (size_t)(n + 1) * (n + 1) * (n + 1);
+    IBIM_HIP(hipMemcpyAsync(ctx->level.ptr, level_dev, sizeof(double) * nodes, hipMemcpyDeviceToDevice, ctx->stream));
+    return IBIM_OK;
+}
+
+int ibim_geometry(ibim_ctx* ctx, int weno_order) {
+    CHECK(ctx);
+    if (!ctx->n) {
+        set_error("ibim_geometry: no level set");
+        return IBIM_ERR_ARG;
+    }
+    if (weno_order != 3 && weno_order != 5) {
+        set_error("weno_order must be 3 or 5");
+        return IBIM_ERR_ARG;
+    }
+    int rc = launch_gradient(ctx, weno_order);
+    if (!rc) rc = launch_curvatures(ctx);
+    if (!rc) ctx->have_geometry = true;
+    return rc;
+}
+
+int ibim_get_grid(ibim_ctx* ctx, double* gx, double* gy, double* gz, double* mc, double* gc) {
+    CHECK(ctx);
+    const size_t bytes = sizeof(double) * (size_t)(ctx->n + 1) * (ctx->n + 1) * (ctx->n + 1);
+    double* dst[5] = {gx, gy, gz, mc, gc};
+    DevBuf* src[5] = {&ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc};
+    for (int i = 0; i < 5; ++i)
+        if (dst[i]) IBIM_HIP(hipMemcpyAsync(dst[i], src[i]->ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    return IBIM_OK;
+}
+
+int ibim_set_grid_fields_host(ibim_ctx* ctx, const double* gx, const double* gy, const double* gz, const double* mc,
+                              const double* gc) {
+    CHECK(ctx);
+    if (!ctx->n) return IBIM_ERR_ARG;
+    const size_t bytes = sizeof(double) * (size_t)(ctx->n + 1) * (ctx->n + 1) * (ctx->n + 1);
+    const double* srcs[5] = {gx, gy, gz, mc, gc};
+    DevBuf* dst[5] = {&ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc};
+    for (int i = 0; i < 5; ++i) {
+        if (!srcs[i]) {
+            set_error("all five grid fields are required");
+            return IBIM_ERR_ARG;
+        }
+        IBIM_HIP(hipMemcpyAsync(dst[i]->ptr, srcs[i], bytes, hipMemcpyHostToDevice, ctx->stream));
+    }
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->have_geometry = true;
+    return IBIM_OK;
+}
+
+int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
+    CHECK(ctx);
+    if (!prm || !count || (kind != IBIM_TUBE_TRADITIONAL && kind != IBIM_TUBE_COMBO)) return IBIM_ERR_ARG;
+    if (!ctx->have_geometry) {
+        set_error("ibim_tube: run ibim_geometry first");
+        return IBIM_ERR_ARG;
+    }
+    ctx->prm = *prm;
+    ctx->kind = kind;
+    ctx->mat_precision = -1;
+    int c = 0;
+    int rc = launch_tube(ctx, &c);
+    if (rc) return rc;
+    ctx->count = c;
+    ctx->ld = ((size_t)c + 63) / 64 * 64;
+    *count = c;
+    return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
+}
+
+int ibim_get_tube(ibim_ctx* ctx, int* node, double* fields) {
+    CHECK(ctx);
+    if (!ctx->count) return IBIM_ERR_ARG;
+    if (node)
+        IBIM_HIP(hipMemcpyAsync(node, ctx->node.ptr, sizeof(int) * ctx->count, hipMemcpyDeviceToHost, ctx->stream));
+    if (fields)
+        IBIM_HIP(hipMemcpyAsync(fields, ctx->tube.ptr, sizeof(double) * IBIM_NF * ctx->count, hipMemcpyDeviceToHost,
+                                ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    return IBIM_OK;
+}
+
+const double* ibim_tube_device(ibim_ctx* ctx) { return ctx ? ctx->tube.as<double>() : nullptr; }
+
+int ibim_assemble(ibim_ctx* ctx, int precision) {
+    CHECK(ctx);
+    if (!ctx->count || (precision != IBIM_C128 && precision != IBIM_C64)) {
+        set_error("ibim_assemble: needs a tube and a precision");
+        return IBIM_ERR_ARG;
+    }
+    return launch_assemble(ctx, precision);
+}
+
+int ibim_regcount(ibim_ctx* ctx) { return ctx ? ctx->regcount : 0; }
+
+int ibim_get_rows(ibim_ctx* ctx, int row0, int nrows, double* re, double* im) {
+    CHECK(ctx);
+    const int n = ctx->count;
+    if (ctx->mat_precision < 0 || row0 < 0 || nrows < 0 || row0 + nrows > n) return IBIM_ERR_ARG;
+    const size_t esz = ctx->mat_precision == IBIM_C128 ? 16 : 8;
+    std::vector<char> buf(esz * ctx->ld * (size_t)nrows);
+    IBIM_HIP(hipMemcpyAsync(buf.data(), ctx->mat.as<char>() + esz * ctx->ld * (size_t)row0, buf.size(),
+                            hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < nrows; ++r)
+        for (int j = 0; j < n; ++j) {
+            size_t o = (size_t)r * ctx->ld + j;
+            if (esz == 16) {
+                const double* v = (const double*)buf.data() + 2 * o;
+                re[(size_t)r * n + j] = v[0];
+                im[(size_t)r * n + j] = v[1];
+            } else {
+                const float* v = (const float*)buf.data() + 2 * o;
+                re[(size_t)r * n + j] = v[0];
+                im[(size_t)r * n + j] = v[1];
+            }
+        }
+    return IBIM_OK;
+}
+
+int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const double* im, int precision) {
+    CHECK(ctx);
+    if (count <= 0 || !re || !im || (precision != IBIM_C128 && precision != IBIM_C64)) return IBIM_ERR_ARG;
+    ctx->count = count;
+    ctx->ld = ((size_t)count + 63) / 64 * 64;
+    const size_t esz = precision == IBIM_C128 ? 16 : 8;
+    if (ctx->mat.ensure(esz * ctx->ld * (size_t)count)) return IBIM_ERR_NOMEM;
+    // interleave row blocks through a pinned staging buffer
+    const int rows_per = (int)((64u << 20) / (esz * ctx->ld)) + 1;
+    void* stage = nullptr;
+    size_t stage_bytes = esz * ctx->ld * (size_t)rows_per;
+    IBIM_HIP(hipHostMalloc(&stage, stage_bytes, hipHostMallocDefault));
+    for (int r0 = 0; r0 < count; r0 += rows_per) {
+        int nr = count - r0 < rows_per ? count - r0 : rows_per;
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        for (int r = 0; r < nr; ++r)
+            for (size_t j = 0; j < ctx->ld; ++j) {
+                size_t o = (size_t)r * ctx->ld + j;
+                double a = 0.0, b = 0.0;
+                if (j < (size_t)count) {
+                    a = re[(size_t)(r0 + r) * count + j];
+                    b = im[(size_t)(r0 + r) * count + j];
+                }
+                if (esz == 16) {
+                    ((double*)stage)[2 * o] = a;
+                    ((double*)stage)[2 * o + 1] = b;
+                } else {
+                    ((float*)stage)[2 * o] = (float)a;
+                    ((float*)stage)[2 * o + 1] = (float)b;
+                }
+            }
+        IBIM_HIP(hipMemcpyAsync(ctx->mat.as<char>() + esz * ctx->ld * (size_t)r0, stage, esz * ctx->ld * (size_t)nr,
+                                hipMemcpyHostToDevice, ctx->stream));
+    }
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    (void)hipHostFree(stage);
+    ctx->mat_precision = precision;
+    return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
+}
+
+int ibim_apply(ibim_ctx* ctx, int op, const double* pre, const double* pim, const double* qre, const double* qim,
+               double* yre, double* yim, double* zre, double* zim) {
+    CHECK(ctx);
+    const int n = ctx->count;
+    if (!n || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 || (op == IBIM_OP_STORED && ctx->mat_precision < 0))
+        return IBIM_ERR_ARG;
+    if (ctx->xfer.ensure(sizeof(double2) * 4 * (size_t)n)) return IBIM_ERR_NOMEM;
+    double2* d = ctx->xfer.as<double2>();
+    std::vector<double> t(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        t[2 * i] = pre ? pre[i] : 0.0;
+        t[2 * i + 1] = pim ? pim[i] : 0.0;
+        t[2 * (size_t)n + 2 * i] = qre ? qre[i] : 0.0;
+        t[2 * (size_t)n + 2 * i + 1] = qim ? qim[i] : 0.0;
+    }
+    IBIM_HIP(hipMemcpyAsync(d, t.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, ctx->stream));
+    const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
+    int rc = matvec_op(ctx, kinds[op], d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
+    if (rc) return rc;
+    IBIM_HIP(hipMemcpyAsync(t.data(), d + 2 * (size_t)n, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < n; ++i) {
+        if (yre) yre[i] = t[2 * i];
+        if (yim) yim[i] = t[2 * i + 1];
+        if (zre) zre[i] = t[2 * (size_t)n + 2 * i];
+        if (zim) zim[i] = t[2 * (size_t)n + 2 * i + 1];
+    }
+    return IBIM_OK;
+}
+
+int ibim_set_nrhs(ibim_ctx* ctx, int nrhs) {
+    CHECK(ctx);
+    if (nrhs <= 0 || !ctx->count) return IBIM_ERR_ARG;
+    const size_t bytes = sizeof(double2) * (size_t)ctx->count * nrhs;
+    if (ctx->rhs.ensure(bytes) || ctx->density.ensure(bytes)) return IBIM_ERR_NOMEM;
+    IBIM_HIP(hipMemsetAsync(ctx->rhs.ptr, 0, bytes, ctx->stream));
+    IBIM_HIP(hipMemsetAsync(ctx->density.ptr, 0, bytes, ctx->stream));
+    ctx->nrhs = nrhs;
+    return IBIM_OK;
+}
+
+int ibim_rhs(ibim_ctx* ctx, int kind, const double* x0, double amp, const double* dir, int col) {
+    CHECK(ctx);
+    if (!ctx->count || col < 0 || col >= ctx->nrhs || kind < 0 || kind > 2) return IBIM_ERR_ARG;
+    return launch_rhs(ctx, kind, x0, amp, dir, col);
+}
+
+static int copy_col(ibim_ctx* ctx, DevBuf& buf, int col, double* re, double* im, bool to_device) {
+    const int n = ctx->count;
+    std::vector<double> tmp(2 * (size_t)n);
+    double2* d = buf.as<double2>() + (size_t)col * n;
+    if (to_device) {
+        for (int i = 0; i < n; ++i) {
+            tmp[2 * i] = re ? re[i] : 0.0;
+            tmp[2 * i + 1] = im ? im[i] : 0.0;
+        }
+        IBIM_HIP(hipMemcpyAsync(d, tmp.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    } else {
+        IBIM_HIP(hipMemcpyAsync(tmp.data(), d, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        for (int i = 0; i < n; ++i) {
+            if (re) re[i] = tmp[2 * i];
+            if (im) im[i] = tmp[2 * i + 1];
+        }
+    }
+    return IBIM_OK;
+}
+
+int ibim_set_rhs_host(ibim_ctx* ctx, int col, const double* bre, const double* bim) {
+    CHECK(ctx);
+    if (col < 0 || col >= ctx->nrhs) return IBIM_ERR_ARG;
+    return copy_col(ctx, ctx->rhs, col, (double*)bre, (double*)bim, true);
+}
+
+int ibim_get_rhs(ibim_ctx* ctx, int col, double* bre, double* bim) {
+    CHECK(ctx);
+    if (col < 0 || col >= ctx->nrhs) return IBIM_ERR_ARG;
+    return copy_col(ctx, ctx->rhs, col, bre, bim, false);
+}
+
+int ibim_get_density(ibim_ctx* ctx, int col, double* re, double* im) {
+    CHECK(ctx);
+    if (col < 0 || col >= ctx->nrhs) return IBIM_ERR_ARG;
+    return copy_col(ctx, ctx->density, col, re, im, false);
+}
+
+int ibim_set_density_host(ibim_ctx* ctx, int col, const double* re, const double* im) {
+    CHECK(ctx);
+    if (col < 0 || col >= ctx->nrhs) return IBIM_ERR_ARG;
+    return copy_col(ctx, ctx->density, col, (double*)re, (double*)im, true);
+}
+
+int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    CHECK(ctx);
+    if (!opts || !info || opts->maxit <= 0 || !ctx->count) return IBIM_ERR_ARG;
+    memset(info, 0, sizeof *info);
+    if (opts->method == IBIM_SOLVE_BICG) return run_bicg(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_REFINED) return run_refined(ctx, opts, info, hist);
+    return IBIM_ERR_ARG;
+}
+
+int ibim_evaluate(ibim_ctx* ctx, int mode, int col, int nrecv, const double* xyz, double* ure, double* uim) {
+    CHECK(ctx);
+    if (!ctx->count || nrecv <= 0 || !xyz || col < 0 || col >= ctx->nrhs || mode < 0 || mode > 2)
+        return IBIM_ERR_ARG;
+    if (ctx->xfer.ensure(sizeof(double) * 5 * (size_t)nrecv)) return IBIM_ERR_NOMEM;
+    double* dxyz = ctx->xfer.as<double>();
+    double* dout = dxyz + 3 * (size_t)nrecv;
+    IBIM_HIP(hipMemcpyAsync(dxyz, xyz, sizeof(double) * 3 * nrecv, hipMemcpyHostToDevice, ctx->stream));
+    int rc = launch_eval(ctx, mode, col, nrecv, dxyz, dout);
+    if (rc) return rc;
+    std::vector<double> tmp(2 * (size_t)nrecv);
+    IBIM_HIP(hipMemcpyAsync(tmp.data(), dout, sizeof(double) * 2 * nrecv, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < nrecv; ++i) {
+        if (ure) ure[i] = tmp[2 * i];
+        if (uim) uim[i] = tmp[2 * i + 1];
+    }
+    return IBIM_OK;
+}
+
+int ibim_set_profiling(ibim_ctx* ctx, int on) {
+    CHECK(ctx);
+    ctx->profile = on != 0;
+    return IBIM_OK;
+}
+
+int ibim_kernel_time(ibim_ctx* ctx, const char* name, double* ms_total, long* launches) {
+    CHECK(ctx);
+    collect_timers(ctx);
+    for (auto& t : ctx->timers)
+        if (t.name == name) {
+            if (ms_total) *ms_total = t.ms_total;
+            if (launches) *launches = t.launches;
+            return IBIM_OK;
+        }
+    if (ms_total) *ms_total = 0.0;
+    if (launches) *launches = 0;
+    return IBIM_OK;
+}
+
+int ibim_reset_timers(ibim_ctx* ctx) {
+    CHECK(ctx);
+    collect_timers(ctx);
+    ctx->timers.clear();
+    return IBIM_OK;
+}
+
+}  // extern "C"

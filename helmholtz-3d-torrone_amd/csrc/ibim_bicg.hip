@@ -1,0 +1,512 @@
+// ibim_bicg.hip -- the reference's complex BiCG (BICGSTAB.c:5-339/344-765) on a
+// dense operator in HBM, with the matrix streamed once per iteration.
+//
+// Fused sweep. One pass over A gives both v = A p and w = A^H p~. A workgroup
+// owns a kMvRows x kMvCols tile; thread t owns columns c0 + t + 256 m
+// (m < kMvCpt), so every row of the tile is read as kMvCpt coalesced 1 KiB
+// (c128) wavefront loads. Per row, the thread's A p contribution is reduced
+// across the wavefront with an xor butterfly and parked in LDS; the A^H p~
+// contribution stays in registers for the whole tile (p~ of the row is a
+// wave-uniform load). Tile results go to two partial planes
+// (ypart[column tile][row], zpart[row tile][column]) which the next kernel sums
+// in a fixed order: no atomics, so repeated solves are bitwise identical.
+//
+// BiCG scalars never leave the device: each iteration is four launches
+// (sweep, dot, update x/r/r~, update p/p~); every workgroup of the update
+// kernels re-reduces the previous kernel's per-block partials in the same fixed
+// order, so all of them hold identical alpha / beta. A per-iteration record
+// (BicgScalars) carries rho, alpha, beta, the residual and the stop flag; once a
+// stop rule fires, later launches of the chunk return at entry.
+#include <cmath>
+#include <vector>
+
+#include "ibim_internal.hpp"
+
+namespace ibim {
+
+namespace {
+
+constexpr int kMvThreads = 256;
+constexpr int kMvCpt = 4;
+constexpr int kMvCols = kMvThreads * kMvCpt;
+constexpr int kMvRows = kSweepRows;
+static_assert(kMvCols == kSweepCols, "sweep tiles must match the partial planes");
+constexpr int kMvUnroll = 4;
+constexpr int kVecThreads = 256;
+constexpr int kMaxVecBlocks = 1024;
+
+template <class T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+template <class MT>
+struct MatTraits;
+template <>
+struct MatTraits<double2> {
+    using acc = double;
+};
+template <>
+struct MatTraits<float2> {
+    using acc = float;
+};
+
+template <class MT>
+__global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, size_t ld, int count,
+                                                      const double2* __restrict__ p, const double2* __restrict__ q,
+                                                      double2* __restrict__ ypart, double2* __restrict__ zpart,
+                                                      const int* __restrict__ done) {
+    using AT = typename MatTraits<MT>::acc;
+    if (done != nullptr && *done) return;
+    __shared__ AT ysh[2][kMvThreads / 64][kMvRows];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int c0 = blockIdx.x * kMvCols;
+    const int r0 = blockIdx.y * kMvRows;
+    const int nrow = min(kMvRows, count - r0);
+
+    AT pre[kMvCpt], pim[kMvCpt], zre[kMvCpt], zim[kMvCpt];
+    bool cok[kMvCpt];
+#pragma unroll
+    for (int m = 0; m < kMvCpt; ++m) {
+        int c = c0 + t + kMvThreads * m;
+        cok[m] = c < count;
+        double2 pv = cok[m] ? p[c] : make_double2(0.0, 0.0);
+        pre[m] = (AT)pv.x;
+        pim[m] = (AT)pv.y;
+        zre[m] = zim[m] = (AT)0;
+    }
+    const MT* Ab = A + (size_t)r0 * ld + c0 + t;
+    for (int rr = 0; rr < nrow; rr += kMvUnroll) {
+        MT a[kMvUnroll][kMvCpt];
+#pragma unroll
+        for (int u = 0; u < kMvUnroll; ++u)
+#pragma unroll
+            for (int m = 0; m < kMvCpt; ++m) {
+                if (rr + u < nrow && cok[m]) {
+                    a[u][m] = Ab[(size_t)(rr + u) * ld + kMvThreads * m];
+                } else {
+                    a[u][m].x = 0;
+                    a[u][m].y = 0;
+                }
+            }
+#pragma unroll
+        for (int u = 0; u < kMvUnroll; ++u) {
+            const int row = rr + u;
+            double2 qv = row < nrow ? q[r0 + row] : make_double2(0.0, 0.0);
+            AT qre = (AT)qv.x, qim = (AT)qv.y;
+            AT sre = 0, sim = 0;
+#pragma unroll
+            for (int m = 0; m < kMvCpt; ++m) {
+                AT ar = (AT)a[u][m].x, ai = (AT)a[u][m].y;
+                sre += ar * pre[m] - ai * pim[m];
+                sim += ar * pim[m] + ai * pre[m];
+                zre[m] += ar * qre + ai * qim;  // conj(a) q
+                zim[m] += ar * qim - ai * qre;
+            }
+            sre = wave_sum(sre);
+            sim = wave_sum(sim);
+            if (lane == 0 && row < nrow) {
+                ysh[0][wave][row] = sre;
+                ysh[1][wave][row] = sim;
+            }
+        }
+    }
+    __syncthreads();
+    for (int row = t; row < nrow; row += kMvThreads) {
+        double sre = 0.0, sim = 0.0;
+#pragma unroll
+        for (int w = 0; w < kMvThreads / 64; ++w) {
+            sre += (double)ysh[0][w][row];
+            sim += (double)ysh[1][w][row];
+        }
+        ypart[(size_t)blockIdx.x * ld + r0 + row] = make_double2(sre, sim);
+    }
+#pragma unroll
+    for (int m = 0; m < kMvCpt; ++m) {
+        int c = c0 + t + kMvThreads * m;
+        if (cok[m]) zpart[(size_t)blockIdx.y * ld + c] = make_double2((double)zre[m], (double)zim[m]);
+    }
+}
+
+// Fixed-order block reduction of NV doubles per thread; result valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_reduce(double (&v)[NV], double* sh /* NV * 4 */) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        double s = wave_sum(v[q]);
+        if (lane == 0) sh[q * 4 + wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < NV; ++q) v[q] = (sh[q * 4 + 0] + sh[q * 4 + 1]) + (sh[q * 4 + 2] + sh[q * 4 + 3]);
+    }
+}
+
+// Every workgroup reduces the NV per-block partials of the previous kernel in
+// the same order (32 threads per value, blocks strided by 32, xor tree).
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ part, int nblocks, double* out /* LDS NV */) {
+    static_assert(NV * 32 <= kVecThreads, "too many partial values");
+    const int t = threadIdx.x, q = t >> 5, s = t & 31;
+    double acc = 0.0;
+    if (q < NV)
+        for (int b = s; b < nblocks; b += 32) acc += part[(size_t)b * NV + q];
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) acc += __shfl_xor(acc, off, 32);
+    if (q < NV && s == 0) out[q] = acc;
+    __syncthreads();
+}
+
+struct VecArgs {
+    double2 *x, *r, *rt, *p, *pt, *v, *w;
+    const double2* b;
+    double2 *ypart, *zpart;
+    int ncol_tiles, nrow_tiles;
+    size_t ld;
+    double* part;  // per-block partials
+    BicgScalars* sc;
+    double* norms;  // [0] = ||b||, [1] = sum |re|+|im| of b
+    int* done;
+    int count;
+};
+
+// x = 0, r = p = b, r~ = p~ = conj(b) (BICGSTAB.c:104-117); partials of
+// <r, r~> (4 sums), ||b||^2 (2 sums) and sum |b_re| + |b_im|.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_init(VecArgs a) {
+    __shared__ double sh[7 * 4];
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        double2 b = a.b[j];
+        double2 c = make_double2(b.x, (-1.0) * b.y);
+        a.x[j] = make_double2(0.0, 0.0);
+        a.r[j] = b;
+        a.p[j] = b;
+        a.rt[j] = c;
+        a.pt[j] = c;
+        v[0] += b.x * c.x;
+        v[1] += b.y * c.y;
+        v[2] += b.y * c.x;
+        v[3] += b.x * c.y;
+        v[4] += b.x * b.x;
+        v[5] += b.y * b.y;
+        v[6] += fabs(b.x) + fabs(b.y);
+    }
+    block_reduce<7>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 7; ++q) a.part[(size_t)blockIdx.x * 7 + q] = v[q];
+}
+
+__global__ void __launch_bounds__(kVecThreads) k_bicg_init_finish(VecArgs a, int nblocks) {
+    __shared__ double tot[7];
+    reduce_partials<7>(a.part, nblocks, tot);
+    if (threadIdx.x == 0) {
+        BicgScalars s{};
+        s.rho_re = tot[0] + tot[1];  // innerrold (BICGSTAB.c:127-128)
+        s.rho_im = tot[2] - tot[3];
+        s.err = INFINITY;            // no previous error before iteration 1
+        a.sc[0] = s;
+        a.norms[0] = sqrt(tot[4] + tot[5]);
+        a.norms[1] = tot[6];
+        *a.done = 0;
+    }
+}
+
+// The zero-data perturbation of BICGSTAB.c:138-145 (after rho was taken).
+__global__ void k_bicg_perturb(VecArgs a) {
+    a.r[0] = make_double2(0.000000001, 0.000000001);
+    a.rt[0] = make_double2(0.000000001, 0.000000001);
+}
+
+// v, w from the partial planes; partials of <r, r~> and <v, p~>.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_dot(VecArgs a) {
+    if (*a.done) return;
+    __shared__ double sh[8 * 4];
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        double yr = 0.0, yi = 0.0, zr = 0.0, zi = 0.0;
+        for (int c = 0; c < a.ncol_tiles; ++c) {
+            double2 y = a.ypart[(size_t)c * a.ld + j];
+            yr += y.x;
+            yi += y.y;
+        }
+        for (int c = 0; c < a.nrow_tiles; ++c) {
+            double2 z = a.zpart[(size_t)c * a.ld + j];
+            zr += z.x;
+            zi += z.y;
+        }
+        a.v[j] = make_double2(yr, yi);
+        a.w[j] = make_double2(zr, zi);
+        double2 r = a.r[j], rt = a.rt[j], pt = a.pt[j];
+        v[0] += r.x * rt.x;
+        v[1] += r.y * rt.y;
+        v[2] += r.y * rt.x;
+        v[3] += r.x * rt.y;
+        v[4] += yr * pt.x;
+        v[5] += yi * pt.y;
+        v[6] += yi * pt.x;
+        v[7] += yr * pt.y;
+    }
+    block_reduce<8>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 8; ++q) a.part[(size_t)blockIdx.x * 8 + q] = v[q];
+}
+
+// alpha = <r, r~> / <v, p~> (BICGSTAB.c:217-229); x += alpha p, r -= alpha v,
+// r~ -= conj(alpha) w (231-242); partials of the new <r, r~> and ||r||^2.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_update_xr(VecArgs a, int nblocks, int it, double* part_out) {
+    if (*a.done) return;
+    __shared__ double tot[8];
+    __shared__ double sh[6 * 4];
+    reduce_partials<8>(a.part, nblocks, tot);
+    const double ca = tot[0] + tot[1], cb = tot[2] - tot[3];
+    const double cc = tot[4] + tot[5], cd = tot[6] - tot[7];
+    const double num = ca * cc + cb * cd, numi = cb * cc - ca * cd, den = cc * cc + cd * cd;
+    const double al = num / den, ali = numi / den;
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        double2 x = a.x[j], p = a.p[j], r = a.r[j], vv = a.v[j], rt = a.rt[j], ww = a.w[j];
+        x.x = x.x + al * p.x - ali * p.y;
+        x.y = x.y + ali * p.x + al * p.y;
+        r.x = r.x - al * vv.x + ali * vv.y;
+        r.y = r.y - ali * vv.x - al * vv.y;
+        rt.x = rt.x - al * ww.x - ali * ww.y;
+        rt.y = rt.y + ali * ww.x - al * ww.y;
+        a.x[j] = x;
+        a.r[j] = r;
+        a.rt[j] = rt;
+        v[0] += r.x * rt.x;
+        v[1] += r.y * rt.y;
+        v[2] += r.y * rt.x;
+        v[3] += r.x * rt.y;
+        v[4] += r.x * r.x;
+        v[5] += r.y * r.y;
+    }
+    block_reduce<6>(v, sh);
+    if (threadIdx.x == 0) {
+        for (int q = 0; q < 6; ++q) part_out[(size_t)blockIdx.x * 6 + q] = v[q];
+        if (blockIdx.x == 0) {
+            a.sc[it].alpha_re = al;
+            a.sc[it].alpha_im = ali;
+        }
+    }
+}
+
+// beta = <r, r~>_new / <r, r~>_old (BICGSTAB.c:246-259); p = r + beta p,
+// p~ = r~ + conj(beta) p~ (271-278); error and stop rules (289-306).
+__global__ void __launch_bounds__(kVecThreads) k_bicg_update_p(VecArgs a, const double* __restrict__ part_in,
+                                                               int nblocks, int it, double tol) {
+    if (*a.done) return;
+    __shared__ double tot[6];
+    reduce_partials<6>(part_in, nblocks, tot);
+    const BicgScalars old = a.sc[it - 1];
+    const double ca = tot[0] + tot[1], cb = tot[2] - tot[3];
+    const double num = ca * old.rho_re + cb * old.rho_im, numi = cb * old.rho_re - ca * old.rho_im;
+    const double den = old.rho_re * old.rho_re + old.rho_im * old.rho_im;
+    const double be = num / den, bei = numi / den;
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        double2 r = a.r[j], p = a.p[j], rt = a.rt[j], pt = a.pt[j];
+        double2 np, npt;
+        np.x = r.x + be * p.x - bei * p.y;
+        np.y = r.y + bei * p.x + be * p.y;
+        npt.x = rt.x + be * pt.x + bei * pt.y;
+        npt.y = rt.y - bei * pt.x + be * pt.y;
+        a.p[j] = np;
+        a.pt[j] = npt;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const double err = sqrt(tot[4] + tot[5]) / a.norms[0];
+        BicgScalars s = a.sc[it];
+        s.rho_re = ca;
+        s.rho_im = cb;
+        s.beta_re = be;
+        s.beta_im = bei;
+        s.err = err;
+        int stop = 0;
+        if (err < tol) stop = 1;
+        else if (err != err) stop = 2;
+        else if (fabs(old.err - err) < 1e-3 * tol) stop = 3;
+        s.stop = stop;
+        a.sc[it] = s;
+        if (stop) *a.done = it;
+    }
+}
+
+unsigned vec_blocks(int count) {
+    unsigned nb = (unsigned)((count + kVecThreads - 1) / kVecThreads);
+    return nb > (unsigned)kMaxVecBlocks ? (unsigned)kMaxVecBlocks : nb;
+}
+
+}  // namespace
+
+// One sweep y = A p, z = A^H q of operator `op` into the partial planes.
+static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
+                     const int* done) {
+    if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
+    const int count = ctx->count;
+    dim3 grid((unsigned)((count + kMvCols - 1) / kMvCols), (unsigned)((count + kMvRows - 1) / kMvRows));
+    TimedScope ts(ctx, "matvec");
+    if (ctx->mat_precision == IBIM_C128)
+        hipLaunchKernelGGL(k_sweep<double2>, grid, dim3(kMvThreads), 0, ctx->stream, ctx->mat.as<double2>(), ctx->ld,
+                           count, p, q, ypart, zpart, done);
+    else
+        hipLaunchKernelGGL(k_sweep<float2>, grid, dim3(kMvThreads), 0, ctx->stream, ctx->mat.as<float2>(), ctx->ld,
+                           count, p, q, ypart, zpart, done);
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+namespace {
+
+struct Workspace {
+    VecArgs a{};
+    unsigned nvb = 0;
+    double* part2 = nullptr;
+};
+
+int prepare(Context* ctx, int maxit, Workspace* ws) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const int nct = (count + kMvCols - 1) / kMvCols, nrt = (count + kMvRows - 1) / kMvRows;
+    const unsigned nvb = vec_blocks(count);
+    const size_t vbytes = sizeof(double2) * ld;
+    if (ctx->work.ensure(vbytes * 7) || ctx->partial.ensure(sizeof(double2) * ld * (size_t)(nct + nrt)) ||
+        ctx->scal.ensure(sizeof(BicgScalars) * (size_t)(maxit + 1) + sizeof(double) * (size_t)nvb * 16 + 64) ||
+        ctx->done.ensure(sizeof(int) * 4))
+        return IBIM_ERR_NOMEM;
+    char* w = ctx->work.as<char>();
+    VecArgs& a = ws->a;
+    a.x = (double2*)(w + 0 * vbytes);
+    a.r = (double2*)(w + 1 * vbytes);
+    a.rt = (double2*)(w + 2 * vbytes);
+    a.p = (double2*)(w + 3 * vbytes);
+    a.pt = (double2*)(w + 4 * vbytes);
+    a.v = (double2*)(w + 5 * vbytes);
+    a.w = (double2*)(w + 6 * vbytes);
+    a.ypart = ctx->partial.as<double2>();
+    a.zpart = a.ypart + ld * (size_t)nct;
+    a.ncol_tiles = nct;
+    a.nrow_tiles = nrt;
+    a.ld = ld;
+    a.sc = ctx->scal.as<BicgScalars>();
+    double* tail = (double*)(a.sc + (maxit + 1));
+    a.part = tail;
+    ws->part2 = tail + (size_t)nvb * 8;
+    a.norms = ws->part2 + (size_t)nvb * 6;
+    a.done = ctx->done.as<int>();
+    a.count = count;
+    ws->nvb = nvb;
+    return IBIM_OK;
+}
+
+// One BiCG solve of A x = b for b, x on the device (length count). Returns
+// iterations and status in info; hist (host, may be NULL) gets the residuals.
+int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* x_out, int maxit, double tol,
+                int chunk, ibim_solve_info* info, double* hist) {
+    VecArgs a = ws.a;
+    a.b = b;
+    const unsigned nvb = ws.nvb;
+    hipLaunchKernelGGL(k_bicg_init, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_bicg_init_finish, dim3(1), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb);
+    IBIM_HIP(hipGetLastError());
+    double norms[2];
+    IBIM_HIP(hipMemcpyAsync(norms, a.norms, sizeof norms, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    info->iterations = 0;
+    info->rel_residual = 0.0;
+    if (norms[1] < 1e-13) hipLaunchKernelGGL(k_bicg_perturb, dim3(1), dim3(1), 0, ctx->stream, a);
+    if (norms[1] < 1e-15) {  // zero data: zero density (BICGSTAB.c:147-156)
+        IBIM_HIP(hipMemsetAsync(x_out, 0, sizeof(double2) * ctx->count, ctx->stream));
+        info->status = 4;
+        return IBIM_OK;
+    }
+    int it = 1, done = 0;
+    if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
+    while (it <= maxit && !done) {
+        int last = it + chunk - 1 < maxit ? it + chunk - 1 : maxit;
+        for (; it <= last; ++it) {
+            int rc = sweep_any(ctx, op, a.p, a.pt, a.ypart, a.zpart, a.done);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_bicg_dot, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_bicg_update_xr, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb, it,
+                               ws.part2);
+            hipLaunchKernelGGL(k_bicg_update_p, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, ws.part2, (int)nvb,
+                               it, tol);
+        }
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipMemcpyAsync(&done, a.done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    const int iters = done ? done : maxit;
+    std::vector<BicgScalars> sc((size_t)iters + 1);
+    IBIM_HIP(hipMemcpyAsync(sc.data(), a.sc, sizeof(BicgScalars) * sc.size(), hipMemcpyDeviceToHost, ctx->stream));
+    if (x_out != a.x)
+        IBIM_HIP(hipMemcpyAsync(x_out, a.x, sizeof(double2) * ctx->count, hipMemcpyDeviceToDevice, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    if (hist)
+        for (int i = 1; i <= iters; ++i) hist[i - 1] = sc[i].err;
+    info->iterations = iters;
+    info->rel_residual = sc[iters].err;
+    info->status = done ? (sc[iters].stop == 1 ? 0 : sc[iters].stop) : 1;
+    return IBIM_OK;
+}
+
+}  // namespace
+
+int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev) {
+    Workspace ws;
+    int rc = prepare(ctx, 1, &ws);
+    if (rc) return rc;
+    rc = sweep_any(ctx, op, (const double2*)p_dev, (const double2*)q_dev, ws.a.ypart, ws.a.zpart, nullptr);
+    if (rc) return rc;
+    // sum the partial planes with the dot kernel (its dot products are unused)
+    VecArgs a = ws.a;
+    a.v = (double2*)y_dev;
+    a.w = (double2*)z_dev;
+    a.r = (double2*)p_dev;
+    a.rt = (double2*)q_dev;
+    a.pt = (double2*)q_dev;
+    IBIM_HIP(hipMemsetAsync(a.done, 0, sizeof(int), ctx->stream));
+    hipLaunchKernelGGL(k_bicg_dot, dim3(ws.nvb), dim3(kVecThreads), 0, ctx->stream, a);
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev) {
+    return matvec_op(ctx, OP_DENSE, p_dev, q_dev, y_dev, z_dev);
+}
+
+int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
+                  ibim_solve_info* info, double* hist) {
+    Workspace ws;
+    int rc = prepare(ctx, maxit, &ws);
+    if (rc) return rc;
+    return bicg_device(ctx, op, ws, (const double2*)b_dev, (double2*)x_dev, maxit, tol, chunk, info, hist);
+}
+
+int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    // the stored operator when there is one, else the FP64 entries recomputed per sweep
+    const int op = ctx->mat_precision >= 0 ? OP_DENSE : OP_FREE64;
+    Workspace ws;
+    int rc = prepare(ctx, opts->maxit, &ws);
+    if (rc) return rc;
+    int total = 0;
+    ibim_solve_info last{};
+    for (int c = 0; c < ctx->nrhs; ++c) {
+        const double2* b = ctx->rhs.as<double2>() + (size_t)c * ctx->count;
+        double2* x = ctx->density.as<double2>() + (size_t)c * ctx->count;
+        rc = bicg_device(ctx, op, ws, b, x, opts->maxit, opts->tol, opts->chunk, &last, c == 0 ? hist : nullptr);
+        if (rc) return rc;
+        total += last.iterations;
+    }
+    *info = last;
+    info->iterations = total;
+    info->outer = 1;
+    ctx->last_iters = total;
+    return IBIM_OK;
+}
+
+}  // namespace ibim

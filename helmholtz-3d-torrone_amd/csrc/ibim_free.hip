@@ -1,0 +1,317 @@
+// ibim_free.hip -- matrix-free fused sweeps and FP64 iterative refinement.
+//
+// When the count x count operator does not fit (or should not be stored), each
+// BiCG iteration recomputes the entries of a kSweepRows x kSweepCols tile in
+// registers and applies them to p (row sums) and p~ (column sums) from that one
+// evaluation -- the same tile / partial-plane structure as the dense sweep, so
+// the BiCG vector kernels are shared.
+//
+// OP_FREE64 evaluates the reference's FP64 entries (ibim_entries.hpp).
+// OP_FREE32 evaluates the traditional entry in FP32 except for the phase: the
+// separation |x*_i - y*_j|^2 is formed in FP64, its square root is refined by
+// one FP64 Newton step, the phase k rho / (2 pi) is reduced modulo 1 in FP64,
+// and only the reduced revolution fraction goes to the FP32 sin / cos units
+// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry and both
+// products accumulate in FP32 over a tile; tiles are summed in FP64.
+//
+// Refinement: x = 0, r = b; repeat { solve A32 d = r by BiCG to inner_tol;
+// x += d; r = b - A64 x with the FP64 operator } until ||r|| / ||b|| < tol.
+#include <cmath>
+#include <vector>
+
+#include "ibim_entries.hpp"
+#include "ibim_internal.hpp"
+
+namespace ibim {
+
+namespace {
+
+constexpr int kFT = 256;
+constexpr int kFCpt = kSweepCols / kFT;
+constexpr int kFRows = kSweepRows;
+constexpr int kStage = 64;
+
+struct RowStage {
+    double sx[kStage], sy[kStage], sz[kStage], nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage],
+        dist[kStage];
+    double qre[kStage], qim[kStage];
+};
+
+template <class T>
+__device__ __forceinline__ T wsum(T v) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+
+// Traditional entry with FP32 arithmetic around an FP64-reduced phase.
+__device__ __forceinline__ void trad_entry_f32(const PointData& a, const PointData& b, bool diag, double kappa,
+                                               float kf, double tau2, int dirichlet, int interior, float* re,
+                                               float* im) {
+    const double d1 = a.sx - b.sx, d2 = a.sy - b.sy, d3 = a.sz - b.sz;
+    const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+    float pr, pim;
+    if (rho2 > tau2) {
+        const float r = sqrtf((float)rho2);
+        const float ir = 1.0f / r;
+        const double rd = (double)r;
+        const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);  // one Newton step in FP64
+        const double t = rho * kappa;                                            // phase in revolutions
+        const float f = (float)(t - rint(t));                                    // in [-1/2, 1/2]
+        const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
+        const float den = (float)(1.0 / (4.0 * kPi)) * ir * ir;
+        const float c1 = -(kf * sv + cv * ir) * den;
+        const float c2 = (kf * cv - sv * ir) * den;
+        const float ip = dirichlet ? (float)d1 * (float)b.nx + (float)d2 * (float)b.ny + (float)d3 * (float)b.nz
+                                   : (float)d1 * (float)a.nx + (float)d2 * (float)a.ny + (float)d3 * (float)a.nz;
+        pr = interior ? -c1 * ip : c1 * ip;
+        pim = interior ? -c2 * ip : c2 * ip;
+    } else {
+        pr = (float)(dirichlet ? b.reg : a.reg);
+        pim = 0.0f;
+    }
+    *re = (float)b.w * pr;
+    *im = (float)b.w * pim;
+    if (diag) *re += 0.5f;
+}
+
+template <int Kind, int Op>
+__global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ tube, int count, ibim_params prm,
+                                                    const double2* __restrict__ p, const double2* __restrict__ q,
+                                                    double2* __restrict__ ypart, double2* __restrict__ zpart,
+                                                    size_t pld, const int* __restrict__ done) {
+    using AT = typename std::conditional<Op == OP_FREE32, float, double>::type;
+    if (done != nullptr && *done) return;
+    __shared__ RowStage st;
+    __shared__ AT ysh[2][kFT / 64][kFRows];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int c0 = blockIdx.x * kSweepCols;
+    const int r0 = blockIdx.y * kFRows;
+    const int nrow = min(kFRows, count - r0);
+    const size_t cnt = (size_t)count;
+    const double kappa = prm.k / (2.0 * kPi);
+    const float kf = (float)prm.k;
+    const double tau2 = prm.tau * prm.tau;
+
+    PointData col[kFCpt];
+    AT pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
+    int cj[kFCpt];
+#pragma unroll
+    for (int m = 0; m < kFCpt; ++m) {
+        cj[m] = c0 + t + kFT * m;
+        if (cj[m] < count) {
+            col[m] = load_point(tube, cnt, cj[m]);
+            double2 pv = p[cj[m]];
+            pre[m] = (AT)pv.x;
+            pim[m] = (AT)pv.y;
+        } else {
+            col[m] = PointData{};
+            pre[m] = pim[m] = (AT)0;
+        }
+        zre[m] = zim[m] = (AT)0;
+    }
+    for (int s0 = 0; s0 < nrow; s0 += kStage) {
+        const int ns = min(kStage, nrow - s0);
+        __syncthreads();
+        if (t < ns) {
+            const int i = r0 + s0 + t;
+            st.sx[t] = tube[IBIM_F_SX * cnt + i];
+            st.sy[t] = tube[IBIM_F_SY * cnt + i];
+            st.sz[t] = tube[IBIM_F_SZ * cnt + i];
+            st.nx[t] = tube[IBIM_F_NX * cnt + i];
+            st.ny[t] = tube[IBIM_F_NY * cnt + i];
+            st.nz[t] = tube[IBIM_F_NZ * cnt + i];
+            st.w[t] = tube[IBIM_F_W * cnt + i];
+            st.reg[t] = tube[IBIM_F_REG * cnt + i];
+            st.dist[t] = tube[IBIM_F_DIST * cnt + i];
+            double2 qv = q[i];
+            st.qre[t] = qv.x;
+            st.qim[t] = qv.y;
+        }
+        __syncthreads();
+        for (int s = 0; s < ns; ++s) {
+            const int i = r0 + s0 + s;
+            PointData a;
+            a.sx = st.sx[s];
+            a.sy = st.sy[s];
+            a.sz = st.sz[s];
+            a.nx = st.nx[s];
+            a.ny = st.ny[s];
+            a.nz = st.nz[s];
+            a.w = st.w[s];
+            a.reg = st.reg[s];
+            a.dist = st.dist[s];
+            const AT qre = (AT)st.qre[s], qim = (AT)st.qim[s];
+            AT sre = 0, sim = 0;
+#pragma unroll
+            for (int m = 0; m < kFCpt; ++m) {
+                AT er = 0, ei = 0;
+                if (cj[m] < count) {
+                    if (Op == OP_FREE32) {
+                        float fr, fi;
+                        trad_entry_f32(a, col[m], i == cj[m], kappa, kf, tau2, prm.dirichlet, prm.interior, &fr, &fi);
+                        er = (AT)fr;
+                        ei = (AT)fi;
+                    } else {
+                        double dr, di;
+                        if (Kind == IBIM_TUBE_TRADITIONAL)
+                            trad_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.dirichlet, prm.interior, &dr, &di);
+                        else
+                            combo_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.eta, &dr, &di);
+                        er = (AT)dr;
+                        ei = (AT)di;
+                    }
+                }
+                sre += er * pre[m] - ei * pim[m];
+                sim += er * pim[m] + ei * pre[m];
+                zre[m] += er * qre + ei * qim;
+                zim[m] += er * qim - ei * qre;
+            }
+            sre = wsum(sre);
+            sim = wsum(sim);
+            if (lane == 0) {
+                ysh[0][wave][s0 + s] = sre;
+                ysh[1][wave][s0 + s] = sim;
+            }
+        }
+    }
+    __syncthreads();
+    for (int row = t; row < nrow; row += kFT) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int w = 0; w < kFT / 64; ++w) {
+            a += (double)ysh[0][w][row];
+            b += (double)ysh[1][w][row];
+        }
+        ypart[(size_t)blockIdx.x * pld + r0 + row] = make_double2(a, b);
+    }
+#pragma unroll
+    for (int m = 0; m < kFCpt; ++m)
+        if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2((double)zre[m], (double)zim[m]);
+}
+
+// r = b - y, partial sums of |r|^2 per block.
+__global__ void __launch_bounds__(256) k_residual(const double2* __restrict__ b, const double2* __restrict__ y,
+                                                  double2* __restrict__ r, int count, double* __restrict__ part) {
+    __shared__ double sh[4];
+    double s = 0.0;
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
+        double2 bb = b[j], yy = y[j];
+        double2 rr = make_double2(bb.x - yy.x, bb.y - yy.y);
+        r[j] = rr;
+        s += rr.x * rr.x + rr.y * rr.y;
+    }
+    s = wsum(s);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+__global__ void __launch_bounds__(256) k_add(double2* __restrict__ x, const double2* __restrict__ d, int count) {
+    for (int j = blockIdx.x * 256 + threadIdx.x; j < count; j += gridDim.x * 256) {
+        double2 a = x[j], e = d[j];
+        x[j] = make_double2(a.x + e.x, a.y + e.y);
+    }
+}
+
+unsigned nblocks(int count) {
+    unsigned nb = (unsigned)((count + 255) / 256);
+    return nb > 1024u ? 1024u : nb;
+}
+
+}  // namespace
+
+int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+    const int count = ctx->count;
+    dim3 grid((unsigned)((count + kSweepCols - 1) / kSweepCols), (unsigned)((count + kFRows - 1) / kFRows));
+    const double* tube = ctx->tube.as<double>();
+    auto P = (const double2*)p;
+    auto Q = (const double2*)q;
+    auto Y = (double2*)ypart;
+    auto Z = (double2*)zpart;
+    if (op == OP_FREE32 && ctx->kind != IBIM_TUBE_TRADITIONAL) op = OP_FREE64;  // FP32 entries: traditional only
+    TimedScope ts(ctx, op == OP_FREE32 ? "sweep_free32" : "sweep_free64");
+    if (op == OP_FREE32)
+        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_TRADITIONAL, OP_FREE32>), grid, dim3(kFT), 0, ctx->stream, tube,
+                           count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
+    else if (ctx->kind == IBIM_TUBE_TRADITIONAL)
+        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_TRADITIONAL, OP_FREE64>), grid, dim3(kFT), 0, ctx->stream, tube,
+                           count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
+    else
+        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_COMBO, OP_FREE64>), grid, dim3(kFT), 0, ctx->stream, tube, count,
+                           ctx->prm, P, Q, Y, Z, ctx->ld, done);
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const int inner = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
+    const int outer_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    if (inner == OP_DENSE && outer_op == OP_DENSE) return IBIM_ERR_ARG;  // unreachable: one precision stored
+    if (ctx->refine.ensure(sizeof(double2) * ld * 4 + sizeof(double) * 1024)) return IBIM_ERR_NOMEM;
+    double2* r = ctx->refine.as<double2>();
+    double2* d = r + ld;
+    double2* y = d + ld;
+    double2* z = y + ld;
+    double* part = (double*)(z + ld);
+    const unsigned nb = nblocks(count);
+    std::vector<double> hpart(nb);
+    const double inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-4;
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    int total_its = 0, outer = 0, hist_at = 0;
+    double res = 0.0;
+    int status = 1;
+    for (int c = 0; c < ctx->nrhs; ++c) {
+        const double2* b = ctx->rhs.as<double2>() + (size_t)c * count;
+        double2* x = ctx->density.as<double2>() + (size_t)c * count;
+        IBIM_HIP(hipMemsetAsync(x, 0, sizeof(double2) * count, ctx->stream));
+        // ||b||
+        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, ctx->stream, b, x, r, count, part);
+        IBIM_HIP(hipMemcpyAsync(hpart.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        double nb2 = 0.0;
+        for (unsigned i = 0; i < nb; ++i) nb2 += hpart[i];
+        const double normb = sqrt(nb2);
+        if (normb == 0.0) {
+            status = 4;
+            continue;
+        }
+        status = 1;
+        for (outer = 1; outer <= max_outer; ++outer) {
+            ibim_solve_info in{};
+            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, inner_tol, opts->chunk, &in, nullptr);
+            if (rc) return rc;
+            total_its += in.iterations;
+            hipLaunchKernelGGL(k_add, dim3(nb), dim3(256), 0, ctx->stream, x, d, count);
+            rc = matvec_op(ctx, outer_op, x, x, y, z);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_residual, dim3(nb), dim3(256), 0, ctx->stream, b, y, r, count, part);
+            IBIM_HIP(hipGetLastError());
+            IBIM_HIP(hipMemcpyAsync(hpart.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, ctx->stream));
+            IBIM_HIP(hipStreamSynchronize(ctx->stream));
+            double s = 0.0;
+            for (unsigned i = 0; i < nb; ++i) s += hpart[i];
+            res = sqrt(s) / normb;
+            if (hist && c == 0 && hist_at < opts->maxit) hist[hist_at++] = res;
+            if (res < opts->tol) {
+                status = 0;
+                break;
+            }
+            if (res != res) {
+                status = 2;
+                break;
+            }
+        }
+        if (outer > max_outer) outer = max_outer;
+    }
+    info->iterations = total_its;
+    info->outer = outer;
+    info->rel_residual = res;
+    info->status = status;
+    ctx->last_iters = total_its;
+    return IBIM_OK;
+}
+
+}  // namespace ibim

@@ -1,0 +1,140 @@
+// ibim_internal.hpp -- device-side state of one IBIM problem and the launch
+// helpers shared by the translation units of libibim.so.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <deque>
+#include <string>
+#include <vector>
+
+#include "../../include/ibim.h"
+
+namespace ibim {
+
+void set_error(const char* fmt, ...);
+
+#define IBIM_HIP(call)                                                                          \
+    do {                                                                                        \
+        hipError_t e_ = (call);                                                                 \
+        if (e_ != hipSuccess) {                                                                 \
+            ::ibim::set_error("%s failed: %s (%s:%d)", #call, hipGetErrorString(e_), __FILE__, \
+                              __LINE__);                                                        \
+            return IBIM_ERR_HIP;                                                                \
+        }                                                                                       \
+    } while (0)
+
+// A device buffer that only grows.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want);
+    void release();
+    template <class T>
+    T* as() const {
+        return static_cast<T*>(ptr);
+    }
+};
+
+// Per-iteration scalars of the device-resident BiCG (one record per iteration).
+struct BicgScalars {
+    double rho_re, rho_im;    // <r, r~> after the iteration
+    double alpha_re, alpha_im;
+    double beta_re, beta_im;
+    double err;               // ||r|| / ||b||
+    int stop;                 // 1 once a stop rule fired at this iteration
+    int pad;
+};
+
+// Timing record of one kernel class (hipEvent pairs on the context stream).
+struct KernelTimer {
+    std::string name;
+    std::vector<hipEvent_t> begin, end;
+    double ms_total = 0.0;
+    long launches = 0;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+
+    // grid
+    int n = 0;
+    double lo = -2.0, h = 0.0;
+    DevBuf level, gx, gy, gz, mc, gc;
+    bool have_geometry = false;
+
+    // tube
+    ibim_params prm{};
+    int kind = 0;
+    int count = 0;
+    DevBuf node, tube, blockcnt, err_flag;
+
+    // operator
+    int mat_precision = -1;  // IBIM_C128 / IBIM_C64, -1 when not assembled
+    size_t ld = 0;
+    DevBuf mat;
+    int regcount = 0;
+
+    // right-hand sides and densities: nrhs columns of length count (complex FP64)
+    int nrhs = 0;
+    DevBuf rhs, density;
+
+    // solver workspace
+    DevBuf work, partial, scal, done;
+    DevBuf xfer;    // receivers and other small transfers
+    DevBuf refine;  // refinement vectors
+    int last_iters = 0;
+
+    // profiling
+    bool profile = false;
+    std::deque<KernelTimer> timers;  // stable addresses for open TimedScopes
+
+    // scratch host copies
+    std::vector<double> host_tmp;
+};
+
+// Timed launch on ctx->stream: records hipEvents around the launch when
+// profiling is on, so the per-kernel average comes from the stream the kernel
+// runs on.
+KernelTimer* timer_for(Context* ctx, const char* name);
+void timer_begin(Context* ctx, KernelTimer* t);
+void timer_end(Context* ctx, KernelTimer* t);
+
+struct TimedScope {
+    Context* ctx;
+    KernelTimer* t;
+    TimedScope(Context* c, const char* name) : ctx(c), t(c->profile ? timer_for(c, name) : nullptr) {
+        if (t) timer_begin(ctx, t);
+    }
+    ~TimedScope() {
+        if (t) timer_end(ctx, t);
+    }
+};
+
+// Kernel launchers (defined in the .hip translation units).
+int launch_gradient(Context* ctx, int order);
+int launch_curvatures(Context* ctx);
+int launch_tube(Context* ctx, int* count_out);
+int launch_assemble(Context* ctx, int precision);
+int launch_rhs(Context* ctx, int kind, const double* x0, double amp, const double* dir, int col);
+int launch_eval(Context* ctx, int mode, int col, int nrecv, const double* xyz_dev, double* out_dev);
+int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
+
+// Operator used by a sweep: the stored matrix, or entries recomputed on the fly.
+enum OpKind { OP_DENSE = 0, OP_FREE64 = 1, OP_FREE32 = 2 };
+// Tile sizes shared by every sweep (the partial planes have these shapes).
+constexpr int kSweepCols = 1024;
+constexpr int kSweepRows = 512;
+int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done);
+int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
+int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
+                  ibim_solve_info* info, double* hist);
+
+}  // namespace ibim
+
+// Handle type of the public API.
+struct ibim_ctx : ibim::Context {};

@@ -1,0 +1,228 @@
+/*
+ * ibim.h -- C ABI of libibim.so, the MI355X (gfx950) implementation of the
+ * implicit boundary integral (IBIM) hot path of moorhsum/Helmholtz-3D-torrone.
+ *
+ * Two layers:
+ *
+ *  1. Drop-in entry points with the reference's names and argument lists
+ *     (HelmholtzKernel.h:20-39, BICGSTAB.h:15-17, weno.h:26-31,
+ *     distance_application3D.h:9-12). They take the reference's host data
+ *     structures (double*** grids, double** row-pointer matrices), run the work
+ *     on the GPU and return the same outputs. libibim.so exports each of them
+ *     twice: with C linkage (declared here, for C / ctypes / FFI callers) and with
+ *     the C++ linkage the reference's own build produces (its .c files are
+ *     compiled as C++, Makefile:1,10-11), so a program built from the
+ *     reference's headers links against libibim.so unchanged. Like the
+ *     reference, these read the caller-defined globals N, SIZE, H, HCUBED, WAVE
+ *     (HelmholtzKernel.h:16-18); libibim.so carries weak definitions that a
+ *     caller's own definitions override.
+ *
+ *  2. A device-resident handle API (ibim_*) for callers that keep the problem
+ *     in HBM: level set -> geometry -> tube -> operator -> right-hand sides ->
+ *     solve -> field evaluation, with every array resident on the GPU between
+ *     calls. The reference has no such API (its driver, large3Dexample2.c:97-1215,
+ *     calls file-static functions); it is what the drop-in entries are built on.
+ *
+ * All functions return 0 (IBIM_OK) on success and a negative IBIM_ERR_* code
+ * otherwise -- where the reference prints and calls exit(0) -- with a message
+ * available from ibim_last_error(). Calls on one handle are not thread-safe;
+ * each handle owns one HIP stream on the device current at ibim_create().
+ */
+#ifndef IBIM_H
+#define IBIM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- errors --- */
+#define IBIM_OK 0
+#define IBIM_ERR_ARG (-1)        /* bad argument or call order                     */
+#define IBIM_ERR_HIP (-2)        /* HIP runtime error                              */
+#define IBIM_ERR_SIZE (-3)       /* size != tube count (HelmholtzKernel.c:113-117) */
+#define IBIM_ERR_GEOMETRY (-4)   /* closest point outside the grid / NaN
+                                    (interpolate3D.c:31-36, 81-85)                 */
+#define IBIM_ERR_NOMEM (-5)      /* device allocation failed                       */
+#define IBIM_ERR_NODEVICE (-6)   /* no HIP device                                  */
+
+/* ------------------------------------------------------------ data model --- */
+/* Tube fields, field-major (field f of point p at fields[f * count + p]). */
+enum ibim_field {
+    IBIM_F_ZX = 0, IBIM_F_ZY, IBIM_F_ZZ, /* grid node z                        */
+    IBIM_F_SX, IBIM_F_SY, IBIM_F_SZ,     /* closest point z* = z - d grad d    */
+    IBIM_F_DIST,                         /* d(z)                               */
+    IBIM_F_NX, IBIM_F_NY, IBIM_F_NZ,     /* unit normal at z* (grad d, inward) */
+    IBIM_F_DELTA,                        /* delta_eps(d), or the combo weight W */
+    IBIM_F_JAC,                          /* J = 1 + 2 d H + d^2 K at z         */
+    IBIM_F_W,                            /* quadrature weight h^3 delta J, or R */
+    IBIM_F_REG,                          /* near-field value                   */
+    IBIM_F_HM, IBIM_F_GK,                /* mean / Gaussian curvature at z*    */
+    IBIM_NF
+};
+
+/* Run-time replacement of the reference's compile-time switches (PDE.h:6-30,
+ * functions.h:20-30) plus the tube / regularisation sizes it passes around. */
+typedef struct ibim_params {
+    double k;          /* wavenumber (WAVE)                                   */
+    double eps;        /* tube half width epsilon (FACTOR * H)                */
+    double tau;        /* near-field radius (TAU_FACTOR * H)                  */
+    double delta_in;   /* inner cut-off of the combo tube (delta0)            */
+    double eta;        /* combined-field coupling (DEFAULT_ETA)               */
+    double sine_root;  /* inner root of the Sine weight; 0.1 = the reference  */
+    int interior;      /* INTERIOR_MODE                                       */
+    int dirichlet;     /* DIRICHLET: double-layer normal at the source point  */
+    int one_sided;     /* TRAD_ONESIDED / COMBO_ONESIDED                      */
+} ibim_params;
+
+#define IBIM_TUBE_TRADITIONAL 0 /* |d| <= eps, cosine delta (large3Dexample2.c:4340-4452) */
+#define IBIM_TUBE_COMBO 1       /* delta_in <= |d| <= eps, Sine weight (HelmholtzKernel.c:223-400) */
+
+#define IBIM_C128 0 /* complex FP64 storage, 16 B per entry */
+#define IBIM_C64 1  /* complex FP32 storage,  8 B per entry */
+
+#define IBIM_RHS_POINT_SOURCE 0    /* Phi(x*, x0) = e^{ik|x*-x0|} / (4 pi |x*-x0|)        */
+#define IBIM_RHS_PLANE_NEUMANN 1   /* amp d/dn e^{ik dir.x} at x* (cal_partialu*)        */
+#define IBIM_RHS_PLANE_DIRICHLET 2 /* -amp e^{ik dir.x*} (sound-soft data)               */
+
+#define IBIM_EVAL_SINGLE 0 /* sum_j -Phi(x,y*_j) phi_j w_j   (NeumannTest, large3Dexample2.c:1810-1824) */
+#define IBIM_EVAL_DOUBLE 1 /* sum_j w_j grad_x Phi(x,y*_j).n_j phi_j (the Dirichlet kernel's layer)        */
+#define IBIM_EVAL_COMBO 2  /* Combo_NeumannTest sample kernel (large3Dexample2.c:1980-2058)                  */
+
+#define IBIM_SOLVE_BICG 0    /* the reference's complex BiCG on the stored operator      */
+#define IBIM_SOLVE_REFINED 1 /* FP64 iterative refinement around a complex-FP32 BiCG      */
+
+typedef struct ibim_solve_opts {
+    int method;        /* IBIM_SOLVE_*                                        */
+    int maxit;         /* BiCG iteration cap (1000 / 2000 in the reference)   */
+    double tol;        /* relative residual target (DEFAULTACC = 1e-7)        */
+    double inner_tol;  /* refinement: inner FP32 BiCG relative tolerance      */
+    int max_outer;     /* refinement: outer FP64 steps                        */
+    int chunk;         /* iterations launched between host checks (0 = auto)  */
+} ibim_solve_opts;
+
+typedef struct ibim_solve_info {
+    int iterations;      /* BiCG iterations (summed over refinement steps)   */
+    int outer;           /* refinement steps                                 */
+    double rel_residual; /* last ||r|| / ||b|| (recursive residual for BiCG, */
+                         /* true FP64 residual for refinement)               */
+    int status;          /* 0 converged, 1 iteration cap, 2 NaN, 3 stagnation, 4 zero RHS */
+} ibim_solve_info;
+
+/* ------------------------------------------------------ handle lifecycle --- */
+typedef struct ibim_ctx ibim_ctx;
+
+int ibim_device_count(void);
+const char *ibim_last_error(void);
+int ibim_create(ibim_ctx **out);
+void ibim_destroy(ibim_ctx *ctx);
+int ibim_sync(ibim_ctx *ctx);
+
+/* ------------------------------------------------------------- geometry --- */
+/* Level set on the (n+1)^3 grid over [lo, lo + n h]^3, layout [i=y][j=x][k=z]
+ * (large3Dexample2.c:405-424). _host copies from host memory, _device from a
+ * device pointer on the handle's device. */
+int ibim_set_level_host(ibim_ctx *ctx, int n, double lo, double h, const double *level);
+int ibim_set_level_device(ibim_ctx *ctx, int n, double lo, double h, const double *level_dev);
+/* WENO-3/5 Godunov gradient and curvatures on every node
+ * (ASSIGN_GRADIENT distance_application3D.c:56-141; BothCurvs :35-54). */
+int ibim_geometry(ibim_ctx *ctx, int weno_order);
+/* Host copies of the grid fields (any pointer may be NULL). */
+int ibim_get_grid(ibim_ctx *ctx, double *gx, double *gy, double *gz, double *mc, double *gc);
+/* Supply gradient and curvatures computed elsewhere (host memory). */
+int ibim_set_grid_fields_host(ibim_ctx *ctx, const double *gx, const double *gy, const double *gz, const double *mc,
+                              const double *gc);
+
+/* ----------------------------------------------------------------- tube --- */
+/* Extract the tube (raster order), project, weight and regularise on the GPU. */
+int ibim_tube(ibim_ctx *ctx, const ibim_params *prm, int kind, int *count);
+int ibim_get_tube(ibim_ctx *ctx, int *node, double *fields);
+/* Device pointer to the field-major tube fields (IBIM_NF x count doubles). */
+const double *ibim_tube_device(ibim_ctx *ctx);
+
+/* ------------------------------------------------------------- operator --- */
+/* Dense count x count operator in HBM (traditional kernel for
+ * IBIM_TUBE_TRADITIONAL, combined-field kernel for IBIM_TUBE_COMBO). */
+int ibim_assemble(ibim_ctx *ctx, int precision);
+/* Host copy of rows [row0, row0 + nrows) as split planes (row-major, count wide). */
+int ibim_get_rows(ibim_ctx *ctx, int row0, int nrows, double *re, double *im);
+/* Upload a dense operator given as split planes (row-major, count x count). */
+int ibim_set_matrix_host(ibim_ctx *ctx, int count, const double *re, const double *im, int precision);
+/* Near-field entry count of the last assembly (the combo kernel's regcount). */
+int ibim_regcount(ibim_ctx *ctx);
+
+/* One application y = A p, z = A^H q (host vectors of length count, split
+ * re / im) of the stored operator (op IBIM_OP_STORED) or of the entries
+ * recomputed on the fly in FP64 (IBIM_OP_FREE64) or in FP32 around an FP64
+ * phase (IBIM_OP_FREE32, traditional kernel only). */
+#define IBIM_OP_STORED 0
+#define IBIM_OP_FREE64 1
+#define IBIM_OP_FREE32 2
+int ibim_apply(ibim_ctx *ctx, int op, const double *pre, const double *pim, const double *qre, const double *qim,
+               double *yre, double *yim, double *zre, double *zim);
+
+/* ------------------------------------------------- right-hand sides/solve --- */
+int ibim_set_nrhs(ibim_ctx *ctx, int nrhs);
+int ibim_rhs(ibim_ctx *ctx, int kind, const double *x0, double amp, const double *dir, int col);
+int ibim_set_rhs_host(ibim_ctx *ctx, int col, const double *bre, const double *bim);
+int ibim_get_rhs(ibim_ctx *ctx, int col, double *bre, double *bim);
+int ibim_solve(ibim_ctx *ctx, const ibim_solve_opts *opts, ibim_solve_info *info, double *hist);
+int ibim_get_density(ibim_ctx *ctx, int col, double *re, double *im);
+int ibim_set_density_host(ibim_ctx *ctx, int col, const double *re, const double *im);
+
+/* ------------------------------------------------------------ evaluation --- */
+/* u at nrecv receivers xyz (host, 3 per receiver) from density column col. */
+int ibim_evaluate(ibim_ctx *ctx, int mode, int col, int nrecv, const double *xyz, double *ure, double *uim);
+
+/* ------------------------------------------------------------- profiling --- */
+/* When on, kernels are bracketed by hipEvents on the handle's stream. */
+int ibim_set_profiling(ibim_ctx *ctx, int on);
+/* Total milliseconds and launch count of a kernel class ("matvec", "assemble", ...). */
+int ibim_kernel_time(ibim_ctx *ctx, const char *name, double *ms_total, long *launches);
+int ibim_reset_timers(ibim_ctx *ctx);
+
+/* ------------------------------------------------ reference drop-in layer --- */
+/* weno.h:26-31 */
+void WENO_DX(double ***uvalue, const int i, const int j, const int k, const int N, const double H, double *ux_p,
+             double *ux_n, const int order);
+void WENO_DY(double ***uvalue, const int i, const int j, const int k, const int N, const double H, double *ux_p,
+             double *ux_n, const int order);
+void WENO_DZ(double ***uvalue, const int i, const int j, const int k, const int N, const double H, double *ux_p,
+             double *ux_n, const int order);
+void weno(double *dfp, double *dfn, double fi[], const double dx, const int order);
+
+/* distance_application3D.h:9-12 (ASSIGN_GRADIENT runs on the GPU) */
+double GaussianCurv(double ***dist, int i, int j, int k, int N, double H);
+double MeanCurv(double ***dist, int i, int j, int k, int N, double H);
+int BothCurvs(double ***dist, int i, int j, int k, int N, double H, double *meancurv, double *gausscurv);
+int ASSIGN_GRADIENT(int N, double H, double ***dist, double ***graddx, double ***graddy, double ***graddz,
+                    int weno_mode);
+
+/* BICGSTAB.h:15-17 (complex BiCG on the GPU; chunk is accepted and unused) */
+int CompBiCG(int size, double *density, double *densityimg, double **Kernel, double **Kernelimg, double *b,
+             double *bimg);
+int omp_CompBiCG(int size, double *density, double *densityimg, double **Kernel, double **Kernelimg, double *b,
+                 double *bimg, int chunk);
+
+/* HelmholtzKernel.h:20-39 (tube build and N^2 fill on the GPU) */
+int new_HelmholtzKernel(int size, double delta, double epsilon, double *zxlist, double *zylist, double *zzlist,
+                        double *zxstarlist, double *zystarlist, double *zzstarlist, double *dist, double *gradx,
+                        double *grady, double *gradz, double *curv, double *Jacobian, double *PolyW, double *SineW,
+                        double *PolyResult, double *SineResult, double **PolyKernelH, double **PolyKernelHimg,
+                        double **SineKernelH, double **SineKernelHimg);
+int new_HelmholtzKernelCombo(int size, double delta, double epsilon, double ***level, double ***leveldx,
+                             double ***leveldy, double ***leveldz, double ***levelmcurv, double ***levelgcurv,
+                             double *zxlist, double *zylist, double *zzlist, double *zxstarlist, double *zystarlist,
+                             double *zzstarlist, double *dist, double *gradx, double *grady, double *gradz,
+                             double *Jacobian, double *regfactor, double *regfactorimg, double *PolyW, double *SineW,
+                             double *PolyResult, double *SineResult, double **PolyKernelH, double **PolyKernelHimg,
+                             double **SineKernelH, double **SineKernelHimg, double wavenum, double eta);
+int DoubleLayerKernel(int size, double epsilon, double ***level, double ***leveldx, double ***leveldy,
+                      double ***leveldz, double ***levelmcurv, double ***levelgcurv, double *zxlist, double *zylist,
+                      double *zzlist, double *zxstarlist, double *zystarlist, double *zzstarlist, double *dist,
+                      double *gradx, double *grady, double *gradz, double *delta, double *Jacobian, double *regfactor,
+                      double *regfactorimg, double *result, double **Kernel, double **Kernelimg, double wavenum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* IBIM_H */

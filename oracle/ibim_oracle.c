@@ -256,7 +256,10 @@ int ibo_interp_grad(int n, double h, double lo, const double *gx, const double *
     const int S = n + 1;
     int i = (int)((p[1] - lo) / h), j = (int)((p[0] - lo) / h), k = (int)((p[2] - lo) / h);
     double lx = p[0] - lo - (double)j * h, ly = p[1] - lo - (double)i * h, lz = p[2] - lo - (double)k * h;
-    if (lx > h || lx < 0.0 || ly > h || ly < 0.0 || lz < 0.0 || lz > h) return -1;
+    /* the reference exits when x* is outside its cell (interpolate3D.c:31-36);
+     * rounding of (x - lo) / h can put it a few ulps outside, which is accepted */
+    const double slack = 1e-12 * h;
+    if (lx > h + slack || lx < -slack || ly > h + slack || ly < -slack || lz < -slack || lz > h + slack) return -1;
     if (i < 0 || j < 0 || k < 0 || i >= n || j >= n || k >= n) return -1;
     double vx = trilin(gx, S, i, j, k, lx, ly, lz, h);
     double vy = trilin(gy, S, i, j, k, lx, ly, lz, h);
@@ -605,6 +608,33 @@ static void sweep_omp(int n, const double *are, const double *aim, const double 
         double s = 0.0, si = 0.0;
         for (int k = 0; k < n; ++k) {
             double a = are[(size_t)k * n + j], b = aim[(size_t)k * n + j];
+            s += (a * qr[k] + b * qi[k]);
+            si += (a * qi[k] - b * qr[k]);
+        }
+        wr[j] = s;
+        wi[j] = si;
+    }
+}
+
+void ibo_sweep_rows(int nrows, int ncols, const double *are, const double *aim, const double *pr, const double *pi,
+                    const double *qr, const double *qi, double *vr, double *vi, double *wr, double *wi)
+{
+#pragma omp parallel for schedule(static, 16)
+    for (int j = 0; j < nrows; ++j) {
+        double s = 0.0, si = 0.0;
+        const double *ar = are + (size_t)j * ncols, *ai = aim + (size_t)j * ncols;
+        for (int k = 0; k < ncols; ++k) {
+            s += (ar[k] * pr[k] - ai[k] * pi[k]);
+            si += (ar[k] * pi[k] + ai[k] * pr[k]);
+        }
+        vr[j] = s;
+        vi[j] = si;
+    }
+#pragma omp parallel for schedule(static, 16)
+    for (int j = 0; j < ncols; ++j) {
+        double s = 0.0, si = 0.0;
+        for (int k = 0; k < nrows; ++k) {
+            double a = are[(size_t)k * ncols + j], b = aim[(size_t)k * ncols + j];
             s += (a * qr[k] + b * qi[k]);
             si += (a * qi[k] - b * qr[k]);
         }

@@ -89,6 +89,12 @@ int ibo_bicg(int n, const double *are, const double *aim, const double *bre, con
 int ibo_bicg_omp(int n, const double *are, const double *aim, const double *bre, const double *bim, double *xre,
                  double *xim, int maxit, double tol, double *hist);
 
+/* One reference-style sweep over a block of nrows x ncols rows of A (split
+ * planes): v = A p over the rows, w += A^H q restricted to the block (the loop
+ * nests of large3Dexample2.c:3262-3309, OpenMP rows / columns). For timing. */
+void ibo_sweep_rows(int nrows, int ncols, const double *are, const double *aim, const double *pr, const double *pi,
+                    const double *qr, const double *qi, double *vr, double *vi, double *wr, double *wi);
+
 /* Right-hand sides at the tube's closest points. */
 void ibo_rhs_point_source(int count, const double *tube, double k, const double *x0, double *bre, double *bim);
 void ibo_rhs_plane_neumann(int count, const double *tube, const double *interp_n, double k, double amp,

@@ -72,6 +72,7 @@ def lib():
         for f in (L.ibo_bicg, L.ibo_bicg_omp):
             f.argtypes = [I, P, P, P, P, P, P, I, D, P]
             f.restype = I
+        L.ibo_sweep_rows.argtypes = [I, I, P, P, P, P, P, P, P, P, P, P]
         L.ibo_rhs_point_source.argtypes = [I, P, D, P, P, P]
         L.ibo_rhs_plane_neumann.argtypes = [I, P, P, D, D, P, P, P]
         L.ibo_rhs_plane_dirichlet.argtypes = [I, P, D, D, P, P, P]
@@ -168,6 +169,17 @@ def bicg(are, aim, bre, bim, maxit=1000, tol=1e-7, omp=False):
     f = lib().ibo_bicg_omp if omp else lib().ibo_bicg
     its = f(n, _p(are), _p(aim), _p(bre), _p(bim), _p(x), _p(xi), maxit, tol, _p(hist))
     return x, xi, its, hist[:its]
+
+
+def sweep_rows(are, aim, p, q):
+    """v = A p over a row block, w = A^H q restricted to it (reference loop nests)."""
+    nr, nc = are.shape
+    p = np.asarray(p, dtype=np.complex128)
+    q = np.asarray(q, dtype=np.complex128)
+    ins = [np.ascontiguousarray(a) for a in (p.real, p.imag, q.real, q.imag)]
+    vr, vi, wr, wi = np.zeros(nr), np.zeros(nr), np.zeros(nc), np.zeros(nc)
+    lib().ibo_sweep_rows(nr, nc, _p(are), _p(aim), *[_p(a) for a in ins], _p(vr), _p(vi), _p(wr), _p(wi))
+    return vr + 1j * vi, wr + 1j * wi
 
 
 def rhs_point_source(flat, count, k, x0):
