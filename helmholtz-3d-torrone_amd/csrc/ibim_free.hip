@@ -190,6 +190,132 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
         if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2((double)zre[m], (double)zim[m]);
 }
 
+// FP32 operator sweep (traditional kernel). Column points keep FP64 positions
+// and FP32 normal / weight / near-field value in registers; row points are
+// staged in LDS the same way. Dirichlet / interior are compile-time switches.
+struct RowStage32 {
+    double sx[kStage], sy[kStage], sz[kStage];
+    float nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage], qre[kStage], qim[kStage];
+};
+
+template <int Dirichlet, int Interior>
+__global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__ tube, int count, ibim_params prm,
+                                                      const double2* __restrict__ p, const double2* __restrict__ q,
+                                                      double2* __restrict__ ypart, double2* __restrict__ zpart,
+                                                      size_t pld, const int* __restrict__ done) {
+    if (done != nullptr && *done) return;
+    __shared__ RowStage32 st;
+    __shared__ float ysh[2][kFT / 64][kFRows];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int c0 = blockIdx.x * kSweepCols;
+    const int r0 = blockIdx.y * kFRows;
+    const int nrow = min(kFRows, count - r0);
+    const size_t cnt = (size_t)count;
+    const double kappa = prm.k / (2.0 * kPi);
+    const float kf = (float)prm.k;
+    const float rfp = (float)(1.0 / (4.0 * kPi));
+    const double tau2 = prm.tau * prm.tau;
+
+    double csx[kFCpt], csy[kFCpt], csz[kFCpt];
+    float cnx[kFCpt], cny[kFCpt], cnz[kFCpt], cw[kFCpt], creg[kFCpt];
+    float pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
+    int cj[kFCpt];
+#pragma unroll
+    for (int m = 0; m < kFCpt; ++m) {
+        cj[m] = c0 + t + kFT * m;
+        const bool ok = cj[m] < count;
+        const int j = ok ? cj[m] : 0;
+        csx[m] = tube[IBIM_F_SX * cnt + j];
+        csy[m] = tube[IBIM_F_SY * cnt + j];
+        csz[m] = tube[IBIM_F_SZ * cnt + j];
+        cnx[m] = (float)tube[IBIM_F_NX * cnt + j];
+        cny[m] = (float)tube[IBIM_F_NY * cnt + j];
+        cnz[m] = (float)tube[IBIM_F_NZ * cnt + j];
+        cw[m] = ok ? (float)tube[IBIM_F_W * cnt + j] : 0.0f;  // w = 0 silences padding columns
+        creg[m] = (float)tube[IBIM_F_REG * cnt + j];
+        double2 pv = ok ? p[j] : make_double2(0.0, 0.0);
+        pre[m] = (float)pv.x;
+        pim[m] = (float)pv.y;
+        zre[m] = zim[m] = 0.0f;
+    }
+    for (int s0 = 0; s0 < nrow; s0 += kStage) {
+        const int ns = min(kStage, nrow - s0);
+        __syncthreads();
+        if (t < ns) {
+            const int i = r0 + s0 + t;
+            st.sx[t] = tube[IBIM_F_SX * cnt + i];
+            st.sy[t] = tube[IBIM_F_SY * cnt + i];
+            st.sz[t] = tube[IBIM_F_SZ * cnt + i];
+            st.nx[t] = (float)tube[IBIM_F_NX * cnt + i];
+            st.ny[t] = (float)tube[IBIM_F_NY * cnt + i];
+            st.nz[t] = (float)tube[IBIM_F_NZ * cnt + i];
+            st.w[t] = (float)tube[IBIM_F_W * cnt + i];
+            st.reg[t] = (float)tube[IBIM_F_REG * cnt + i];
+            double2 qv = q[i];
+            st.qre[t] = (float)qv.x;
+            st.qim[t] = (float)qv.y;
+        }
+        __syncthreads();
+        for (int s = 0; s < ns; ++s) {
+            const int i = r0 + s0 + s;
+            const double ax = st.sx[s], ay = st.sy[s], az = st.sz[s];
+            const float anx = st.nx[s], any = st.ny[s], anz = st.nz[s], areg = st.reg[s];
+            const float qre = st.qre[s], qim = st.qim[s];
+            float sre = 0.0f, sim = 0.0f;
+#pragma unroll
+            for (int m = 0; m < kFCpt; ++m) {
+                const double d1 = ax - csx[m], d2 = ay - csy[m], d3 = az - csz[m];
+                const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+                float pr, pi_;
+                if (rho2 > tau2) {
+                    const float r = sqrtf((float)rho2);
+                    const float ir = 1.0f / r;
+                    const double rd = (double)r;
+                    const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);
+                    const double tph = rho * kappa;
+                    const float f = (float)(tph - rint(tph));
+                    const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
+                    const float den = rfp * ir * ir;
+                    const float c1 = -(kf * sv + cv * ir) * den;
+                    const float c2 = (kf * cv - sv * ir) * den;
+                    const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+                    const float ip = Dirichlet ? f1 * cnx[m] + f2 * cny[m] + f3 * cnz[m] : f1 * anx + f2 * any + f3 * anz;
+                    pr = Interior ? -c1 * ip : c1 * ip;
+                    pi_ = Interior ? -c2 * ip : c2 * ip;
+                } else {
+                    pr = Dirichlet ? creg[m] : areg;
+                    pi_ = 0.0f;
+                }
+                float er = cw[m] * pr, ei = cw[m] * pi_;
+                if (i == cj[m]) er += 0.5f;
+                sre += er * pre[m] - ei * pim[m];
+                sim += er * pim[m] + ei * pre[m];
+                zre[m] += er * qre + ei * qim;
+                zim[m] += er * qim - ei * qre;
+            }
+            sre = wsum(sre);
+            sim = wsum(sim);
+            if (lane == 0) {
+                ysh[0][wave][s0 + s] = sre;
+                ysh[1][wave][s0 + s] = sim;
+            }
+        }
+    }
+    __syncthreads();
+    for (int row = t; row < nrow; row += kFT) {
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int w = 0; w < kFT / 64; ++w) {
+            a += (double)ysh[0][w][row];
+            b += (double)ysh[1][w][row];
+        }
+        ypart[(size_t)blockIdx.x * pld + r0 + row] = make_double2(a, b);
+    }
+#pragma unroll
+    for (int m = 0; m < kFCpt; ++m)
+        if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2((double)zre[m], (double)zim[m]);
+}
+
 // r = b - y, partial sums of |r|^2 per block.
 __global__ void __launch_bounds__(256) k_residual(const double2* __restrict__ b, const double2* __restrict__ y,
                                                   double2* __restrict__ r, int count, double* __restrict__ part) {
@@ -231,10 +357,12 @@ int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* 
     auto Z = (double2*)zpart;
     if (op == OP_FREE32 && ctx->kind != IBIM_TUBE_TRADITIONAL) op = OP_FREE64;  // FP32 entries: traditional only
     TimedScope ts(ctx, op == OP_FREE32 ? "sweep_free32" : "sweep_free64");
-    if (op == OP_FREE32)
-        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_TRADITIONAL, OP_FREE32>), grid, dim3(kFT), 0, ctx->stream, tube,
-                           count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
-    else if (ctx->kind == IBIM_TUBE_TRADITIONAL)
+    if (op == OP_FREE32) {
+        const int dir = ctx->prm.dirichlet != 0, in = ctx->prm.interior != 0;
+        auto kern = dir ? (in ? k_sweep_free32<1, 1> : k_sweep_free32<1, 0>)
+                        : (in ? k_sweep_free32<0, 1> : k_sweep_free32<0, 0>);
+        hipLaunchKernelGGL(kern, grid, dim3(kFT), 0, ctx->stream, tube, count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
+    } else if (ctx->kind == IBIM_TUBE_TRADITIONAL)
         hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_TRADITIONAL, OP_FREE64>), grid, dim3(kFT), 0, ctx->stream, tube,
                            count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
     else

@@ -90,6 +90,11 @@ def lib():
             "ibim_evaluate": ([vp, I, I, I, P, P, P], I),
             "ibim_set_profiling": ([vp, I], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
+            "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
+            "ibim_read_rhs_text": ([C.c_char_p, C.c_char_p, I, P, P], I),
+            "ibim_write_rhs_text": ([C.c_char_p, C.c_char_p, I, P, P], I),
+            "ibim_read_density_text": ([C.c_char_p, C.c_char_p, I, P, P], I),
+            "ibim_write_density_text": ([C.c_char_p, C.c_char_p, I, P, P], I),
             "weno": ([P, P, P, D, I], None),
             "WENO_DX": ([PPP, I, I, I, I, D, P, P, I], None),
             "WENO_DY": ([PPP, I, I, I, I, D, P, P, I], None),
@@ -277,6 +282,51 @@ class Problem:
 
     def sync(self):
         _check(lib().ibim_sync(self.h), "ibim_sync")
+
+
+# ---------------------------------------------------------------------------
+# The reference driver's text files.
+# ---------------------------------------------------------------------------
+def reference_names(n, k):
+    """File names the reference driver uses for grid size n and wavenumber k
+    (large3Dexample2.c:407, 607-608, 3001-3002)."""
+    return {"distance": "distance%d.txt" % n,
+            "rhs": ("Rel_bN%dk%d.txt" % (n, int(k)), "Rel_bimgN%dk%d.txt" % (n, int(k))),
+            "density": ("Rel_DensityN%dk%d.txt" % (n, int(k)), "Rel_DensityimgN%dk%d.txt" % (n, int(k)))}
+
+
+def read_distance_file(path, n):
+    out = np.zeros((n + 1,) * 3)
+    _check(lib().ibim_read_distance_file(path.encode(), n, _p(out)), "ibim_read_distance_file")
+    return out
+
+
+def write_distance_file(path, level):
+    level = np.ascontiguousarray(level, dtype=np.float64)
+    _check(lib().ibim_write_distance_file(path.encode(), level.shape[0] - 1, _p(level)), "ibim_write_distance_file")
+
+
+def _text_pair(fn, paths, count, a=None, b=None):
+    a = np.zeros(count) if a is None else np.ascontiguousarray(a, dtype=np.float64)
+    b = np.zeros(count) if b is None else np.ascontiguousarray(b, dtype=np.float64)
+    _check(fn(paths[0].encode(), paths[1].encode(), count, _p(a), _p(b)), fn.__name__)
+    return a, b
+
+
+def read_rhs_text(paths, count):
+    return _text_pair(lib().ibim_read_rhs_text, paths, count)
+
+
+def write_rhs_text(paths, bre, bim):
+    _text_pair(lib().ibim_write_rhs_text, paths, len(bre), bre, bim)
+
+
+def read_density_text(paths, count):
+    return _text_pair(lib().ibim_read_density_text, paths, count)
+
+
+def write_density_text(paths, re, im):
+    _text_pair(lib().ibim_write_density_text, paths, len(re), re, im)
 
 
 # ---------------------------------------------------------------------------

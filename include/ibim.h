@@ -180,6 +180,19 @@ int ibim_set_profiling(ibim_ctx *ctx, int on);
 int ibim_kernel_time(ibim_ctx *ctx, const char *name, double *ms_total, long *launches);
 int ibim_reset_timers(ibim_ctx *ctx);
 
+/* ------------------------------------------------- reference file formats --- */
+/* Host-only readers / writers of the reference driver's text files, so its
+ * inputs and outputs interoperate with this library:
+ * distance<N>.txt (large3Dexample2.c:405-424), Rel_bN<N>k<k>.txt and
+ * Rel_bimgN<N>k<k>.txt (print_f :1235-1335, read at :3108-3127),
+ * Rel_DensityN<N>k<k>.txt and Rel_DensityimgN<N>k<k>.txt (:3446-3474, :858-866). */
+int ibim_read_distance_file(const char *path, int n, double *level);
+int ibim_write_distance_file(const char *path, int n, const double *level);
+int ibim_read_rhs_text(const char *path_re, const char *path_im, int count, double *bre, double *bim);
+int ibim_write_rhs_text(const char *path_re, const char *path_im, int count, const double *bre, const double *bim);
+int ibim_read_density_text(const char *path_re, const char *path_im, int count, double *re, double *im);
+int ibim_write_density_text(const char *path_re, const char *path_im, int count, const double *re, const double *im);
+
 /* ------------------------------------------------ reference drop-in layer --- */
 /* weno.h:26-31 */
 void WENO_DX(double ***uvalue, const int i, const int j, const int k, const int N, const double H, double *ux_p,
