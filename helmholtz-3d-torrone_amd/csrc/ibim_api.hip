@@ -137,7 +137,7 @@ void ibim_destroy(ibim_ctx* ctx) {
     collect_timers(ctx);
     for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
                       &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
-                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine})
+                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj})
         b->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -161,6 +161,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->lo = lo;
     ctx->h = h;
     ctx->have_geometry = false;
+    ctx->weno_order = 0;
     ctx->count = 0;
     ctx->mat_precision = -1;
     return IBIM_OK;
@@ -185,6 +186,14 @@ int ibim_set_level_device(ibim_ctx* ctx, int n, double lo, double h, const doubl
     return IBIM_OK;
 }
 
+int ibim_set_level_shape(ibim_ctx* ctx, int n, double lo, double h, const ibim_shape* shape) {
+    CHECK(ctx);
+    if (!shape || shape->kind < IBIM_SHAPE_SPHERE || shape->kind > IBIM_SHAPE_STAR) return IBIM_ERR_ARG;
+    int rc = set_grid(ctx, n, lo, h);
+    if (rc) return rc;
+    return launch_sdf(ctx, shape);
+}
+
 int ibim_geometry(ibim_ctx* ctx, int weno_order) {
     CHECK(ctx);
     if (!ctx->n) {
@@ -198,7 +207,17 @@ int ibim_geometry(ibim_ctx* ctx, int weno_order) {
     int rc = launch_gradient(ctx, weno_order);
     if (!rc) rc = launch_curvatures(ctx);
     if (!rc) ctx->have_geometry = true;
+    ctx->weno_order = rc ? 0 : weno_order;
     return rc;
+}
+
+int ibim_get_level(ibim_ctx* ctx, double* level) {
+    CHECK(ctx);
+    if (!ctx->n || !level) return IBIM_ERR_ARG;
+    const size_t bytes = sizeof(double) * (size_t)(ctx->n + 1) * (ctx->n + 1) * (ctx->n + 1);
+    IBIM_HIP(hipMemcpyAsync(level, ctx->level.ptr, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    return IBIM_OK;
 }
 
 int ibim_get_grid(ibim_ctx* ctx, double* gx, double* gy, double* gz, double* mc, double* gc) {
@@ -228,6 +247,7 @@ int ibim_set_grid_fields_host(ibim_ctx* ctx, const double* gx, const double* gy,
     }
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
     ctx->have_geometry = true;
+    ctx->weno_order = 0;
     return IBIM_OK;
 }
 
@@ -384,6 +404,14 @@ int ibim_set_nrhs(ibim_ctx* ctx, int nrhs) {
 int ibim_rhs(ibim_ctx* ctx, int kind, const double* x0, double amp, const double* dir, int col) {
     CHECK(ctx);
     if (!ctx->count || col < 0 || col >= ctx->nrhs || kind < 0 || kind > 2) return IBIM_ERR_ARG;
+    ctx->rhs_desc.resize(ctx->nrhs);
+    auto& d = ctx->rhs_desc[col];
+    d.kind = kind;
+    d.amp = amp;
+    for (int i = 0; i < 3; ++i) {
+        d.x0[i] = x0 ? x0[i] : 0.0;
+        d.dir[i] = dir ? dir[i] : 0.0;
+    }
     return launch_rhs(ctx, kind, x0, amp, dir, col);
 }
 
@@ -412,6 +440,8 @@ static int copy_col(ibim_ctx* ctx, DevBuf& buf, int col, double* re, double* im,
 int ibim_set_rhs_host(ibim_ctx* ctx, int col, const double* bre, const double* bim) {
     CHECK(ctx);
     if (col < 0 || col >= ctx->nrhs) return IBIM_ERR_ARG;
+    ctx->rhs_desc.resize(ctx->nrhs);
+    ctx->rhs_desc[col].kind = -1;
     return copy_col(ctx, ctx->rhs, col, (double*)bre, (double*)bim, true);
 }
 
@@ -460,6 +490,34 @@ int ibim_evaluate(ibim_ctx* ctx, int mode, int col, int nrecv, const double* xyz
         if (uim) uim[i] = tmp[2 * i + 1];
     }
     return IBIM_OK;
+}
+
+int ibim_set_receivers(ibim_ctx* ctx, int nrecv, const double* xyz, const double* tre, const double* tim) {
+    CHECK(ctx);
+    if (nrecv <= 0 || !xyz) return IBIM_ERR_ARG;
+    if (ctx->recv.ensure(sizeof(double) * 5 * (size_t)nrecv)) return IBIM_ERR_NOMEM;
+    std::vector<double> t(5 * (size_t)nrecv);
+    for (int i = 0; i < 3 * nrecv; ++i) t[i] = xyz[i];
+    for (int i = 0; i < nrecv; ++i) {
+        t[3 * (size_t)nrecv + 2 * i] = tre ? tre[i] : 0.0;
+        t[3 * (size_t)nrecv + 2 * i + 1] = tim ? tim[i] : 0.0;
+    }
+    IBIM_HIP(hipMemcpyAsync(ctx->recv.ptr, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->nrecv = nrecv;
+    return IBIM_OK;
+}
+
+int ibim_gradient(ibim_ctx* ctx, int col, int mode, const ibim_solve_opts* opts, double* J, double* dlevel,
+                  double* dk) {
+    CHECK(ctx);
+    if (!ctx->count || col < 0 || col >= ctx->nrhs || !ctx->nrecv || !opts || (mode != IBIM_EVAL_SINGLE &&
+        mode != IBIM_EVAL_DOUBLE) || ctx->kind != IBIM_TUBE_TRADITIONAL || !ctx->weno_order) {
+        set_error("ibim_gradient: needs a solved traditional problem on WENO geometry, receivers and a "
+                  "single/double-layer mode");
+        return IBIM_ERR_ARG;
+    }
+    return run_gradient(ctx, col, mode, opts, J, dlevel, dk);
 }
 
 int ibim_set_profiling(ibim_ctx* ctx, int on) {

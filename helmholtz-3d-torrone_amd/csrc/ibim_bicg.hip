@@ -405,9 +405,19 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
 // One BiCG solve of A x = b for b, x on the device (length count). Returns
 // iterations and status in info; hist (host, may be NULL) gets the residuals.
 int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* x_out, int maxit, double tol,
-                int chunk, ibim_solve_info* info, double* hist) {
+                int chunk, ibim_solve_info* info, double* hist, bool adjoint = false) {
     VecArgs a = ws.a;
     a.b = b;
+    // The operator's adjoint system A^H x = b runs the same recurrences: one
+    // sweep with (p~, p) in place of (p, p~) gives A p~ and A^H p, so the dot
+    // kernel reads the two partial planes the other way round.
+    VecArgs ad = a;
+    if (adjoint) {
+        ad.ypart = a.zpart;
+        ad.zpart = a.ypart;
+        ad.ncol_tiles = a.nrow_tiles;
+        ad.nrow_tiles = a.ncol_tiles;
+    }
     const unsigned nvb = ws.nvb;
     hipLaunchKernelGGL(k_bicg_init, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
     hipLaunchKernelGGL(k_bicg_init_finish, dim3(1), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb);
@@ -428,9 +438,10 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     while (it <= maxit && !done) {
         int last = it + chunk - 1 < maxit ? it + chunk - 1 : maxit;
         for (; it <= last; ++it) {
-            int rc = sweep_any(ctx, op, a.p, a.pt, a.ypart, a.zpart, a.done);
+            int rc = adjoint ? sweep_any(ctx, op, a.pt, a.p, a.ypart, a.zpart, a.done)
+                             : sweep_any(ctx, op, a.p, a.pt, a.ypart, a.zpart, a.done);
             if (rc) return rc;
-            hipLaunchKernelGGL(k_bicg_dot, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_bicg_dot, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, adjoint ? ad : a);
             hipLaunchKernelGGL(k_bicg_update_xr, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb, it,
                                ws.part2);
             hipLaunchKernelGGL(k_bicg_update_p, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, ws.part2, (int)nvb,
@@ -480,11 +491,11 @@ int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev
 }
 
 int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
-                  ibim_solve_info* info, double* hist) {
+                  ibim_solve_info* info, double* hist, bool adjoint) {
     Workspace ws;
     int rc = prepare(ctx, maxit, &ws);
     if (rc) return rc;
-    return bicg_device(ctx, op, ws, (const double2*)b_dev, (double2*)x_dev, maxit, tol, chunk, info, hist);
+    return bicg_device(ctx, op, ws, (const double2*)b_dev, (double2*)x_dev, maxit, tol, chunk, info, hist, adjoint);
 }
 
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {

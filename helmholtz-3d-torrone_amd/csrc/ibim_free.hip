@@ -409,7 +409,7 @@ int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
         status = 1;
         for (outer = 1; outer <= max_outer; ++outer) {
             ibim_solve_info in{};
-            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, inner_tol, opts->chunk, &in, nullptr);
+            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, inner_tol, opts->chunk, &in, nullptr, false);
             if (rc) return rc;
             total_its += in.iterations;
             hipLaunchKernelGGL(k_add, dim3(nb), dim3(256), 0, ctx->stream, x, d, count);

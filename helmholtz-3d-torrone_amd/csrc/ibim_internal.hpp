@@ -64,6 +64,7 @@ struct Context {
     double lo = -2.0, h = 0.0;
     DevBuf level, gx, gy, gz, mc, gc;
     bool have_geometry = false;
+    int weno_order = 0;  // order of the WENO gradient on the grid, 0 when set from the host
 
     // tube
     ibim_params prm{};
@@ -80,6 +81,17 @@ struct Context {
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;
     DevBuf rhs, density;
+
+    // how each right-hand-side column was made (the adjoint needs db / dgeometry)
+    struct RhsDesc {
+        int kind = -1;  // IBIM_RHS_*, or -1 when set from the host
+        double x0[3] = {0, 0, 0}, amp = 1.0, dir[3] = {0, 0, 0};
+    };
+    std::vector<RhsDesc> rhs_desc;
+
+    // receivers and targets of the misfit objective, adjoint workspace
+    int nrecv = 0;
+    DevBuf recv, adj, gadj;
 
     // solver workspace
     DevBuf work, partial, scal, done;
@@ -114,12 +126,14 @@ struct TimedScope {
 };
 
 // Kernel launchers (defined in the .hip translation units).
+int launch_sdf(Context* ctx, const ibim_shape* shape);
 int launch_gradient(Context* ctx, int order);
 int launch_curvatures(Context* ctx);
 int launch_tube(Context* ctx, int* count_out);
 int launch_assemble(Context* ctx, int precision);
 int launch_rhs(Context* ctx, int kind, const double* x0, double amp, const double* dir, int col);
 int launch_eval(Context* ctx, int mode, int col, int nrecv, const double* xyz_dev, double* out_dev);
+int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, double* J, double* dlevel, double* dk);
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
@@ -132,7 +146,7 @@ constexpr int kSweepRows = 512;
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done);
 int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
 int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
-                  ibim_solve_info* info, double* hist);
+                  ibim_solve_info* info, double* hist, bool adjoint = false);
 
 }  // namespace ibim
 

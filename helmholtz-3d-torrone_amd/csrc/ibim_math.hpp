@@ -22,40 +22,142 @@ constexpr double kPi = 3.14159265358979323846264338328;  // constants.h:6
 
 IBIM_HD size_t node_at(int S, int i, int j, int k) { return ((size_t)i * S + j) * S + k; }
 
+// Magnitude used by every branch decision; for dual numbers the branch is taken
+// on the value, so derivatives are those of the branch the value selects.
+IBIM_HD double absv(double a) { return fabs(a); }
+
+// Forward-mode dual number with ND derivative directions (exact vector-Jacobian
+// products of the piecewise-smooth stencils, used by the adjoint).
+template <int ND>
+struct Dual {
+    double v;
+    double d[ND];
+    IBIM_HD Dual() : v(0.0) {
+        for (int q = 0; q < ND; ++q) d[q] = 0.0;
+    }
+    IBIM_HD Dual(double x) : v(x) {
+        for (int q = 0; q < ND; ++q) d[q] = 0.0;
+    }
+    IBIM_HD static Dual seed(double x, int q) {
+        Dual r(x);
+        r.d[q] = 1.0;
+        return r;
+    }
+};
+template <int ND>
+IBIM_HD double absv(const Dual<ND>& a) { return fabs(a.v); }
+template <int ND>
+IBIM_HD Dual<ND> operator+(const Dual<ND>& a, const Dual<ND>& b) {
+    Dual<ND> r(a.v + b.v);
+    for (int q = 0; q < ND; ++q) r.d[q] = a.d[q] + b.d[q];
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator-(const Dual<ND>& a, const Dual<ND>& b) {
+    Dual<ND> r(a.v - b.v);
+    for (int q = 0; q < ND; ++q) r.d[q] = a.d[q] - b.d[q];
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator-(const Dual<ND>& a) {
+    Dual<ND> r(-a.v);
+    for (int q = 0; q < ND; ++q) r.d[q] = -a.d[q];
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator*(const Dual<ND>& a, const Dual<ND>& b) {
+    Dual<ND> r(a.v * b.v);
+    for (int q = 0; q < ND; ++q) r.d[q] = a.d[q] * b.v + a.v * b.d[q];
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator*(double a, const Dual<ND>& b) {
+    Dual<ND> r(a * b.v);
+    for (int q = 0; q < ND; ++q) r.d[q] = a * b.d[q];
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator*(const Dual<ND>& a, double b) {
+    return b * a;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator/(const Dual<ND>& a, const Dual<ND>& b) {
+    Dual<ND> r(a.v / b.v);
+    const double ib = 1.0 / b.v;
+    for (int q = 0; q < ND; ++q) r.d[q] = (a.d[q] - r.v * b.d[q]) * ib;
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator/(const Dual<ND>& a, double b) {
+    Dual<ND> r(a.v / b);
+    for (int q = 0; q < ND; ++q) r.d[q] = a.d[q] / b;
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator/(double a, const Dual<ND>& b) {
+    return Dual<ND>(a) / b;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator+(double a, const Dual<ND>& b) {
+    Dual<ND> r = b;
+    r.v = a + b.v;
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator+(const Dual<ND>& a, double b) {
+    return b + a;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator-(const Dual<ND>& a, double b) {
+    Dual<ND> r = a;
+    r.v = a.v - b;
+    return r;
+}
+template <int ND>
+IBIM_HD Dual<ND> operator-(double a, const Dual<ND>& b) {
+    return Dual<ND>(a) - b;
+}
+template <int ND>
+IBIM_HD Dual<ND>& operator+=(Dual<ND>& a, const Dual<ND>& b) {
+    a = a + b;
+    return a;
+}
+
 // ---------------------------------------------------------------------------
 // WENO one-sided derivatives (weno.c:259-345): Jiang-Peng HJ-WENO3 / WENO5,
 // eps = 1e-12, ideal weights 1:6:3. f holds 5 (order 3) or 7 (order 5) samples.
 // ---------------------------------------------------------------------------
-IBIM_HD void weno_1d(const double* f, double dx, int order, double* dplus, double* dminus) {
+template <class T>
+IBIM_HD void weno_1d(const T* f, double dx, int order, T* dplus, T* dminus) {
     const double eps = 1e-12;
     const double ih = 1.0 / dx;
     if (order == 3) {
-        double d0 = f[1] - f[0], d1 = f[2] - f[1], d2 = f[3] - f[2], d3 = f[4] - f[3];
-        double central = d1 + d2;
-        double s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2;
-        double r = (eps + s2 * s2) / (eps + s1 * s1);
-        double w = 1. / (1. + 2. * r * r);
+        T d0 = f[1] - f[0], d1 = f[2] - f[1], d2 = f[3] - f[2], d3 = f[4] - f[3];
+        T central = d1 + d2;
+        T s0 = d1 - d0, s1 = d2 - d1, s2 = d3 - d2;
+        T r = (eps + s2 * s2) / (eps + s1 * s1);
+        T w = 1. / (1. + 2. * r * r);
         *dplus = .5 * ih * (central - w * (d3 - 2. * d2 + d1));
         r = (eps + s0 * s0) / (eps + s1 * s1);
         w = 1 / (1 + 2. * r * r);
         *dminus = .5 * ih * (central - w * (d0 - 2. * d1 + d2));
         return;
     }
-    double q[6];
+    T q[6];
 #pragma unroll
     for (int m = 0; m < 6; ++m) q[m] = (f[m + 1] - f[m]) * ih;
-    double central = (-q[1] + 7. * (q[2] + q[3]) - q[4]) / 12.;
-    double s[5];
+    T central = (-q[1] + 7. * (q[2] + q[3]) - q[4]) / 12.;
+    T s[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) s[m] = q[m + 1] - q[m];
-    double b0 = 13. * (s[4] - s[3]) * (s[4] - s[3]) + 3. * (s[4] - 3. * s[3]) * (s[4] - 3. * s[3]);
-    double b1 = 13. * (s[3] - s[2]) * (s[3] - s[2]) + 3. * (s[3] + s[2]) * (s[3] + s[2]);
-    double b2 = 13. * (s[2] - s[1]) * (s[2] - s[1]) + 3. * (3. * s[2] - s[1]) * (3. * s[2] - s[1]);
-    double a0 = 1. / ((eps + b0) * (eps + b0));
-    double a1 = 6. / ((eps + b1) * (eps + b1));
-    double a2 = 3. / ((eps + b2) * (eps + b2));
-    double inv = 1. / (a0 + a1 + a2);
-    double w0 = a0 * inv, w2 = a2 * inv;
+    T b0 = 13. * (s[4] - s[3]) * (s[4] - s[3]) + 3. * (s[4] - 3. * s[3]) * (s[4] - 3. * s[3]);
+    T b1 = 13. * (s[3] - s[2]) * (s[3] - s[2]) + 3. * (s[3] + s[2]) * (s[3] + s[2]);
+    T b2 = 13. * (s[2] - s[1]) * (s[2] - s[1]) + 3. * (3. * s[2] - s[1]) * (3. * s[2] - s[1]);
+    T a0 = 1. / ((eps + b0) * (eps + b0));
+    T a1 = 6. / ((eps + b1) * (eps + b1));
+    T a2 = 3. / ((eps + b2) * (eps + b2));
+    T inv = 1. / (a0 + a1 + a2);
+    T w0 = a0 * inv, w2 = a2 * inv;
     *dplus = central + w0 * (s[4] - 2. * s[3] + s[2]) / 3. + (w2 - .5) * (s[3] - 2. * s[2] + s[1]) / 6.;
     b0 = 13. * (s[0] - s[1]) * (s[0] - s[1]) + 3. * (s[0] - 3. * s[1]) * (s[0] - 3. * s[1]);
     b1 = 13. * (s[1] - s[2]) * (s[1] - s[2]) + 3. * (s[1] + s[2]) * (s[1] + s[2]);
@@ -88,32 +190,45 @@ struct FlatPlane {
 };
 
 // One axis of WENO_DX / WENO_DY / WENO_DZ (weno.c:117-254).
-template <class Line>
-IBIM_HD void weno_line(const Line& V, int c, int n, double h, int order, double* dp, double* dm) {
-    double f[7];
+template <class Line, class T>
+IBIM_HD void weno_line(const Line& V, int c, int n, double h, int order, T* dp, T* dm) {
+    T f[7];
     const int half = order == 5 ? 3 : 2;
     for (int o = -half; o <= half; ++o) f[o + half] = V(mirror(c + o, n));
     weno_1d(f, h, order, dp, dm);
 }
 
 // Godunov upwind choice (distance_application3D.c:88-112).
-IBIM_HD double godunov_pick(double dp, double dn, double d) {
-    double p_hi = 0.0 >= dp ? 0.0 : dp, p_lo = 0.0 <= dp ? 0.0 : dp;
-    double n_hi = 0.0 >= dn ? 0.0 : dn, n_lo = 0.0 <= dn ? 0.0 : dn;
-    if (d > 0.0) return fabs(p_lo) > fabs(n_hi) ? p_lo : n_hi;
-    return fabs(p_hi) > fabs(n_lo) ? p_hi : n_lo;
+IBIM_HD double valof(double a) { return a; }
+template <int ND>
+IBIM_HD double valof(const Dual<ND>& a) { return a.v; }
+
+template <class T>
+IBIM_HD T godunov_pick(T dp, T dn, double d) {
+    const T zero(0.0);
+    T p_hi = 0.0 >= valof(dp) ? zero : dp, p_lo = 0.0 <= valof(dp) ? zero : dp;
+    T n_hi = 0.0 >= valof(dn) ? zero : dn, n_lo = 0.0 <= valof(dn) ? zero : dn;
+    if (d > 0.0) return absv(p_lo) > absv(n_hi) ? p_lo : n_hi;
+    return absv(p_hi) > absv(n_lo) ? p_hi : n_lo;
 }
 
 // ---------------------------------------------------------------------------
 // Limited second derivatives (finite_difference3D.c:220-721).
 // ---------------------------------------------------------------------------
-IBIM_HD double pick_smaller(double a, double b) { return fabs(a) < fabs(b) ? a : b; }
-IBIM_HD double pick_smallest(double a, double b, double c) { return pick_smaller(pick_smaller(a, b), c); }
+template <class T>
+IBIM_HD T pick_smaller(T a, T b) {
+    return absv(a) < absv(b) ? a : b;
+}
+template <class T>
+IBIM_HD T pick_smallest(T a, T b, T c) {
+    return pick_smaller(pick_smaller(a, b), c);
+}
 
 // XDER2 / YDER2 / ZDER2 along a strided line; tri_order keeps each function's
 // argument order of the three-way min-abs (XDER2: fwd, cen, bwd; others fwd, bwd, cen).
 template <class Line>
-IBIM_HD double d2_line(const Line& V, int c, int n, double h, int tri_order) {
+IBIM_HD auto d2_line(const Line& V, int c, int n, double h, int tri_order) -> decltype(V(0)) {
+    using T = decltype(V(0));
 #define IBIM_V(m) V(m)
     const double hh = h * h;
     if (c < 0) {
@@ -127,22 +242,23 @@ IBIM_HD double d2_line(const Line& V, int c, int n, double h, int tri_order) {
         return (IBIM_V(a + 1) - 2.0 * IBIM_V(a) + IBIM_V(a - 1)) / hh;
     }
     if (c < 3) {
-        double fwd = (2.0 * IBIM_V(c) - 5.0 * IBIM_V(c + 1) + 4.0 * IBIM_V(c + 2) - IBIM_V(c + 3)) / hh;
-        double cen = (IBIM_V(c + 1) - 2.0 * IBIM_V(c) + IBIM_V(c - 1)) / hh;
+        T fwd = (2.0 * IBIM_V(c) - 5.0 * IBIM_V(c + 1) + 4.0 * IBIM_V(c + 2) - IBIM_V(c + 3)) / hh;
+        T cen = (IBIM_V(c + 1) - 2.0 * IBIM_V(c) + IBIM_V(c - 1)) / hh;
         return pick_smaller(fwd, cen);
     }
     if (c > n - 3) {
-        double bwd = (2.0 * IBIM_V(c) - 5.0 * IBIM_V(c - 1) + 4.0 * IBIM_V(c - 2) - IBIM_V(c - 3)) / hh;
-        double cen = (IBIM_V(c + 1) - 2.0 * IBIM_V(c) + IBIM_V(c - 1)) / hh;
+        T bwd = (2.0 * IBIM_V(c) - 5.0 * IBIM_V(c - 1) + 4.0 * IBIM_V(c - 2) - IBIM_V(c - 3)) / hh;
+        T cen = (IBIM_V(c + 1) - 2.0 * IBIM_V(c) + IBIM_V(c - 1)) / hh;
         return pick_smaller(bwd, cen);
     }
-    double v_m3 = IBIM_V(c - 3), v_m2 = IBIM_V(c - 2), v_m1 = IBIM_V(c - 1), v0 = IBIM_V(c);
-    double v1 = IBIM_V(c + 1), v2 = IBIM_V(c + 2), v3 = IBIM_V(c + 3);
+    T v_m3 = IBIM_V(c - 3), v_m2 = IBIM_V(c - 2), v_m1 = IBIM_V(c - 1), v0 = IBIM_V(c);
+    T v1 = IBIM_V(c + 1), v2 = IBIM_V(c + 2), v3 = IBIM_V(c + 3);
 #undef IBIM_V
-    double jpp = v2 - v1, jp = v1 - v0, jn = v0 - v_m1, jnn = v_m1 - v_m2;
-    double fwd = (2.0 * v0 - 5.0 * v1 + 4.0 * v2 - v3) / hh;
-    double cen = (v1 - 2.0 * v0 + v_m1) / hh;
-    double bwd = (2.0 * v0 - 5.0 * v_m1 + 4.0 * v_m2 - v_m3) / hh;
+    const double jpp = valof(v2) - valof(v1), jp = valof(v1) - valof(v0);
+    const double jn = valof(v0) - valof(v_m1), jnn = valof(v_m1) - valof(v_m2);
+    T fwd = (2.0 * v0 - 5.0 * v1 + 4.0 * v2 - v3) / hh;
+    T cen = (v1 - 2.0 * v0 + v_m1) / hh;
+    T bwd = (2.0 * v0 - 5.0 * v_m1 + 4.0 * v_m2 - v_m3) / hh;
     if (jp * jnn <= 0.0 && jpp * jn <= 0.0)
         return tri_order == 0 ? pick_smallest(fwd, cen, bwd) : pick_smallest(fwd, bwd, cen);
     if (jp * jnn <= 0.0) return pick_smaller(fwd, cen);
@@ -155,16 +271,17 @@ IBIM_HD double d2_line(const Line& V, int c, int n, double h, int tri_order) {
 // one-sided first differences ({1,-4,3}); the smallest in magnitude wins,
 // scanned in (da, db) order starting from 1e6 (LARGENUM).
 template <class Plane>
-IBIM_HD double dmixed(const Plane& Wp, int ia, int ib, int n, double h) {
+IBIM_HD auto dmixed(const Plane& Wp, int ia, int ib, int n, double h) -> decltype(Wp(0, 0)) {
+    using T = decltype(Wp(0, 0));
 #define IBIM_W(p, q) Wp((p), (q))
     const double c3[3] = {1.0, -4.0, 3.0};
-    if (ia == 0 || ia == n || ib == 0 || ib == n) return 0.0;
+    if (ia == 0 || ia == n || ib == 0 || ib == n) return T(0.0);
     if (ia < 2 || ia > n - 2 || ib < 2 || ib > n - 2)
         return (IBIM_W(1, 1) - IBIM_W(-1, 1) - IBIM_W(1, -1) + IBIM_W(-1, -1)) / (4.0 * h * h);
-    double best = 1000000;
+    T best(1000000.0);
     for (int da = 0; da < 3; ++da)
         for (int db = 0; db < 3; ++db) {
-            double acc = 0.0;
+            T acc(0.0);
             if (da == 1 && db == 1) {
                 acc = (IBIM_W(1, 1) - IBIM_W(-1, 1) - IBIM_W(1, -1) + IBIM_W(-1, -1));
             } else if (da == 1) {
@@ -181,7 +298,7 @@ IBIM_HD double dmixed(const Plane& Wp, int ia, int ib, int n, double h) {
                     for (int mb = 2; mb >= 0; --mb)
                         acc += (c3[2 - ma] * c3[2 - mb] * (double)(sa_ * sb_) * IBIM_W(sa_ * ma, sb_ * mb));
             }
-            if (fabs(acc) < fabs(best)) best = acc;
+            if (absv(acc) < absv(best)) best = acc;
         }
 #undef IBIM_W
     return best / (4.0 * h * h);

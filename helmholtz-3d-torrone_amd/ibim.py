@@ -49,6 +49,18 @@ class Params(C.Structure):
                 ("interior", I), ("dirichlet", I), ("one_sided", I)]
 
 
+SHAPE_SPHERE, SHAPE_TORUS, SHAPE_ELLIPSOID, SHAPE_STAR = 0, 1, 2, 3
+
+
+class Shape(C.Structure):
+    """ibim_shape: an analytic surface whose signed distance the GPU builds."""
+    _fields_ = [("kind", I), ("variant", I), ("m", I), ("n", I), ("center", D * 3), ("a", D), ("b", D), ("c", D)]
+
+
+def shape(kind, a=1.0, b=0.0, c=0.0, m=0, n=0, variant=0, center=(0.0, 0.0, 0.0)):
+    return Shape(kind, variant, m, n, (D * 3)(*center), a, b, c)
+
+
 class SolveOpts(C.Structure):
     _fields_ = [("method", I), ("maxit", I), ("tol", D), ("inner_tol", D), ("max_outer", I), ("chunk", I)]
 
@@ -76,7 +88,8 @@ def lib():
             "ibim_device_count": ([], I), "ibim_last_error": ([], C.c_char_p),
             "ibim_create": ([C.POINTER(vp)], I), "ibim_destroy": ([vp], None), "ibim_sync": ([vp], I),
             "ibim_set_level_host": ([vp, I, D, D, P], I), "ibim_set_level_device": ([vp, I, D, D, vp], I),
-            "ibim_geometry": ([vp, I], I), "ibim_get_grid": ([vp, P, P, P, P, P], I),
+            "ibim_set_level_shape": ([vp, I, D, D, C.POINTER(Shape)], I),
+            "ibim_geometry": ([vp, I], I), "ibim_get_level": ([vp, P], I), "ibim_get_grid": ([vp, P, P, P, P, P], I),
             "ibim_set_grid_fields_host": ([vp, P, P, P, P, P], I),
             "ibim_tube": ([vp, C.POINTER(Params), I, IP], I), "ibim_get_tube": ([vp, IP, P], I),
             "ibim_tube_device": ([vp], vp),
@@ -88,6 +101,8 @@ def lib():
             "ibim_solve": ([vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), P], I),
             "ibim_get_density": ([vp, I, P, P], I), "ibim_set_density_host": ([vp, I, P, P], I),
             "ibim_evaluate": ([vp, I, I, I, P, P, P], I),
+            "ibim_set_receivers": ([vp, I, P, P, P], I),
+            "ibim_gradient": ([vp, I, I, C.POINTER(SolveOpts), P, P, P], I),
             "ibim_set_profiling": ([vp, I], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
             "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
@@ -160,6 +175,20 @@ class Problem:
         _check(lib().ibim_set_level_host(self.h, n, lo, h, _p(level)), "ibim_set_level_host")
         self.n = n
         return self
+
+    def set_level_shape(self, shp, n, lo=-2.0, h=None):
+        """Build the signed distance of `shp` (ibim.shape(...)) on the GPU."""
+        h = (-2.0 * lo) / n if h is None else h
+        _check(lib().ibim_set_level_shape(self.h, n, lo, h, C.byref(shp)), "ibim_set_level_shape")
+        self.n = n
+        return self
+
+    def level(self):
+        """Host copy of the level set."""
+        S = self.n + 1
+        out = np.zeros((S, S, S))
+        _check(lib().ibim_get_level(self.h, _p(out)), "ibim_get_level")
+        return out
 
     def geometry(self, weno_order=5):
         _check(lib().ibim_geometry(self.h, weno_order), "ibim_geometry")
@@ -267,6 +296,26 @@ class Problem:
         a, b = np.zeros(m), np.zeros(m)
         _check(lib().ibim_evaluate(self.h, mode, col, m, _p(xyz), _p(a), _p(b)), "ibim_evaluate")
         return a, b
+
+    # misfit gradient (adjoint) ---------------------------------------------------
+    def set_receivers(self, xyz, target=None):
+        """Receivers (m, 3) and complex targets t of J = sum |u(x_r) - t_r|^2."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        m = xyz.shape[0]
+        t = np.zeros(m, complex) if target is None else np.asarray(target, complex)
+        tre, tim = np.ascontiguousarray(t.real), np.ascontiguousarray(t.imag)
+        _check(lib().ibim_set_receivers(self.h, m, _p(xyz), _p(tre), _p(tim)), "ibim_set_receivers")
+        self.nrecv = m
+
+    def gradient(self, mode=EVAL_DOUBLE, col=0, maxit=1000, tol=1e-12, chunk=0):
+        """J and dJ/d(level-set values) (shape (n+1)^3, [y][x][z]) and dJ/dk for
+        the solved column `col` (adjoint BiCG to `tol`)."""
+        o = SolveOpts(SOLVE_BICG, maxit, tol, 1e-4, 20, chunk)
+        J, dk = np.zeros(1), np.zeros(1)
+        S = self.n + 1
+        g = np.zeros((S, S, S))
+        _check(lib().ibim_gradient(self.h, col, mode, C.byref(o), _p(J), _p(g), _p(dk)), "ibim_gradient")
+        return float(J[0]), g, float(dk[0])
 
     # profiling --------------------------------------------------------------------
     def set_profiling(self, on=True):

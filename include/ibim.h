@@ -123,9 +123,26 @@ int ibim_sync(ibim_ctx *ctx);
  * device pointer on the handle's device. */
 int ibim_set_level_host(ibim_ctx *ctx, int n, double lo, double h, const double *level);
 int ibim_set_level_device(ibim_ctx *ctx, int n, double lo, double h, const double *level_dev);
+/* Signed-distance level set (positive inside) of an analytic surface, built on
+ * the GPU straight into the handle's grid. The reference reads a precomputed
+ * distance file instead (large3Dexample2.c:405-424); its formulas assume
+ * |grad d| = 1, which these are. */
+#define IBIM_SHAPE_SPHERE 0    /* radius a                                            */
+#define IBIM_SHAPE_TORUS 1     /* major radius a, minor radius b, axis z              */
+#define IBIM_SHAPE_ELLIPSOID 2 /* semi-axes a, b, c                                    */
+#define IBIM_SHAPE_STAR 3      /* r(theta, phi) = a + b sin(m theta) cos(n phi)  (variant 0)
+                                  r(theta, phi) = a + b cos(m theta) sin(n phi)  (variant 1) */
+typedef struct ibim_shape {
+    int kind, variant, m, n;
+    double center[3];
+    double a, b, c;
+} ibim_shape;
+int ibim_set_level_shape(ibim_ctx *ctx, int n, double lo, double h, const ibim_shape *shape);
 /* WENO-3/5 Godunov gradient and curvatures on every node
  * (ASSIGN_GRADIENT distance_application3D.c:56-141; BothCurvs :35-54). */
 int ibim_geometry(ibim_ctx *ctx, int weno_order);
+/* Host copy of the level set. */
+int ibim_get_level(ibim_ctx *ctx, double *level);
 /* Host copies of the grid fields (any pointer may be NULL). */
 int ibim_get_grid(ibim_ctx *ctx, double *gx, double *gy, double *gz, double *mc, double *gc);
 /* Supply gradient and curvatures computed elsewhere (host memory). */
@@ -172,6 +189,22 @@ int ibim_set_density_host(ibim_ctx *ctx, int col, const double *re, const double
 /* ------------------------------------------------------------ evaluation --- */
 /* u at nrecv receivers xyz (host, 3 per receiver) from density column col. */
 int ibim_evaluate(ibim_ctx *ctx, int mode, int col, int nrecv, const double *xyz, double *ure, double *uim);
+
+/* -------------------------------------------------------------- adjoint --- */
+/* Misfit objective J = sum_r |u(x_r) - t_r|^2 of the field of density column col
+ * (mode IBIM_EVAL_SINGLE or IBIM_EVAL_DOUBLE) at nrecv receivers with targets t,
+ * and its gradient with respect to every level-set grid value and the
+ * wavenumber k. The adjoint solves A^H lambda = E^H (u - t) with the same BiCG,
+ * differentiates the operator, evaluator and right-hand side entries exactly on
+ * the fly (no stored derivative matrices), then back-propagates through the
+ * quadrature weights, projection, trilinear sampling, WENO / Godunov gradient
+ * and the limited curvature stencils (branch choices held fixed). Requires a
+ * traditional tube, a solved column, and a right-hand side made by ibim_rhs
+ * (point source or Dirichlet plane wave) or set from the host. dlevel: host
+ * array of (n+1)^3 doubles (may be NULL). */
+int ibim_set_receivers(ibim_ctx *ctx, int nrecv, const double *xyz, const double *tre, const double *tim);
+int ibim_gradient(ibim_ctx *ctx, int col, int mode, const ibim_solve_opts *opts, double *J, double *dlevel,
+                  double *dk);
 
 /* ------------------------------------------------------------- profiling --- */
 /* When on, kernels are bracketed by hipEvents on the handle's stream. */

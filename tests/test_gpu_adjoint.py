@@ -1,0 +1,145 @@
+"""The misfit gradient (ibim_gradient) against central finite differences of
+the whole device pipeline: level set -> WENO geometry -> tube -> operator ->
+solve -> receivers.
+
+Objective: J = sum_r |u(x_r) - t_r|^2 on seed-random receivers on |x| = 2.5,
+targets from a second surface. The finite differences re-run the forward
+path with FP64 iterative refinement to a relative residual of 1e-15, so their
+noise (about 1e-15 J / step) stays far below the tolerances below:
+
+  single entries and k: |fd - g| <= 1e-5 |g| + 1e-8 max|g|   (step 1e-6)
+  a smooth direction:   relative 1e-6                         (step 1e-6)
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N = 32
+H = 4.0 / N
+X0 = (0.1, 0.2, 0.3)
+
+
+def receivers(m, radius=2.5, seed=0):
+    v = np.random.default_rng(seed).standard_normal((m, 3))
+    return radius * v / np.linalg.norm(v, axis=1, keepdims=True)
+
+
+class Pipeline:
+    def __init__(self, ib, dirichlet, interior, mode, k):
+        self.ib, self.dirichlet, self.interior, self.mode, self.k = ib, dirichlet, interior, mode, k
+        self.pb = ib.Problem()
+        self.recv = receivers(60)
+
+    def star(self, b):
+        ib = self.ib
+        self.pb.set_level_shape(ib.shape(ib.SHAPE_STAR, a=1.0, b=b, m=3, n=2, variant=0), N)
+        return self.pb.level()
+
+    def prepare(self, level, k):
+        ib, pb = self.ib, self.pb
+        pb.set_level(level)
+        pb.geometry(5)
+        pb.tube(ib.params(k, 3 * H, 2 * H, interior=self.interior, dirichlet=self.dirichlet))
+        pb.assemble(ib.C128)
+        pb.rhs(ib.RHS_POINT_SOURCE, x0=X0)
+
+    def field(self, level, k):
+        """u at the receivers with a solve accurate to the FP64 floor."""
+        self.prepare(level, k)
+        info, _ = self.pb.solve(method=self.ib.SOLVE_REFINED, tol=1e-15, inner_tol=1e-5, max_outer=10)
+        assert info.rel_residual < 1e-13, info.rel_residual
+        re, im = self.pb.evaluate(self.recv, self.mode)
+        return re + 1j * im
+
+    def J(self, level, k, target):
+        return float(np.sum(np.abs(self.field(level, k) - target) ** 2))
+
+    def gradient(self, level, k, target):
+        self.prepare(level, k)
+        info, _ = self.pb.solve(tol=1e-13, maxit=2000)
+        assert info.status == 0
+        self.pb.set_receivers(self.recv, target)
+        return self.pb.gradient(mode=self.mode, tol=1e-13, maxit=2000)
+
+
+CASES = [
+    # exterior sound-soft problem, double-layer representation
+    pytest.param(1, 0, 1, 3.0, id="dirichlet-exterior-double"),
+    # interior Neumann operator, single-layer evaluation (row-normal and row-reg paths)
+    pytest.param(0, 1, 0, 2.5, id="neumann-interior-single"),
+]
+
+
+@pytest.mark.parametrize("dirichlet,interior,mode,k", CASES)
+def test_gradient_matches_central_differences(gpu, dirichlet, interior, mode, k):
+    P = Pipeline(gpu, dirichlet, interior, mode, k)
+    target = P.field(P.star(0.10), k)
+    level = P.star(0.15)
+    J, g, dk = P.gradient(level, k, target)
+    J_ref = P.J(level, k, target)
+    assert abs(J - J_ref) <= 1e-10 * J_ref, (J, J_ref)
+    assert np.isfinite(g).all() and np.abs(g).max() > 0
+
+    step = 1e-6
+    gmax = np.abs(g).max()
+    rng = np.random.default_rng(1)
+    live = np.argwhere(np.abs(g) > 1e-3 * gmax)
+    picks = live[rng.choice(len(live), 25, replace=False)]
+    worst = 0.0
+    for idx in picks:
+        idx = tuple(idx)
+        lp, lm = level.copy(), level.copy()
+        lp[idx] += step
+        lm[idx] -= step
+        fd = (P.J(lp, k, target) - P.J(lm, k, target)) / (2 * step)
+        err = abs(fd - g[idx])
+        worst = max(worst, err / abs(g[idx]))
+        assert err <= 1e-5 * abs(g[idx]) + 1e-8 * gmax, (idx, fd, g[idx])
+    print(f"worst relative entry error {worst:.2e} over {len(picks)} entries")
+
+    fdk = (P.J(level, k + step, target) - P.J(level, k - step, target)) / (2 * step)
+    assert abs(fdk - dk) <= 1e-5 * abs(dk), (fdk, dk)
+
+    # a smooth direction touching every node
+    x = np.linspace(-2, 2, N + 1)
+    Y, X, Z = np.meshgrid(x, x, x, indexing="ij")
+    v = np.sin(1.3 * X + 0.4) * np.cos(0.7 * Y - 0.2) + 0.5 * np.sin(0.9 * Z + 1.1)
+    fdv = (P.J(level + step * v, k, target) - P.J(level - step * v, k, target)) / (2 * step)
+    gv = float(np.sum(g * v))
+    assert abs(fdv - gv) <= 1e-6 * abs(gv), (fdv, gv)
+
+
+def test_nodes_outside_every_stencil_have_zero_gradient(gpu):
+    P = Pipeline(gpu, 1, 0, 1, 3.0)
+    target = P.field(P.star(0.10), 3.0)
+    level = P.star(0.15)
+    _, g, _ = P.gradient(level, 3.0, target)
+    J0 = P.J(level, 3.0, target)
+    far = np.argwhere(np.abs(level) > 1.2)
+    for idx in far[np.random.default_rng(2).choice(len(far), 3, replace=False)]:
+        idx = tuple(idx)
+        assert g[idx] == 0.0
+        lp = level.copy()
+        lp[idx] += 1e-3
+        assert P.J(lp, 3.0, target) == J0
+
+
+def test_autograd_function_returns_the_adjoint_gradient(gpu):
+    import torch
+    import ibim_torch
+
+    P = Pipeline(gpu, 1, 0, 1, 3.0)
+    target = P.field(P.star(0.10), 3.0)
+    level = P.star(0.15)
+    J, g, dk = P.gradient(level, 3.0, target)
+    spec = ibim_torch.MisfitSpec(receivers=P.recv, target=target, eps=3 * H, tau=2 * H, x0=X0, tol=1e-13)
+    for dev in ("cpu", "cuda"):
+        lt = torch.tensor(level, dtype=torch.float64, device=dev, requires_grad=True)
+        kt = torch.tensor(3.0, dtype=torch.float64, device=dev, requires_grad=True)
+        Jt = ibim_torch.misfit(lt, kt, spec)
+        (2.0 * Jt).backward()
+        assert abs(Jt.item() - J) <= 1e-12 * J
+        assert lt.grad.device.type == dev
+        np.testing.assert_allclose(lt.grad.cpu().numpy(), 2.0 * g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
+        assert abs(kt.grad.item() - 2.0 * dk) <= 1e-9 * abs(dk)
