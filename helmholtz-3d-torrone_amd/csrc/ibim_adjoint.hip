@@ -497,7 +497,7 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
     if (J) *J = Jv;
     IBIM_HIP(hipMemcpyAsync(e, hu.data(), sizeof(double2) * nrecv, hipMemcpyHostToDevice, ctx->stream));
     const unsigned nb = (unsigned)((count + kT - 1) / kT);
-    TimedScope ts(ctx, "gradient");
+    TimedScope ts(ctx, "adjoint");
     hipLaunchKernelGGL(k_adj_eval, dim3(nb), dim3(kT), 0, ctx->stream, ctx->tube.as<double>(), count, ctx->prm, mode,
                        phi, xyz, e, nrecv, g, adj);
     IBIM_HIP(hipGetLastError());

@@ -469,6 +469,7 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     memset(info, 0, sizeof *info);
     if (opts->method == IBIM_SOLVE_BICG) return run_bicg(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_REFINED) return run_refined(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_RELIABLE) return run_reliable(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
 }
 

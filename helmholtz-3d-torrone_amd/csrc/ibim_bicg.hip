@@ -42,40 +42,76 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Products of a stored entry: FP64 entries in FP64; FP32 entries in FP32 with
+// the vector rounded to FP32 once per tile (6e-8 relative per product). Every
+// sum longer than one thread's kMvCpt columns or kMvUnroll rows runs in FP64:
+// long FP32 chains make the operator vector-dependent at the 1e-5 level and
+// cost BiCG 2x the iterations.
 template <class MT>
 struct MatTraits;
 template <>
 struct MatTraits<double2> {
-    using acc = double;
+    using prod = double;
 };
 template <>
 struct MatTraits<float2> {
-    using acc = float;
+    using prod = float;
 };
+
+// Sum 8 per-lane values over the wavefront, transposed: three halving
+// exchanges (xor 32, 16, 8) leave each lane with one value index, then a
+// three-step butterfly (xor 4, 2, 1) completes it. Lane l ends with the full
+// sum of value ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1):
+// 10 shuffles instead of 48 for eight independent butterflies.
+__device__ __forceinline__ double wave_sum8_transposed(const double (&v)[8], int lane) {
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+    double w[4], u[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const double send = h5 ? v[i] : v[i + 4];
+        const double keep = h5 ? v[i + 4] : v[i];
+        w[i] = keep + __shfl_xor(send, 32);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const double send = h4 ? w[i] : w[i + 2];
+        const double keep = h4 ? w[i + 2] : w[i];
+        u[i] = keep + __shfl_xor(send, 16);
+    }
+    double z = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    return z;
+}
+
+static_assert(kMvUnroll == 4, "the transposed row reduction packs 4 rows x (re, im)");
 
 template <class MT>
 __global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, size_t ld, int count,
                                                       const double2* __restrict__ p, const double2* __restrict__ q,
                                                       double2* __restrict__ ypart, double2* __restrict__ zpart,
                                                       const int* __restrict__ done) {
-    using AT = typename MatTraits<MT>::acc;
+    using PT = typename MatTraits<MT>::prod;
     if (done != nullptr && *done) return;
-    __shared__ AT ysh[2][kMvThreads / 64][kMvRows];
+    __shared__ double ysh[2][kMvThreads / 64][kMvRows];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int c0 = blockIdx.x * kMvCols;
     const int r0 = blockIdx.y * kMvRows;
     const int nrow = min(kMvRows, count - r0);
+    const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
 
-    AT pre[kMvCpt], pim[kMvCpt], zre[kMvCpt], zim[kMvCpt];
+    PT pre[kMvCpt], pim[kMvCpt];
+    double zre[kMvCpt], zim[kMvCpt];
     bool cok[kMvCpt];
 #pragma unroll
     for (int m = 0; m < kMvCpt; ++m) {
         int c = c0 + t + kMvThreads * m;
         cok[m] = c < count;
         double2 pv = cok[m] ? p[c] : make_double2(0.0, 0.0);
-        pre[m] = (AT)pv.x;
-        pim[m] = (AT)pv.y;
-        zre[m] = zim[m] = (AT)0;
+        pre[m] = (PT)pv.x;
+        pim[m] = (PT)pv.y;
+        zre[m] = zim[m] = 0.0;
     }
     const MT* Ab = A + (size_t)r0 * ld + c0 + t;
     for (int rr = 0; rr < nrow; rr += kMvUnroll) {
@@ -91,42 +127,50 @@ __global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, 
                     a[u][m].y = 0;
                 }
             }
+        double rows[2 * kMvUnroll];
+        PT zr[kMvCpt], zi[kMvCpt];
+#pragma unroll
+        for (int m = 0; m < kMvCpt; ++m) zr[m] = zi[m] = (PT)0;
 #pragma unroll
         for (int u = 0; u < kMvUnroll; ++u) {
             const int row = rr + u;
             double2 qv = row < nrow ? q[r0 + row] : make_double2(0.0, 0.0);
-            AT qre = (AT)qv.x, qim = (AT)qv.y;
-            AT sre = 0, sim = 0;
+            PT qre = (PT)qv.x, qim = (PT)qv.y;
+            PT sre = 0, sim = 0;
 #pragma unroll
             for (int m = 0; m < kMvCpt; ++m) {
-                AT ar = (AT)a[u][m].x, ai = (AT)a[u][m].y;
+                PT ar = (PT)a[u][m].x, ai = (PT)a[u][m].y;
                 sre += ar * pre[m] - ai * pim[m];
                 sim += ar * pim[m] + ai * pre[m];
-                zre[m] += ar * qre + ai * qim;  // conj(a) q
-                zim[m] += ar * qim - ai * qre;
+                zr[m] += ar * qre + ai * qim;  // conj(a) q
+                zi[m] += ar * qim - ai * qre;
             }
-            sre = wave_sum(sre);
-            sim = wave_sum(sim);
-            if (lane == 0 && row < nrow) {
-                ysh[0][wave][row] = sre;
-                ysh[1][wave][row] = sim;
-            }
+            rows[2 * u] = (double)sre;
+            rows[2 * u + 1] = (double)sim;
         }
+#pragma unroll
+        for (int m = 0; m < kMvCpt; ++m) {
+            zre[m] += (double)zr[m];
+            zim[m] += (double)zi[m];
+        }
+        const double tot = wave_sum8_transposed(rows, lane);
+        const int row = rr + (my_val >> 1);
+        if ((lane & 7) == 0 && row < nrow) ysh[my_val & 1][wave][row] = tot;
     }
     __syncthreads();
     for (int row = t; row < nrow; row += kMvThreads) {
         double sre = 0.0, sim = 0.0;
 #pragma unroll
         for (int w = 0; w < kMvThreads / 64; ++w) {
-            sre += (double)ysh[0][w][row];
-            sim += (double)ysh[1][w][row];
+            sre += ysh[0][w][row];
+            sim += ysh[1][w][row];
         }
         ypart[(size_t)blockIdx.x * ld + r0 + row] = make_double2(sre, sim);
     }
 #pragma unroll
     for (int m = 0; m < kMvCpt; ++m) {
         int c = c0 + t + kMvThreads * m;
-        if (cok[m]) zpart[(size_t)blockIdx.y * ld + c] = make_double2((double)zre[m], (double)zim[m]);
+        if (cok[m]) zpart[(size_t)blockIdx.y * ld + c] = make_double2(zre[m], zim[m]);
     }
 }
 
@@ -335,6 +379,31 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_update_p(VecArgs a, const 
     }
 }
 
+// r = b - A x from the partial planes of a sweep over x (FP64 residual
+// replacement), with per-block partials of |r|^2.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_replace(VecArgs a) {
+    __shared__ double sh[2 * 4];
+    double v[2] = {0.0, 0.0};
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        double yr = 0.0, yi = 0.0;
+        for (int c = 0; c < a.ncol_tiles; ++c) {
+            double2 y = a.ypart[(size_t)c * a.ld + j];
+            yr += y.x;
+            yi += y.y;
+        }
+        const double2 bb = a.b[j];
+        const double2 rr = make_double2(bb.x - yr, bb.y - yi);
+        a.r[j] = rr;
+        v[0] += rr.x * rr.x;
+        v[1] += rr.y * rr.y;
+    }
+    block_reduce<2>(v, sh);
+    if (threadIdx.x == 0) {
+        a.part[(size_t)blockIdx.x * 2] = v[0];
+        a.part[(size_t)blockIdx.x * 2 + 1] = v[1];
+    }
+}
+
 unsigned vec_blocks(int count) {
     unsigned nb = (unsigned)((count + kVecThreads - 1) / kVecThreads);
     return nb > (unsigned)kMaxVecBlocks ? (unsigned)kMaxVecBlocks : nb;
@@ -404,8 +473,37 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
 
 // One BiCG solve of A x = b for b, x on the device (length count). Returns
 // iterations and status in info; hist (host, may be NULL) gets the residuals.
+// FP64 residual replacement (van der Vorst and Ye, SIAM J. Sci. Comput. 22,
+// 2000): BiCG runs on a lower-precision operator; whenever its recursive
+// residual has dropped by `factor` since the last replacement, and when it
+// claims convergence, r is replaced by the true residual b - A x of the FP64
+// operator `true_op`. The Krylov recurrences carry on (no restart).
+struct Replace {
+    int true_op;
+    double factor;
+    int replacements;
+    double true_err;
+};
+
+// r = b - A_true x into the BiCG workspace; returns ||r|| / ||b||.
+int true_residual(Context* ctx, const VecArgs& a, unsigned nvb, Replace* rep, double normb, double* err) {
+    int rc = sweep_any(ctx, rep->true_op, a.x, a.x, a.ypart, a.zpart, nullptr);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_bicg_replace, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
+    IBIM_HIP(hipGetLastError());
+    std::vector<double> h(2 * (size_t)nvb);
+    IBIM_HIP(hipMemcpyAsync(h.data(), a.part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    double s2 = 0.0;
+    for (double v : h) s2 += v;
+    *err = sqrt(s2) / normb;
+    rep->replacements++;
+    rep->true_err = *err;
+    return IBIM_OK;
+}
+
 int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* x_out, int maxit, double tol,
-                int chunk, ibim_solve_info* info, double* hist, bool adjoint = false) {
+                int chunk, ibim_solve_info* info, double* hist, bool adjoint = false, Replace* rep = nullptr) {
     VecArgs a = ws.a;
     a.b = b;
     // The operator's adjoint system A^H x = b runs the same recurrences: one
@@ -435,6 +533,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     }
     int it = 1, done = 0;
     if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
+    double next_replace = rep ? rep->factor : 0.0;
     while (it <= maxit && !done) {
         int last = it + chunk - 1 < maxit ? it + chunk - 1 : maxit;
         for (; it <= last; ++it) {
@@ -449,7 +548,31 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
         }
         IBIM_HIP(hipGetLastError());
         IBIM_HIP(hipMemcpyAsync(&done, a.done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        BicgScalars at_last{};
+        if (rep) IBIM_HIP(hipMemcpyAsync(&at_last, a.sc + last, sizeof at_last, hipMemcpyDeviceToHost, ctx->stream));
         IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        if (!rep) continue;
+        BicgScalars cur = at_last;
+        if (done) {
+            IBIM_HIP(hipMemcpyAsync(&cur, a.sc + done, sizeof cur, hipMemcpyDeviceToHost, ctx->stream));
+            IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        }
+        const bool claims = done && cur.stop == 1;
+        if (done && !claims) break;  // NaN or stagnation: no replacement helps
+        if (!claims && !(cur.err < next_replace)) continue;
+        double terr = 0.0;
+        int rc = true_residual(ctx, a, nvb, rep, norms[0], &terr);
+        if (rc) return rc;
+        if (claims && terr < tol) break;
+        // carry on from the last real iteration with the true residual
+        if (done) {
+            it = done + 1;
+            cur.stop = 0;
+            IBIM_HIP(hipMemcpyAsync(a.sc + done, &cur, sizeof cur, hipMemcpyHostToDevice, ctx->stream));
+            IBIM_HIP(hipMemsetAsync(a.done, 0, sizeof(int), ctx->stream));
+            done = 0;
+        }
+        next_replace = terr * rep->factor;
     }
     const int iters = done ? done : maxit;
     std::vector<BicgScalars> sc((size_t)iters + 1);
@@ -460,7 +583,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     if (hist)
         for (int i = 1; i <= iters; ++i) hist[i - 1] = sc[i].err;
     info->iterations = iters;
-    info->rel_residual = sc[iters].err;
+    info->rel_residual = rep && rep->replacements ? rep->true_err : sc[iters].err;
     info->status = done ? (sc[iters].stop == 1 ? 0 : sc[iters].stop) : 1;
     return IBIM_OK;
 }
@@ -496,6 +619,34 @@ int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxi
     int rc = prepare(ctx, maxit, &ws);
     if (rc) return rc;
     return bicg_device(ctx, op, ws, (const double2*)b_dev, (double2*)x_dev, maxit, tol, chunk, info, hist, adjoint);
+}
+
+// BiCG on the FP32 operator (the stored complex-FP32 matrix when there is one,
+// else FP32 entries recomputed per sweep) with FP64 residual replacement by the
+// FP64 operator (stored complex-FP64, else recomputed FP64 entries).
+int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    const int op = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
+    const int true_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    Workspace ws;
+    int rc = prepare(ctx, opts->maxit, &ws);
+    if (rc) return rc;
+    int total = 0, outer = 0;
+    ibim_solve_info last{};
+    for (int c = 0; c < ctx->nrhs; ++c) {
+        const double2* b = ctx->rhs.as<double2>() + (size_t)c * ctx->count;
+        double2* x = ctx->density.as<double2>() + (size_t)c * ctx->count;
+        Replace rep{true_op, opts->inner_tol > 0 && opts->inner_tol < 1 ? opts->inner_tol : 1e-2, 0, 0.0};
+        rc = bicg_device(ctx, op, ws, b, x, opts->maxit, opts->tol, opts->chunk, &last, c == 0 ? hist : nullptr,
+                         false, &rep);
+        if (rc) return rc;
+        total += last.iterations;
+        outer += rep.replacements;
+    }
+    *info = last;
+    info->iterations = total;
+    info->outer = outer;
+    ctx->last_iters = total;
+    return IBIM_OK;
 }
 
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {

@@ -11,8 +11,10 @@
 // separation |x*_i - y*_j|^2 is formed in FP64, its square root is refined by
 // one FP64 Newton step, the phase k rho / (2 pi) is reduced modulo 1 in FP64,
 // and only the reduced revolution fraction goes to the FP32 sin / cos units
-// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry and both
-// products accumulate in FP32 over a tile; tiles are summed in FP64.
+// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry and its
+// products with the (FP32-rounded) vectors are FP32; sums over more than one
+// thread's 4 columns or 4 rows are FP64, which keeps BiCG's iteration count
+// close to the FP64 operator's.
 //
 // Refinement: x = 0, r = b; repeat { solve A32 d = r by BiCG to inner_tol;
 // x += d; r = b - A64 x with the FP64 operator } until ||r|| / ||b|| < tol.
@@ -198,6 +200,27 @@ struct RowStage32 {
     float nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage], qre[kStage], qim[kStage];
 };
 
+// Sum 8 per-lane values over the wavefront (transposed: halving exchanges
+// xor 32 / 16 / 8, then a butterfly over xor 4 / 2 / 1). Lane l ends with the
+// sum of value ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
+__device__ __forceinline__ double wsum8_transposed(const double (&v)[8], int lane) {
+    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
+    double w[4], u[2];
+#pragma unroll
+    for (int i = 0; i < 4; ++i) w[i] = (h5 ? v[i + 4] : v[i]) + __shfl_xor(h5 ? v[i] : v[i + 4], 32);
+#pragma unroll
+    for (int i = 0; i < 2; ++i) u[i] = (h4 ? w[i + 2] : w[i]) + __shfl_xor(h4 ? w[i] : w[i + 2], 16);
+    double z = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8);
+    z += __shfl_xor(z, 4);
+    z += __shfl_xor(z, 2);
+    z += __shfl_xor(z, 1);
+    return z;
+}
+
+constexpr int kRowBatch = 4;  // rows per transposed reduction (4 rows x re, im)
+
+// Products in FP32 (the vectors rounded to FP32 once per tile); sums longer
+// than one thread's kFCpt columns or kRowBatch rows run in FP64.
 template <int Dirichlet, int Interior>
 __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__ tube, int count, ibim_params prm,
                                                       const double2* __restrict__ p, const double2* __restrict__ q,
@@ -205,7 +228,7 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
                                                       size_t pld, const int* __restrict__ done) {
     if (done != nullptr && *done) return;
     __shared__ RowStage32 st;
-    __shared__ float ysh[2][kFT / 64][kFRows];
+    __shared__ double ysh[2][kFT / 64][kFRows];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int c0 = blockIdx.x * kSweepCols;
     const int r0 = blockIdx.y * kFRows;
@@ -215,10 +238,12 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
     const float kf = (float)prm.k;
     const float rfp = (float)(1.0 / (4.0 * kPi));
     const double tau2 = prm.tau * prm.tau;
+    const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
 
     double csx[kFCpt], csy[kFCpt], csz[kFCpt];
     float cnx[kFCpt], cny[kFCpt], cnz[kFCpt], cw[kFCpt], creg[kFCpt];
-    float pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
+    float pre[kFCpt], pim[kFCpt];
+    double zre[kFCpt], zim[kFCpt];
     int cj[kFCpt];
 #pragma unroll
     for (int m = 0; m < kFCpt; ++m) {
@@ -236,13 +261,17 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
         double2 pv = ok ? p[j] : make_double2(0.0, 0.0);
         pre[m] = (float)pv.x;
         pim[m] = (float)pv.y;
-        zre[m] = zim[m] = 0.0f;
+        zre[m] = zim[m] = 0.0;
     }
     for (int s0 = 0; s0 < nrow; s0 += kStage) {
         const int ns = min(kStage, nrow - s0);
         __syncthreads();
-        if (t < ns) {
-            const int i = r0 + s0 + t;
+        if (t < kStage) {
+            // rows past the tile repeat the tile's first row with q = 0: finite
+            // entries times zero add nothing to the column sums, and their row
+            // sums are dropped
+            const bool ok = t < ns;
+            const int i = r0 + s0 + (ok ? t : 0);
             st.sx[t] = tube[IBIM_F_SX * cnt + i];
             st.sy[t] = tube[IBIM_F_SY * cnt + i];
             st.sz[t] = tube[IBIM_F_SZ * cnt + i];
@@ -251,54 +280,67 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
             st.nz[t] = (float)tube[IBIM_F_NZ * cnt + i];
             st.w[t] = (float)tube[IBIM_F_W * cnt + i];
             st.reg[t] = (float)tube[IBIM_F_REG * cnt + i];
-            double2 qv = q[i];
+            double2 qv = ok ? q[i] : make_double2(0.0, 0.0);
             st.qre[t] = (float)qv.x;
             st.qim[t] = (float)qv.y;
         }
         __syncthreads();
-        for (int s = 0; s < ns; ++s) {
-            const int i = r0 + s0 + s;
-            const double ax = st.sx[s], ay = st.sy[s], az = st.sz[s];
-            const float anx = st.nx[s], any = st.ny[s], anz = st.nz[s], areg = st.reg[s];
-            const float qre = st.qre[s], qim = st.qim[s];
-            float sre = 0.0f, sim = 0.0f;
+        for (int sb = 0; sb < ns; sb += kRowBatch) {
+            double rows[2 * kRowBatch];
+            float zr[kFCpt], zi[kFCpt];
+#pragma unroll
+            for (int m = 0; m < kFCpt; ++m) zr[m] = zi[m] = 0.0f;
+#pragma unroll
+            for (int u = 0; u < kRowBatch; ++u) {
+                const int s = sb + u;  // < kStage; rows >= ns carry q = 0 and their sums are dropped
+                const int i = r0 + s0 + s;
+                const double ax = st.sx[s], ay = st.sy[s], az = st.sz[s];
+                const float anx = st.nx[s], any = st.ny[s], anz = st.nz[s], areg = st.reg[s];
+                const float qre = st.qre[s], qim = st.qim[s];
+                float sre = 0.0f, sim = 0.0f;
+#pragma unroll
+                for (int m = 0; m < kFCpt; ++m) {
+                    const double d1 = ax - csx[m], d2 = ay - csy[m], d3 = az - csz[m];
+                    const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+                    float pr, pi_;
+                    if (rho2 > tau2) {
+                        const float r = sqrtf((float)rho2);
+                        const float ir = 1.0f / r;
+                        const double rd = (double)r;
+                        const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);
+                        const double tph = rho * kappa;
+                        const float f = (float)(tph - rint(tph));
+                        const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
+                        const float den = rfp * ir * ir;
+                        const float c1 = -(kf * sv + cv * ir) * den;
+                        const float c2 = (kf * cv - sv * ir) * den;
+                        const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
+                        const float ip = Dirichlet ? f1 * cnx[m] + f2 * cny[m] + f3 * cnz[m]
+                                                   : f1 * anx + f2 * any + f3 * anz;
+                        pr = Interior ? -c1 * ip : c1 * ip;
+                        pi_ = Interior ? -c2 * ip : c2 * ip;
+                    } else {
+                        pr = Dirichlet ? creg[m] : areg;
+                        pi_ = 0.0f;
+                    }
+                    float er = cw[m] * pr, ei = cw[m] * pi_;
+                    if (i == cj[m]) er += 0.5f;
+                    sre += er * pre[m] - ei * pim[m];
+                    sim += er * pim[m] + ei * pre[m];
+                    zr[m] += er * qre + ei * qim;
+                    zi[m] += er * qim - ei * qre;
+                }
+                rows[2 * u] = (double)sre;
+                rows[2 * u + 1] = (double)sim;
+            }
 #pragma unroll
             for (int m = 0; m < kFCpt; ++m) {
-                const double d1 = ax - csx[m], d2 = ay - csy[m], d3 = az - csz[m];
-                const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
-                float pr, pi_;
-                if (rho2 > tau2) {
-                    const float r = sqrtf((float)rho2);
-                    const float ir = 1.0f / r;
-                    const double rd = (double)r;
-                    const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);
-                    const double tph = rho * kappa;
-                    const float f = (float)(tph - rint(tph));
-                    const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
-                    const float den = rfp * ir * ir;
-                    const float c1 = -(kf * sv + cv * ir) * den;
-                    const float c2 = (kf * cv - sv * ir) * den;
-                    const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-                    const float ip = Dirichlet ? f1 * cnx[m] + f2 * cny[m] + f3 * cnz[m] : f1 * anx + f2 * any + f3 * anz;
-                    pr = Interior ? -c1 * ip : c1 * ip;
-                    pi_ = Interior ? -c2 * ip : c2 * ip;
-                } else {
-                    pr = Dirichlet ? creg[m] : areg;
-                    pi_ = 0.0f;
-                }
-                float er = cw[m] * pr, ei = cw[m] * pi_;
-                if (i == cj[m]) er += 0.5f;
-                sre += er * pre[m] - ei * pim[m];
-                sim += er * pim[m] + ei * pre[m];
-                zre[m] += er * qre + ei * qim;
-                zim[m] += er * qim - ei * qre;
+                zre[m] += (double)zr[m];
+                zim[m] += (double)zi[m];
             }
-            sre = wsum(sre);
-            sim = wsum(sim);
-            if (lane == 0) {
-                ysh[0][wave][s0 + s] = sre;
-                ysh[1][wave][s0 + s] = sim;
-            }
+            const double tot = wsum8_transposed(rows, lane);
+            const int s = sb + (my_val >> 1);
+            if ((lane & 7) == 0 && s < ns) ysh[my_val & 1][wave][s0 + s] = tot;
         }
     }
     __syncthreads();
@@ -306,14 +348,14 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
         double a = 0.0, b = 0.0;
 #pragma unroll
         for (int w = 0; w < kFT / 64; ++w) {
-            a += (double)ysh[0][w][row];
-            b += (double)ysh[1][w][row];
+            a += ysh[0][w][row];
+            b += ysh[1][w][row];
         }
         ypart[(size_t)blockIdx.x * pld + r0 + row] = make_double2(a, b);
     }
 #pragma unroll
     for (int m = 0; m < kFCpt; ++m)
-        if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2((double)zre[m], (double)zim[m]);
+        if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2(zre[m], zim[m]);
 }
 
 // r = b - y, partial sums of |r|^2 per block.
@@ -408,8 +450,10 @@ int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
         }
         status = 1;
         for (outer = 1; outer <= max_outer; ++outer) {
+            // later corrections only need to bring the residual from `res` to tol
+            const double itol = outer == 1 ? inner_tol : fmax(inner_tol, fmin(0.5, 0.3 * opts->tol / res));
             ibim_solve_info in{};
-            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, inner_tol, opts->chunk, &in, nullptr, false);
+            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, itol, opts->chunk, &in, nullptr, false);
             if (rc) return rc;
             total_its += in.iterations;
             hipLaunchKernelGGL(k_add, dim3(nb), dim3(256), 0, ctx->stream, x, d, count);

@@ -136,6 +136,7 @@ int launch_eval(Context* ctx, int mode, int col, int nrecv, const double* xyz_de
 int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, double* J, double* dlevel, double* dk);
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
 
 // Operator used by a sweep: the stored matrix, or entries recomputed on the fly.

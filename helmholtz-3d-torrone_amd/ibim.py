@@ -35,7 +35,7 @@ TUBE_TRADITIONAL, TUBE_COMBO = 0, 1
 C128, C64 = 0, 1
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
 EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO = 0, 1, 2
-SOLVE_BICG, SOLVE_REFINED = 0, 1
+SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2
 OP_STORED, OP_FREE64, OP_FREE32 = 0, 1, 2
 
 

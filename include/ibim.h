@@ -90,12 +90,16 @@ typedef struct ibim_params {
 
 #define IBIM_SOLVE_BICG 0    /* the reference's complex BiCG on the stored operator      */
 #define IBIM_SOLVE_REFINED 1 /* FP64 iterative refinement around a complex-FP32 BiCG      */
+#define IBIM_SOLVE_RELIABLE 2 /* complex-FP32 BiCG with FP64 residual replacement: every */
+                              /* inner_tol drop of the residual (default 1e-2) and at    */
+                              /* convergence r = b - A64 x; no restarts                  */
 
 typedef struct ibim_solve_opts {
     int method;        /* IBIM_SOLVE_*                                        */
     int maxit;         /* BiCG iteration cap (1000 / 2000 in the reference)   */
     double tol;        /* relative residual target (DEFAULTACC = 1e-7)        */
-    double inner_tol;  /* refinement: inner FP32 BiCG relative tolerance      */
+    double inner_tol;  /* refinement: first inner FP32 BiCG relative tolerance; */
+                       /* later ones use max(inner_tol, 0.3 tol / residual)      */
     int max_outer;     /* refinement: outer FP64 steps                        */
     int chunk;         /* iterations launched between host checks (0 = auto)  */
 } ibim_solve_opts;
