@@ -42,21 +42,11 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
-// Products of a stored entry: FP64 entries in FP64; FP32 entries in FP32 with
-// the vector rounded to FP32 once per tile (6e-8 relative per product). Every
-// sum longer than one thread's kMvCpt columns or kMvUnroll rows runs in FP64:
-// long FP32 chains make the operator vector-dependent at the 1e-5 level and
-// cost BiCG 2x the iterations.
-template <class MT>
-struct MatTraits;
-template <>
-struct MatTraits<double2> {
-    using prod = double;
-};
-template <>
-struct MatTraits<float2> {
-    using prod = float;
-};
+// Entries are applied in FP64 whatever their storage: a stored complex64
+// operator is then one fixed matrix for BiCG (FP32 products or sums make it
+// vector-dependent at the 1e-5 level, which doubled the iteration count on a
+// 65k-point torus). With the transposed row reduction below the FP64 sweep over
+// complex64 storage still streams at the HBM read rate.
 
 // Sum 8 per-lane values over the wavefront, transposed: three halving
 // exchanges (xor 32, 16, 8) leave each lane with one value index, then a
@@ -87,12 +77,66 @@ __device__ __forceinline__ double wave_sum8_transposed(const double (&v)[8], int
 
 static_assert(kMvUnroll == 4, "the transposed row reduction packs 4 rows x (re, im)");
 
+// The row loop of one tile. Guard = false for interior tiles (all kMvRows rows
+// and kMvCols columns in range): unpredicated loads keep the complex64 sweep at
+// the HBM rate (per-element guards cost it 40%).
+template <bool Guard, class MT>
+__device__ __forceinline__ void sweep_rows(const MT* __restrict__ Ab, size_t ld, int nrow, const bool (&cok)[kMvCpt],
+                                           const double (&pre)[kMvCpt], const double (&pim)[kMvCpt],
+                                           double (&zre)[kMvCpt], double (&zim)[kMvCpt],
+                                           const double2* __restrict__ qt, double (&ysh)[2][kMvThreads / 64][kMvRows],
+                                           int lane, int wave, int my_val) {
+    for (int rr = 0; rr < nrow; rr += kMvUnroll) {
+        MT a[kMvUnroll][kMvCpt];
+#pragma unroll
+        for (int u = 0; u < kMvUnroll; ++u)
+#pragma unroll
+            for (int m = 0; m < kMvCpt; ++m) {
+                if (!Guard || (rr + u < nrow && cok[m])) {
+                    a[u][m] = Ab[(size_t)(rr + u) * ld + kMvThreads * m];
+                } else {
+                    a[u][m].x = 0;
+                    a[u][m].y = 0;
+                }
+            }
+        double rows[2 * kMvUnroll];
+        double zr[kMvCpt], zi[kMvCpt];
+#pragma unroll
+        for (int m = 0; m < kMvCpt; ++m) zr[m] = zi[m] = 0.0;
+#pragma unroll
+        for (int u = 0; u < kMvUnroll; ++u) {
+            const double2 qv = (!Guard || rr + u < nrow) ? qt[rr + u] : make_double2(0.0, 0.0);
+            double sre = 0.0, sim = 0.0;
+#pragma unroll
+            for (int m = 0; m < kMvCpt; ++m) {
+                // explicit FMAs: the file is built without contraction for the
+                // BiCG vector updates, and separate multiplies and adds would
+                // drop the complex64 sweep off the HBM rate
+                const double ar = (double)a[u][m].x, ai = (double)a[u][m].y;
+                sre = fma(ar, pre[m], fma(-ai, pim[m], sre));
+                sim = fma(ar, pim[m], fma(ai, pre[m], sim));
+                zr[m] = fma(ar, qv.x, fma(ai, qv.y, zr[m]));  // conj(a) q
+                zi[m] = fma(ar, qv.y, fma(-ai, qv.x, zi[m]));
+            }
+            rows[2 * u] = sre;
+            rows[2 * u + 1] = sim;
+        }
+#pragma unroll
+        for (int m = 0; m < kMvCpt; ++m) {
+            zre[m] += zr[m];
+            zim[m] += zi[m];
+        }
+        const double tot = wave_sum8_transposed(rows, lane);
+        const int row = rr + (my_val >> 1);
+        if ((lane & 7) == 0 && (!Guard || row < nrow)) ysh[my_val & 1][wave][row] = tot;
+    }
+}
+
 template <class MT>
 __global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, size_t ld, int count,
                                                       const double2* __restrict__ p, const double2* __restrict__ q,
                                                       double2* __restrict__ ypart, double2* __restrict__ zpart,
                                                       const int* __restrict__ done) {
-    using PT = typename MatTraits<MT>::prod;
     if (done != nullptr && *done) return;
     __shared__ double ysh[2][kMvThreads / 64][kMvRows];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
@@ -101,62 +145,22 @@ __global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, 
     const int nrow = min(kMvRows, count - r0);
     const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
 
-    PT pre[kMvCpt], pim[kMvCpt];
-    double zre[kMvCpt], zim[kMvCpt];
+    double pre[kMvCpt], pim[kMvCpt], zre[kMvCpt], zim[kMvCpt];
     bool cok[kMvCpt];
 #pragma unroll
     for (int m = 0; m < kMvCpt; ++m) {
         int c = c0 + t + kMvThreads * m;
         cok[m] = c < count;
         double2 pv = cok[m] ? p[c] : make_double2(0.0, 0.0);
-        pre[m] = (PT)pv.x;
-        pim[m] = (PT)pv.y;
+        pre[m] = pv.x;
+        pim[m] = pv.y;
         zre[m] = zim[m] = 0.0;
     }
     const MT* Ab = A + (size_t)r0 * ld + c0 + t;
-    for (int rr = 0; rr < nrow; rr += kMvUnroll) {
-        MT a[kMvUnroll][kMvCpt];
-#pragma unroll
-        for (int u = 0; u < kMvUnroll; ++u)
-#pragma unroll
-            for (int m = 0; m < kMvCpt; ++m) {
-                if (rr + u < nrow && cok[m]) {
-                    a[u][m] = Ab[(size_t)(rr + u) * ld + kMvThreads * m];
-                } else {
-                    a[u][m].x = 0;
-                    a[u][m].y = 0;
-                }
-            }
-        double rows[2 * kMvUnroll];
-        PT zr[kMvCpt], zi[kMvCpt];
-#pragma unroll
-        for (int m = 0; m < kMvCpt; ++m) zr[m] = zi[m] = (PT)0;
-#pragma unroll
-        for (int u = 0; u < kMvUnroll; ++u) {
-            const int row = rr + u;
-            double2 qv = row < nrow ? q[r0 + row] : make_double2(0.0, 0.0);
-            PT qre = (PT)qv.x, qim = (PT)qv.y;
-            PT sre = 0, sim = 0;
-#pragma unroll
-            for (int m = 0; m < kMvCpt; ++m) {
-                PT ar = (PT)a[u][m].x, ai = (PT)a[u][m].y;
-                sre += ar * pre[m] - ai * pim[m];
-                sim += ar * pim[m] + ai * pre[m];
-                zr[m] += ar * qre + ai * qim;  // conj(a) q
-                zi[m] += ar * qim - ai * qre;
-            }
-            rows[2 * u] = (double)sre;
-            rows[2 * u + 1] = (double)sim;
-        }
-#pragma unroll
-        for (int m = 0; m < kMvCpt; ++m) {
-            zre[m] += (double)zr[m];
-            zim[m] += (double)zi[m];
-        }
-        const double tot = wave_sum8_transposed(rows, lane);
-        const int row = rr + (my_val >> 1);
-        if ((lane & 7) == 0 && row < nrow) ysh[my_val & 1][wave][row] = tot;
-    }
+    if (nrow == kMvRows && c0 + kMvCols <= count)
+        sweep_rows<false>(Ab, ld, nrow, cok, pre, pim, zre, zim, q + r0, ysh, lane, wave, my_val);
+    else
+        sweep_rows<true>(Ab, ld, nrow, cok, pre, pim, zre, zim, q + r0, ysh, lane, wave, my_val);
     __syncthreads();
     for (int row = t; row < nrow; row += kMvThreads) {
         double sre = 0.0, sim = 0.0;

@@ -11,10 +11,9 @@
 // separation |x*_i - y*_j|^2 is formed in FP64, its square root is refined by
 // one FP64 Newton step, the phase k rho / (2 pi) is reduced modulo 1 in FP64,
 // and only the reduced revolution fraction goes to the FP32 sin / cos units
-// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry and its
-// products with the (FP32-rounded) vectors are FP32; sums over more than one
-// thread's 4 columns or 4 rows are FP64, which keeps BiCG's iteration count
-// close to the FP64 operator's.
+// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry is FP32;
+// it is applied to the FP64 vectors in FP64, so the FP32 operator is one fixed
+// matrix and BiCG keeps close to the FP64 operator's iteration count.
 //
 // Refinement: x = 0, r = b; repeat { solve A32 d = r by BiCG to inner_tol;
 // x += d; r = b - A64 x with the FP64 operator } until ||r|| / ||b|| < tol.
@@ -197,7 +196,8 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
 // staged in LDS the same way. Dirichlet / interior are compile-time switches.
 struct RowStage32 {
     double sx[kStage], sy[kStage], sz[kStage];
-    float nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage], qre[kStage], qim[kStage];
+    float nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage];
+    double qre[kStage], qim[kStage];
 };
 
 // Sum 8 per-lane values over the wavefront (transposed: halving exchanges
@@ -219,8 +219,8 @@ __device__ __forceinline__ double wsum8_transposed(const double (&v)[8], int lan
 
 constexpr int kRowBatch = 4;  // rows per transposed reduction (4 rows x re, im)
 
-// Products in FP32 (the vectors rounded to FP32 once per tile); sums longer
-// than one thread's kFCpt columns or kRowBatch rows run in FP64.
+// The FP32 entry is applied to the FP64 vectors in FP64 (products and sums), so
+// the recomputed FP32 operator is one fixed matrix for BiCG.
 template <int Dirichlet, int Interior>
 __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__ tube, int count, ibim_params prm,
                                                       const double2* __restrict__ p, const double2* __restrict__ q,
@@ -242,7 +242,7 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
 
     double csx[kFCpt], csy[kFCpt], csz[kFCpt];
     float cnx[kFCpt], cny[kFCpt], cnz[kFCpt], cw[kFCpt], creg[kFCpt];
-    float pre[kFCpt], pim[kFCpt];
+    double pre[kFCpt], pim[kFCpt];
     double zre[kFCpt], zim[kFCpt];
     int cj[kFCpt];
 #pragma unroll
@@ -259,8 +259,8 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
         cw[m] = ok ? (float)tube[IBIM_F_W * cnt + j] : 0.0f;  // w = 0 silences padding columns
         creg[m] = (float)tube[IBIM_F_REG * cnt + j];
         double2 pv = ok ? p[j] : make_double2(0.0, 0.0);
-        pre[m] = (float)pv.x;
-        pim[m] = (float)pv.y;
+        pre[m] = pv.x;
+        pim[m] = pv.y;
         zre[m] = zim[m] = 0.0;
     }
     for (int s0 = 0; s0 < nrow; s0 += kStage) {
@@ -281,23 +281,20 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
             st.w[t] = (float)tube[IBIM_F_W * cnt + i];
             st.reg[t] = (float)tube[IBIM_F_REG * cnt + i];
             double2 qv = ok ? q[i] : make_double2(0.0, 0.0);
-            st.qre[t] = (float)qv.x;
-            st.qim[t] = (float)qv.y;
+            st.qre[t] = qv.x;
+            st.qim[t] = qv.y;
         }
         __syncthreads();
         for (int sb = 0; sb < ns; sb += kRowBatch) {
             double rows[2 * kRowBatch];
-            float zr[kFCpt], zi[kFCpt];
-#pragma unroll
-            for (int m = 0; m < kFCpt; ++m) zr[m] = zi[m] = 0.0f;
 #pragma unroll
             for (int u = 0; u < kRowBatch; ++u) {
                 const int s = sb + u;  // < kStage; rows >= ns carry q = 0 and their sums are dropped
                 const int i = r0 + s0 + s;
                 const double ax = st.sx[s], ay = st.sy[s], az = st.sz[s];
                 const float anx = st.nx[s], any = st.ny[s], anz = st.nz[s], areg = st.reg[s];
-                const float qre = st.qre[s], qim = st.qim[s];
-                float sre = 0.0f, sim = 0.0f;
+                const double qre = st.qre[s], qim = st.qim[s];
+                double sre = 0.0, sim = 0.0;
 #pragma unroll
                 for (int m = 0; m < kFCpt; ++m) {
                     const double d1 = ax - csx[m], d2 = ay - csy[m], d3 = az - csz[m];
@@ -323,20 +320,16 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
                         pr = Dirichlet ? creg[m] : areg;
                         pi_ = 0.0f;
                     }
-                    float er = cw[m] * pr, ei = cw[m] * pi_;
-                    if (i == cj[m]) er += 0.5f;
+                    float erf = cw[m] * pr;
+                    if (i == cj[m]) erf += 0.5f;
+                    const double er = (double)erf, ei = (double)(cw[m] * pi_);
                     sre += er * pre[m] - ei * pim[m];
                     sim += er * pim[m] + ei * pre[m];
-                    zr[m] += er * qre + ei * qim;
-                    zi[m] += er * qim - ei * qre;
+                    zre[m] += er * qre + ei * qim;
+                    zim[m] += er * qim - ei * qre;
                 }
-                rows[2 * u] = (double)sre;
-                rows[2 * u + 1] = (double)sim;
-            }
-#pragma unroll
-            for (int m = 0; m < kFCpt; ++m) {
-                zre[m] += (double)zr[m];
-                zim[m] += (double)zi[m];
+                rows[2 * u] = sre;
+                rows[2 * u + 1] = sim;
             }
             const double tot = wsum8_transposed(rows, lane);
             const int s = sb + (my_val >> 1);

@@ -276,7 +276,7 @@ class Problem:
         info = SolveInfo()
         hist = np.zeros(max(maxit, max_outer))
         _check(lib().ibim_solve(self.h, C.byref(o), C.byref(info), _p(hist)), "ibim_solve")
-        n = info.iterations if method == SOLVE_BICG and self.nrhs == 1 else info.outer
+        n = info.iterations if method in (SOLVE_BICG, SOLVE_RELIABLE) and self.nrhs == 1 else info.outer
         return info, hist[:n]
 
     def density(self, col=0):
