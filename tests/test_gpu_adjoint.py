@@ -7,18 +7,18 @@ targets from a second surface. The finite differences re-run the forward
 path with FP64 iterative refinement to a relative residual of 1e-15, so their
 noise (about 1e-15 J / step) stays far below the tolerances below.
 
-J is only piecewise smooth in the level-set values: the Godunov upwind choice
-and the limited second differences switch stencils where two candidates tie,
-and the WENO weights of a nearly linear line (a distance function along its
-normal) vary on the scale of the data's fourth differences. A central
-difference measures the derivative only where it has converged, so each of
-the 50 sampled entries is differenced at steps 1e-6 and 5e-7: entries whose two
-differences agree (|f1 - f2| <= 1e-7 max|g|) are resolved and must satisfy
-
-  |R - g| <= 1e-5 |g| + 1e-9 max|g|,   R = (4 f(5e-7) - f(1e-6)) / 3,
-
-and at least 60% of the entries must be resolved. dJ/dk is compared the same
-way (always resolved).
+J is only piecewise smooth in the level-set values, and sharply curved between
+the kinks: the Godunov upwind choice and the limited second differences switch
+stencils where two candidates tie, and the WENO weights of a nearly linear
+line (a distance function along its normal) vary on the scale of the data's
+fourth differences. One-sided differences at steps 1e-8 .. 3e-6 converge to
+the adjoint value as the step shrinks, but a step of 1e-6 often straddles a
+kink. The check therefore uses central differences at 1e-8 (noise about
+1e-10 in dJ/dd, 1e-8 max|g|) on 50 seeded entries with |g| > 1e-2 max|g|; an
+entry whose differences at 1e-8 and 2e-8 disagree by more than
+5e-6 |g| + 3e-8 max|g| straddles a kink and is counted, not compared. The
+rest must satisfy |fd - g| <= 1e-5 |g| + 2e-8 max|g|, and at least 80% of the
+entries must qualify. dJ/dk uses the same rule at steps 1e-6 / 5e-7.
 """
 import numpy as np
 import pytest
@@ -104,21 +104,20 @@ def test_gradient_matches_central_differences(gpu, dirichlet, interior, mode, k)
         return (P.J(lp, k, target) - P.J(lm, k, target)) / (2 * step)
 
     rng = np.random.default_rng(1)
-    live = np.argwhere(np.abs(g) > 1e-3 * gmax)
+    live = np.argwhere(np.abs(g) > 1e-2 * gmax)
     picks = live[rng.choice(len(live), 50, replace=False)]
     resolved, worst = 0, 0.0
     for idx in picks:
         idx = tuple(idx)
-        f1, f2 = fd(idx, 1e-6), fd(idx, 5e-7)
-        if abs(f1 - f2) > 1e-7 * gmax:
+        f1, f2 = fd(idx, 1e-8), fd(idx, 2e-8)
+        if abs(f1 - f2) > 5e-6 * abs(g[idx]) + 3e-8 * gmax:
             continue
         resolved += 1
-        R = (4 * f2 - f1) / 3
-        err = abs(R - g[idx])
+        err = abs(f1 - g[idx])
         worst = max(worst, err / abs(g[idx]))
-        assert err <= 1e-5 * abs(g[idx]) + 1e-9 * gmax, (idx, R, g[idx])
+        assert err <= 1e-5 * abs(g[idx]) + 2e-8 * gmax, (idx, f1, f2, g[idx])
     print(f"{resolved} of {len(picks)} entries resolved; worst relative error {worst:.2e}")
-    assert resolved >= 0.6 * len(picks)
+    assert resolved >= 0.8 * len(picks)
 
     fk1 = (P.J(level, k + 1e-6, target) - P.J(level, k - 1e-6, target)) / 2e-6
     fk2 = (P.J(level, k + 5e-7, target) - P.J(level, k - 5e-7, target)) / 1e-6
@@ -149,7 +148,9 @@ def test_autograd_function_returns_the_adjoint_gradient(gpu):
     level = P.star(0.15)
     J, g, dk = P.gradient(level, 3.0, target)
     spec = ibim_torch.MisfitSpec(receivers=P.recv, target=target, eps=3 * H, tau=2 * H, x0=X0, tol=1e-13)
-    for dev in ("cpu", "cuda"):
+    # the library owns its device memory; torch tensors on either side are accepted
+    devices = ["cpu"] + (["cuda"] if torch.cuda.is_available() else [])
+    for dev in devices:
         lt = torch.tensor(level, dtype=torch.float64, device=dev, requires_grad=True)
         kt = torch.tensor(3.0, dtype=torch.float64, device=dev, requires_grad=True)
         Jt = ibim_torch.misfit(lt, kt, spec)
