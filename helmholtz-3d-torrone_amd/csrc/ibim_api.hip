@@ -137,7 +137,7 @@ void ibim_destroy(ibim_ctx* ctx) {
     collect_timers(ctx);
     for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
                       &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
-                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj})
+                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch})
         b->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -390,6 +390,16 @@ int ibim_apply(ibim_ctx* ctx, int op, const double* pre, const double* pim, cons
     return IBIM_OK;
 }
 
+int ibim_apply_batch(ibim_ctx* ctx, int op, int adjoint, int nvec, const double* pre, const double* pim, double* yre,
+                     double* yim) {
+    CHECK(ctx);
+    if (!ctx->count || nvec < 1 || nvec > 16 || !pre || !yre || !yim || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 ||
+        (op == IBIM_OP_STORED && ctx->mat_precision < 0) || (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL))
+        return IBIM_ERR_ARG;
+    const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
+    return batch_apply_host(ctx, kinds[op], adjoint != 0, nvec, pre, pim, yre, yim);
+}
+
 int ibim_set_nrhs(ibim_ctx* ctx, int nrhs) {
     CHECK(ctx);
     if (nrhs <= 0 || !ctx->count) return IBIM_ERR_ARG;
@@ -467,7 +477,8 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     CHECK(ctx);
     if (!opts || !info || opts->maxit <= 0 || !ctx->count) return IBIM_ERR_ARG;
     memset(info, 0, sizeof *info);
-    if (opts->method == IBIM_SOLVE_BICG) return run_bicg(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_BICG)
+        return ctx->nrhs > 1 && opts->batch == 0 ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_REFINED) return run_refined(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_RELIABLE) return run_reliable(ctx, opts, info, hist);
     return IBIM_ERR_ARG;

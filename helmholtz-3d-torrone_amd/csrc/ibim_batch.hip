@@ -1,0 +1,677 @@
+// ibim_batch.hip -- BiCG on up to 16 right-hand sides per pass over the operator.
+//
+// One pass over the operator applies it to 16 vectors at once, as a complex
+// GEMM on the FP64 matrix cores (v_mfma_f64_16x16x4f64, products and sums in
+// FP64, so every column sees the same fixed operator as the single-vector
+// path). The matrix-vector product Y = A P and the adjoint product Z = A^H Q
+// are two passes; at 16 right-hand sides both are matrix-core bound, so
+// fusing them would save no time and would need partial planes 16x larger.
+//
+//   row pass  (Y = A P):   a wave owns 16 rows and runs over all columns; the
+//                          16 x 64 block of A is staged through LDS (coalesced
+//                          row loads) and read back in the MFMA A-operand layout.
+//   column pass (Z = A^H Q): a wave owns 16 columns and runs over all rows; the
+//                          MFMA A operand is 4 rows x 16 consecutive columns,
+//                          loaded directly.
+// The vector operand (a step x 16 block of P or Q) is staged in LDS once per
+// workgroup and shared by its four waves.
+// Matrix-free variants evaluate the entry in the same layout instead of
+// loading it (traditional tube, FP64 or FP32 entries); the points of the
+// streamed index are staged in LDS per workgroup.
+//
+// Vectors are RHS-interleaved: element (point j, right-hand side r) at
+// [j * 16 + r], with ld rows so padding rows read as zero. Each column keeps
+// its own BiCG scalars and stop state (the reference's recurrences and stop
+// rules per column, BICGSTAB.c:104-306); finished columns freeze.
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <vector>
+
+#include "ibim_entries.hpp"
+#include "ibim_internal.hpp"
+
+namespace ibim {
+
+namespace {
+
+constexpr int kR = 16;     // right-hand sides per batch: the MFMA N dimension
+constexpr int kBT = 256;   // threads per workgroup (4 waves)
+constexpr int kKT = 64;    // columns (row pass) or rows (column pass) staged per step
+constexpr int kMaxVB = 1024;
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+// D = A B + C for a 16x4 A, 4x16 B, 16x16 D. Lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16], and D[4 v + l / 16][l % 16] in component v.
+__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// Complex accumulate with three real products (Gauss): for a = ar + i ai and
+// b = br + i bi, k1 += br (ar + ai), k2 += ar (bi - br), k3 += ai (br + bi);
+// then re = k1 - k3, im = k1 + k2. Three independent accumulators also keep
+// consecutive matrix-core instructions free of dependences.
+struct Acc3 {
+    d4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
+    __device__ __forceinline__ void add(double ar, double ai, double br, double bi) {
+        k1 = mfma(ar + ai, br, k1);
+        k2 = mfma(ar, bi - br, k2);
+        k3 = mfma(ai, br + bi, k3);
+    }
+    __device__ __forceinline__ double re(int v) const { return k1[v] - k3[v]; }
+    __device__ __forceinline__ double im(int v) const { return k1[v] + k2[v]; }
+};
+
+struct BState {
+    double rho[kR][2];
+    double alpha[kR][2];
+    double err[kR];
+    double normb[kR];
+    double sumb[kR];
+    int done[kR];  // 0 running; else the iteration it stopped at (-1: zero data)
+    int stop[kR];  // stop rule that fired (1 tol, 2 NaN, 3 stagnation, 4 zero data)
+    int all_done;
+};
+
+// ------------------------------------------------------------ stored passes --
+// Row pass: a workgroup owns 64 rows (16 per wave) and steps over the columns
+// KT at a time; each wave stages its 16 x KT block of A through LDS, and the
+// KT x 16 block of P is staged once per workgroup (every wave reuses it).
+template <class MT>
+struct RowStep {
+    static constexpr int kt = sizeof(MT) == 8 ? 64 : 32;  // 33 KB of A tiles either way
+};
+
+template <class MT>
+__global__ void __launch_bounds__(kBT) k_rows_stored(const MT* __restrict__ A, size_t ld, int count,
+                                                     const double2* __restrict__ P, double2* __restrict__ Y,
+                                                     const BState* __restrict__ st) {
+    constexpr int KT = RowStep<MT>::kt;
+    if (st != nullptr && st->all_done) return;
+    __shared__ MT tile[4][16][KT + 1];
+    __shared__ double2 ps[KT][kR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row0 = (blockIdx.x * 4 + w) * 16;
+    Acc3 y;
+    for (int k0 = 0; k0 < count; k0 += KT) {
+        // k0 + KT <= ld: both are multiples of KT (ld of 64) and k0 < count <= ld
+#pragma unroll 4
+        for (int e = lane; e < 16 * KT; e += 64) {
+            const int rr = e / KT, cc = e % KT, row = row0 + rr;
+            MT v;
+            v.x = 0;
+            v.y = 0;
+            if (row < count) v = A[(size_t)row * ld + k0 + cc];
+            tile[w][rr][cc] = v;
+        }
+        for (int e = threadIdx.x; e < KT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];
+        __syncthreads();
+#pragma unroll 4
+        for (int s = 0; s < KT / 4; ++s) {
+            const MT a = tile[w][li][4 * s + lk];
+            const double2 p = ps[4 * s + lk][li];
+            y.add((double)a.x, (double)a.y, p.x, p.y);
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int row = row0 + 4 * v + lk;
+        if (row < count) Y[(size_t)row * kR + li] = make_double2(y.re(v), y.im(v));
+    }
+}
+
+// Column pass: a wave owns 16 columns and steps over the rows; the kKT x 16
+// block of Q is staged once per workgroup.
+template <class MT>
+__global__ void __launch_bounds__(kBT) k_cols_stored(const MT* __restrict__ A, size_t ld, int count,
+                                                     const double2* __restrict__ Q, double2* __restrict__ Z,
+                                                     const BState* __restrict__ st) {
+    if (st != nullptr && st->all_done) return;
+    __shared__ double2 qs[kKT][kR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int c0 = (blockIdx.x * 4 + w) * 16;
+    const bool act = c0 < count;  // then c0 + 16 <= ld (multiples of 16 and 64)
+    const int col = c0 + li;
+    Acc3 z;
+    for (int i0 = 0; i0 < count; i0 += kKT) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kKT * kR; e += kBT) {
+            const int i = i0 + e / kR;
+            qs[e / kR][e % kR] = i < count ? Q[(size_t)i0 * kR + e] : make_double2(0.0, 0.0);
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int s = 0; s < kKT / 4; ++s) {
+            const int row = i0 + 4 * s + lk;
+            const double2 q = qs[4 * s + lk][li];
+            double ar = 0.0, ai = 0.0;
+            if (act && row < count) {
+                const MT a = A[(size_t)row * ld + col];
+                ar = (double)a.x;
+                ai = (double)a.y;
+            }
+            z.add(ar, -ai, q.x, q.y);  // conj(a) q
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = c0 + 4 * v + lk;
+        if (c < count) Z[(size_t)c * kR + li] = make_double2(z.re(v), z.im(v));
+    }
+}
+
+// ------------------------------------------------------- matrix-free passes --
+template <int Op>
+__device__ __forceinline__ void free_entry(const PointData& a, const PointData& b, bool diag, const ibim_params& prm,
+                                           double kappa, float kf, double tau2, double* re, double* im) {
+    if (Op == OP_FREE32) {
+        float fr, fi;
+        trad_entry_f32(a, b, diag, kappa, kf, tau2, prm.dirichlet, prm.interior, &fr, &fi);
+        *re = (double)fr;
+        *im = (double)fi;
+    } else {
+        trad_entry(a, b, diag, prm.k, prm.tau, prm.dirichlet, prm.interior, re, im);
+    }
+}
+
+template <int Op>
+__global__ void __launch_bounds__(kBT) k_rows_free(const double* __restrict__ tube, int count, ibim_params prm,
+                                                   const double2* __restrict__ P, double2* __restrict__ Y,
+                                                   const BState* __restrict__ st) {
+    if (st != nullptr && st->all_done) return;
+    __shared__ PointData cp[kKT];
+    __shared__ double2 ps[kKT][kR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row0 = (blockIdx.x * 4 + w) * 16;
+    const size_t cnt = (size_t)count;
+    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
+    const float kf = (float)prm.k;
+    const int row = row0 + li;
+    const bool rok = row < count;
+    const PointData a = load_point(tube, cnt, rok ? row : 0);
+    Acc3 y;
+    for (int k0 = 0; k0 < count; k0 += kKT) {
+        __syncthreads();
+        if (threadIdx.x < kKT) {
+            const int j = k0 + threadIdx.x;
+            cp[threadIdx.x] = load_point(tube, cnt, j < count ? j : 0);
+        }
+        for (int e = threadIdx.x; e < kKT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];  // rows < ld
+        __syncthreads();
+#pragma unroll 2
+        for (int s = 0; s < kKT / 4; ++s) {
+            const int j = k0 + 4 * s + lk;
+            double er = 0.0, ei = 0.0;
+            if (rok && j < count) free_entry<Op>(a, cp[4 * s + lk], row == j, prm, kappa, kf, tau2, &er, &ei);
+            const double2 p = ps[4 * s + lk][li];
+            y.add(er, ei, p.x, p.y);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 4 * v + lk;
+        if (r < count) Y[(size_t)r * kR + li] = make_double2(y.re(v), y.im(v));
+    }
+}
+
+template <int Op>
+__global__ void __launch_bounds__(kBT) k_cols_free(const double* __restrict__ tube, int count, ibim_params prm,
+                                                   const double2* __restrict__ Q, double2* __restrict__ Z,
+                                                   const BState* __restrict__ st) {
+    if (st != nullptr && st->all_done) return;
+    __shared__ PointData rp[kKT];
+    __shared__ double2 qs[kKT][kR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int c0 = (blockIdx.x * 4 + w) * 16;
+    const size_t cnt = (size_t)count;
+    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
+    const float kf = (float)prm.k;
+    const int col = c0 + li;
+    const bool cok = col < count;
+    const PointData b = load_point(tube, cnt, cok ? col : 0);
+    Acc3 z;
+    for (int i0 = 0; i0 < count; i0 += kKT) {
+        __syncthreads();
+        if (threadIdx.x < kKT) {
+            const int i = i0 + threadIdx.x;
+            rp[threadIdx.x] = load_point(tube, cnt, i < count ? i : 0);
+        }
+        for (int e = threadIdx.x; e < kKT * kR; e += kBT) qs[e / kR][e % kR] = Q[(size_t)i0 * kR + e];  // rows < ld
+        __syncthreads();
+#pragma unroll 2
+        for (int s = 0; s < kKT / 4; ++s) {
+            const int i = i0 + 4 * s + lk;
+            double ar = 0.0, ai = 0.0;
+            if (cok && i < count) free_entry<Op>(rp[4 * s + lk], b, i == col, prm, kappa, kf, tau2, &ar, &ai);
+            const double2 q = qs[4 * s + lk][li];
+            z.add(ar, -ai, q.x, q.y);  // conj(a) q
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int c = c0 + 4 * v + lk;
+        if (c < count) Z[(size_t)c * kR + li] = make_double2(z.re(v), z.im(v));
+    }
+}
+
+// ------------------------------------------------------------ vector kernels --
+struct BArgs {
+    double2 *x, *r, *rt, *p, *pt, *v, *w;
+    const double2* b;
+    double* part;     // per-block partials [block][kR][8]
+    BState* st;
+    double* hist;     // [maxit + 1][kR] relative residuals
+    int count;
+};
+
+// Thread t of a block works on right-hand side t % 16 for points
+// blockIdx.x * 16 + t / 16 (+ a grid stride of gridDim.x * 16); per-column sums
+// are reduced over the 4 lanes of a wave and the 4 waves in a fixed order.
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double* out /* [kR][8] */) {
+    __shared__ double sh[4][kR][NV];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, r = threadIdx.x & 15;
+#pragma unroll
+    for (int q = 0; q < NV; ++q) {
+        v[q] += __shfl_xor(v[q], 16);
+        v[q] += __shfl_xor(v[q], 32);
+    }
+    if (lane < 16)
+#pragma unroll
+        for (int q = 0; q < NV; ++q) sh[wave][r][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x < kR * NV) {
+        const int rr = threadIdx.x / NV, q = threadIdx.x % NV;
+        out[rr * 8 + q] = (sh[0][rr][q] + sh[1][rr][q]) + (sh[2][rr][q] + sh[3][rr][q]);
+    }
+}
+
+// Reduce partials [nb][kR][8] to tot[kR][8] in LDS (one workgroup, fixed order).
+__device__ __forceinline__ void reduce_all(const double* __restrict__ part, int nb, int nv, double (*tot)[8]) {
+    __shared__ double half[2][kR * 8];
+    const int t = threadIdx.x, idx = t & 127, h = t >> 7;
+    double s = 0.0;
+    if ((idx & 7) < nv)
+        for (int b = h; b < nb; b += 2) s += part[(size_t)b * kR * 8 + idx];
+    half[h][idx] = s;
+    __syncthreads();
+    if (t < kR * 8) tot[t >> 3][t & 7] = half[0][t] + half[1][t];
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(kBT) kb_init(BArgs a) {
+    const int r = threadIdx.x & 15;
+    double v[7] = {0, 0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 b = a.b[e];
+        const double2 c = make_double2(b.x, -b.y);
+        a.x[e] = make_double2(0.0, 0.0);
+        a.r[e] = b;
+        a.p[e] = b;
+        a.rt[e] = c;
+        a.pt[e] = c;
+        v[0] += b.x * c.x;
+        v[1] += b.y * c.y;
+        v[2] += b.y * c.x;
+        v[3] += b.x * c.y;
+        v[4] += b.x * b.x;
+        v[5] += b.y * b.y;
+        v[6] += fabs(b.x) + fabs(b.y);
+    }
+    block_partials<7>(v, a.part + (size_t)blockIdx.x * kR * 8);
+}
+
+__global__ void __launch_bounds__(kBT) kb_init_finish(BArgs a, int nb) {
+    __shared__ double tot[kR][8];
+    reduce_all(a.part, nb, 7, tot);
+    if (threadIdx.x < kR) {
+        const int r = threadIdx.x;
+        BState* s = a.st;
+        s->rho[r][0] = tot[r][0] + tot[r][1];
+        s->rho[r][1] = tot[r][2] - tot[r][3];
+        s->normb[r] = sqrt(tot[r][4] + tot[r][5]);
+        s->sumb[r] = tot[r][6];
+        s->err[r] = INFINITY;
+        const bool zero = tot[r][6] < 1e-15;  // BICGSTAB.c:147-156: zero density
+        s->done[r] = zero ? -1 : 0;
+        s->stop[r] = zero ? 4 : 0;
+        if (!zero && tot[r][6] < 1e-13) {     // BICGSTAB.c:138-145
+            a.r[r] = make_double2(0.000000001, 0.000000001);
+            a.rt[r] = make_double2(0.000000001, 0.000000001);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int r = 0; r < kR; ++r) all &= a.st->done[r] != 0;
+        a.st->all_done = all;
+    }
+}
+
+// partials of <r, r~> and <v, p~> per column
+__global__ void __launch_bounds__(kBT) kb_dot(BArgs a) {
+    if (a.st->all_done) return;
+    const int r = threadIdx.x & 15;
+    double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 rr = a.r[e], rt = a.rt[e], y = a.v[e], pt = a.pt[e];
+        v[0] += rr.x * rt.x;
+        v[1] += rr.y * rt.y;
+        v[2] += rr.y * rt.x;
+        v[3] += rr.x * rt.y;
+        v[4] += y.x * pt.x;
+        v[5] += y.y * pt.y;
+        v[6] += y.y * pt.x;
+        v[7] += y.x * pt.y;
+    }
+    block_partials<8>(v, a.part + (size_t)blockIdx.x * kR * 8);
+}
+
+// alpha = <r, r~> / <v, p~> per running column (BICGSTAB.c:217-229)
+__global__ void __launch_bounds__(kBT) kb_alpha(BArgs a, int nb) {
+    if (a.st->all_done) return;
+    __shared__ double tot[kR][8];
+    reduce_all(a.part, nb, 8, tot);
+    if (threadIdx.x < kR) {
+        const int r = threadIdx.x;
+        const double ca = tot[r][0] + tot[r][1], cb = tot[r][2] - tot[r][3];
+        const double cc = tot[r][4] + tot[r][5], cd = tot[r][6] - tot[r][7];
+        const double den = cc * cc + cd * cd;
+        a.st->alpha[r][0] = (ca * cc + cb * cd) / den;
+        a.st->alpha[r][1] = (cb * cc - ca * cd) / den;
+    }
+}
+
+// x += alpha p, r -= alpha v, r~ -= conj(alpha) w (BICGSTAB.c:231-242); partials
+// of the new <r, r~> and ||r||^2.
+__global__ void __launch_bounds__(kBT) kb_update_xr(BArgs a) {
+    if (a.st->all_done) return;
+    const int r = threadIdx.x & 15;
+    const bool run = a.st->done[r] == 0;
+    const double al = a.st->alpha[r][0], ali = a.st->alpha[r][1];
+    double v[6] = {0, 0, 0, 0, 0, 0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        double2 x = a.x[e], p = a.p[e], rr = a.r[e], vv = a.v[e], rt = a.rt[e], ww = a.w[e];
+        if (run) {
+            x.x = x.x + al * p.x - ali * p.y;
+            x.y = x.y + ali * p.x + al * p.y;
+            rr.x = rr.x - al * vv.x + ali * vv.y;
+            rr.y = rr.y - ali * vv.x - al * vv.y;
+            rt.x = rt.x - al * ww.x - ali * ww.y;
+            rt.y = rt.y + ali * ww.x - al * ww.y;
+            a.x[e] = x;
+            a.r[e] = rr;
+            a.rt[e] = rt;
+        }
+        v[0] += rr.x * rt.x;
+        v[1] += rr.y * rt.y;
+        v[2] += rr.y * rt.x;
+        v[3] += rr.x * rt.y;
+        v[4] += rr.x * rr.x;
+        v[5] += rr.y * rr.y;
+    }
+    block_partials<6>(v, a.part + (size_t)blockIdx.x * kR * 8);
+}
+
+// beta = <r, r~>_new / <r, r~>_old (BICGSTAB.c:246-259), errors and stop rules
+// (289-306) per running column; beta goes to alpha's slot for kb_update_p.
+__global__ void __launch_bounds__(kBT) kb_beta(BArgs a, int nb, int it, double tol) {
+    if (a.st->all_done) return;
+    __shared__ double tot[kR][8];
+    reduce_all(a.part, nb, 6, tot);
+    if (threadIdx.x < kR) {
+        const int r = threadIdx.x;
+        BState* s = a.st;
+        if (s->done[r] == 0) {
+            const double ca = tot[r][0] + tot[r][1], cb = tot[r][2] - tot[r][3];
+            const double o0 = s->rho[r][0], o1 = s->rho[r][1];
+            const double den = o0 * o0 + o1 * o1;
+            s->alpha[r][0] = (ca * o0 + cb * o1) / den;
+            s->alpha[r][1] = (cb * o0 - ca * o1) / den;
+            s->rho[r][0] = ca;
+            s->rho[r][1] = cb;
+            const double err = sqrt(tot[r][4] + tot[r][5]) / s->normb[r];
+            int stop = 0;
+            if (err < tol) stop = 1;
+            else if (err != err) stop = 2;
+            else if (fabs(s->err[r] - err) < 1e-3 * tol) stop = 3;
+            s->err[r] = err;
+            if (a.hist) a.hist[(size_t)it * kR + r] = err;
+            if (stop) {
+                s->done[r] = it;
+                s->stop[r] = stop;
+            }
+        } else {
+            s->alpha[r][0] = s->alpha[r][1] = 0.0;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int r = 0; r < kR; ++r) all &= a.st->done[r] != 0;
+        a.st->all_done = all;
+    }
+}
+
+// p = r + beta p, p~ = r~ + conj(beta) p~ (BICGSTAB.c:271-278)
+__global__ void __launch_bounds__(kBT) kb_update_p(BArgs a) {
+    const int r = threadIdx.x & 15;
+    if (a.st->done[r] != 0) return;  // per-thread: the column is finished (no barriers below)
+    const double be = a.st->alpha[r][0], bei = a.st->alpha[r][1];
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 rr = a.r[e], p = a.p[e], rt = a.rt[e], pt = a.pt[e];
+        a.p[e] = make_double2(rr.x + be * p.x - bei * p.y, rr.y + bei * p.x + be * p.y);
+        a.pt[e] = make_double2(rt.x + be * pt.x + bei * pt.y, rt.y - bei * pt.x + be * pt.y);
+    }
+}
+
+// column-major (count per column) <-> RHS-interleaved, zero-filling unused columns
+__global__ void kb_pack(const double2* __restrict__ src, int count, int c0, int ncol, double2* __restrict__ dst,
+                        size_t rows) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= rows * kR) return;
+    const size_t j = e / kR;
+    const int r = (int)(e % kR);
+    dst[e] = (j < (size_t)count && r < ncol) ? src[(size_t)(c0 + r) * count + j] : make_double2(0.0, 0.0);
+}
+
+__global__ void kb_unpack(const double2* __restrict__ src, int count, int c0, int ncol, double2* __restrict__ dst) {
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)count * kR) return;
+    const size_t j = e / kR;
+    const int r = (int)(e % kR);
+    if (r < ncol) dst[(size_t)(c0 + r) * count + j] = src[e];
+}
+
+}  // namespace
+
+// Y = op(A) P for 16 RHS-interleaved vectors (rows = ld, padding rows zero).
+int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y, const void* state) {
+    const int count = ctx->count;
+    const unsigned nb = (unsigned)((count + 63) / 64);  // 64 rows / columns per workgroup
+    const BState* st = (const BState*)state;
+    if (op == OP_DENSE) {
+        TimedScope ts(ctx, adjoint ? "batch_cols" : "batch_rows");
+        if (ctx->mat_precision == IBIM_C128) {
+            if (adjoint)
+                hipLaunchKernelGGL(k_cols_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
+                                   ctx->ld, count, P, Y, st);
+            else
+                hipLaunchKernelGGL(k_rows_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
+                                   ctx->ld, count, P, Y, st);
+        } else {
+            if (adjoint)
+                hipLaunchKernelGGL(k_cols_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
+                                   ctx->ld, count, P, Y, st);
+            else
+                hipLaunchKernelGGL(k_rows_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
+                                   ctx->ld, count, P, Y, st);
+        }
+    } else {
+        if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
+            set_error("batched matrix-free products need the traditional tube");
+            return IBIM_ERR_ARG;
+        }
+        TimedScope ts(ctx, adjoint ? "batch_cols_free" : "batch_rows_free");
+        const double* tube = ctx->tube.as<double>();
+        if (op == OP_FREE32) {
+            if (adjoint)
+                hipLaunchKernelGGL(k_cols_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
+                                   P, Y, st);
+            else
+                hipLaunchKernelGGL(k_rows_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
+                                   P, Y, st);
+        } else {
+            if (adjoint)
+                hipLaunchKernelGGL(k_cols_free<OP_FREE64>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
+                                   P, Y, st);
+            else
+                hipLaunchKernelGGL(k_rows_free<OP_FREE64>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
+                                   P, Y, st);
+        }
+    }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+// Host entry of the batched products: nvec <= 16 column-major host vectors.
+int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double* pre, const double* pim, double* yre,
+                     double* yim) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld, vec = ld * kR;
+    if (ctx->xfer.ensure(sizeof(double2) * vec * 2)) return IBIM_ERR_NOMEM;
+    double2* P = ctx->xfer.as<double2>();
+    double2* Y = P + vec;
+    std::vector<double2> h(vec, make_double2(0.0, 0.0));
+    for (int r = 0; r < nvec; ++r)
+        for (int j = 0; j < count; ++j)
+            h[(size_t)j * kR + r] = make_double2(pre[(size_t)r * count + j], pim ? pim[(size_t)r * count + j] : 0.0);
+    IBIM_HIP(hipMemcpyAsync(P, h.data(), sizeof(double2) * vec, hipMemcpyHostToDevice, ctx->stream));
+    IBIM_HIP(hipMemsetAsync(Y, 0, sizeof(double2) * vec, ctx->stream));
+    int rc = batch_apply(ctx, op, adjoint, P, Y, nullptr);
+    if (rc) return rc;
+    IBIM_HIP(hipMemcpyAsync(h.data(), Y, sizeof(double2) * vec, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < nvec; ++r)
+        for (int j = 0; j < count; ++j) {
+            yre[(size_t)r * count + j] = h[(size_t)j * kR + r].x;
+            yim[(size_t)r * count + j] = h[(size_t)j * kR + r].y;
+        }
+    return IBIM_OK;
+}
+
+// BiCG on columns [c0, c0 + ncol) of ctx->rhs (ncol <= 16) with operator `op`;
+// densities into the same columns of ctx->density. Per-column iteration counts,
+// residuals and stop reasons are accumulated into the out arrays.
+static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
+                      double* errs, int* stops, double* hist) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const size_t vec = ld * kR;  // double2 per batched vector
+    const unsigned nvb = (unsigned)std::min<size_t>(((size_t)count + 15) / 16, kMaxVB);
+    const size_t want = sizeof(double2) * vec * 8 + sizeof(double) * (size_t)nvb * kR * 8 + sizeof(BState) + 256 +
+                        sizeof(double) * (size_t)(maxit + 1) * kR;
+    if (ctx->batch.ensure(want)) return IBIM_ERR_NOMEM;
+    char* base = ctx->batch.as<char>();
+    BArgs a{};
+    double2* v0 = (double2*)base;
+    a.x = v0;
+    a.r = v0 + vec;
+    a.rt = v0 + 2 * vec;
+    a.p = v0 + 3 * vec;
+    a.pt = v0 + 4 * vec;
+    a.v = v0 + 5 * vec;
+    a.w = v0 + 6 * vec;
+    double2* b = v0 + 7 * vec;
+    a.b = b;
+    a.part = (double*)(v0 + 8 * vec);
+    a.st = (BState*)(((uintptr_t)(a.part + (size_t)nvb * kR * 8) + 255) & ~(uintptr_t)255);
+    a.hist = (double*)(a.st + 1);
+    a.count = count;
+    // padding rows (count..ld) of every vector read as zero in the passes
+    IBIM_HIP(hipMemsetAsync(v0, 0, sizeof(double2) * vec * 7, ctx->stream));
+    const unsigned pb = (unsigned)((vec + 255) / 256);
+    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol, b, ld);
+    hipLaunchKernelGGL(kb_init, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+    hipLaunchKernelGGL(kb_init_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
+    IBIM_HIP(hipGetLastError());
+    if (chunk <= 0) chunk = count > 20000 ? 2 : 16;
+    BState hs{};
+    int it = 1;
+    IBIM_HIP(hipMemcpyAsync(&hs, a.st, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    while (!hs.all_done && it <= maxit) {
+        const int last = std::min(it + chunk - 1, maxit);
+        for (; it <= last; ++it) {
+            int rc = batch_apply(ctx, op, false, a.p, a.v, a.st);
+            if (!rc) rc = batch_apply(ctx, op, true, a.pt, a.w, a.st);
+            if (rc) return rc;
+            hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+            hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
+            hipLaunchKernelGGL(kb_update_xr, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+            hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, it, tol);
+            hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+        }
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipMemcpyAsync(&hs, a.st, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    hipLaunchKernelGGL(kb_unpack, dim3((unsigned)(((size_t)count * kR + 255) / 256)), dim3(256), 0, ctx->stream, a.x,
+                       count, c0, ncol, ctx->density.as<double2>());
+    IBIM_HIP(hipGetLastError());
+    std::vector<double> h;
+    if (hist) {
+        h.resize((size_t)(maxit + 1) * kR);
+        IBIM_HIP(hipMemcpyAsync(h.data(), a.hist, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    }
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < ncol; ++r) {
+        const int d = hs.done[r];
+        iters[c0 + r] = d > 0 ? d : (d < 0 ? 0 : maxit);
+        errs[c0 + r] = d < 0 ? 0.0 : hs.err[r];
+        stops[c0 + r] = d == 0 ? 1 : (hs.stop[r] == 1 ? 0 : hs.stop[r]);
+        if (hist && c0 + r == 0)
+            for (int i = 1; i <= iters[0]; ++i) hist[i - 1] = h[(size_t)i * kR];
+    }
+    return IBIM_OK;
+}
+
+// All right-hand sides, 16 per batch. The operator is the stored one when
+// assembled, else the FP64 entries recomputed per pass (as run_bicg).
+int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    const int op = ctx->mat_precision >= 0 ? OP_DENSE : OP_FREE64;
+    if (op != OP_DENSE && ctx->kind != IBIM_TUBE_TRADITIONAL) return run_bicg(ctx, opts, info, hist);
+    const int n = ctx->nrhs;
+    std::vector<int> iters(n), stops(n);
+    std::vector<double> errs(n);
+    for (int c0 = 0; c0 < n; c0 += kR) {
+        int rc = bicg_batch(ctx, op, c0, std::min(kR, n - c0), opts->maxit, opts->tol, opts->chunk, iters.data(),
+                            errs.data(), stops.data(), c0 == 0 ? hist : nullptr);
+        if (rc) return rc;
+    }
+    int total = 0, status = 0;
+    double worst = 0.0;
+    for (int c = 0; c < n; ++c) {
+        total += iters[c];
+        worst = std::max(worst, errs[c]);
+        if (stops[c] && stops[c] != 4) status = stops[c];
+    }
+    info->iterations = total;
+    info->outer = 1;
+    info->rel_residual = worst;
+    info->status = status;
+    ctx->last_iters = total;
+    return IBIM_OK;
+}
+
+}  // namespace ibim

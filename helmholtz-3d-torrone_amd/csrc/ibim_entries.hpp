@@ -104,4 +104,37 @@ __device__ __forceinline__ int combo_entry(const PointData& a, const PointData& 
     return near;
 }
 
+// Traditional entry with FP32 arithmetic around an FP64-reduced phase.
+// kappa = k / (2 pi): the phase in revolutions, reduced modulo 1 in FP64 before
+// the FP32 sin / cos units (which take revolutions).
+__device__ __forceinline__ void trad_entry_f32(const PointData& a, const PointData& b, bool diag, double kappa,
+                                               float kf, double tau2, int dirichlet, int interior, float* re,
+                                               float* im) {
+    const double d1 = a.sx - b.sx, d2 = a.sy - b.sy, d3 = a.sz - b.sz;
+    const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+    float pr, pim;
+    if (rho2 > tau2) {
+        const float r = sqrtf((float)rho2);
+        const float ir = 1.0f / r;
+        const double rd = (double)r;
+        const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);  // one Newton step in FP64
+        const double t = rho * kappa;                                            // phase in revolutions
+        const float f = (float)(t - rint(t));                                    // in [-1/2, 1/2]
+        const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
+        const float den = (float)(1.0 / (4.0 * kPi)) * ir * ir;
+        const float c1 = -(kf * sv + cv * ir) * den;
+        const float c2 = (kf * cv - sv * ir) * den;
+        const float ip = dirichlet ? (float)d1 * (float)b.nx + (float)d2 * (float)b.ny + (float)d3 * (float)b.nz
+                                   : (float)d1 * (float)a.nx + (float)d2 * (float)a.ny + (float)d3 * (float)a.nz;
+        pr = interior ? -c1 * ip : c1 * ip;
+        pim = interior ? -c2 * ip : c2 * ip;
+    } else {
+        pr = (float)(dirichlet ? b.reg : a.reg);
+        pim = 0.0f;
+    }
+    *re = (float)b.w * pr;
+    *im = (float)b.w * pim;
+    if (diag) *re += 0.5f;
+}
+
 }  // namespace ibim

@@ -97,6 +97,7 @@ struct Context {
     DevBuf work, partial, scal, done;
     DevBuf xfer;    // receivers and other small transfers
     DevBuf refine;  // refinement vectors
+    DevBuf batch;   // batched (16 right-hand sides) BiCG workspace
     int last_iters = 0;
 
     // profiling
@@ -137,6 +138,11 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+// Y = op(A) P (adjoint: A^H) for 16 right-hand-side-interleaved vectors of ld rows.
+int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y, const void* state);
+int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double* pre, const double* pim, double* yre,
+                     double* yim);
 int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
 
 // Operator used by a sweep: the stored matrix, or entries recomputed on the fly.

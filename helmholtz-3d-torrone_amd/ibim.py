@@ -62,7 +62,8 @@ def shape(kind, a=1.0, b=0.0, c=0.0, m=0, n=0, variant=0, center=(0.0, 0.0, 0.0)
 
 
 class SolveOpts(C.Structure):
-    _fields_ = [("method", I), ("maxit", I), ("tol", D), ("inner_tol", D), ("max_outer", I), ("chunk", I)]
+    _fields_ = [("method", I), ("maxit", I), ("tol", D), ("inner_tol", D), ("max_outer", I), ("chunk", I),
+                ("batch", I)]
 
 
 class SolveInfo(C.Structure):
@@ -96,6 +97,7 @@ def lib():
             "ibim_assemble": ([vp, I], I), "ibim_get_rows": ([vp, I, I, P, P], I),
             "ibim_set_matrix_host": ([vp, I, P, P, I], I), "ibim_regcount": ([vp], I),
             "ibim_apply": ([vp, I, P, P, P, P, P, P, P, P], I),
+            "ibim_apply_batch": ([vp, I, I, I, P, P, P, P], I),
             "ibim_set_nrhs": ([vp, I], I), "ibim_rhs": ([vp, I, P, D, P, I], I),
             "ibim_set_rhs_host": ([vp, I, P, P], I), "ibim_get_rhs": ([vp, I, P, P], I),
             "ibim_solve": ([vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), P], I),
@@ -248,6 +250,17 @@ class Problem:
         _check(lib().ibim_apply(self.h, op, *[_p(a) for a in arrs], *[_p(a) for a in out]), "ibim_apply")
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
 
+    def apply_batch(self, P, op=0, adjoint=False):
+        """op(A) P for a (count, m <= 16) complex array, on the matrix cores."""
+        P = np.asarray(P, dtype=complex)
+        m = P.shape[1]
+        pre = np.ascontiguousarray(P.real.T)
+        pim = np.ascontiguousarray(P.imag.T)
+        yre, yim = np.zeros((m, self.count)), np.zeros((m, self.count))
+        _check(lib().ibim_apply_batch(self.h, op, 1 if adjoint else 0, m, _p(pre), _p(pim), _p(yre), _p(yim)),
+               "ibim_apply_batch")
+        return (yre + 1j * yim).T
+
     def regcount(self):
         return lib().ibim_regcount(self.h)
 
@@ -271,8 +284,10 @@ class Problem:
         _check(lib().ibim_get_rhs(self.h, col, _p(a), _p(b)), "ibim_get_rhs")
         return a, b
 
-    def solve(self, method=SOLVE_BICG, maxit=1000, tol=1e-7, inner_tol=1e-4, max_outer=20, chunk=0):
-        o = SolveOpts(method, maxit, tol, inner_tol, max_outer, chunk)
+    def solve(self, method=SOLVE_BICG, maxit=1000, tol=1e-7, inner_tol=1e-4, max_outer=20, chunk=0, batch=0):
+        """batch 0: several right-hand sides go 16 per pass over the operator
+        (BiCG); 1: one column at a time."""
+        o = SolveOpts(method, maxit, tol, inner_tol, max_outer, chunk, batch)
         info = SolveInfo()
         hist = np.zeros(max(maxit, max_outer))
         _check(lib().ibim_solve(self.h, C.byref(o), C.byref(info), _p(hist)), "ibim_solve")
@@ -310,7 +325,7 @@ class Problem:
     def gradient(self, mode=EVAL_DOUBLE, col=0, maxit=1000, tol=1e-12, chunk=0):
         """J and dJ/d(level-set values) (shape (n+1)^3, [y][x][z]) and dJ/dk for
         the solved column `col` (adjoint BiCG to `tol`)."""
-        o = SolveOpts(SOLVE_BICG, maxit, tol, 1e-4, 20, chunk)
+        o = SolveOpts(SOLVE_BICG, maxit, tol, 1e-4, 20, chunk, 1)
         J, dk = np.zeros(1), np.zeros(1)
         S = self.n + 1
         g = np.zeros((S, S, S))

@@ -102,6 +102,8 @@ typedef struct ibim_solve_opts {
                        /* later ones use max(inner_tol, 0.3 tol / residual)      */
     int max_outer;     /* refinement: outer FP64 steps                        */
     int chunk;         /* iterations launched between host checks (0 = auto)  */
+    int batch;         /* several right-hand sides: 0 = solve 16 per pass over */
+                       /* the operator (BiCG), 1 = one column at a time        */
 } ibim_solve_opts;
 
 typedef struct ibim_solve_info {
@@ -180,6 +182,12 @@ int ibim_regcount(ibim_ctx *ctx);
 #define IBIM_OP_FREE32 2
 int ibim_apply(ibim_ctx *ctx, int op, const double *pre, const double *pim, const double *qre, const double *qim,
                double *yre, double *yim, double *zre, double *zim);
+/* Y = A P (adjoint = 0) or Y = A^H P (adjoint = 1) for nvec <= 16 vectors at
+ * once (host arrays, column-major, count per column, split re / im), on the
+ * FP64 matrix cores: the product the batched BiCG applies per pass. No
+ * counterpart in the reference (it solves one right-hand side at a time). */
+int ibim_apply_batch(ibim_ctx *ctx, int op, int adjoint, int nvec, const double *pre, const double *pim, double *yre,
+                     double *yim);
 
 /* ------------------------------------------------- right-hand sides/solve --- */
 int ibim_set_nrhs(ibim_ctx *ctx, int nrhs);

@@ -1,0 +1,89 @@
+"""Batched right-hand sides: the 16-wide matrix-core products and the batched
+BiCG against dense numpy products and the one-column-at-a-time solver."""
+import numpy as np
+import pytest
+
+import shapes
+
+pytestmark = pytest.mark.gpu
+
+N = 28  # 3,7xx tube points: not a multiple of 16 or 64, so padding is exercised
+
+
+def problem(gpu, k=4.0, dirichlet=1):
+    pb = gpu.Problem()
+    pb.set_level(shapes.sphere(N))
+    pb.geometry(5)
+    h = 4.0 / N
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=dirichlet))
+    return pb
+
+
+def dense(pb):
+    re, im = pb.rows(0, pb.count)
+    return re + 1j * im
+
+
+def relnorm(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("nvec", [5, 16])
+def test_batched_products_match_dense(gpu, nvec):
+    pb = problem(gpu)
+    rng = np.random.default_rng(nvec)
+    P = rng.standard_normal((pb.count, nvec)) + 1j * rng.standard_normal((pb.count, nvec))
+    pb.assemble(gpu.C128)
+    A = dense(pb)
+    for adjoint in (False, True):
+        ref = (A.conj().T if adjoint else A) @ P
+        assert relnorm(pb.apply_batch(P, gpu.OP_STORED, adjoint), ref) < 1e-13
+        assert relnorm(pb.apply_batch(P, gpu.OP_FREE64, adjoint), ref) < 1e-13
+        assert relnorm(pb.apply_batch(P, gpu.OP_FREE32, adjoint), ref) < 1e-6
+    pb.assemble(gpu.C64)
+    A32 = dense(pb)  # the stored complex64 values
+    assert relnorm(A32, A) < 1e-7
+    for adjoint in (False, True):
+        ref = (A32.conj().T if adjoint else A32) @ P
+        assert relnorm(pb.apply_batch(P, gpu.OP_STORED, adjoint), ref) < 1e-13
+
+
+def fill_rhs(gpu, pb, n):
+    pb.set_nrhs(n)
+    rng = np.random.default_rng(7)
+    for c in range(n):
+        if c == 3:
+            continue  # a zero column: zero density, no iterations
+        if c % 2:
+            d = rng.standard_normal(3)
+            pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=d / np.linalg.norm(d), col=c)
+        else:
+            pb.rhs(gpu.RHS_POINT_SOURCE, x0=0.3 * rng.standard_normal(3), col=c)
+
+
+@pytest.mark.parametrize("store", ["c128", "c64", "free"])
+def test_batched_bicg_matches_column_solves(gpu, store):
+    pb = problem(gpu)
+    if store == "c128":
+        pb.assemble(gpu.C128)
+    elif store == "c64":
+        pb.assemble(gpu.C64)
+    n = 19  # one full batch of 16 and one of 3
+    fill_rhs(gpu, pb, n)
+    info_b, hist_b = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+    xb = [pb.density(c) for c in range(n)]
+    info_s, hist_s = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=1)
+    xs = [pb.density(c) for c in range(n)]
+    assert info_b.status == 0 and info_s.status == 0
+    for c in range(n):
+        b = xb[c][0] + 1j * xb[c][1]
+        s = xs[c][0] + 1j * xs[c][1]
+        if c == 3:
+            assert not b.any() and not s.any()
+        else:
+            assert relnorm(b, s) < 1e-10, (c, relnorm(b, s))
+    # the same recurrences: iteration counts agree up to rounding-order effects
+    assert abs(info_b.iterations - info_s.iterations) <= 0.05 * info_s.iterations + n
+    # column 0's residual history follows the single-column solver's
+    m = min(len(hist_b), len(hist_s), 10)
+    np.testing.assert_allclose(hist_b[:m], hist_s[:m], rtol=1e-6)
