@@ -505,7 +505,7 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
     // adjoint solve A^H lambda = E^H e
     const int op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
     ibim_solve_info info{};
-    rc = bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true);
+    rc = bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true, false);
     if (rc) return rc;
 
     hipLaunchKernelGGL(k_adj_operator<1>, dim3(nb), dim3(kT), 0, ctx->stream, ctx->tube.as<double>(), count,

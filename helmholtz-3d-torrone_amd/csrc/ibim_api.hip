@@ -479,7 +479,10 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     memset(info, 0, sizeof *info);
     if (opts->method == IBIM_SOLVE_BICG)
         return ctx->nrhs > 1 && opts->batch == 0 ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
-    if (opts->method == IBIM_SOLVE_REFINED) return run_refined(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_REFINED)
+        return ctx->nrhs > 1 && opts->batch == 0 && (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C64)
+                   ? run_refined_batched(ctx, opts, info)
+                   : run_refined(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_RELIABLE) return run_reliable(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
 }

@@ -268,6 +268,7 @@ struct BArgs {
     BState* st;
     double* hist;     // [maxit + 1][kR] relative residuals
     int count;
+    int ref_guards;   // 1: the reference's small-data rules; 0: only exact zeros (correction solves)
 };
 
 // Thread t of a block works on right-hand side t % 16 for points
@@ -339,10 +340,12 @@ __global__ void __launch_bounds__(kBT) kb_init_finish(BArgs a, int nb) {
         s->normb[r] = sqrt(tot[r][4] + tot[r][5]);
         s->sumb[r] = tot[r][6];
         s->err[r] = INFINITY;
-        const bool zero = tot[r][6] < 1e-15;  // BICGSTAB.c:147-156: zero density
+        // BICGSTAB.c:147-156: zero density; BICGSTAB.c:138-145: perturbation of
+        // tiny data (skipped for refinement corrections, whose data are small)
+        const bool zero = a.ref_guards ? tot[r][6] < 1e-15 : tot[r][6] == 0.0;
         s->done[r] = zero ? -1 : 0;
         s->stop[r] = zero ? 4 : 0;
-        if (!zero && tot[r][6] < 1e-13) {     // BICGSTAB.c:138-145
+        if (a.ref_guards && !zero && tot[r][6] < 1e-13) {
             a.r[r] = make_double2(0.000000001, 0.000000001);
             a.rt[r] = make_double2(0.000000001, 0.000000001);
         }
@@ -570,47 +573,50 @@ int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double*
     return IBIM_OK;
 }
 
-// BiCG on columns [c0, c0 + ncol) of ctx->rhs (ncol <= 16) with operator `op`;
-// densities into the same columns of ctx->density. Per-column iteration counts,
-// residuals and stop reasons are accumulated into the out arrays.
-static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
-                      double* errs, int* stops, double* hist) {
+// Workspace of one batch: x, r, r~, p, p~, v, w, b and `extra` more
+// interleaved vectors, per-block partials, the state and the residual history.
+static int batch_workspace(Context* ctx, int maxit, int extra, BArgs* a, unsigned* nvb_out, double2** extra_out) {
     const int count = ctx->count;
-    const size_t ld = ctx->ld;
-    const size_t vec = ld * kR;  // double2 per batched vector
+    const size_t vec = ctx->ld * kR;
     const unsigned nvb = (unsigned)std::min<size_t>(((size_t)count + 15) / 16, kMaxVB);
-    const size_t want = sizeof(double2) * vec * 8 + sizeof(double) * (size_t)nvb * kR * 8 + sizeof(BState) + 256 +
+    const int nv = 8 + extra;
+    const size_t want = sizeof(double2) * vec * nv + sizeof(double) * (size_t)nvb * kR * 8 + sizeof(BState) + 256 +
                         sizeof(double) * (size_t)(maxit + 1) * kR;
     if (ctx->batch.ensure(want)) return IBIM_ERR_NOMEM;
-    char* base = ctx->batch.as<char>();
-    BArgs a{};
-    double2* v0 = (double2*)base;
-    a.x = v0;
-    a.r = v0 + vec;
-    a.rt = v0 + 2 * vec;
-    a.p = v0 + 3 * vec;
-    a.pt = v0 + 4 * vec;
-    a.v = v0 + 5 * vec;
-    a.w = v0 + 6 * vec;
-    double2* b = v0 + 7 * vec;
-    a.b = b;
-    a.part = (double*)(v0 + 8 * vec);
-    a.st = (BState*)(((uintptr_t)(a.part + (size_t)nvb * kR * 8) + 255) & ~(uintptr_t)255);
-    a.hist = (double*)(a.st + 1);
-    a.count = count;
+    double2* v0 = ctx->batch.as<double2>();
+    *a = BArgs{};
+    a->x = v0;
+    a->r = v0 + vec;
+    a->rt = v0 + 2 * vec;
+    a->p = v0 + 3 * vec;
+    a->pt = v0 + 4 * vec;
+    a->v = v0 + 5 * vec;
+    a->w = v0 + 6 * vec;
+    a->b = v0 + 7 * vec;
+    if (extra_out) *extra_out = v0 + 8 * vec;
+    a->part = (double*)(v0 + (size_t)nv * vec);
+    a->st = (BState*)(((uintptr_t)(a->part + (size_t)nvb * kR * 8) + 255) & ~(uintptr_t)255);
+    a->hist = (double*)(a->st + 1);
+    a->count = count;
+    a->ref_guards = 1;
+    *nvb_out = nvb;
     // padding rows (count..ld) of every vector read as zero in the passes
-    IBIM_HIP(hipMemsetAsync(v0, 0, sizeof(double2) * vec * 7, ctx->stream));
-    const unsigned pb = (unsigned)((vec + 255) / 256);
-    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol, b, ld);
+    IBIM_HIP(hipMemsetAsync(v0, 0, sizeof(double2) * vec * nv, ctx->stream));
+    return IBIM_OK;
+}
+
+// BiCG for the 16 interleaved right-hand sides in a.b (padding rows zero);
+// the solutions are left in a.x and the final state in *hs.
+static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, int maxit, double tol, int chunk,
+                           BState* hs) {
     hipLaunchKernelGGL(kb_init, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
     hipLaunchKernelGGL(kb_init_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
     IBIM_HIP(hipGetLastError());
-    if (chunk <= 0) chunk = count > 20000 ? 2 : 16;
-    BState hs{};
+    if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
     int it = 1;
-    IBIM_HIP(hipMemcpyAsync(&hs, a.st, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    while (!hs.all_done && it <= maxit) {
+    while (!hs->all_done && it <= maxit) {
         const int last = std::min(it + chunk - 1, maxit);
         for (; it <= last; ++it) {
             int rc = batch_apply(ctx, op, false, a.p, a.v, a.st);
@@ -623,9 +629,28 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
             hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
         }
         IBIM_HIP(hipGetLastError());
-        IBIM_HIP(hipMemcpyAsync(&hs, a.st, sizeof hs, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
         IBIM_HIP(hipStreamSynchronize(ctx->stream));
     }
+    return IBIM_OK;
+}
+
+// BiCG on columns [c0, c0 + ncol) of ctx->rhs (ncol <= 16) with operator `op`;
+// densities into the same columns of ctx->density. Per-column iteration counts,
+// residuals and stop reasons go to the out arrays.
+static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
+                      double* errs, int* stops, double* hist) {
+    const int count = ctx->count;
+    BArgs a;
+    unsigned nvb;
+    int rc = batch_workspace(ctx, maxit, 0, &a, &nvb, nullptr);
+    if (rc) return rc;
+    const unsigned pb = (unsigned)((ctx->ld * kR + 255) / 256);
+    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol,
+                       (double2*)a.b, ctx->ld);
+    BState hs{};
+    rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs);
+    if (rc) return rc;
     hipLaunchKernelGGL(kb_unpack, dim3((unsigned)(((size_t)count * kR + 255) / 256)), dim3(256), 0, ctx->stream, a.x,
                        count, c0, ncol, ctx->density.as<double2>());
     IBIM_HIP(hipGetLastError());
@@ -643,6 +668,160 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
         if (hist && c0 + r == 0)
             for (int i = 1; i <= iters[0]; ++i) hist[i - 1] = h[(size_t)i * kR];
     }
+    return IBIM_OK;
+}
+
+// X += D; and R = B - Y with per-block partials of |R|^2 (2 per column), the
+// columns flagged in `conv` getting R = 0 (their correction is then zero).
+__global__ void __launch_bounds__(kBT) kb_accumulate(double2* __restrict__ X, const double2* __restrict__ D, int count) {
+    const int r = threadIdx.x & 15;
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 x = X[e], d = D[e];
+        X[e] = make_double2(x.x + d.x, x.y + d.y);
+    }
+}
+
+struct ConvMask {
+    int c[kR];
+};
+
+__global__ void __launch_bounds__(kBT) kb_residual(const double2* __restrict__ B, const double2* __restrict__ Y,
+                                                   double2* __restrict__ R, int count, ConvMask conv,
+                                                   double* __restrict__ part) {
+    const int r = threadIdx.x & 15;
+    double v[2] = {0.0, 0.0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 b = B[e], y = Y[e];
+        const double2 rr = make_double2(b.x - y.x, b.y - y.y);
+        R[e] = conv.c[r] ? make_double2(0.0, 0.0) : rr;
+        v[0] += rr.x * rr.x;
+        v[1] += rr.y * rr.y;
+    }
+    block_partials<2>(v, part + (size_t)blockIdx.x * kR * 8);
+}
+
+// Per-column sums of the first nv partial values (host, fixed order).
+static int column_sums(Context* ctx, const double* part, unsigned nvb, int nv, std::vector<double>& out) {
+    std::vector<double> h((size_t)nvb * kR * 8);
+    IBIM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    out.assign((size_t)kR * nv, 0.0);
+    for (unsigned b = 0; b < nvb; ++b)
+        for (int r = 0; r < kR; ++r)
+            for (int q = 0; q < nv; ++q) out[(size_t)r * nv + q] += h[((size_t)b * kR + r) * 8 + q];
+    return IBIM_OK;
+}
+
+// FP64 iterative refinement around batched BiCG on the FP32 operator, 16
+// columns at a time: x = 0, r = b; repeat { A32 d = r by batched BiCG; x += d;
+// r = b - A64 x by a batched FP64 pass } until every column's ||r|| / ||b|| < tol.
+static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol, const ibim_solve_opts* opts,
+                         int* iters, double* errs, int* stops, int* outers) {
+    const int count = ctx->count;
+    BArgs a;
+    unsigned nvb;
+    double2* ex;
+    int rc = batch_workspace(ctx, opts->maxit, 2, &a, &nvb, &ex);
+    if (rc) return rc;
+    a.ref_guards = 0;
+    const size_t vec = ctx->ld * kR;
+    double2* X = ex;        // accumulated solution
+    double2* B = ex + vec;  // the right-hand sides
+    double2* Rb = (double2*)a.b;
+    const unsigned pb = (unsigned)((vec + 255) / 256);
+    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol, B,
+                       ctx->ld);
+    IBIM_HIP(hipMemcpyAsync(Rb, B, sizeof(double2) * vec, hipMemcpyDeviceToDevice, ctx->stream));
+    // ||b|| per column
+    ConvMask none{};
+    hipLaunchKernelGGL(kb_residual, dim3(nvb), dim3(kBT), 0, ctx->stream, B, X, a.v, count, none, a.part);
+    IBIM_HIP(hipGetLastError());
+    std::vector<double> sums;
+    rc = column_sums(ctx, a.part, nvb, 2, sums);
+    if (rc) return rc;
+    double normb[kR], res[kR];
+    ConvMask conv{};
+    for (int r = 0; r < kR; ++r) {
+        normb[r] = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]);
+        res[r] = normb[r] > 0.0 ? 1.0 : 0.0;
+        conv.c[r] = normb[r] == 0.0 || r >= ncol;
+    }
+    const double inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-4;
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    int outer = 0, total = 0;
+    for (outer = 1; outer <= max_outer; ++outer) {
+        double worst = 0.0;
+        for (int r = 0; r < ncol; ++r)
+            if (!conv.c[r]) worst = std::max(worst, res[r]);
+        if (worst == 0.0) break;
+        const double itol = outer == 1 ? inner_tol : std::max(inner_tol, std::min(0.5, 0.3 * opts->tol / worst));
+        BState hs{};
+        rc = bicg_batch_core(ctx, inner, a, nvb, opts->maxit, itol, opts->chunk, &hs);
+        if (rc) return rc;
+        int most = 0;
+        for (int r = 0; r < kR; ++r) most = std::max(most, hs.done[r] > 0 ? hs.done[r] : (hs.done[r] < 0 ? 0 : opts->maxit));
+        total += most;
+        hipLaunchKernelGGL(kb_accumulate, dim3(nvb), dim3(kBT), 0, ctx->stream, X, a.x, count);
+        rc = batch_apply(ctx, outer_op, false, X, a.v, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(kb_residual, dim3(nvb), dim3(kBT), 0, ctx->stream, B, a.v, Rb, count, conv, a.part);
+        IBIM_HIP(hipGetLastError());
+        rc = column_sums(ctx, a.part, nvb, 2, sums);
+        if (rc) return rc;
+        bool all = true;
+        for (int r = 0; r < ncol; ++r) {
+            if (conv.c[r]) continue;
+            res[r] = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / normb[r];
+            if (res[r] < opts->tol) conv.c[r] = 1;
+            all = all && conv.c[r];
+        }
+        // columns that converged now get a zero right-hand side next round
+        hipLaunchKernelGGL(kb_residual, dim3(nvb), dim3(kBT), 0, ctx->stream, B, a.v, Rb, count, conv, a.part);
+        IBIM_HIP(hipGetLastError());
+        if (all) break;
+    }
+    if (outer > max_outer) outer = max_outer;
+    hipLaunchKernelGGL(kb_unpack, dim3((unsigned)(((size_t)count * kR + 255) / 256)), dim3(256), 0, ctx->stream, X,
+                       count, c0, ncol, ctx->density.as<double2>());
+    IBIM_HIP(hipGetLastError());
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int r = 0; r < ncol; ++r) {
+        iters[c0 + r] = total;
+        errs[c0 + r] = res[r];
+        stops[c0 + r] = normb[r] == 0.0 ? 4 : (res[r] < opts->tol ? 0 : 1);
+        outers[c0 + r] = outer;
+    }
+    return IBIM_OK;
+}
+
+// Refinement for several right-hand sides, 16 per batch (see refined_batch).
+int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info) {
+    if (ctx->kind != IBIM_TUBE_TRADITIONAL && ctx->mat_precision != IBIM_C64) return IBIM_ERR_ARG;
+    const int inner = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
+    const int outer_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    const int n = ctx->nrhs;
+    std::vector<int> iters(n), stops(n), outers(n);
+    std::vector<double> errs(n);
+    for (int c0 = 0; c0 < n; c0 += kR) {
+        int rc = refined_batch(ctx, inner, outer_op, c0, std::min(kR, n - c0), opts, iters.data(), errs.data(),
+                               stops.data(), outers.data());
+        if (rc) return rc;
+    }
+    int total = 0, status = 0, outer = 0;
+    double worst = 0.0;
+    for (int c0 = 0; c0 < n; c0 += kR) total += iters[c0];
+    for (int c = 0; c < n; ++c) {
+        worst = std::max(worst, errs[c]);
+        outer = std::max(outer, outers[c]);
+        if (stops[c] && stops[c] != 4) status = stops[c];
+    }
+    info->iterations = total;
+    info->outer = outer;
+    info->rel_residual = worst;
+    info->status = status;
+    ctx->last_iters = total;
     return IBIM_OK;
 }
 

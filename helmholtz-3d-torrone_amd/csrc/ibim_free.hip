@@ -415,7 +415,7 @@ int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
             // later corrections only need to bring the residual from `res` to tol
             const double itol = outer == 1 ? inner_tol : fmax(inner_tol, fmin(0.5, 0.3 * opts->tol / res));
             ibim_solve_info in{};
-            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, itol, opts->chunk, &in, nullptr, false);
+            int rc = bicg_solve_op(ctx, inner, r, d, opts->maxit, itol, opts->chunk, &in, nullptr, false, false);
             if (rc) return rc;
             total_its += in.iterations;
             hipLaunchKernelGGL(k_add, dim3(nb), dim3(256), 0, ctx->stream, x, d, count);

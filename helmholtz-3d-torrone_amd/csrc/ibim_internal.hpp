@@ -139,6 +139,7 @@ int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, d
 int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info);
 // Y = op(A) P (adjoint: A^H) for 16 right-hand-side-interleaved vectors of ld rows.
 int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y, const void* state);
 int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double* pre, const double* pim, double* yre,
@@ -152,8 +153,9 @@ constexpr int kSweepCols = 1024;
 constexpr int kSweepRows = 512;
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done);
 int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
+// ref_guards = false for correction / adjoint solves (only exact zeros are zero data).
 int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
-                  ibim_solve_info* info, double* hist, bool adjoint = false);
+                  ibim_solve_info* info, double* hist, bool adjoint = false, bool ref_guards = true);
 
 }  // namespace ibim
 

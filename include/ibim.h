@@ -102,8 +102,8 @@ typedef struct ibim_solve_opts {
                        /* later ones use max(inner_tol, 0.3 tol / residual)      */
     int max_outer;     /* refinement: outer FP64 steps                        */
     int chunk;         /* iterations launched between host checks (0 = auto)  */
-    int batch;         /* several right-hand sides: 0 = solve 16 per pass over */
-                       /* the operator (BiCG), 1 = one column at a time        */
+    int batch;         /* several right-hand sides: 0 = 16 per pass over the   */
+                       /* operator (BiCG, refinement), 1 = one at a time       */
 } ibim_solve_opts;
 
 typedef struct ibim_solve_info {

@@ -87,3 +87,25 @@ def test_batched_bicg_matches_column_solves(gpu, store):
     # column 0's residual history follows the single-column solver's
     m = min(len(hist_b), len(hist_s), 10)
     np.testing.assert_allclose(hist_b[:m], hist_s[:m], rtol=1e-6)
+
+
+@pytest.mark.parametrize("store", ["c64", "free32"])
+def test_batched_refinement_reaches_the_fp64_solution(gpu, store):
+    pb = problem(gpu)
+    n = 19
+    fill_rhs(gpu, pb, n)
+    pb.assemble(gpu.C128)
+    info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-13, batch=0)
+    ref = [pb.density(c) for c in range(n)]
+    if store == "c64":
+        pb.assemble(gpu.C64)  # inner: stored complex64; outer: FP64 matrix-free
+    info, _ = pb.solve(gpu.SOLVE_REFINED, maxit=500, tol=1e-10, inner_tol=1e-6, max_outer=10, batch=0)
+    assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 2
+    for c in range(n):
+        x = pb.density(c)
+        x = x[0] + 1j * x[1]
+        r = ref[c][0] + 1j * ref[c][1]
+        if c == 3:
+            assert not x.any()
+        else:
+            assert relnorm(x, r) < 1e-8, (c, relnorm(x, r))
