@@ -42,13 +42,16 @@ C_EVAL = 26  # flops per entry evaluation (fixed convention)
 WORKLOADS = {
     # Torus (R=1, r=0.4) on 128^3 over [-2,2]^3, eps = tau = 2h, k = 10, interior
     # point source at (1, 0, 0.05): exterior Dirichlet problem, double layer.
-    "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05)),
-    # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3).
-    "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3)),
+    "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05), tol=1e-8),
+    # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10.
+    "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-10),
     # small case for quick checks
-    "sphere32": dict(shape="sphere", n=32, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3)),
+    "sphere32": dict(shape="sphere", n=32, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-8),
 }
-TOL = 1e-8          # relative residual target of both timed paths
+# workloads with their own drivers (below): many right-hand sides and surfaces,
+# the misfit gradient, and the 256^3 matrix-free case
+EXTRA = ("batch", "adjoint96", "star256")
+TOL = 1e-8          # default relative residual target of both timed paths
 REF_TOL = 1e-11     # the FP64 solve the densities are compared against
 REPLACE = 1e-2      # residual drop between FP64 residual replacements
 NRECV = 500
@@ -83,9 +86,9 @@ def step(pb, w, recv, mixed):
         pb.assemble(ibim.C64)
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if mixed:
-        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=TOL, inner_tol=REPLACE, chunk=1)
+        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"], inner_tol=REPLACE, chunk=1)
     else:
-        info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=TOL, chunk=1)
+        info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
     u = pb.evaluate(recv, ibim.EVAL_DOUBLE)
     x = pb.density()
     return info, x, u
@@ -145,18 +148,243 @@ def cpu_baseline(w, count, iters, budget_s=20.0):
     }
 
 
+def fibonacci_directions(m):
+    """Unit vectors on a Fibonacci sphere: z = 1 - (2i+1)/m, phi = i pi (3 - sqrt 5)."""
+    i = np.arange(m)
+    z = 1 - (2 * i + 1) / m
+    ph = i * np.pi * (3 - np.sqrt(5))
+    s = np.sqrt(1 - z * z)
+    return np.stack([s * np.cos(ph), s * np.sin(ph), z], -1)
+
+
+def emit(line):
+    print(json.dumps(line), flush=True)
+
+
+def progress(pb):
+    pb.set_progress(lambda what, it, res: print("[bench]   %s iteration %d, largest residual %.2e" % (what, it, res),
+                                                file=sys.stderr, flush=True))
+
+
+def run_batch(args):
+    """Many right-hand sides and surfaces: the torus (128^3, k = 10) with 64 plane
+    waves on a Fibonacci sphere, plus 8 spheres / ellipsoids with seed-random
+    radii in [0.6, 1.2] on 48^3-96^3 grids with 16 plane waves each; sound-soft
+    data, tol 1e-7. FP64 path: batched BiCG on the stored complex128 operator.
+    Mixed path: batched FP64 refinement around batched BiCG on the stored
+    complex64 operator (FP64 matrix-free outer residuals). 16 right-hand sides
+    per pass over the operator on the FP64 matrix cores; surfaces one after the
+    other."""
+    tol, k = 1e-7, 10.0
+    rng = np.random.default_rng(0)
+    surfaces = [("torus128", ibim.shape(ibim.SHAPE_TORUS, a=1.0, b=0.4), 128, 64)]
+    grids = [48, 56, 64, 72, 80, 88, 96, 96]
+    for i in range(8):
+        r = rng.uniform(0.6, 1.2, 3)
+        if i % 2 == 0:
+            shp = ibim.shape(ibim.SHAPE_SPHERE, a=float(r[0]))
+            name = "sphere R=%.2f" % r[0]
+        else:
+            shp = ibim.shape(ibim.SHAPE_ELLIPSOID, a=float(r[0]), b=float(r[1]), c=float(r[2]))
+            name = "ellipsoid %.2f/%.2f/%.2f" % tuple(r)
+        surfaces.append((name, shp, grids[i], 16))
+    dirs = fibonacci_directions(64)
+    totals = {"fp64": 0.0, "mixed": 0.0}
+    rows, n_rhs, worst_err = [], 0, 0.0
+    for name, shp, n, m in surfaces:
+        h = 4.0 / n
+        res = {}
+        for path in ("fp64", "mixed"):
+            pb = ibim.Problem()
+            progress(pb)
+            pb.set_level_shape(shp, n)
+            pb.sync()
+            t0 = time.perf_counter()
+            pb.geometry(5)
+            count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1))
+            pb.assemble(ibim.C128 if path == "fp64" else ibim.C64)
+            pb.set_nrhs(m)
+            for c in range(m):
+                pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+            if path == "fp64":
+                info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=tol, batch=0)
+            else:
+                info, _ = pb.solve(ibim.SOLVE_REFINED, maxit=3000, tol=tol, inner_tol=1e-4, max_outer=10, batch=0)
+            x = []
+            for c in range(m):
+                d = pb.density(c)
+                x.append(d[0] + 1j * d[1])
+            x = np.stack(x)
+            pb.sync()
+            res[path] = (time.perf_counter() - t0, info, x, count)
+            pb.close()
+        err = float(max(np.linalg.norm(res["mixed"][2][c] - res["fp64"][2][c]) / np.linalg.norm(res["fp64"][2][c])
+                        for c in range(m)))
+        worst_err = max(worst_err, err)
+        for path in res:
+            totals[path] += res[path][0]
+        n_rhs += m
+        rows.append({"surface": name, "n": n, "tube_points": res["fp64"][3], "rhs": m,
+                     "fp64_s": res["fp64"][0], "fp64_iterations": res["fp64"][1].iterations,
+                     "fp64_status": res["fp64"][1].status, "mixed_s": res["mixed"][0],
+                     "mixed_iterations": res["mixed"][1].iterations, "mixed_status": res["mixed"][1].status,
+                     "mixed_rel_err_vs_fp64": err})
+        print("[bench] %s: N_t=%d, %d rhs, fp64 %.2f s, mixed %.2f s, rel err %.1e" %
+              (name, res["fp64"][3], m, res["fp64"][0], res["mixed"][0], err), file=sys.stderr, flush=True)
+    emit({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / totals["mixed"],
+          "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": totals["mixed"] * 1e3,
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+          "dtype": "complex64 operator, fp64 matrix-core products, fp64 refinement",
+          "data": "synthetic: torus + 8 seed-random spheres/ellipsoids, Fibonacci plane waves",
+          "config": {"workload": "batch: torus 128^3 x 64 plane waves + 8 surfaces on 48^3-96^3 x 16 plane waves, "
+                                 "k=10, eps=tau=2h, tol %g" % tol, "rhs": n_rhs, "parallelism": "single GPU"},
+          "fp64_solves_per_s": n_rhs / totals["fp64"], "speedup_mixed_vs_fp64": totals["fp64"] / totals["mixed"],
+          "worst_rel_err_mixed_vs_fp64": worst_err, "surfaces": rows})
+
+
+def run_adjoint(args):
+    """Misfit gradient: perturbed sphere r = 1 + 0.15 sin3t cos2p on 96^3, k = 8,
+    point source, 200 seed-random receivers on |x| = 2.5, target field from
+    r = 1 + 0.1 sin3t cos2p. Times the forward solve and the gradient with
+    respect to every level-set value and k (adjoint BiCG + adjoint kernels), and
+    checks 50 seed-random entries against central differences (step 1e-8, the
+    forward path re-solved by FP64 refinement to the FP64 floor)."""
+    n, k = 96, 8.0
+    h = 4.0 / n
+    ctr = (0.0137, -0.0219, 0.0071)  # off the grid's symmetry planes
+    v = np.random.default_rng(0).standard_normal((200, 3))
+    recv = 2.5 * v / np.linalg.norm(v, axis=1, keepdims=True)
+    x0 = (0.1, 0.2, 0.3)
+    pb = ibim.Problem()
+    progress(pb)
+
+    def prepare(level, kk):
+        pb.set_level(level)
+        pb.geometry(5)
+        pb.tube(ibim.params(kk, 2 * h, 2 * h, dirichlet=1))
+        pb.assemble(ibim.C128)
+        pb.rhs(ibim.RHS_POINT_SOURCE, x0=x0)
+
+    def field(level, kk, tol_refined=True):
+        prepare(level, kk)
+        if tol_refined:
+            pb.solve(ibim.SOLVE_REFINED, tol=1e-15, inner_tol=1e-5, max_outer=10)
+        re, im = pb.evaluate(recv, ibim.EVAL_DOUBLE)
+        return re + 1j * im
+
+    pb.set_level_shape(ibim.shape(ibim.SHAPE_STAR, a=1.0, b=0.10, m=3, n=2, variant=0, center=ctr), n)
+    target = field(pb.level(), k)
+    pb.set_level_shape(ibim.shape(ibim.SHAPE_STAR, a=1.0, b=0.15, m=3, n=2, variant=0, center=ctr), n)
+    level = pb.level()
+    times = []
+    for _ in range(2):
+        pb.sync()
+        t0 = time.perf_counter()
+        prepare(level, k)
+        info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=1e-12)
+        t1 = time.perf_counter()
+        pb.set_receivers(recv, target)
+        J, g, dk = pb.gradient(mode=ibim.EVAL_DOUBLE, tol=1e-12, maxit=3000)
+        pb.sync()
+        times.append((t1 - t0, time.perf_counter() - t1))
+    count = pb.count
+    t_fwd, t_adj = times[-1]
+
+    def J_of(lv, kk):
+        return float(np.sum(np.abs(field(lv, kk) - target) ** 2))
+
+    gmax = np.abs(g).max()
+    live = np.argwhere(np.abs(g) > 1e-2 * gmax)
+    picks = live[np.random.default_rng(1).choice(len(live), 50, replace=False)]
+    errs, resolved = [], 0
+    for idx in picks:
+        idx = tuple(idx)
+        f = []
+        for st in (1e-8, 2e-8):
+            lp, lm = level.copy(), level.copy()
+            lp[idx] += st
+            lm[idx] -= st
+            f.append((J_of(lp, k) - J_of(lm, k)) / (2 * st))
+        if abs(f[0] - f[1]) <= 5e-6 * abs(g[idx]) + 3e-8 * gmax:
+            resolved += 1
+            errs.append(abs(f[0] - g[idx]) / abs(g[idx]))
+    fk = (J_of(level, k + 1e-6) - J_of(level, k - 1e-6)) / 2e-6
+    print("[bench] adjoint96: N_t=%d forward %.3f s gradient %.3f s; %d/50 entries resolved, worst rel err %.1e" %
+          (count, t_fwd, t_adj, resolved, max(errs) if errs else float("nan")), file=sys.stderr, flush=True)
+    emit({"metric": "misfit gradients/s (forward solve + adjoint)", "value": 1.0 / (t_fwd + t_adj),
+          "unit": "gradients/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": (t_fwd + t_adj) * 1e3,
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
+          "data": "synthetic: perturbed-sphere signed distance, point source, 200 receivers",
+          "config": {"workload": "adjoint96: r=1+0.15 sin3t cos2p on 96^3, k=8, eps=tau=2h, target from b=0.1, "
+                                 "200 receivers on |x|=2.5", "tube_points": count, "parallelism": "single GPU"},
+          "forward_s": t_fwd, "gradient_s": t_adj, "forward_iterations": info.iterations, "J": J,
+          "fd_entries_resolved": resolved, "fd_worst_rel_err": max(errs) if errs else None,
+          "dk": dk, "dk_fd": fk, "dk_rel_err": abs(fk - dk) / abs(dk)})
+
+
+def run_star(args):
+    """The 256^3 matrix-free case: star r = 1 + 0.2 cos4t sin3p, k = 40,
+    eps = tau = 2h, 16 plane waves (Fibonacci), nothing stored. Times batched
+    BiCG iterations on the FP32 (mixed) and FP64 matrix-free operators, each
+    pass applying the operator to all 16 right-hand sides on the matrix cores,
+    over a fixed iteration budget (BiCG on this k = 40 problem needs more than
+    300 iterations), and reports the residuals reached."""
+    n, k, m, its = 256, 40.0, 16, args.star_iterations
+    h = 4.0 / n
+    dirs = fibonacci_directions(m)
+    out = {}
+    for path in ("mixed", "fp64"):
+        pb = ibim.Problem()
+        progress(pb)
+        pb.set_level_shape(ibim.shape(ibim.SHAPE_STAR, a=1.0, b=0.2, m=4, n=3, variant=1), n)
+        pb.sync()
+        t0 = time.perf_counter()
+        pb.geometry(5)
+        count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1))
+        pb.set_nrhs(m)
+        for c in range(m):
+            pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+        pb.sync()
+        t_setup = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        if path == "mixed":
+            info, _ = pb.solve(ibim.SOLVE_REFINED, maxit=its, tol=1e-6, inner_tol=1e-6, max_outer=1, batch=0)
+        else:
+            info, _ = pb.solve(ibim.SOLVE_BICG, maxit=its, tol=1e-6, batch=0)
+        pb.sync()
+        out[path] = (time.perf_counter() - t0, info, count, t_setup)
+        print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations %.1f s, residual %.2e" %
+              (path, count, t_setup, its, out[path][0], info.rel_residual), file=sys.stderr, flush=True)
+        pb.close()
+    per_it = {p: out[p][0] / its for p in out}
+    emit({"metric": "batched BiCG iterations/s (16 right-hand sides, matrix-free)", "value": 1.0 / per_it["mixed"],
+          "unit": "iterations/s", "n_gpus": 1, "steps": its, "warmup": 0, "ms_per_step": per_it["mixed"] * 1e3,
+          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+          "dtype": "fp32 entries (fp64 phase), fp64 matrix-core products",
+          "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
+          "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=40, eps=tau=2h, 16 plane waves",
+                     "tube_points": out["mixed"][2], "parallelism": "single GPU"},
+          "fp64_iterations_per_s": 1.0 / per_it["fp64"], "speedup_mixed_vs_fp64": per_it["fp64"] / per_it["mixed"],
+          "residual_after_budget": {p: out[p][1].rel_residual for p in out},
+          "converged": {p: out[p][1].status == 0 for p in out}, "setup_s": out["mixed"][3]})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS) + list(EXTRA))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--star-iterations", type=int, default=60, help="iteration budget of the star256 workload")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.workload in EXTRA:
+        {"batch": run_batch, "adjoint96": run_adjoint, "star256": run_star}[args.workload](args)
+        return
     w = WORKLOADS[args.workload]
     level = level_of(w)
     recv = shapes.shell_receivers(NRECV, 1.5, 3.0, seed=0)
@@ -230,7 +458,7 @@ def main():
         "dtype": "fp32 operator (fp64 phase, fp64 sums) + fp64 residual replacement",
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
         "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g" % (
-            args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, TOL),
+            args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"]),
             "tube_points": count, "parallelism": "single GPU",
             "mixed_operator": "stored complex64" if stored else "recomputed FP32 entries"},
         "mixed": {"solves_per_s": 1.0 / tm, "iterations": infom.iterations, "replacements": infom.outer,

@@ -1,4 +1,5 @@
 // ibim_api.hip -- the device-resident handle API of libibim.so (include/ibim.h).
+#include <chrono>
 #include <cstdarg>
 #include <cstring>
 #include <new>
@@ -46,6 +47,15 @@ KernelTimer* timer_for(Context* ctx, const char* name) {
     ctx->timers.push_back(KernelTimer{});
     ctx->timers.back().name = name;
     return &ctx->timers.back();
+}
+
+void report_progress(Context* ctx, const char* what, int iteration, double residual) {
+    if (!ctx->progress) return;
+    const double now =
+        std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    if (ctx->progress_last != 0.0 && now - ctx->progress_last < 5.0) return;
+    ctx->progress_last = now;
+    ctx->progress(what, iteration, residual, ctx->progress_user);
 }
 
 void timer_begin(Context* ctx, KernelTimer* t) {
@@ -533,6 +543,14 @@ int ibim_gradient(ibim_ctx* ctx, int col, int mode, const ibim_solve_opts* opts,
         return IBIM_ERR_ARG;
     }
     return run_gradient(ctx, col, mode, opts, J, dlevel, dk);
+}
+
+int ibim_set_progress(ibim_ctx* ctx, ibim_progress_fn fn, void* user) {
+    CHECK(ctx);
+    ctx->progress = fn;
+    ctx->progress_user = user;
+    ctx->progress_last = 0.0;
+    return IBIM_OK;
 }
 
 int ibim_set_profiling(ibim_ctx* ctx, int on) {

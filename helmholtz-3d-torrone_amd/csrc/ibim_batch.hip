@@ -631,6 +631,10 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
         IBIM_HIP(hipGetLastError());
         IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
         IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        double worst = 0.0;
+        for (int r = 0; r < kR; ++r)
+            if (hs->done[r] == 0) worst = std::max(worst, hs->err[r]);
+        if (!hs->all_done) report_progress(ctx, "batched BiCG", it - 1, worst);
     }
     return IBIM_OK;
 }

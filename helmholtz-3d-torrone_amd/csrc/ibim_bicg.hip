@@ -559,8 +559,10 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
         IBIM_HIP(hipGetLastError());
         IBIM_HIP(hipMemcpyAsync(&done, a.done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
         BicgScalars at_last{};
-        if (rep) IBIM_HIP(hipMemcpyAsync(&at_last, a.sc + last, sizeof at_last, hipMemcpyDeviceToHost, ctx->stream));
+        if (rep || ctx->progress)
+            IBIM_HIP(hipMemcpyAsync(&at_last, a.sc + last, sizeof at_last, hipMemcpyDeviceToHost, ctx->stream));
         IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        if (!done) report_progress(ctx, adjoint ? "adjoint BiCG" : "BiCG", last, at_last.err);
         if (!rep) continue;
         BicgScalars cur = at_last;
         if (done) {

@@ -100,6 +100,11 @@ struct Context {
     DevBuf batch;   // batched (16 right-hand sides) BiCG workspace
     int last_iters = 0;
 
+    // progress reports from long solves (at host checks, at most every few seconds)
+    ibim_progress_fn progress = nullptr;
+    void* progress_user = nullptr;
+    double progress_last = 0.0;
+
     // profiling
     bool profile = false;
     std::deque<KernelTimer> timers;  // stable addresses for open TimedScopes
@@ -127,6 +132,9 @@ struct TimedScope {
 };
 
 // Kernel launchers (defined in the .hip translation units).
+// Calls the progress callback if one is set and enough time has passed.
+void report_progress(Context* ctx, const char* what, int iteration, double residual);
+
 int launch_sdf(Context* ctx, const ibim_shape* shape);
 int launch_gradient(Context* ctx, int order);
 int launch_curvatures(Context* ctx);

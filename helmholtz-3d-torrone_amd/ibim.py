@@ -39,6 +39,9 @@ SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2
 OP_STORED, OP_FREE64, OP_FREE32 = 0, 1, 2
 
 
+PROGRESS_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_int, C.c_double, C.c_void_p)
+
+
 class IbimError(RuntimeError):
     pass
 
@@ -105,7 +108,8 @@ def lib():
             "ibim_evaluate": ([vp, I, I, I, P, P, P], I),
             "ibim_set_receivers": ([vp, I, P, P, P], I),
             "ibim_gradient": ([vp, I, I, C.POINTER(SolveOpts), P, P, P], I),
-            "ibim_set_profiling": ([vp, I], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
+            "ibim_set_profiling": ([vp, I], I),
+            "ibim_set_progress": ([vp, PROGRESS_FN, vp], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
             "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
             "ibim_read_rhs_text": ([C.c_char_p, C.c_char_p, I, P, P], I),
@@ -333,6 +337,15 @@ class Problem:
         return float(J[0]), g, float(dk[0])
 
     # profiling --------------------------------------------------------------------
+    def set_progress(self, fn):
+        """fn(what, iteration, residual) at most every 5 s during long solves (None: off)."""
+        if fn is None:
+            self._progress = None
+            _check(lib().ibim_set_progress(self.h, PROGRESS_FN(), None), "ibim_set_progress")
+            return
+        self._progress = PROGRESS_FN(lambda what, it, res, user: fn(what.decode(), it, res))
+        _check(lib().ibim_set_progress(self.h, self._progress, None), "ibim_set_progress")
+
     def set_profiling(self, on=True):
         _check(lib().ibim_set_profiling(self.h, 1 if on else 0), "ibim_set_profiling")
 

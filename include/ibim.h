@@ -189,6 +189,12 @@ int ibim_apply(ibim_ctx *ctx, int op, const double *pre, const double *pim, cons
 int ibim_apply_batch(ibim_ctx *ctx, int op, int adjoint, int nvec, const double *pre, const double *pim, double *yre,
                      double *yim);
 
+/* Progress of long solves: called from ibim_solve / ibim_gradient at host
+ * checks, at most every 5 s, with the iteration and the largest running
+ * residual. No counterpart in the reference (it prints each iteration). */
+typedef void (*ibim_progress_fn)(const char *what, int iteration, double residual, void *user);
+int ibim_set_progress(ibim_ctx *ctx, ibim_progress_fn fn, void *user);
+
 /* ------------------------------------------------- right-hand sides/solve --- */
 int ibim_set_nrhs(ibim_ctx *ctx, int nrhs);
 int ibim_rhs(ibim_ctx *ctx, int kind, const double *x0, double amp, const double *dir, int col);
