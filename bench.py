@@ -369,6 +369,39 @@ def run_star(args):
           "converged": {p: out[p][1].status == 0 for p in out}, "setup_s": out["mixed"][3]})
 
 
+def measured_traffic(kernel, workload):
+    """HBM bytes per launch of `kernel` from the committed counter summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    import re
+    path = os.path.join(REPO, "profiles", "pmc_sweeps_%s.md" % workload)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        m = re.match(r"\s*traffic (\S+) (\S+) = ([0-9.e+-]+)", line)
+        if m and m.group(1) == kernel and m.group(2) == workload:
+            return float(m.group(3))
+    return None
+
+
+def calibrate_copy(nbytes=1 << 30, reps=3):
+    """A plain device-to-device copy of known size (the runtime's blit kernel),
+    the reference point for the FETCH_SIZE / WRITE_SIZE counters."""
+    import ctypes as C
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    if hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) or hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)):
+        raise RuntimeError("hipMalloc failed")
+    hip.hipMemset(a, 0, C.c_size_t(nbytes))
+    hip.hipDeviceSynchronize()
+    for _ in range(reps):
+        if hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    emit({"calibration": "device-to-device copy", "bytes_read": nbytes, "bytes_written": nbytes, "reps": reps})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -377,11 +410,16 @@ def main():
     ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS) + list(EXTRA))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--star-iterations", type=int, default=60, help="iteration budget of the star256 workload")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.workload in EXTRA:
         {"batch": run_batch, "adjoint96": run_adjoint, "star256": run_star}[args.workload](args)
         return
@@ -432,7 +470,7 @@ def main():
         dom_bytes = n2 * 8 + vec_bytes
         achieved = dom_bytes / avg_dom / 1e9 if n_dom else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_sweep<float2>", args.workload),
                 "kernel": "k_sweep<float2>: 8 B/entry x %d^2 entries + 64 B/point vectors per launch" % count}
     else:
         flops = n2 * (C_EVAL + 8 * 2)
