@@ -159,3 +159,67 @@ def test_autograd_function_returns_the_adjoint_gradient(gpu):
         assert lt.grad.device.type == dev
         np.testing.assert_allclose(lt.grad.cpu().numpy(), 2.0 * g, rtol=1e-9, atol=1e-12 * np.abs(g).max())
         assert abs(kt.grad.item() - 2.0 * dk) <= 1e-9 * abs(dk)
+
+
+def test_gradient_matches_differences_of_the_oracle_pipeline(gpu):
+    """The GPU adjoint against central differences of J computed entirely by the
+    CPU oracle (the reference's geometry, tube, kernel entries and evaluation,
+    restated in C and pinned to the reference's golden vectors), with a direct
+    LU solve in place of BiCG. Small grid (24^3) so the oracle is fast."""
+    import oracle_ctypes as O
+
+    n, k = 24, 3.0
+    h = 4.0 / n
+    recv = receivers(40, seed=3)
+    pb = gpu.Problem()
+    pb.set_level_shape(gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.10, m=3, n=2, variant=0, center=CENTER), n)
+    lt = pb.level()
+    pb.set_level_shape(gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.15, m=3, n=2, variant=0, center=CENTER), n)
+    level = pb.level()
+    prm = O.make_params(k=k, eps=3 * h, tau=2 * h, dirichlet=1)
+
+    def oracle_field(lv, kk):
+        p = O.make_params(k=kk, eps=3 * h, tau=2 * h, dirichlet=1)
+        node, flat = O.tube(lv, O.geometry(lv, n), n, p, 0)
+        cnt = len(node)
+        are, aim = O.kernel_rows(flat, cnt, p)
+        bre, bim = O.rhs_point_source(flat, cnt, kk, X0)
+        phi = np.linalg.solve(are + 1j * aim, bre + 1j * bim)
+        ure, uim = O.evaluate(flat, cnt, p, phi.real.copy(), phi.imag.copy(), recv, 1)
+        return ure + 1j * uim
+
+    target = oracle_field(lt, k)
+
+    def J_oracle(lv, kk):
+        return float(np.sum(np.abs(oracle_field(lv, kk) - target) ** 2))
+
+    pb.set_level(level)
+    pb.geometry(5)
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=1))
+    pb.assemble(gpu.C128)
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=X0)
+    info, _ = pb.solve(tol=1e-14, maxit=2000)
+    pb.set_receivers(recv, target)
+    J, g, dk = pb.gradient(mode=gpu.EVAL_DOUBLE, tol=1e-14, maxit=2000)
+    J0 = J_oracle(level, k)
+    assert abs(J - J0) <= 1e-9 * J0, (J, J0)
+
+    gmax = np.abs(g).max()
+    live = np.argwhere(np.abs(g) > 1e-2 * gmax)
+    picks = live[np.random.default_rng(5).choice(len(live), 10, replace=False)]
+    resolved = 0
+    for idx in picks:
+        idx = tuple(idx)
+        f = []
+        for st in (1e-8, 2e-8):
+            lp, lm = level.copy(), level.copy()
+            lp[idx] += st
+            lm[idx] -= st
+            f.append((J_oracle(lp, k) - J_oracle(lm, k)) / (2 * st))
+        if abs(f[0] - f[1]) > 5e-6 * abs(g[idx]) + 3e-8 * gmax:
+            continue
+        resolved += 1
+        assert abs(f[0] - g[idx]) <= 1e-5 * abs(g[idx]) + 2e-8 * gmax, (idx, f, g[idx])
+    assert resolved >= 7
+    fk = (J_oracle(level, k + 1e-6) - J_oracle(level, k - 1e-6)) / 2e-6
+    assert abs(fk - dk) <= 1e-5 * abs(dk), (fk, dk)
