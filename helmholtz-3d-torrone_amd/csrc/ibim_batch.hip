@@ -16,7 +16,7 @@
 // The vector operand (a step x 16 block of P or Q) is staged in LDS once per
 // workgroup and shared by its four waves.
 // Matrix-free variants evaluate the entry in the same layout instead of
-// loading it (traditional tube, FP64 or FP32 entries); the points of the
+// loading it (traditional tube; FP64 library entries or the fast ones); the points of the
 // streamed index are staged in LDS per workgroup.
 //
 // Vectors are RHS-interleaved: element (point j, right-hand side r) at
@@ -167,15 +167,11 @@ __global__ void __launch_bounds__(kBT) k_cols_stored(const MT* __restrict__ A, s
 // ------------------------------------------------------- matrix-free passes --
 template <int Op>
 __device__ __forceinline__ void free_entry(const PointData& a, const PointData& b, bool diag, const ibim_params& prm,
-                                           double kappa, float kf, double tau2, double* re, double* im) {
-    if (Op == OP_FREE32) {
-        float fr, fi;
-        trad_entry_f32(a, b, diag, kappa, kf, tau2, prm.dirichlet, prm.interior, &fr, &fi);
-        *re = (double)fr;
-        *im = (double)fi;
-    } else {
+                                           double kappa, double tau2, double* re, double* im) {
+    if (Op == OP_FREE32)
+        trad_entry_fast(a, b, diag, prm.k, kappa, tau2, prm.dirichlet, prm.interior, re, im);
+    else
         trad_entry(a, b, diag, prm.k, prm.tau, prm.dirichlet, prm.interior, re, im);
-    }
 }
 
 template <int Op>
@@ -190,7 +186,6 @@ __global__ void __launch_bounds__(kBT) k_rows_free(const double* __restrict__ tu
     const int row0 = (blockIdx.x * 4 + w) * 16;
     const size_t cnt = (size_t)count;
     const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
-    const float kf = (float)prm.k;
     const int row = row0 + li;
     const bool rok = row < count;
     const PointData a = load_point(tube, cnt, rok ? row : 0);
@@ -207,7 +202,7 @@ __global__ void __launch_bounds__(kBT) k_rows_free(const double* __restrict__ tu
         for (int s = 0; s < kKT / 4; ++s) {
             const int j = k0 + 4 * s + lk;
             double er = 0.0, ei = 0.0;
-            if (rok && j < count) free_entry<Op>(a, cp[4 * s + lk], row == j, prm, kappa, kf, tau2, &er, &ei);
+            if (rok && j < count) free_entry<Op>(a, cp[4 * s + lk], row == j, prm, kappa, tau2, &er, &ei);
             const double2 p = ps[4 * s + lk][li];
             y.add(er, ei, p.x, p.y);
         }
@@ -231,7 +226,6 @@ __global__ void __launch_bounds__(kBT) k_cols_free(const double* __restrict__ tu
     const int c0 = (blockIdx.x * 4 + w) * 16;
     const size_t cnt = (size_t)count;
     const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
-    const float kf = (float)prm.k;
     const int col = c0 + li;
     const bool cok = col < count;
     const PointData b = load_point(tube, cnt, cok ? col : 0);
@@ -248,7 +242,7 @@ __global__ void __launch_bounds__(kBT) k_cols_free(const double* __restrict__ tu
         for (int s = 0; s < kKT / 4; ++s) {
             const int i = i0 + 4 * s + lk;
             double ar = 0.0, ai = 0.0;
-            if (cok && i < count) free_entry<Op>(rp[4 * s + lk], b, i == col, prm, kappa, kf, tau2, &ar, &ai);
+            if (cok && i < count) free_entry<Op>(rp[4 * s + lk], b, i == col, prm, kappa, tau2, &ar, &ai);
             const double2 q = qs[4 * s + lk][li];
             z.add(ar, -ai, q.x, q.y);  // conj(a) q
         }

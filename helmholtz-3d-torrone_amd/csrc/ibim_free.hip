@@ -7,13 +7,11 @@
 // the BiCG vector kernels are shared.
 //
 // OP_FREE64 evaluates the reference's FP64 entries (ibim_entries.hpp).
-// OP_FREE32 evaluates the traditional entry in FP32 except for the phase: the
-// separation |x*_i - y*_j|^2 is formed in FP64, its square root is refined by
-// one FP64 Newton step, the phase k rho / (2 pi) is reduced modulo 1 in FP64,
-// and only the reduced revolution fraction goes to the FP32 sin / cos units
-// (v_sin_f32 / v_cos_f32 take revolutions). The rest of the entry is FP32;
-// it is applied to the FP64 vectors in FP64, so the FP32 operator is one fixed
-// matrix and BiCG keeps close to the FP64 operator's iteration count.
+// OP_FREE32 (the mixed-precision operator) evaluates the traditional entry
+// with FP32 sin / cos, r.n and weight products and FP64 for the separation,
+// rho, 1/rho, the phase reduction and the radial factors (trad_entry_fast,
+// ibim_entries.hpp), and applies it to the FP64 vectors in FP64: one fixed
+// matrix whose error is that of the complex64-rounded FP64 operator.
 //
 // Refinement: x = 0, r = b; repeat { solve A32 d = r by BiCG to inner_tol;
 // x += d; r = b - A64 x with the FP64 operator } until ||r|| / ||b|| < tol.
@@ -50,7 +48,7 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
                                                     const double2* __restrict__ p, const double2* __restrict__ q,
                                                     double2* __restrict__ ypart, double2* __restrict__ zpart,
                                                     size_t pld, const int* __restrict__ done) {
-    using AT = typename std::conditional<Op == OP_FREE32, float, double>::type;
+    using AT = double;
     if (done != nullptr && *done) return;
     __shared__ RowStage st;
     __shared__ AT ysh[2][kFT / 64][kFRows];
@@ -60,7 +58,6 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
     const int nrow = min(kFRows, count - r0);
     const size_t cnt = (size_t)count;
     const double kappa = prm.k / (2.0 * kPi);
-    const float kf = (float)prm.k;
     const double tau2 = prm.tau * prm.tau;
 
     PointData col[kFCpt];
@@ -118,8 +115,9 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
                 AT er = 0, ei = 0;
                 if (cj[m] < count) {
                     if (Op == OP_FREE32) {
-                        float fr, fi;
-                        trad_entry_f32(a, col[m], i == cj[m], kappa, kf, tau2, prm.dirichlet, prm.interior, &fr, &fi);
+                        double fr, fi;
+                        trad_entry_fast(a, col[m], i == cj[m], prm.k, kappa, tau2, prm.dirichlet, prm.interior, &fr,
+                                        &fi);
                         er = (AT)fr;
                         ei = (AT)fi;
                     } else {
@@ -163,12 +161,6 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
 // FP32 operator sweep (traditional kernel). Column points keep FP64 positions
 // and FP32 normal / weight / near-field value in registers; row points are
 // staged in LDS the same way. Dirichlet / interior are compile-time switches.
-struct RowStage32 {
-    double sx[kStage], sy[kStage], sz[kStage];
-    float nx[kStage], ny[kStage], nz[kStage], w[kStage], reg[kStage];
-    double qre[kStage], qim[kStage];
-};
-
 // Sum 8 per-lane values over the wavefront (transposed: halving exchanges
 // xor 32 / 16 / 8, then a butterfly over xor 4 / 2 / 1). Lane l ends with the
 // sum of value ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
@@ -188,46 +180,52 @@ __device__ __forceinline__ double wsum8_transposed(const double (&v)[8], int lan
 
 constexpr int kRowBatch = 4;  // rows per transposed reduction (4 rows x re, im)
 
-// The FP32 entry is applied to the FP64 vectors in FP64 (products and sums), so
-// the recomputed FP32 operator is one fixed matrix for BiCG.
+// The fast operator for the traditional tube, specialised on the
+// Dirichlet / interior switches: entries by trad_entry_fast, applied to the
+// FP64 vectors in FP64 (products and sums), so the recomputed operator is one
+// fixed matrix for BiCG. Column points stay in registers, row points are
+// staged in LDS; rows go through the transposed 8-value wave reduction four at
+// a time.
+// A column point as the mixed-precision entry reads it (FP32 where the entry
+// rounds to FP32 anyway): fewer registers per column.
+struct ColPoint {
+    double sx, sy, sz;
+    float nx, ny, nz, w, reg;
+};
+
 template <int Dirichlet, int Interior>
 __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__ tube, int count, ibim_params prm,
                                                       const double2* __restrict__ p, const double2* __restrict__ q,
                                                       double2* __restrict__ ypart, double2* __restrict__ zpart,
                                                       size_t pld, const int* __restrict__ done) {
     if (done != nullptr && *done) return;
-    __shared__ RowStage32 st;
+    __shared__ RowStage st;
     __shared__ double ysh[2][kFT / 64][kFRows];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int c0 = blockIdx.x * kSweepCols;
     const int r0 = blockIdx.y * kFRows;
     const int nrow = min(kFRows, count - r0);
     const size_t cnt = (size_t)count;
-    const double kappa = prm.k / (2.0 * kPi);
-    const float kf = (float)prm.k;
-    const float rfp = (float)(1.0 / (4.0 * kPi));
-    const double tau2 = prm.tau * prm.tau;
+    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau, k = prm.k;
     const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
 
-    double csx[kFCpt], csy[kFCpt], csz[kFCpt];
-    float cnx[kFCpt], cny[kFCpt], cnz[kFCpt], cw[kFCpt], creg[kFCpt];
-    double pre[kFCpt], pim[kFCpt];
-    double zre[kFCpt], zim[kFCpt];
+    ColPoint col[kFCpt];
+    double pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
     int cj[kFCpt];
 #pragma unroll
     for (int m = 0; m < kFCpt; ++m) {
         cj[m] = c0 + t + kFT * m;
         const bool ok = cj[m] < count;
         const int j = ok ? cj[m] : 0;
-        csx[m] = tube[IBIM_F_SX * cnt + j];
-        csy[m] = tube[IBIM_F_SY * cnt + j];
-        csz[m] = tube[IBIM_F_SZ * cnt + j];
-        cnx[m] = (float)tube[IBIM_F_NX * cnt + j];
-        cny[m] = (float)tube[IBIM_F_NY * cnt + j];
-        cnz[m] = (float)tube[IBIM_F_NZ * cnt + j];
-        cw[m] = ok ? (float)tube[IBIM_F_W * cnt + j] : 0.0f;  // w = 0 silences padding columns
-        creg[m] = (float)tube[IBIM_F_REG * cnt + j];
-        double2 pv = ok ? p[j] : make_double2(0.0, 0.0);
+        col[m].sx = tube[IBIM_F_SX * cnt + j];
+        col[m].sy = tube[IBIM_F_SY * cnt + j];
+        col[m].sz = tube[IBIM_F_SZ * cnt + j];
+        col[m].nx = (float)tube[IBIM_F_NX * cnt + j];
+        col[m].ny = (float)tube[IBIM_F_NY * cnt + j];
+        col[m].nz = (float)tube[IBIM_F_NZ * cnt + j];
+        col[m].w = ok ? (float)tube[IBIM_F_W * cnt + j] : 0.0f;  // silences padding columns
+        col[m].reg = (float)tube[IBIM_F_REG * cnt + j];
+        const double2 pv = ok ? p[cj[m]] : make_double2(0.0, 0.0);
         pre[m] = pv.x;
         pim[m] = pv.y;
         zre[m] = zim[m] = 0.0;
@@ -244,12 +242,13 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
             st.sx[t] = tube[IBIM_F_SX * cnt + i];
             st.sy[t] = tube[IBIM_F_SY * cnt + i];
             st.sz[t] = tube[IBIM_F_SZ * cnt + i];
-            st.nx[t] = (float)tube[IBIM_F_NX * cnt + i];
-            st.ny[t] = (float)tube[IBIM_F_NY * cnt + i];
-            st.nz[t] = (float)tube[IBIM_F_NZ * cnt + i];
-            st.w[t] = (float)tube[IBIM_F_W * cnt + i];
-            st.reg[t] = (float)tube[IBIM_F_REG * cnt + i];
-            double2 qv = ok ? q[i] : make_double2(0.0, 0.0);
+            st.nx[t] = tube[IBIM_F_NX * cnt + i];
+            st.ny[t] = tube[IBIM_F_NY * cnt + i];
+            st.nz[t] = tube[IBIM_F_NZ * cnt + i];
+            st.w[t] = tube[IBIM_F_W * cnt + i];
+            st.reg[t] = tube[IBIM_F_REG * cnt + i];
+            st.dist[t] = 0.0;
+            const double2 qv = ok ? q[i] : make_double2(0.0, 0.0);
             st.qre[t] = qv.x;
             st.qim[t] = qv.y;
         }
@@ -260,38 +259,22 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
             for (int u = 0; u < kRowBatch; ++u) {
                 const int s = sb + u;  // < kStage; rows >= ns carry q = 0 and their sums are dropped
                 const int i = r0 + s0 + s;
-                const double ax = st.sx[s], ay = st.sy[s], az = st.sz[s];
-                const float anx = st.nx[s], any = st.ny[s], anz = st.nz[s], areg = st.reg[s];
+                PointData a;
+                a.sx = st.sx[s];
+                a.sy = st.sy[s];
+                a.sz = st.sz[s];
+                a.nx = st.nx[s];
+                a.ny = st.ny[s];
+                a.nz = st.nz[s];
+                a.w = st.w[s];
+                a.reg = st.reg[s];
+                a.dist = 0.0;
                 const double qre = st.qre[s], qim = st.qim[s];
                 double sre = 0.0, sim = 0.0;
 #pragma unroll
                 for (int m = 0; m < kFCpt; ++m) {
-                    const double d1 = ax - csx[m], d2 = ay - csy[m], d3 = az - csz[m];
-                    const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
-                    float pr, pi_;
-                    if (rho2 > tau2) {
-                        const float r = sqrtf((float)rho2);
-                        const float ir = 1.0f / r;
-                        const double rd = (double)r;
-                        const double rho = fma(fma(-rd, rd, rho2), (double)(0.5f * ir), rd);
-                        const double tph = rho * kappa;
-                        const float f = (float)(tph - rint(tph));
-                        const float sv = __builtin_amdgcn_sinf(f), cv = __builtin_amdgcn_cosf(f);
-                        const float den = rfp * ir * ir;
-                        const float c1 = -(kf * sv + cv * ir) * den;
-                        const float c2 = (kf * cv - sv * ir) * den;
-                        const float f1 = (float)d1, f2 = (float)d2, f3 = (float)d3;
-                        const float ip = Dirichlet ? f1 * cnx[m] + f2 * cny[m] + f3 * cnz[m]
-                                                   : f1 * anx + f2 * any + f3 * anz;
-                        pr = Interior ? -c1 * ip : c1 * ip;
-                        pi_ = Interior ? -c2 * ip : c2 * ip;
-                    } else {
-                        pr = Dirichlet ? creg[m] : areg;
-                        pi_ = 0.0f;
-                    }
-                    float erf = cw[m] * pr;
-                    if (i == cj[m]) erf += 0.5f;
-                    const double er = (double)erf, ei = (double)(cw[m] * pi_);
+                    double er, ei;
+                    trad_entry_fast(a, col[m], i == cj[m], k, kappa, tau2, Dirichlet, Interior, &er, &ei);
                     sre += er * pre[m] - ei * pim[m];
                     sim += er * pim[m] + ei * pre[m];
                     zre[m] += er * qre + ei * qim;
