@@ -53,7 +53,7 @@ WORKLOADS = {
 EXTRA = ("batch", "adjoint96", "star256")
 TOL = 1e-8          # default relative residual target of both timed paths
 REF_TOL = 1e-11     # the FP64 solve the densities are compared against
-REPLACE = 1e-2      # residual drop between FP64 residual replacements
+REPLACE = 1e-3      # residual drop between FP64 residual replacements
 NRECV = 500
 C64_MAX_BYTES = 120e9  # store the complex64 operator up to this size
 

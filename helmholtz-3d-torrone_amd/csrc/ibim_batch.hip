@@ -8,8 +8,8 @@
 // fusing them would save no time and would need partial planes 16x larger.
 //
 //   row pass  (Y = A P):   a wave owns 16 rows and runs over all columns; the
-//                          16 x 64 block of A is staged through LDS (coalesced
-//                          row loads) and read back in the MFMA A-operand layout.
+//                          MFMA A operand is 16 rows x 4 adjacent columns,
+//                          loaded directly.
 //   column pass (Z = A^H Q): a wave owns 16 columns and runs over all rows; the
 //                          MFMA A operand is 4 rows x 16 consecutive columns,
 //                          loaded directly.
@@ -76,50 +76,44 @@ struct BState {
 
 // ------------------------------------------------------------ stored passes --
 // Row pass: a workgroup owns 64 rows (16 per wave) and steps over the columns
-// KT at a time; each wave stages its 16 x KT block of A through LDS, and the
-// KT x 16 block of P is staged once per workgroup (every wave reuses it).
-template <class MT>
-struct RowStep {
-    static constexpr int kt = sizeof(MT) == 8 ? 64 : 32;  // 33 KB of A tiles either way
-};
-
+// kKT at a time. The MFMA A operand (16 rows x 4 consecutive columns) is loaded
+// directly: four lanes read four adjacent entries of one row (64 B for
+// complex128, 32 B for complex64), and the next three k-steps read the rest of
+// those cache lines. The kKT x 16 block of P is staged once per workgroup.
 template <class MT>
 __global__ void __launch_bounds__(kBT) k_rows_stored(const MT* __restrict__ A, size_t ld, int count,
                                                      const double2* __restrict__ P, double2* __restrict__ Y,
                                                      const BState* __restrict__ st) {
-    constexpr int KT = RowStep<MT>::kt;
     if (st != nullptr && st->all_done) return;
-    __shared__ MT tile[4][16][KT + 1];
-    __shared__ double2 ps[KT][kR];
+    __shared__ double2 ps[kKT][kR];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
     const int row0 = (blockIdx.x * 4 + w) * 16;
+    const int row = row0 + li;
+    const bool rok = row < count;
+    const MT* Ar = A + (size_t)(rok ? row : 0) * ld + lk;
     Acc3 y;
-    for (int k0 = 0; k0 < count; k0 += KT) {
-        // k0 + KT <= ld: both are multiples of KT (ld of 64) and k0 < count <= ld
-#pragma unroll 4
-        for (int e = lane; e < 16 * KT; e += 64) {
-            const int rr = e / KT, cc = e % KT, row = row0 + rr;
-            MT v;
-            v.x = 0;
-            v.y = 0;
-            if (row < count) v = A[(size_t)row * ld + k0 + cc];
-            tile[w][rr][cc] = v;
-        }
-        for (int e = threadIdx.x; e < KT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];
+    for (int k0 = 0; k0 < count; k0 += kKT) {
+        // k0 + kKT <= ld: both are multiples of 64 and k0 < count <= ld
+        __syncthreads();
+        for (int e = threadIdx.x; e < kKT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];
         __syncthreads();
 #pragma unroll 4
-        for (int s = 0; s < KT / 4; ++s) {
-            const MT a = tile[w][li][4 * s + lk];
+        for (int s = 0; s < kKT / 4; ++s) {
+            double ar = 0.0, ai = 0.0;
+            if (rok) {
+                const MT a = Ar[k0 + 4 * s];
+                ar = (double)a.x;
+                ai = (double)a.y;
+            }
             const double2 p = ps[4 * s + lk][li];
-            y.add((double)a.x, (double)a.y, p.x, p.y);
+            y.add(ar, ai, p.x, p.y);
         }
-        __syncthreads();
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
-        const int row = row0 + 4 * v + lk;
-        if (row < count) Y[(size_t)row * kR + li] = make_double2(y.re(v), y.im(v));
+        const int r = row0 + 4 * v + lk;
+        if (r < count) Y[(size_t)r * kR + li] = make_double2(y.re(v), y.im(v));
     }
 }
 
