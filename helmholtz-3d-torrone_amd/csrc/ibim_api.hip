@@ -271,6 +271,7 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
+    ctx->count = 0;  // a failed build leaves no stale tube behind
     int c = 0;
     int rc = launch_tube(ctx, &c);
     if (rc) return rc;

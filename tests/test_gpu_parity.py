@@ -252,6 +252,50 @@ def test_refinement_matches_fp64_solution(gpu):
     assert relnorm(xr[0] + 1j * xr[1], x64[0] + 1j * x64[1]) <= 1e-6
 
 
+@pytest.mark.parametrize("store", ["c64", "free32"])
+def test_residual_replacement_matches_fp64_solution(gpu, store):
+    """BiCG on the mixed-precision operator with FP64 residual replacement reaches
+    the FP64 operator's solution (true residual below tol) with an iteration count
+    close to the FP64 solve's."""
+    pb, prm, flat, cnt = _sphere_problem(gpu, n=32, k=6.0, dirichlet=1)
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3])
+    info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-12)
+    x64 = pb.density()
+    if store == "c64":
+        pb.assemble(gpu.C64)  # the replacements then use the FP64 matrix-free operator
+    seen = []
+    pb.set_progress(lambda what, it, res: seen.append((what, it, res)))
+    info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=2000, tol=1e-10, inner_tol=1e-3, chunk=1)
+    pb.set_progress(None)
+    x = pb.density()
+    assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 1
+    assert len(hist) == info.iterations and hist[-1] < 1e-10
+    assert info.iterations <= 1.5 * info64.iterations
+    assert relnorm(x[0] + 1j * x[1], x64[0] + 1j * x64[1]) <= 1e-8
+    assert seen and seen[0][0] == "BiCG"  # the first host check always reports
+
+
+def test_degenerate_inputs_are_errors_not_faults(gpu):
+    """A level set whose tube is empty, solves before a right-hand side exists,
+    and out-of-range columns come back as IbimError (the reference exits)."""
+    n = 16
+    far = np.full((n + 1,) * 3, 5.0)  # every node 5 inside: no |d| <= eps
+    pb = gpu.Problem().set_level(far).geometry(5)
+    with pytest.raises(gpu.IbimError):
+        pb.tube(gpu.params(k=2.0, eps=0.75, tau=0.5, dirichlet=1))
+    with pytest.raises(gpu.IbimError):
+        pb.assemble(gpu.C128)
+    with pytest.raises(gpu.IbimError):
+        pb.solve(gpu.SOLVE_BICG, maxit=10, tol=1e-8)
+    pb2, prm, flat, cnt = _sphere_problem(gpu, n=16, k=2.0, dirichlet=1)
+    with pytest.raises(gpu.IbimError):
+        pb2.rhs(gpu.RHS_POINT_SOURCE, x0=[0.0, 0.0, 0.0], col=1)  # one column only
+    with pytest.raises(gpu.IbimError):
+        pb2.set_nrhs(0)
+    with pytest.raises(gpu.IbimError):
+        pb2.apply_batch(np.zeros((cnt, 17)), gpu.OP_STORED)  # at most 16 vectors
+
+
 # ------------------------------------------------------------------ drop-ins --
 def test_compbicg_dropin_matches_reference(gpu):
     b = golden("bicg.npz")
