@@ -43,25 +43,25 @@ __device__ __forceinline__ T wsum(T v) {
     return v;
 }
 
-template <int Kind, int Op>
-__global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ tube, int count, ibim_params prm,
+// The FP64 operator (the reference's entries, either tube kind) recomputed per
+// sweep: column points in registers, row points staged in LDS, one wavefront
+// reduction per row (the FP64 entry evaluation dominates this kernel).
+template <int Kind>
+__global__ void __launch_bounds__(kFT) k_sweep_free64(const double* __restrict__ tube, int count, ibim_params prm,
                                                     const double2* __restrict__ p, const double2* __restrict__ q,
                                                     double2* __restrict__ ypart, double2* __restrict__ zpart,
                                                     size_t pld, const int* __restrict__ done) {
-    using AT = double;
     if (done != nullptr && *done) return;
     __shared__ RowStage st;
-    __shared__ AT ysh[2][kFT / 64][kFRows];
+    __shared__ double ysh[2][kFT / 64][kFRows];
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
     const int c0 = blockIdx.x * kSweepCols;
     const int r0 = blockIdx.y * kFRows;
     const int nrow = min(kFRows, count - r0);
     const size_t cnt = (size_t)count;
-    const double kappa = prm.k / (2.0 * kPi);
-    const double tau2 = prm.tau * prm.tau;
 
     PointData col[kFCpt];
-    AT pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
+    double pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
     int cj[kFCpt];
 #pragma unroll
     for (int m = 0; m < kFCpt; ++m) {
@@ -69,13 +69,13 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
         if (cj[m] < count) {
             col[m] = load_point(tube, cnt, cj[m]);
             double2 pv = p[cj[m]];
-            pre[m] = (AT)pv.x;
-            pim[m] = (AT)pv.y;
+            pre[m] = pv.x;
+            pim[m] = pv.y;
         } else {
             col[m] = PointData{};
-            pre[m] = pim[m] = (AT)0;
+            pre[m] = pim[m] = 0.0;
         }
-        zre[m] = zim[m] = (AT)0;
+        zre[m] = zim[m] = 0.0;
     }
     for (int s0 = 0; s0 < nrow; s0 += kStage) {
         const int ns = min(kStage, nrow - s0);
@@ -108,27 +108,16 @@ __global__ void __launch_bounds__(kFT) k_sweep_free(const double* __restrict__ t
             a.w = st.w[s];
             a.reg = st.reg[s];
             a.dist = st.dist[s];
-            const AT qre = (AT)st.qre[s], qim = (AT)st.qim[s];
-            AT sre = 0, sim = 0;
+            const double qre = st.qre[s], qim = st.qim[s];
+            double sre = 0.0, sim = 0.0;
 #pragma unroll
             for (int m = 0; m < kFCpt; ++m) {
-                AT er = 0, ei = 0;
+                double er = 0.0, ei = 0.0;
                 if (cj[m] < count) {
-                    if (Op == OP_FREE32) {
-                        double fr, fi;
-                        trad_entry_fast(a, col[m], i == cj[m], prm.k, kappa, tau2, prm.dirichlet, prm.interior, &fr,
-                                        &fi);
-                        er = (AT)fr;
-                        ei = (AT)fi;
-                    } else {
-                        double dr, di;
-                        if (Kind == IBIM_TUBE_TRADITIONAL)
-                            trad_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.dirichlet, prm.interior, &dr, &di);
-                        else
-                            combo_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.eta, &dr, &di);
-                        er = (AT)dr;
-                        ei = (AT)di;
-                    }
+                    if (Kind == IBIM_TUBE_TRADITIONAL)
+                        trad_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.dirichlet, prm.interior, &er, &ei);
+                    else
+                        combo_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.eta, &er, &ei);
                 }
                 sre += er * pre[m] - ei * pim[m];
                 sim += er * pim[m] + ei * pre[m];
@@ -350,10 +339,10 @@ int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* 
                         : (in ? k_sweep_free32<0, 1> : k_sweep_free32<0, 0>);
         hipLaunchKernelGGL(kern, grid, dim3(kFT), 0, ctx->stream, tube, count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
     } else if (ctx->kind == IBIM_TUBE_TRADITIONAL)
-        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_TRADITIONAL, OP_FREE64>), grid, dim3(kFT), 0, ctx->stream, tube,
+        hipLaunchKernelGGL((k_sweep_free64<IBIM_TUBE_TRADITIONAL>), grid, dim3(kFT), 0, ctx->stream, tube,
                            count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
     else
-        hipLaunchKernelGGL((k_sweep_free<IBIM_TUBE_COMBO, OP_FREE64>), grid, dim3(kFT), 0, ctx->stream, tube, count,
+        hipLaunchKernelGGL((k_sweep_free64<IBIM_TUBE_COMBO>), grid, dim3(kFT), 0, ctx->stream, tube, count,
                            ctx->prm, P, Q, Y, Z, ctx->ld, done);
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
