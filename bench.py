@@ -323,13 +323,15 @@ def run_adjoint(args):
 
 
 def run_star(args):
-    """The 256^3 matrix-free case: star r = 1 + 0.2 cos4t sin3p, k = 40,
-    eps = tau = 2h, 16 plane waves (Fibonacci), nothing stored. Times batched
-    BiCG iterations on the FP32 (mixed) and FP64 matrix-free operators, each
-    pass applying the operator to all 16 right-hand sides on the matrix cores,
-    over a fixed iteration budget (BiCG on this k = 40 problem needs more than
-    300 iterations), and reports the residuals reached."""
-    n, k, m, its = 256, 40.0, 16, args.star_iterations
+    """The 256^3 matrix-free case: star r = 1 + 0.2 cos4t sin3p, k = 40 (or
+    --star-k), eps = tau = 2h, 16 plane waves (Fibonacci), nothing stored, tol
+    1e-6. Batched BiCG on the mixed-precision and on the FP64 matrix-free
+    operators (the mixed one inside batched FP64 refinement), each pass applying
+    the operator to all 16 right-hand sides on the matrix cores, with an
+    iteration budget (--star-iterations; BiCG does not converge within 300
+    iterations at k = 40). Reports solves/s when every column converged, else
+    batched iterations/s, and the residuals reached."""
+    n, k, m, its = 256, args.star_k, 16, args.star_iterations
     h = 4.0 / n
     dirs = fibonacci_directions(m)
     out = {}
@@ -348,25 +350,39 @@ def run_star(args):
         t_setup = time.perf_counter() - t0
         t0 = time.perf_counter()
         if path == "mixed":
-            info, _ = pb.solve(ibim.SOLVE_REFINED, maxit=its, tol=1e-6, inner_tol=1e-6, max_outer=1, batch=0)
+            info, _ = pb.solve(ibim.SOLVE_REFINED, maxit=its, tol=1e-6, inner_tol=1e-6, max_outer=3, batch=0)
         else:
             info, _ = pb.solve(ibim.SOLVE_BICG, maxit=its, tol=1e-6, batch=0)
         pb.sync()
         out[path] = (time.perf_counter() - t0, info, count, t_setup)
-        print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations %.1f s, residual %.2e" %
-              (path, count, t_setup, its, out[path][0], info.rel_residual), file=sys.stderr, flush=True)
+        print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations (budget %d) %.1f s, residual %.2e, status %d" %
+              (path, count, t_setup, info.iterations, its, out[path][0], info.rel_residual, info.status),
+              file=sys.stderr, flush=True)
         pb.close()
-    per_it = {p: out[p][0] / its for p in out}
-    emit({"metric": "batched BiCG iterations/s (16 right-hand sides, matrix-free)", "value": 1.0 / per_it["mixed"],
-          "unit": "iterations/s", "n_gpus": 1, "steps": its, "warmup": 0, "ms_per_step": per_it["mixed"] * 1e3,
+    converged = {p: out[p][1].status == 0 for p in out}
+    if all(converged.values()):
+        # every column converged: whole solves
+        metric, unit, value = "solves/s (16 right-hand sides, matrix-free)", "solves/s", m / out["mixed"][0]
+        ms = out["mixed"][0] * 1e3
+        fp64_rate = m / out["fp64"][0]
+    else:
+        # iteration budget exhausted: throughput of batched iterations
+        metric, unit, value = "batched BiCG iterations/s (16 right-hand sides, matrix-free)", "iterations/s", \
+            its / out["mixed"][0]
+        ms = out["mixed"][0] / its * 1e3
+        fp64_rate = its / out["fp64"][0]
+    emit({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "fp32 entries (fp64 phase), fp64 matrix-core products",
+          "dtype": "mixed-precision entries (fp64 radial factors and phase), fp64 matrix-core products",
           "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
-          "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=40, eps=tau=2h, 16 plane waves",
+          "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, "
+                                 "%d-iteration budget" % (k, its),
                      "tube_points": out["mixed"][2], "parallelism": "single GPU"},
-          "fp64_iterations_per_s": 1.0 / per_it["fp64"], "speedup_mixed_vs_fp64": per_it["fp64"] / per_it["mixed"],
-          "residual_after_budget": {p: out[p][1].rel_residual for p in out},
-          "converged": {p: out[p][1].status == 0 for p in out}, "setup_s": out["mixed"][3]})
+          "fp64_rate": fp64_rate, "speedup_mixed_vs_fp64": out["fp64"][0] / out["mixed"][0],
+          "seconds": {p: out[p][0] for p in out},
+          "iterations_summed_over_columns": {p: out[p][1].iterations for p in out},
+          "residual": {p: out[p][1].rel_residual for p in out}, "converged": converged,
+          "setup_s": out["mixed"][3]})
 
 
 def measured_traffic(kernel, workload):
@@ -410,6 +426,7 @@ def main():
     ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS) + list(EXTRA))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--star-iterations", type=int, default=60, help="iteration budget of the star256 workload")
+    ap.add_argument("--star-k", type=float, default=40.0, help="wavenumber of the star256 workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     args = ap.parse_args()
