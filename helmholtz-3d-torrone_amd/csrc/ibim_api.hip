@@ -147,7 +147,9 @@ void ibim_destroy(ibim_ctx* ctx) {
     collect_timers(ctx);
     for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
                       &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
-                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch})
+                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch,
+                      &ctx->pair_tiles, &ctx->pair_blk, &ctx->pair_pts, &ctx->near_off, &ctx->near_idx,
+                      &ctx->near_cur})
         b->release();
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -174,6 +176,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->weno_order = 0;
     ctx->count = 0;
     ctx->mat_precision = -1;
+    ctx->mat_pair = false;
     return IBIM_OK;
 }
 
@@ -271,6 +274,7 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
+    ctx->mat_pair = false;
     ctx->count = 0;  // a failed build leaves no stale tube behind
     int c = 0;
     int rc = launch_tube(ctx, &c);
@@ -297,10 +301,12 @@ const double* ibim_tube_device(ibim_ctx* ctx) { return ctx ? ctx->tube.as<double
 
 int ibim_assemble(ibim_ctx* ctx, int precision) {
     CHECK(ctx);
-    if (!ctx->count || (precision != IBIM_C128 && precision != IBIM_C64)) {
+    if (!ctx->count || precision < IBIM_C128 || precision > IBIM_C64_PAIR) {
         set_error("ibim_assemble: needs a tube and a precision");
         return IBIM_ERR_ARG;
     }
+    if (precision == IBIM_C128_PAIR) return launch_assemble_pair(ctx, IBIM_C128);
+    if (precision == IBIM_C64_PAIR) return launch_assemble_pair(ctx, IBIM_C64);
     return launch_assemble(ctx, precision);
 }
 
@@ -310,6 +316,24 @@ int ibim_get_rows(ibim_ctx* ctx, int row0, int nrows, double* re, double* im) {
     CHECK(ctx);
     const int n = ctx->count;
     if (ctx->mat_precision < 0 || row0 < 0 || nrows < 0 || row0 + nrows > n) return IBIM_ERR_ARG;
+    if (ctx->mat_pair) {  // rebuilt on the device, a few rows at a time
+        const int per = (int)((64u << 20) / (sizeof(double2) * (size_t)n)) + 1;
+        if (ctx->xfer.ensure(sizeof(double2) * (size_t)n * per)) return IBIM_ERR_NOMEM;
+        std::vector<double2> h((size_t)n * per);
+        for (int r0 = 0; r0 < nrows; r0 += per) {
+            const int nr = nrows - r0 < per ? nrows - r0 : per;
+            int rc = pair_rows(ctx, row0 + r0, nr, ctx->xfer.as<double2>());
+            if (rc) return rc;
+            IBIM_HIP(hipMemcpyAsync(h.data(), ctx->xfer.ptr, sizeof(double2) * (size_t)n * nr, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+            IBIM_HIP(hipStreamSynchronize(ctx->stream));
+            for (size_t o = 0; o < (size_t)n * nr; ++o) {
+                re[(size_t)r0 * n + o] = h[o].x;
+                im[(size_t)r0 * n + o] = h[o].y;
+            }
+        }
+        return IBIM_OK;
+    }
     const size_t esz = ctx->mat_precision == IBIM_C128 ? 16 : 8;
     std::vector<char> buf(esz * ctx->ld * (size_t)nrows);
     IBIM_HIP(hipMemcpyAsync(buf.data(), ctx->mat.as<char>() + esz * ctx->ld * (size_t)row0, buf.size(),
@@ -368,6 +392,7 @@ int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const doubl
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
     (void)hipHostFree(stage);
     ctx->mat_precision = precision;
+    ctx->mat_pair = false;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
 }
 
@@ -407,6 +432,10 @@ int ibim_apply_batch(ibim_ctx* ctx, int op, int adjoint, int nvec, const double*
     if (!ctx->count || nvec < 1 || nvec > 16 || !pre || !yre || !yim || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 ||
         (op == IBIM_OP_STORED && ctx->mat_precision < 0) || (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL))
         return IBIM_ERR_ARG;
+    if (op == IBIM_OP_STORED && ctx->mat_pair) {
+        set_error("ibim_apply_batch: the batched products need dense or matrix-free operators, not pair storage");
+        return IBIM_ERR_ARG;
+    }
     const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
     return batch_apply_host(ctx, kinds[op], adjoint != 0, nvec, pre, pim, yre, yim);
 }
@@ -488,10 +517,13 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     CHECK(ctx);
     if (!opts || !info || opts->maxit <= 0 || !ctx->count) return IBIM_ERR_ARG;
     memset(info, 0, sizeof *info);
+    // the batched solvers need dense or matrix-free operators; pair storage
+    // solves one column at a time
+    const bool batched = ctx->nrhs > 1 && opts->batch == 0 && !ctx->mat_pair;
     if (opts->method == IBIM_SOLVE_BICG)
-        return ctx->nrhs > 1 && opts->batch == 0 ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
+        return batched ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_REFINED)
-        return ctx->nrhs > 1 && opts->batch == 0 && (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C64)
+        return batched && (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C64)
                    ? run_refined_batched(ctx, opts, info)
                    : run_refined(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_RELIABLE) return run_reliable(ctx, opts, info, hist);

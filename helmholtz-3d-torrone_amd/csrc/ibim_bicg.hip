@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "ibim_internal.hpp"
+#include "ibim_wave.hpp"
 
 namespace ibim {
 
@@ -45,35 +46,8 @@ __device__ __forceinline__ T wave_sum(T v) {
 // Entries are applied in FP64 whatever their storage: a stored complex64
 // operator is then one fixed matrix for BiCG (FP32 products or sums make it
 // vector-dependent at the 1e-5 level, which doubled the iteration count on a
-// 65k-point torus). With the transposed row reduction below the FP64 sweep over
-// complex64 storage still streams at the HBM read rate.
-
-// Sum 8 per-lane values over the wavefront, transposed: three halving
-// exchanges (xor 32, 16, 8) leave each lane with one value index, then a
-// three-step butterfly (xor 4, 2, 1) completes it. Lane l ends with the full
-// sum of value ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1):
-// 10 shuffles instead of 48 for eight independent butterflies.
-__device__ __forceinline__ double wave_sum8_transposed(const double (&v)[8], int lane) {
-    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
-    double w[4], u[2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const double send = h5 ? v[i] : v[i + 4];
-        const double keep = h5 ? v[i + 4] : v[i];
-        w[i] = keep + __shfl_xor(send, 32);
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i) {
-        const double send = h4 ? w[i] : w[i + 2];
-        const double keep = h4 ? w[i + 2] : w[i];
-        u[i] = keep + __shfl_xor(send, 16);
-    }
-    double z = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8);
-    z += __shfl_xor(z, 4);
-    z += __shfl_xor(z, 2);
-    z += __shfl_xor(z, 1);
-    return z;
-}
+// 65k-point torus). With the transposed row reduction (wave_sum8_transposed)
+// the FP64 sweep over complex64 storage still streams at the HBM read rate.
 
 static_assert(kMvUnroll == 4, "the transposed row reduction packs 4 rows x (re, im)");
 
@@ -143,7 +117,7 @@ __global__ void __launch_bounds__(kMvThreads) k_sweep(const MT* __restrict__ A, 
     const int c0 = blockIdx.x * kMvCols;
     const int r0 = blockIdx.y * kMvRows;
     const int nrow = min(kMvRows, count - r0);
-    const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    const int my_val = wave_sum8_index(lane);
 
     double pre[kMvCpt], pim[kMvCpt], zre[kMvCpt], zim[kMvCpt];
     bool cok[kMvCpt];
@@ -212,8 +186,14 @@ __device__ __forceinline__ void reduce_partials(const double* __restrict__ part,
 struct VecArgs {
     double2 *x, *r, *rt, *p, *pt, *v, *w;
     const double2* b;
+    // Partial planes of the last sweep. Dense and matrix-free sweeps: v sums
+    // ncol_tiles planes of ypart, w sums nrow_tiles planes of zpart. Pair
+    // storage (pair = 1): ypart and zpart alike hold ncol_tiles row-side
+    // planes, nrow_tiles column-side planes and a near-field plane, of which
+    // element j sums those its tiles wrote (ibim_pair.hip).
     double2 *ypart, *zpart;
     int ncol_tiles, nrow_tiles;
+    int pair;
     size_t ld;
     double* part;  // per-block partials
     BicgScalars* sc;
@@ -221,6 +201,45 @@ struct VecArgs {
     int* done;
     int count;
 };
+
+__device__ __forceinline__ double2 sum_planes(const double2* __restrict__ part, int nplanes, size_t ld, int j) {
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < nplanes; ++c) {
+        const double2 y = part[(size_t)c * ld + j];
+        sr += y.x;
+        si += y.y;
+    }
+    return make_double2(sr, si);
+}
+
+__device__ __forceinline__ double2 sum_pair_planes(const double2* __restrict__ part, int ncol, int nrow, int count,
+                                                   size_t ld, int j) {
+    const int blk = j / kSweepRows;
+    const int nt = (count - blk * kSweepRows + kSweepCols - 1) / kSweepCols;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < nt; ++c) {
+        const double2 y = part[(size_t)c * ld + j];
+        sr += y.x;
+        si += y.y;
+    }
+    for (int c = 0; c <= blk; ++c) {
+        const double2 y = part[(size_t)(ncol + c) * ld + j];
+        sr += y.x;
+        si += y.y;
+    }
+    const double2 y = part[(size_t)(ncol + nrow) * ld + j];
+    return make_double2(sr + y.x, si + y.y);
+}
+
+// (A p)_j and (A^H q)_j of the last sweep.
+__device__ __forceinline__ double2 sweep_y(const VecArgs& a, int j) {
+    return a.pair ? sum_pair_planes(a.ypart, a.ncol_tiles, a.nrow_tiles, a.count, a.ld, j)
+                  : sum_planes(a.ypart, a.ncol_tiles, a.ld, j);
+}
+__device__ __forceinline__ double2 sweep_z(const VecArgs& a, int j) {
+    return a.pair ? sum_pair_planes(a.zpart, a.ncol_tiles, a.nrow_tiles, a.count, a.ld, j)
+                  : sum_planes(a.zpart, a.nrow_tiles, a.ld, j);
+}
 
 // x = 0, r = p = b, r~ = p~ = conj(b) (BICGSTAB.c:104-117); partials of
 // <r, r~> (4 sums), ||b||^2 (2 sums) and sum |b_re| + |b_im|.
@@ -275,19 +294,10 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_dot(VecArgs a) {
     __shared__ double sh[8 * 4];
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
-        double yr = 0.0, yi = 0.0, zr = 0.0, zi = 0.0;
-        for (int c = 0; c < a.ncol_tiles; ++c) {
-            double2 y = a.ypart[(size_t)c * a.ld + j];
-            yr += y.x;
-            yi += y.y;
-        }
-        for (int c = 0; c < a.nrow_tiles; ++c) {
-            double2 z = a.zpart[(size_t)c * a.ld + j];
-            zr += z.x;
-            zi += z.y;
-        }
-        a.v[j] = make_double2(yr, yi);
-        a.w[j] = make_double2(zr, zi);
+        const double2 y = sweep_y(a, j), z = sweep_z(a, j);
+        const double yr = y.x, yi = y.y;
+        a.v[j] = y;
+        a.w[j] = z;
         double2 r = a.r[j], rt = a.rt[j], pt = a.pt[j];
         v[0] += r.x * rt.x;
         v[1] += r.y * rt.y;
@@ -389,14 +399,9 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_replace(VecArgs a) {
     __shared__ double sh[2 * 4];
     double v[2] = {0.0, 0.0};
     for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
-        double yr = 0.0, yi = 0.0;
-        for (int c = 0; c < a.ncol_tiles; ++c) {
-            double2 y = a.ypart[(size_t)c * a.ld + j];
-            yr += y.x;
-            yi += y.y;
-        }
+        const double2 y = sweep_y(a, j);
         const double2 bb = a.b[j];
-        const double2 rr = make_double2(bb.x - yr, bb.y - yi);
+        const double2 rr = make_double2(bb.x - y.x, bb.y - y.y);
         a.r[j] = rr;
         v[0] += rr.x * rr.x;
         v[1] += rr.y * rr.y;
@@ -419,6 +424,7 @@ unsigned vec_blocks(int count) {
 static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
                      const int* done) {
     if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
+    if (ctx->mat_pair) return sweep_pair_launch(ctx, p, q, ypart, zpart, done);
     const int count = ctx->count;
     dim3 grid((unsigned)((count + kMvCols - 1) / kMvCols), (unsigned)((count + kMvRows - 1) / kMvRows));
     TimedScope ts(ctx, "matvec");
@@ -446,7 +452,9 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
     const int nct = (count + kMvCols - 1) / kMvCols, nrt = (count + kMvRows - 1) / kMvRows;
     const unsigned nvb = vec_blocks(count);
     const size_t vbytes = sizeof(double2) * ld;
-    if (ctx->work.ensure(vbytes * 7) || ctx->partial.ensure(sizeof(double2) * ld * (size_t)(nct + nrt)) ||
+    // pair storage: row-side, column-side and near planes for v and w alike
+    const size_t nplanes = ctx->mat_pair ? 2 * (size_t)(nct + nrt + 1) : (size_t)(nct + nrt);
+    if (ctx->work.ensure(vbytes * 7) || ctx->partial.ensure(sizeof(double2) * ld * nplanes) ||
         ctx->scal.ensure(sizeof(BicgScalars) * (size_t)(maxit + 1) + sizeof(double) * (size_t)nvb * 16 + 64) ||
         ctx->done.ensure(sizeof(int) * 4))
         return IBIM_ERR_NOMEM;
@@ -463,6 +471,7 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
     a.zpart = a.ypart + ld * (size_t)nct;
     a.ncol_tiles = nct;
     a.nrow_tiles = nrt;
+    a.pair = 0;
     a.ld = ld;
     a.sc = ctx->scal.as<BicgScalars>();
     double* tail = (double*)(a.sc + (maxit + 1));
@@ -473,6 +482,14 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
     a.count = count;
     ws->nvb = nvb;
     return IBIM_OK;
+}
+
+// The workspace's vector arguments with the partial-plane layout of operator op.
+VecArgs with_layout(const VecArgs& base, const Context* ctx, int op) {
+    VecArgs a = base;
+    a.pair = op == OP_DENSE && ctx->mat_pair ? 1 : 0;
+    a.zpart = a.ypart + a.ld * (size_t)(a.pair ? a.ncol_tiles + a.nrow_tiles + 1 : a.ncol_tiles);
+    return a;
 }
 
 // One BiCG solve of A x = b for b, x on the device (length count). Returns
@@ -490,7 +507,8 @@ struct Replace {
 };
 
 // r = b - A_true x into the BiCG workspace; returns ||r|| / ||b||.
-int true_residual(Context* ctx, const VecArgs& a, unsigned nvb, Replace* rep, double normb, double* err) {
+int true_residual(Context* ctx, const VecArgs& base, unsigned nvb, Replace* rep, double normb, double* err) {
+    const VecArgs a = with_layout(base, ctx, rep->true_op);
     int rc = sweep_any(ctx, rep->true_op, a.x, a.x, a.ypart, a.zpart, nullptr);
     if (rc) return rc;
     hipLaunchKernelGGL(k_bicg_replace, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
@@ -513,7 +531,7 @@ int true_residual(Context* ctx, const VecArgs& a, unsigned nvb, Replace* rep, do
 int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* x_out, int maxit, double tol,
                 int chunk, ibim_solve_info* info, double* hist, bool adjoint = false, Replace* rep = nullptr,
                 bool ref_guards = true) {
-    VecArgs a = ws.a;
+    VecArgs a = with_layout(ws.a, ctx, op);
     a.b = b;
     // The operator's adjoint system A^H x = b runs the same recurrences: one
     // sweep with (p~, p) in place of (p, p~) gives A p~ and A^H p, so the dot
@@ -522,8 +540,10 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     if (adjoint) {
         ad.ypart = a.zpart;
         ad.zpart = a.ypart;
-        ad.ncol_tiles = a.nrow_tiles;
-        ad.nrow_tiles = a.ncol_tiles;
+        if (!a.pair) {  // the pair layout is the same for v and w
+            ad.ncol_tiles = a.nrow_tiles;
+            ad.nrow_tiles = a.ncol_tiles;
+        }
     }
     const unsigned nvb = ws.nvb;
     hipLaunchKernelGGL(k_bicg_init, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
@@ -606,10 +626,10 @@ int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* 
     Workspace ws;
     int rc = prepare(ctx, 1, &ws);
     if (rc) return rc;
-    rc = sweep_any(ctx, op, (const double2*)p_dev, (const double2*)q_dev, ws.a.ypart, ws.a.zpart, nullptr);
+    VecArgs a = with_layout(ws.a, ctx, op);
+    rc = sweep_any(ctx, op, (const double2*)p_dev, (const double2*)q_dev, a.ypart, a.zpart, nullptr);
     if (rc) return rc;
     // sum the partial planes with the dot kernel (its dot products are unused)
-    VecArgs a = ws.a;
     a.v = (double2*)y_dev;
     a.w = (double2*)z_dev;
     a.r = (double2*)p_dev;

@@ -20,6 +20,7 @@
 
 #include "ibim_entries.hpp"
 #include "ibim_internal.hpp"
+#include "ibim_wave.hpp"
 
 namespace ibim {
 
@@ -150,23 +151,6 @@ __global__ void __launch_bounds__(kFT) k_sweep_free64(const double* __restrict__
 // FP32 operator sweep (traditional kernel). Column points keep FP64 positions
 // and FP32 normal / weight / near-field value in registers; row points are
 // staged in LDS the same way. Dirichlet / interior are compile-time switches.
-// Sum 8 per-lane values over the wavefront (transposed: halving exchanges
-// xor 32 / 16 / 8, then a butterfly over xor 4 / 2 / 1). Lane l ends with the
-// sum of value ((l >> 5) & 1) * 4 + ((l >> 4) & 1) * 2 + ((l >> 3) & 1).
-__device__ __forceinline__ double wsum8_transposed(const double (&v)[8], int lane) {
-    const bool h5 = lane & 32, h4 = lane & 16, h3 = lane & 8;
-    double w[4], u[2];
-#pragma unroll
-    for (int i = 0; i < 4; ++i) w[i] = (h5 ? v[i + 4] : v[i]) + __shfl_xor(h5 ? v[i] : v[i + 4], 32);
-#pragma unroll
-    for (int i = 0; i < 2; ++i) u[i] = (h4 ? w[i + 2] : w[i]) + __shfl_xor(h4 ? w[i] : w[i + 2], 16);
-    double z = (h3 ? u[1] : u[0]) + __shfl_xor(h3 ? u[0] : u[1], 8);
-    z += __shfl_xor(z, 4);
-    z += __shfl_xor(z, 2);
-    z += __shfl_xor(z, 1);
-    return z;
-}
-
 constexpr int kRowBatch = 4;  // rows per transposed reduction (4 rows x re, im)
 
 // The fast operator for the traditional tube, specialised on the
@@ -196,7 +180,7 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
     const int nrow = min(kFRows, count - r0);
     const size_t cnt = (size_t)count;
     const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau, k = prm.k;
-    const int my_val = ((lane >> 5) & 1) * 4 + ((lane >> 4) & 1) * 2 + ((lane >> 3) & 1);
+    const int my_val = wave_sum8_index(lane);
 
     ColPoint col[kFCpt];
     double pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
@@ -272,7 +256,7 @@ __global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__
                 rows[2 * u] = sre;
                 rows[2 * u + 1] = sim;
             }
-            const double tot = wsum8_transposed(rows, lane);
+            const double tot = wave_sum8_transposed(rows, lane);
             const int s = sb + (my_val >> 1);
             if ((lane & 7) == 0 && s < ns) ysh[my_val & 1][wave][s0 + s] = tot;
         }

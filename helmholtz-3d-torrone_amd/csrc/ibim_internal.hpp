@@ -73,10 +73,16 @@ struct Context {
     DevBuf node, tube, blockcnt, err_flag;
 
     // operator
-    int mat_precision = -1;  // IBIM_C128 / IBIM_C64, -1 when not assembled
+    int mat_precision = -1;  // IBIM_C128 / IBIM_C64 (element type), -1 when not assembled
+    bool mat_pair = false;   // pair storage (ibim_pair.hip) instead of dense rows
     size_t ld = 0;
     DevBuf mat;
     int regcount = 0;
+    // pair storage: tile table (row block, tile), first tile of each row block,
+    // point table, near-neighbour lists (offsets, indices, build cursors)
+    int pair_ntiles = 0;
+    long long near_pairs = 0;
+    DevBuf pair_tiles, pair_blk, pair_pts, near_off, near_idx, near_cur;
 
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;
@@ -140,6 +146,12 @@ int launch_gradient(Context* ctx, int order);
 int launch_curvatures(Context* ctx);
 int launch_tube(Context* ctx, int* count_out);
 int launch_assemble(Context* ctx, int precision);
+// Pair storage of the traditional operator (precision IBIM_C128 / IBIM_C64 as the element type).
+int launch_assemble_pair(Context* ctx, int precision);
+size_t pair_tile_count(int count);
+int sweep_pair_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done);
+// Dense rows of the pair operator into out_dev (nrows x count complex FP64).
+int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 int launch_rhs(Context* ctx, int kind, const double* x0, double amp, const double* dir, int col);
 int launch_eval(Context* ctx, int mode, int col, int nrecv, const double* xyz_dev, double* out_dev);
 int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, double* J, double* dlevel, double* dk);

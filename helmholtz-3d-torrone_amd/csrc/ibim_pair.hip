@@ -1,0 +1,530 @@
+// ibim_pair.hip -- the traditional operator stored once per unordered pair
+// (IBIM_C128_PAIR / IBIM_C64_PAIR).
+//
+// For rho_ij > tau the traditional entry (large3Dexample2.c:4463-4564) factors
+// into a part symmetric in (i, j) and two real point factors:
+//   A_ij = F_ij alpha_ij,   A_ji = F_ij beta_ij,   F_ij = sigma (c1 + i c2)(rho_ij)
+// with grad_x Phi = (c1 + i c2) r and sigma = -1 for the interior problem, +1
+// otherwise. With r_ij = x*_i - x*_j:
+//   Dirichlet (normal of the column point): alpha_ij = w_j r_ij.n_j = x_i.m_j - c_j,
+//     beta_ij = w_i r_ji.n_i = x_j.m_i - c_i, per point m = w n, c = x.m;
+//   Neumann (normal of the row point): alpha_ij = w_j r_ij.n_i = w_j (c_i - x_j.m_i),
+//     beta_ij = w_i r_ji.n_j = w_i (c_j - x_i.m_j), per point m = n, c = x.n.
+// Only F of the pairs i < j is stored, so a sweep reads half the bytes of the
+// dense operator and rebuilds alpha, beta from the points (FP64; the points
+// are wave-uniform for the row and in registers for the column). One pass over
+// F gives v = A p and w = A^H q:
+//   row side    v_i += A_ij p_j,        w_i += conj(A_ji) q_j
+//   column side v_j += A_ji p_i,        w_j += conj(A_ij) q_i
+// Near-field pairs (rho <= tau, the diagonal included) store F = 0; their
+// entries w_j reg_q (+1/2 on the diagonal) come from a per-point list of
+// near neighbours (sorted, so the sums have a fixed order) in a small kernel
+// that writes one more partial plane.
+//
+// Storage: row blocks of kSweepRows rows; row block I holds tiles t = 0 ..
+// T_I - 1 of kSweepRows x kSweepCols values covering columns
+// I kSweepRows + t kSweepCols .. + kSweepCols - 1 (the first kSweepRows of them
+// form the diagonal block, where F = 0 for j <= i). Tiles are stored one after
+// another, each row-major; values outside the operator are zero.
+//
+// Partial planes (complex FP64, leading dimension ld), for v in ypart and for
+// w in zpart alike: ncol = ceil(count / kSweepCols) row-side planes (tile t of
+// any row block writes plane t), nrow = ceil(count / kSweepRows) column-side
+// planes (tile (I, t) writes plane ncol + I), and the near-field plane
+// ncol + nrow. Element j of block J = j / kSweepRows sums row-side planes
+// t < T_J, column-side planes I <= J and the near plane, in that order.
+#include <cmath>
+#include <vector>
+
+#include "ibim_entries.hpp"
+#include "ibim_internal.hpp"
+#include "ibim_wave.hpp"
+
+namespace ibim {
+
+namespace {
+
+constexpr int kPT = 256;                 // threads per tile
+constexpr int kPCpt = kSweepCols / kPT;  // columns per thread
+constexpr int kPR = kSweepRows;
+constexpr int kPC = kSweepCols;
+constexpr int kPairAsmRows = 64;  // rows per assembly workgroup
+static_assert(kPCpt == 4, "the column loops below assume 4 columns per thread");
+static_assert(kPR % kPairAsmRows == 0, "assembly row chunks must tile a row block");
+
+// Point data of the pair sweep: position, m and c as defined above, and w.
+struct alignas(64) PairPoint {
+    double x, y, z, mx, my, mz, c, w;
+};
+
+template <class Store>
+__device__ __forceinline__ void store_pair(Store* dst, double re, double im);
+
+template <>
+__device__ __forceinline__ void store_pair<double2>(double2* dst, double re, double im) {
+    *dst = make_double2(re, im);
+}
+
+template <>
+__device__ __forceinline__ void store_pair<float2>(float2* dst, double re, double im) {
+    *dst = make_float2((float)re, (float)im);
+}
+
+// The near-field decision of trad_entry (the same FP64 expression, so the
+// pair operator and the dense one agree on which pairs are near).
+__device__ __forceinline__ double pair_rho(const PointData& a, const PointData& b) {
+    double d1 = a.sx - b.sx, d2 = a.sy - b.sy, d3 = a.sz - b.sz;
+    return sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+}
+
+// alpha_ij (factor of A_ij) and beta_ij (factor of A_ji) for row point R = i
+// and column point C = j.
+template <bool Dirichlet>
+__device__ __forceinline__ void pair_factors(const PairPoint& R, double cx, double cy, double cz, double cmx,
+                                             double cmy, double cmz, double cc, double cw, double* alpha,
+                                             double* beta) {
+    if (Dirichlet) {
+        *alpha = fma(R.x, cmx, fma(R.y, cmy, fma(R.z, cmz, -cc)));
+        *beta = fma(cx, R.mx, fma(cy, R.my, fma(cz, R.mz, -R.c)));
+    } else {
+        *alpha = cw * fma(-cx, R.mx, fma(-cy, R.my, fma(-cz, R.mz, R.c)));
+        *beta = R.w * fma(-R.x, cmx, fma(-R.y, cmy, fma(-R.z, cmz, cc)));
+    }
+}
+
+__global__ void __launch_bounds__(256) k_pair_points(const double* __restrict__ tube, int count, int dirichlet,
+                                                     PairPoint* __restrict__ pts, int npad) {
+    const int p = blockIdx.x * 256 + threadIdx.x;
+    if (p >= npad) return;
+    PairPoint o{};
+    if (p < count) {
+        const PointData d = load_point(tube, (size_t)count, p);
+        o.x = d.sx;
+        o.y = d.sy;
+        o.z = d.sz;
+        o.w = d.w;
+        if (dirichlet) {
+            o.mx = d.w * d.nx;
+            o.my = d.w * d.ny;
+            o.mz = d.w * d.nz;
+        } else {
+            o.mx = d.nx;
+            o.my = d.ny;
+            o.mz = d.nz;
+        }
+        o.c = d.sx * o.mx + d.sy * o.my + d.sz * o.mz;
+    }
+    pts[p] = o;
+}
+
+// F of one tile, kPairAsmRows rows per workgroup; counts near pairs per point.
+template <class Store>
+__global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict__ tube, int count,
+                                                       const int2* __restrict__ tiles, ibim_params prm,
+                                                       Store* __restrict__ F, int* __restrict__ ncnt) {
+    const int2 tl = tiles[blockIdx.x];
+    const int r0 = tl.x * kPR, c0 = r0 + tl.y * kPC;
+    const size_t cnt = (size_t)count;
+    const double sigma = prm.interior ? -1.0 : 1.0;
+    PointData b[kPCpt];
+#pragma unroll
+    for (int m = 0; m < kPCpt; ++m) {
+        const int j = c0 + threadIdx.x + kPT * m;
+        b[m] = j < count ? load_point(tube, cnt, j) : PointData{};
+    }
+    Store* Fb = F + (size_t)blockIdx.x * kPR * kPC + threadIdx.x;
+    const int rb = blockIdx.y * kPairAsmRows;
+    for (int r = rb; r < rb + kPairAsmRows; ++r) {
+        const int i = r0 + r;
+        const PointData a = i < count ? load_point(tube, cnt, i) : PointData{};
+#pragma unroll
+        for (int m = 0; m < kPCpt; ++m) {
+            const int j = c0 + threadIdx.x + kPT * m;
+            double fr = 0.0, fi = 0.0;
+            if (i < j && j < count) {
+                const double rho = pair_rho(a, b[m]);
+                if (rho > prm.tau) {
+                    double c1, c2;
+                    grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                    fr = sigma * c1;
+                    fi = sigma * c2;
+                } else {
+                    atomicAdd(&ncnt[i], 1);
+                    atomicAdd(&ncnt[j], 1);
+                }
+            }
+            store_pair<Store>(Fb + (size_t)r * kPC + kPT * m, fr, fi);
+        }
+    }
+}
+
+// The near-neighbour lists: j enters i's list and i enters j's for every near
+// pair i < j (order fixed afterwards by k_near_sort).
+__global__ void __launch_bounds__(kPT) k_near_fill(const double* __restrict__ tube, int count,
+                                                   const int2* __restrict__ tiles, double tau,
+                                                   const int* __restrict__ off, int* __restrict__ cur,
+                                                   int* __restrict__ idx) {
+    const int2 tl = tiles[blockIdx.x];
+    const int r0 = tl.x * kPR, c0 = r0 + tl.y * kPC;
+    const size_t cnt = (size_t)count;
+    PointData b[kPCpt];
+#pragma unroll
+    for (int m = 0; m < kPCpt; ++m) {
+        const int j = c0 + threadIdx.x + kPT * m;
+        b[m] = j < count ? load_point(tube, cnt, j) : PointData{};
+    }
+    const int rb = blockIdx.y * kPairAsmRows;
+    for (int r = rb; r < rb + kPairAsmRows; ++r) {
+        const int i = r0 + r;
+        if (i >= count) break;
+        const PointData a = load_point(tube, cnt, i);
+#pragma unroll
+        for (int m = 0; m < kPCpt; ++m) {
+            const int j = c0 + threadIdx.x + kPT * m;
+            if (i < j && j < count && !(pair_rho(a, b[m]) > tau)) {
+                // the counting pass made the same decisions; the bound only
+                // keeps the lists in their slots
+                const int si = atomicAdd(&cur[i], 1), sj = atomicAdd(&cur[j], 1);
+                if (si < off[i + 1] - off[i]) idx[off[i] + si] = j;
+                if (sj < off[j + 1] - off[j]) idx[off[j] + sj] = i;
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) k_near_sort(const int* __restrict__ off, int* __restrict__ idx, int count) {
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const int b = off[i], e = off[i + 1];
+    for (int s = b + 1; s < e; ++s) {
+        const int v = idx[s];
+        int t = s - 1;
+        while (t >= b && idx[t] > v) {
+            idx[t + 1] = idx[t];
+            --t;
+        }
+        idx[t + 1] = v;
+    }
+}
+
+// One tile of the pair sweep (see the file comment for the planes written).
+template <class MT, bool Dirichlet>
+__global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
+                                                    const PairPoint* __restrict__ pts, int count, size_t ld,
+                                                    int ncol, const double2* __restrict__ p,
+                                                    const double2* __restrict__ q, double2* __restrict__ ypart,
+                                                    double2* __restrict__ zpart, const int* __restrict__ done) {
+    if (done != nullptr && *done) return;
+    __shared__ double sh[4][kPT / 64][kPR];  // (v re, v im, w re, w im) x wave x row
+    const int2 tl = tiles[blockIdx.x];
+    const int r0 = tl.x * kPR, c0 = r0 + tl.y * kPC;
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int my_val = wave_sum8_index(lane);
+
+    double cx[kPCpt], cy[kPCpt], cz[kPCpt], cmx[kPCpt], cmy[kPCpt], cmz[kPCpt], cc[kPCpt], cw[kPCpt];
+    double pre[kPCpt], pim[kPCpt], qre[kPCpt], qim[kPCpt];
+    double vre[kPCpt], vim[kPCpt], wre[kPCpt], wim[kPCpt];
+#pragma unroll
+    for (int m = 0; m < kPCpt; ++m) {
+        const int j = c0 + t + kPT * m;  // < count + kPC: inside the padded point table
+        const PairPoint P = pts[j];
+        cx[m] = P.x;
+        cy[m] = P.y;
+        cz[m] = P.z;
+        cmx[m] = P.mx;
+        cmy[m] = P.my;
+        cmz[m] = P.mz;
+        cc[m] = P.c;
+        cw[m] = P.w;
+        // F = 0 outside the operator, so clamped loads contribute nothing
+        const int jc = j < count ? j : count - 1;
+        const double2 pv = p[jc], qv = q[jc];
+        pre[m] = pv.x;
+        pim[m] = pv.y;
+        qre[m] = qv.x;
+        qim[m] = qv.y;
+        vre[m] = vim[m] = wre[m] = wim[m] = 0.0;
+    }
+    const MT* Fb = F + (size_t)blockIdx.x * kPR * kPC + t;
+    // the next two rows' values are loaded while the current two are applied
+    MT nxt[2][kPCpt];
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+#pragma unroll
+        for (int m = 0; m < kPCpt; ++m) nxt[u][m] = Fb[(size_t)u * kPC + kPT * m];
+    for (int rr = 0; rr < kPR; rr += 2) {
+        MT a[2][kPCpt];
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int m = 0; m < kPCpt; ++m) a[u][m] = nxt[u][m];
+        if (rr + 2 < kPR) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int m = 0; m < kPCpt; ++m) nxt[u][m] = Fb[(size_t)(rr + 2 + u) * kPC + kPT * m];
+        }
+        double rows[8];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int i = r0 + rr + u;  // < count + kPR: inside the padded point table
+            const PairPoint R = pts[i];
+            const int ic = i < count ? i : count - 1;
+            const double2 pi = p[ic], qi = q[ic];
+            double svr = 0.0, svi = 0.0, swr = 0.0, swi = 0.0;
+#pragma unroll
+            for (int m = 0; m < kPCpt; ++m) {
+                double al, be;
+                pair_factors<Dirichlet>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m], cmz[m], cc[m], cw[m], &al, &be);
+                const double fr = (double)a[u][m].x, fi = (double)a[u][m].y;
+                const double ar = fr * al, ai = fi * al;  // A_ij
+                const double br = fr * be, bi = fi * be;  // A_ji
+                svr = fma(ar, pre[m], fma(-ai, pim[m], svr));  // A_ij p_j
+                svi = fma(ar, pim[m], fma(ai, pre[m], svi));
+                swr = fma(br, qre[m], fma(bi, qim[m], swr));  // conj(A_ji) q_j
+                swi = fma(br, qim[m], fma(-bi, qre[m], swi));
+                vre[m] = fma(br, pi.x, fma(-bi, pi.y, vre[m]));  // A_ji p_i
+                vim[m] = fma(br, pi.y, fma(bi, pi.x, vim[m]));
+                wre[m] = fma(ar, qi.x, fma(ai, qi.y, wre[m]));  // conj(A_ij) q_i
+                wim[m] = fma(ar, qi.y, fma(-ai, qi.x, wim[m]));
+            }
+            rows[4 * u] = svr;
+            rows[4 * u + 1] = svi;
+            rows[4 * u + 2] = swr;
+            rows[4 * u + 3] = swi;
+        }
+        const double tot = wave_sum8_transposed_x(rows, lane);
+        if ((lane & 7) == 0) sh[my_val & 3][wave][rr + (my_val >> 2)] = tot;
+    }
+    __syncthreads();
+    for (int row = t; row < kPR; row += kPT) {
+        const int i = r0 + row;
+        if (i >= count) break;
+        double s[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s[k] = (sh[k][0][row] + sh[k][1][row]) + (sh[k][2][row] + sh[k][3][row]);
+        ypart[(size_t)tl.y * ld + i] = make_double2(s[0], s[1]);
+        zpart[(size_t)tl.y * ld + i] = make_double2(s[2], s[3]);
+    }
+    const size_t cp = (size_t)(ncol + tl.x) * ld;
+#pragma unroll
+    for (int m = 0; m < kPCpt; ++m) {
+        const int j = c0 + t + kPT * m;
+        if (j < count) {
+            ypart[cp + j] = make_double2(vre[m], vim[m]);
+            zpart[cp + j] = make_double2(wre[m], wim[m]);
+        }
+    }
+}
+
+// Near-field entries (w_j reg_q, +1/2 on the diagonal; q = j for Dirichlet,
+// i for Neumann) applied to p and, conjugate-transposed (they are real), to q.
+template <bool Dirichlet>
+__global__ void __launch_bounds__(256) k_pair_near(const double* __restrict__ tube, int count,
+                                                   const int* __restrict__ off, const int* __restrict__ idx,
+                                                   const double2* __restrict__ p, const double2* __restrict__ q,
+                                                   double2* __restrict__ ynear, double2* __restrict__ znear,
+                                                   const int* __restrict__ done) {
+    if (done != nullptr && *done) return;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= count) return;
+    const double* W = tube + (size_t)IBIM_F_W * count;
+    const double* G = tube + (size_t)IBIM_F_REG * count;
+    const double wi = W[i], gi = G[i];
+    const double aii = wi * gi + 0.5;
+    const double2 pi = p[i], qi = q[i];
+    double yr = aii * pi.x, yi = aii * pi.y, zr = aii * qi.x, zi = aii * qi.y;
+    const int e = off[i + 1];
+    for (int s = off[i]; s < e; ++s) {
+        const int j = idx[s];
+        const double wj = W[j], gj = G[j];
+        const double aij = Dirichlet ? wj * gj : wj * gi;
+        const double aji = Dirichlet ? wi * gi : wi * gj;
+        const double2 pj = p[j], qj = q[j];
+        yr += aij * pj.x;
+        yi += aij * pj.y;
+        zr += aji * qj.x;
+        zi += aji * qj.y;
+    }
+    ynear[i] = make_double2(yr, yi);
+    znear[i] = make_double2(zr, zi);
+}
+
+// Dense rows row0 .. row0 + nrows - 1 of the pair operator (the sweep's
+// arithmetic), for inspection and tests.
+template <class MT, bool Dirichlet>
+__global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, const int* __restrict__ blk_start,
+                                                   const PairPoint* __restrict__ pts, const double* __restrict__ tube,
+                                                   int count, ibim_params prm, int row0, int nrows,
+                                                   double2* __restrict__ out) {
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    const int r = blockIdx.y;
+    if (j >= count || r >= nrows) return;
+    const int i = row0 + r;
+    const size_t cnt = (size_t)count;
+    double re, im;
+    const PointData a = load_point(tube, cnt, i), b = load_point(tube, cnt, j);
+    if (!(pair_rho(a, b) > prm.tau)) {
+        re = b.w * (Dirichlet ? b.reg : a.reg);
+        im = 0.0;
+        if (i == j) re += 0.5;
+    } else {
+        const int lo = i < j ? i : j, hi = i < j ? j : i;
+        const int I = lo / kPR;
+        const int t = (hi - I * kPR) / kPC;
+        const size_t tile = (size_t)blk_start[I] + t;
+        const MT f = F[tile * kPR * kPC + (size_t)(lo - I * kPR) * kPC + (hi - I * kPR - t * kPC)];
+        const PairPoint L = pts[lo], H = pts[hi];
+        double al, be;
+        pair_factors<Dirichlet>(L, H.x, H.y, H.z, H.mx, H.my, H.mz, H.c, H.w, &al, &be);
+        const double fac = i < j ? al : be;
+        re = (double)f.x * fac;
+        im = (double)f.y * fac;
+    }
+    out[(size_t)r * count + j] = make_double2(re, im);
+}
+
+template <class MT>
+int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles) {
+    const int count = ctx->count;
+    const unsigned nt = (unsigned)tiles.size();
+    const double* tube = ctx->tube.as<double>();
+    int* cnt = ctx->near_cur.as<int>();
+    IBIM_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * count, ctx->stream));
+    const int npad = count + kPC;
+    hipLaunchKernelGGL(k_pair_points, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, ctx->stream, tube, count,
+                       ctx->prm.dirichlet != 0, ctx->pair_pts.as<PairPoint>(), npad);
+    const dim3 grid(nt, kPR / kPairAsmRows);
+    {
+        TimedScope ts(ctx, "assemble");
+        hipLaunchKernelGGL(k_pair_assemble<MT>, grid, dim3(kPT), 0, ctx->stream, tube, count,
+                           ctx->pair_tiles.as<int2>(), ctx->prm, ctx->mat.as<MT>(), cnt);
+    }
+    IBIM_HIP(hipGetLastError());
+    // near-neighbour list offsets (exclusive scan on the host: count ints)
+    std::vector<int> h((size_t)count + 1, 0);
+    IBIM_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    long long run = 0;
+    for (int i = 0; i < count; ++i) {
+        const int c = h[i];
+        h[i] = (int)run;
+        run += c;
+    }
+    if (run > 0x7fffffffLL) {
+        set_error("ibim_assemble: %lld near-field pairs overflow the 32-bit lists", run);
+        return IBIM_ERR_SIZE;
+    }
+    h[count] = (int)run;
+    if (ctx->near_idx.ensure(sizeof(int) * (size_t)(run > 0 ? run : 1))) return IBIM_ERR_NOMEM;
+    IBIM_HIP(hipMemcpyAsync(ctx->near_off.ptr, h.data(), sizeof(int) * ((size_t)count + 1), hipMemcpyHostToDevice,
+                            ctx->stream));
+    IBIM_HIP(hipMemsetAsync(cnt, 0, sizeof(int) * count, ctx->stream));
+    hipLaunchKernelGGL(k_near_fill, grid, dim3(kPT), 0, ctx->stream, tube, count, ctx->pair_tiles.as<int2>(),
+                       ctx->prm.tau, ctx->near_off.as<int>(), cnt, ctx->near_idx.as<int>());
+    hipLaunchKernelGGL(k_near_sort, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->near_off.as<int>(), ctx->near_idx.as<int>(), count);
+    IBIM_HIP(hipGetLastError());
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->near_pairs = run;
+    return IBIM_OK;
+}
+
+}  // namespace
+
+size_t pair_tile_count(int count) {
+    size_t n = 0;
+    for (int I = 0; I * kPR < count; ++I) n += (size_t)((count - I * kPR + kPC - 1) / kPC);
+    return n;
+}
+
+int launch_assemble_pair(Context* ctx, int precision) {
+    if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
+        set_error("ibim_assemble: pair storage needs the traditional tube");
+        return IBIM_ERR_ARG;
+    }
+    const int count = ctx->count;
+    std::vector<int2> tiles;
+    std::vector<int> blk;
+    for (int I = 0; I * kPR < count; ++I) {
+        blk.push_back((int)tiles.size());
+        const int T = (count - I * kPR + kPC - 1) / kPC;
+        for (int t = 0; t < T; ++t) tiles.push_back(make_int2(I, t));
+    }
+    blk.push_back((int)tiles.size());
+    const size_t esz = precision == IBIM_C128 ? sizeof(double2) : sizeof(float2);
+    ctx->mat_precision = -1;
+    ctx->mat_pair = false;
+    ctx->ld = ((size_t)count + 63) / 64 * 64;
+    if (ctx->mat.ensure(esz * tiles.size() * kPR * kPC) || ctx->pair_tiles.ensure(sizeof(int2) * tiles.size()) ||
+        ctx->pair_blk.ensure(sizeof(int) * blk.size()) ||
+        ctx->pair_pts.ensure(sizeof(PairPoint) * ((size_t)count + kPC)) ||
+        ctx->near_off.ensure(sizeof(int) * ((size_t)count + 1)) || ctx->near_cur.ensure(sizeof(int) * count))
+        return IBIM_ERR_NOMEM;
+    IBIM_HIP(hipMemcpyAsync(ctx->pair_tiles.ptr, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    IBIM_HIP(hipMemcpyAsync(ctx->pair_blk.ptr, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    const int rc = precision == IBIM_C128 ? assemble_pair_as<double2>(ctx, tiles) : assemble_pair_as<float2>(ctx, tiles);
+    if (rc) return rc;
+    ctx->pair_ntiles = (int)tiles.size();
+    ctx->mat_precision = precision;
+    ctx->mat_pair = true;
+    ctx->regcount = 0;
+    return IBIM_OK;
+}
+
+int sweep_pair_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const int ncol = (count + kPC - 1) / kPC, nrow = (count + kPR - 1) / kPR;
+    auto P = (const double2*)p;
+    auto Q = (const double2*)q;
+    auto Y = (double2*)ypart;
+    auto Z = (double2*)zpart;
+    const bool dir = ctx->prm.dirichlet != 0;
+    const int2* tiles = ctx->pair_tiles.as<int2>();
+    const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
+    {
+        TimedScope ts(ctx, "matvec");
+        if (ctx->mat_precision == IBIM_C128) {
+            auto kern = dir ? k_pair_sweep<double2, true> : k_pair_sweep<double2, false>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
+                               ctx->mat.as<double2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done);
+        } else {
+            auto kern = dir ? k_pair_sweep<float2, true> : k_pair_sweep<float2, false>;
+            hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
+                               ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done);
+        }
+    }
+    {
+        TimedScope ts(ctx, "near");
+        const size_t np = (size_t)(ncol + nrow) * ld;
+        auto kern = dir ? k_pair_near<true> : k_pair_near<false>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tube.as<double>(), count, ctx->near_off.as<int>(),
+                           ctx->near_idx.as<int>(), P, Q, Y + np, Z + np, done);
+    }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev) {
+    const bool dir = ctx->prm.dirichlet != 0;
+    const dim3 grid((unsigned)((ctx->count + 255) / 256), (unsigned)nrows);
+    const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
+    const double* tube = ctx->tube.as<double>();
+    const int* blk = ctx->pair_blk.as<int>();
+    if (ctx->mat_precision == IBIM_C128) {
+        auto kern = dir ? k_pair_rows<double2, true> : k_pair_rows<double2, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, ctx->mat.as<double2>(), blk, pts, tube, ctx->count,
+                           ctx->prm, row0, nrows, out_dev);
+    } else {
+        auto kern = dir ? k_pair_rows<float2, true> : k_pair_rows<float2, false>;
+        hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, ctx->mat.as<float2>(), blk, pts, tube, ctx->count,
+                           ctx->prm, row0, nrows, out_dev);
+    }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+}  // namespace ibim

@@ -33,6 +33,8 @@ NF = len(FIELDS)
 
 TUBE_TRADITIONAL, TUBE_COMBO = 0, 1
 C128, C64 = 0, 1
+# pair storage of the traditional operator (half the bytes per sweep)
+C128_PAIR, C64_PAIR = 2, 3
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
 EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO = 0, 1, 2
 SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2

@@ -79,6 +79,13 @@ typedef struct ibim_params {
 
 #define IBIM_C128 0 /* complex FP64 storage, 16 B per entry */
 #define IBIM_C64 1  /* complex FP32 storage,  8 B per entry */
+/* Traditional tube only: the radial factor sigma (c1 + i c2)(rho_ij) of each
+ * unordered pair i < j stored once (complex FP64 / FP32), the entries
+ * A_ij = F_ij w_j r_ij.n_q rebuilt from the points in every sweep, near-field
+ * pairs (rho <= tau) in per-point neighbour lists: half the bytes of
+ * IBIM_C128 / IBIM_C64 per sweep. Not available to the batched solvers. */
+#define IBIM_C128_PAIR 2
+#define IBIM_C64_PAIR 3
 
 #define IBIM_RHS_POINT_SOURCE 0    /* Phi(x*, x0) = e^{ik|x*-x0|} / (4 pi |x*-x0|)        */
 #define IBIM_RHS_PLANE_NEUMANN 1   /* amp d/dn e^{ik dir.x} at x* (cal_partialu*)        */
@@ -164,9 +171,11 @@ const double *ibim_tube_device(ibim_ctx *ctx);
 
 /* ------------------------------------------------------------- operator --- */
 /* Dense count x count operator in HBM (traditional kernel for
- * IBIM_TUBE_TRADITIONAL, combined-field kernel for IBIM_TUBE_COMBO). */
+ * IBIM_TUBE_TRADITIONAL, combined-field kernel for IBIM_TUBE_COMBO), or the
+ * pair storage of the traditional kernel (IBIM_C128_PAIR / IBIM_C64_PAIR). */
 int ibim_assemble(ibim_ctx *ctx, int precision);
-/* Host copy of rows [row0, row0 + nrows) as split planes (row-major, count wide). */
+/* Host copy of rows [row0, row0 + nrows) as split planes (row-major, count wide);
+ * for pair storage the entries as a sweep rebuilds them. */
 int ibim_get_rows(ibim_ctx *ctx, int row0, int nrows, double *re, double *im);
 /* Upload a dense operator given as split planes (row-major, count x count). */
 int ibim_set_matrix_host(ibim_ctx *ctx, int count, const double *re, const double *im, int precision);

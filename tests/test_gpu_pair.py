@@ -1,0 +1,169 @@
+"""Pair storage of the traditional operator (IBIM_C128_PAIR / IBIM_C64_PAIR):
+the radial factor of each unordered pair stored once, entries rebuilt per sweep.
+
+Checked against the dense complex128 operator, whose entries are pinned to the
+reference (test_gpu_parity): rebuilt rows to 1e-13 of the largest entry
+(complex FP64 pairs; the factors w_j r.n are formed as x_i.m_j - c_j, a
+different rounding order) and 1e-7 (complex FP32 pairs); sweeps to 1e-13 in
+norm against the rebuilt rows; solutions, residual-replacement solves and the
+misfit gradient against the dense operator's.
+"""
+import numpy as np
+import pytest
+
+import shapes
+
+pytestmark = pytest.mark.gpu
+
+
+def relnorm(a, b):
+    return np.linalg.norm(np.asarray(a) - np.asarray(b)) / np.linalg.norm(np.asarray(b))
+
+
+def problem(gpu, n, k=4.0, dirichlet=1, interior=0):
+    pb = gpu.Problem()
+    pb.set_level(shapes.sphere(n))
+    pb.geometry(5)
+    h = 4.0 / n
+    # on the coarse grid a 3h tube would reach the sphere's centre; 1.5h keeps
+    # the count below one row block (512)
+    eps = (3.0 if n >= 16 else 1.5) * h
+    pb.tube(gpu.params(k, eps, 1.5 * h if n < 16 else 2 * h, dirichlet=dirichlet, interior=interior))
+    return pb
+
+
+def dense_rows(pb):
+    re, im = pb.rows(0, pb.count)
+    return re + 1j * im
+
+
+# n = 12: one row block (count < 512); n = 28: several row blocks and column tiles
+@pytest.mark.parametrize("n", [12, 28])
+@pytest.mark.parametrize("dirichlet,interior", [(1, 0), (0, 0), (1, 1), (0, 1)])
+def test_pair_rows_match_dense_operator(gpu, n, dirichlet, interior):
+    pb = problem(gpu, n, dirichlet=dirichlet, interior=interior)
+    pb.assemble(gpu.C128)
+    A = dense_rows(pb)
+    scale = np.abs(A).max()
+    pb.assemble(gpu.C128_PAIR)
+    Ap = dense_rows(pb)
+    assert np.abs(Ap - A).max() <= 1e-13 * scale
+    pb.assemble(gpu.C64_PAIR)
+    A32 = dense_rows(pb)
+    assert np.abs(A32 - A).max() <= 1e-7 * scale
+    assert relnorm(A32, A) <= 1e-7
+
+
+@pytest.mark.parametrize("n", [12, 28])
+@pytest.mark.parametrize("dirichlet", [1, 0])
+@pytest.mark.parametrize("precision", ["C128_PAIR", "C64_PAIR"])
+def test_pair_sweep_matches_rebuilt_rows(gpu, n, dirichlet, precision):
+    pb = problem(gpu, n, dirichlet=dirichlet)
+    pb.assemble(getattr(gpu, precision))
+    A = dense_rows(pb)
+    rng = np.random.default_rng(n + 10 * dirichlet)
+    p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    q = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    y, z = pb.apply(p, q, gpu.OP_STORED)
+    assert relnorm(y, A @ p) <= 1e-13
+    assert relnorm(z, A.conj().T @ q) <= 1e-13
+    # repeated sweeps are bitwise identical (fixed-order partial planes)
+    y2, z2 = pb.apply(p, q, gpu.OP_STORED)
+    assert np.array_equal(y, y2) and np.array_equal(z, z2)
+
+
+def test_pair_bicg_matches_dense_solution(gpu):
+    pb = problem(gpu, 28, k=5.0)
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3])
+    pb.assemble(gpu.C128)
+    info_d, hist_d = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12)
+    xd = pb.density()
+    pb.assemble(gpu.C128_PAIR)
+    info_p, hist_p = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12)
+    xp = pb.density()
+    assert info_d.status == 0 and info_p.status == 0
+    assert abs(info_p.iterations - info_d.iterations) <= 2
+    assert relnorm(xp[0] + 1j * xp[1], xd[0] + 1j * xd[1]) <= 1e-10
+    np.testing.assert_allclose(hist_p[:10], hist_d[:10], rtol=1e-8)
+    pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12)
+    xp2 = pb.density()
+    assert np.array_equal(xp[0], xp2[0]) and np.array_equal(xp[1], xp2[1])
+
+
+def test_pair_mixed_solvers_reach_the_fp64_solution(gpu):
+    pb = problem(gpu, 28, k=6.0)
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3])
+    pb.assemble(gpu.C128)
+    info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-12)
+    x64 = pb.density()
+    x64 = x64[0] + 1j * x64[1]
+    pb.assemble(gpu.C64_PAIR)  # inner: FP32 pairs; replacements / corrections: FP64 matrix-free
+    info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=2000, tol=1e-10, inner_tol=1e-3, chunk=1)
+    x = pb.density()
+    assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 1
+    assert info.iterations <= 1.5 * info64.iterations
+    assert relnorm(x[0] + 1j * x[1], x64) <= 1e-8
+    info, _ = pb.solve(gpu.SOLVE_REFINED, maxit=2000, tol=1e-10, inner_tol=1e-4, max_outer=10)
+    x = pb.density()
+    assert info.status == 0 and info.rel_residual < 1e-10
+    assert relnorm(x[0] + 1j * x[1], x64) <= 1e-8
+
+
+def test_pair_several_right_hand_sides_solve_column_by_column(gpu):
+    pb = problem(gpu, 28)
+    pb.set_nrhs(3)
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3], col=0)
+    pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=[0.0, 0.6, 0.8], col=2)  # column 1 stays zero
+    pb.assemble(gpu.C128)
+    pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, batch=1)
+    ref = [pb.density(c) for c in range(3)]
+    pb.assemble(gpu.C128_PAIR)
+    info, _ = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, batch=0)  # pair storage: column by column
+    assert info.status == 0
+    for c in range(3):
+        x = pb.density(c)
+        if c == 1:
+            assert not x[0].any() and not x[1].any()
+        else:
+            assert relnorm(x[0] + 1j * x[1], ref[c][0] + 1j * ref[c][1]) <= 1e-10
+
+
+def test_pair_gradient_matches_dense_gradient(gpu):
+    n, k = 24, 3.0
+    h = 4.0 / n
+    center = (0.0137, -0.0219, 0.0071)
+    star = lambda b: gpu.shape(gpu.SHAPE_STAR, a=1.0, b=b, m=3, n=2, variant=0, center=center)
+    v = np.random.default_rng(0).standard_normal((40, 3))
+    recv = 2.5 * v / np.linalg.norm(v, axis=1, keepdims=True)
+    pb = gpu.Problem()
+    pb.set_level_shape(star(0.15), n)
+    pb.geometry(5)
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=1))
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=(0.1, 0.2, 0.3))
+    target = np.zeros(len(recv), complex) + 0.01
+    out = {}
+    for prec in ("C128", "C128_PAIR"):
+        pb.assemble(getattr(gpu, prec))
+        info, _ = pb.solve(tol=1e-13, maxit=2000)
+        assert info.status == 0
+        pb.set_receivers(recv, target)
+        out[prec] = pb.gradient(mode=gpu.EVAL_DOUBLE, tol=1e-13, maxit=2000)
+    J0, g0, dk0 = out["C128"]
+    J1, g1, dk1 = out["C128_PAIR"]
+    assert abs(J1 - J0) <= 1e-10 * J0
+    assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max()
+    assert abs(dk1 - dk0) <= 1e-8 * abs(dk0)
+
+
+def test_pair_storage_rejects_what_it_cannot_do(gpu):
+    pb = gpu.Problem().set_level(shapes.sphere(16)).geometry(5)
+    h = 4.0 / 16
+    pb.tube(gpu.params(2.0, 3 * h, 2 * h, dirichlet=1), gpu.TUBE_COMBO)
+    with pytest.raises(gpu.IbimError):
+        pb.assemble(gpu.C128_PAIR)  # the combined-field kernel has no pair form
+    pb2 = problem(gpu, 16)
+    pb2.assemble(gpu.C64_PAIR)
+    with pytest.raises(gpu.IbimError):
+        pb2.apply_batch(np.zeros((pb2.count, 2)), gpu.OP_STORED)
+    with pytest.raises(gpu.IbimError):
+        pb2.assemble(7)
