@@ -6,12 +6,13 @@ HBM: WENO gradient + curvatures, tube extraction / projection / weights, the
 right-hand side, the solve, and the field at 500 exterior receivers.
 
 Two paths are timed on the same workload:
-  mixed  complex BiCG on the FP32 operator -- stored as complex64 in HBM while
-         N^2 * 8 B fits, else FP32 entries recomputed per sweep with an FP64
-         range-reduced phase -- with FP64 residual replacement by the FP64
-         operator (method IBIM_SOLVE_RELIABLE);
-  fp64   the plain FP64 GPU path: dense complex128 operator assembled in HBM,
-         the reference's complex BiCG.
+  mixed  complex BiCG on the FP32 operator -- its radial factors stored once per
+         unordered pair as complex64 (IBIM_C64_PAIR) while that fits in HBM,
+         else FP32 entries recomputed per sweep with an FP64 range-reduced
+         phase -- with FP64 residual replacement by the FP64 operator (method
+         IBIM_SOLVE_RELIABLE);
+  fp64   the FP64 GPU path: the operator's radial factors stored once per pair
+         as complex128 (IBIM_C128_PAIR), the reference's complex BiCG.
 Both run with one host check per iteration (chunk 1), so every sweep launch in
 the timed region does a full pass.
 `value` is the mixed path's solves/s; both paths' densities are checked against
@@ -55,7 +56,15 @@ TOL = 1e-8          # default relative residual target of both timed paths
 REF_TOL = 1e-11     # the FP64 solve the densities are compared against
 REPLACE = 1e-3      # residual drop between FP64 residual replacements
 NRECV = 500
-C64_MAX_BYTES = 120e9  # store the complex64 operator up to this size
+STORE_MAX_BYTES = 200e9  # store an operator in HBM up to this size (288 GB per GPU)
+PAIR_ROWS, PAIR_COLS = 512, 1024  # tile shape of the pair storage
+
+
+def pair_bytes(count, esz):
+    """HBM bytes of the pair storage: row blocks of 512 rows, tiles of 512 x 1024
+    values from the diagonal block to the last column."""
+    tiles = sum((count - b * PAIR_ROWS + PAIR_COLS - 1) // PAIR_COLS for b in range((count + PAIR_ROWS - 1) // PAIR_ROWS))
+    return tiles * PAIR_ROWS * PAIR_COLS * esz
 
 
 def level_of(w):
@@ -74,16 +83,16 @@ def params_of(w):
 
 
 def mixed_stores(count):
-    return float(count) * count * 8 <= C64_MAX_BYTES
+    return pair_bytes(count, 8) <= STORE_MAX_BYTES
 
 
 def step(pb, w, recv, mixed):
     pb.geometry(5)
     count = pb.tube(params_of(w), ibim.TUBE_TRADITIONAL)
     if not mixed:
-        pb.assemble(ibim.C128)
+        pb.assemble(ibim.C128_PAIR if pair_bytes(count, 16) <= STORE_MAX_BYTES else ibim.C128)
     elif mixed_stores(count):
-        pb.assemble(ibim.C64)
+        pb.assemble(ibim.C64_PAIR)
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if mixed:
         info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"], inner_tol=REPLACE, chunk=1)
@@ -262,7 +271,7 @@ def run_adjoint(args):
         pb.set_level(level)
         pb.geometry(5)
         pb.tube(ibim.params(kk, 2 * h, 2 * h, dirichlet=1))
-        pb.assemble(ibim.C128)
+        pb.assemble(ibim.C128_PAIR)
         pb.rhs(ibim.RHS_POINT_SOURCE, x0=x0)
 
     def field(level, kk, tol_refined=True):
@@ -446,17 +455,23 @@ def main():
     exact = shapes.point_source(recv, w["x0"], w["k"])
     print("[bench] workload %s: uploading level set" % args.workload, file=sys.stderr, flush=True)
 
-    # FP64 reference density (untimed): dense operator, BiCG to REF_TOL
+    # FP64 reference density (untimed): the dense complex128 operator (the
+    # reference's entries), BiCG to REF_TOL; its sweep time is reported beside
     p64 = make_problem(level, w)
     p64.geometry(5)
     count = p64.tube(params_of(w), ibim.TUBE_TRADITIONAL)
     p64.assemble(ibim.C128)
     p64.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
+    p64.set_profiling(True)
     info_ref, _ = p64.solve(ibim.SOLVE_BICG, maxit=4000, tol=REF_TOL)
+    p64.sync()
+    ms_dense, n_dense = p64.kernel_time("matvec")
+    p64.set_profiling(False)
+    p64.reset_timers()
     xr = p64.density()
     x_ref = xr[0] + 1j * xr[1]
-    print("[bench] count=%d reference BiCG: %d iterations, status %d" % (count, info_ref.iterations, info_ref.status),
-          file=sys.stderr, flush=True)
+    print("[bench] count=%d reference BiCG (dense complex128): %d iterations, status %d" %
+          (count, info_ref.iterations, info_ref.status), file=sys.stderr, flush=True)
 
     t64, info64, x64, u64 = timed(p64, w, recv, False, args.steps, args.warmup)
     ms_mv, n_mv = p64.kernel_time("matvec")
@@ -480,24 +495,27 @@ def main():
     field_err = rel(um[0] + 1j * um[1], exact)
 
     n2 = float(count) * count
+    npairs = float(count) * (count - 1) / 2
     vec_bytes = 4 * 16 * count  # p, p~ in; A p, A^H p~ out (complex128)
     avg_dom = ms_dom / max(n_dom, 1) * 1e-3
     if stored:
-        # dominant kernel: the fused sweep over the complex64 operator (HBM-bound)
-        dom_bytes = n2 * 8 + vec_bytes
+        # dominant kernel: the fused pair sweep over the complex64 radial factors
+        dom_bytes = npairs * 8 + vec_bytes
         achieved = dom_bytes / avg_dom / 1e9 if n_dom else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_sweep<float2>", args.workload),
-                "kernel": "k_sweep<float2>: 8 B/entry x %d^2 entries + 64 B/point vectors per launch" % count}
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_pair_sweep<float2>", args.workload),
+                "kernel": "k_pair_sweep<float2>: 8 B/pair x %d*%d/2 pairs + 64 B/point vectors per launch" %
+                          (count, count - 1)}
     else:
         flops = n2 * (C_EVAL + 8 * 2)
         achieved = flops / avg_dom / 1e12 if n_dom else 0.0
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
                 "kernel": "k_sweep_free32: %d flops/entry x %d^2 entries per launch" % (C_EVAL + 16, count)}
-    # the FP64 path's dense fused sweep (HBM-bound): one pass over the c128 operator
+    # the FP64 path's fused sweep (HBM-bound): one pass over the complex128 pairs
     avg_mv = ms_mv / max(n_mv, 1) * 1e-3
-    mv_gbs = (n2 * 16 + vec_bytes) / avg_mv / 1e9 if n_mv else 0.0
+    mv_gbs = (npairs * 16 + vec_bytes) / avg_mv / 1e9 if n_mv else 0.0
+    avg_dense = ms_dense / max(n_dense, 1) * 1e-3
 
     line = {
         "metric": "solves/s + matvec GB/s (%peak), mixed-prec vs FP64 GPU at rel err <=1e-6",
@@ -510,18 +528,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 operator (fp64 phase, fp64 sums) + fp64 residual replacement",
+        "dtype": "complex64 radial factors per pair (fp64 point factors, products and sums) + fp64 residual "
+                 "replacement",
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
         "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g" % (
             args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"]),
             "tube_points": count, "parallelism": "single GPU",
-            "mixed_operator": "stored complex64" if stored else "recomputed FP32 entries"},
+            "mixed_operator": "complex64 pair storage" if stored else "recomputed FP32 entries",
+            "fp64_operator": "complex128 pair storage" if pair_bytes(count, 16) <= STORE_MAX_BYTES else "dense complex128"},
         "mixed": {"solves_per_s": 1.0 / tm, "iterations": infom.iterations, "replacements": infom.outer,
                   "rel_err_vs_fp64": err_mixed, "field_rel_err_vs_analytic": field_err,
                   "sweep_ms": avg_dom * 1e3, "replacement_sweep_ms": ms_rep / max(n_rep, 1)},
         "fp64": {"solves_per_s": 1.0 / t64, "iterations": info64.iterations, "rel_err_vs_fp64": err_fp64,
                  "assemble_ms": ms_asm / max(n_asm, 1), "matvec_ms": avg_mv * 1e3,
-                 "matvec_GBs": mv_gbs, "matvec_frac_of_peak": mv_gbs / HBM_PEAK_GBS},
+                 "matvec_GBs": mv_gbs, "matvec_frac_of_peak": mv_gbs / HBM_PEAK_GBS,
+                 "matvec_traffic": measured_traffic("k_pair_sweep<double2>", args.workload)},
+        "dense_c128_reference": {"iterations": info_ref.iterations, "tol": REF_TOL, "sweep_ms": avg_dense * 1e3,
+                                 "sweep_GBs": (n2 * 16 + vec_bytes) / avg_dense / 1e9 if n_dense else 0.0},
         "speedup_mixed_vs_fp64": t64 / tm,
         "roofline": roof,
     }

@@ -67,7 +67,7 @@ __device__ __forceinline__ void sweep_rows(const MT* __restrict__ Ab, size_t ld,
 #pragma unroll
             for (int m = 0; m < kMvCpt; ++m) {
                 if (!Guard || (rr + u < nrow && cok[m])) {
-                    a[u][m] = Ab[(size_t)(rr + u) * ld + kMvThreads * m];
+                    a[u][m] = load_stream(Ab + (size_t)(rr + u) * ld + kMvThreads * m);
                 } else {
                     a[u][m].x = 0;
                     a[u][m].y = 0;

@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
 #pragma unroll
     for (int u = 0; u < 2; ++u)
 #pragma unroll
-        for (int m = 0; m < kPCpt; ++m) nxt[u][m] = Fb[(size_t)u * kPC + kPT * m];
+        for (int m = 0; m < kPCpt; ++m) nxt[u][m] = load_stream(Fb + (size_t)u * kPC + kPT * m);
     for (int rr = 0; rr < kPR; rr += 2) {
         MT a[2][kPCpt];
 #pragma unroll
@@ -262,7 +262,7 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
-                for (int m = 0; m < kPCpt; ++m) nxt[u][m] = Fb[(size_t)(rr + 2 + u) * kPC + kPT * m];
+                for (int m = 0; m < kPCpt; ++m) nxt[u][m] = load_stream(Fb + (size_t)(rr + 2 + u) * kPC + kPT * m);
         }
         double rows[8];
 #pragma unroll
