@@ -503,7 +503,7 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
     IBIM_HIP(hipGetLastError());
 
     // adjoint solve A^H lambda = E^H e
-    const int op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    const int op = fp64_operator(ctx);
     ibim_solve_info info{};
     rc = bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true, false);
     if (rc) return rc;

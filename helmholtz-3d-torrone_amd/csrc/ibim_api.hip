@@ -146,7 +146,7 @@ void ibim_destroy(ibim_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     collect_timers(ctx);
     for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
-                      &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
+                      &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->mat64, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
                       &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch,
                       &ctx->pair_tiles, &ctx->pair_blk, &ctx->pair_pts, &ctx->near_off, &ctx->near_idx,
                       &ctx->near_cur})
@@ -176,7 +176,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->weno_order = 0;
     ctx->count = 0;
     ctx->mat_precision = -1;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     return IBIM_OK;
 }
 
@@ -274,7 +274,7 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     ctx->count = 0;  // a failed build leaves no stale tube behind
     int c = 0;
     int rc = launch_tube(ctx, &c);
@@ -301,12 +301,13 @@ const double* ibim_tube_device(ibim_ctx* ctx) { return ctx ? ctx->tube.as<double
 
 int ibim_assemble(ibim_ctx* ctx, int precision) {
     CHECK(ctx);
-    if (!ctx->count || precision < IBIM_C128 || precision > IBIM_C64_PAIR) {
+    if (!ctx->count || precision < IBIM_C128 || precision > IBIM_MIXED_PAIR) {
         set_error("ibim_assemble: needs a tube and a precision");
         return IBIM_ERR_ARG;
     }
-    if (precision == IBIM_C128_PAIR) return launch_assemble_pair(ctx, IBIM_C128);
-    if (precision == IBIM_C64_PAIR) return launch_assemble_pair(ctx, IBIM_C64);
+    if (precision == IBIM_C128_PAIR) return launch_assemble_pair(ctx, IBIM_C128, false);
+    if (precision == IBIM_C64_PAIR) return launch_assemble_pair(ctx, IBIM_C64, false);
+    if (precision == IBIM_MIXED_PAIR) return launch_assemble_pair(ctx, IBIM_C64, true);
     return launch_assemble(ctx, precision);
 }
 
@@ -392,7 +393,7 @@ int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const doubl
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
     (void)hipHostFree(stage);
     ctx->mat_precision = precision;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
 }
 

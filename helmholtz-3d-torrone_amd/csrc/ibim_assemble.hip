@@ -94,7 +94,7 @@ int launch_assemble(Context* ctx, int precision) {
     if (ctx->mat.ensure(esz * ctx->ld * (size_t)ctx->count)) return IBIM_ERR_NOMEM;
     if (ctx->err_flag.ensure(sizeof(int))) return IBIM_ERR_NOMEM;
     ctx->mat_precision = precision;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     return precision == IBIM_C128 ? assemble_as<double2>(ctx) : assemble_as<float2>(ctx);
 }
 

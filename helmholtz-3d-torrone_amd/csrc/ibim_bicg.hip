@@ -423,8 +423,9 @@ unsigned vec_blocks(int count) {
 // One sweep y = A p, z = A^H q of operator `op` into the partial planes.
 static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
                      const int* done) {
+    if (op == OP_PAIR64) return sweep_pair_launch(ctx, true, p, q, ypart, zpart, done);
     if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
-    if (ctx->mat_pair) return sweep_pair_launch(ctx, p, q, ypart, zpart, done);
+    if (ctx->mat_pair) return sweep_pair_launch(ctx, false, p, q, ypart, zpart, done);
     const int count = ctx->count;
     dim3 grid((unsigned)((count + kMvCols - 1) / kMvCols), (unsigned)((count + kMvRows - 1) / kMvRows));
     TimedScope ts(ctx, "matvec");
@@ -487,7 +488,7 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
 // The workspace's vector arguments with the partial-plane layout of operator op.
 VecArgs with_layout(const VecArgs& base, const Context* ctx, int op) {
     VecArgs a = base;
-    a.pair = op == OP_DENSE && ctx->mat_pair ? 1 : 0;
+    a.pair = (op == OP_DENSE && ctx->mat_pair) || op == OP_PAIR64 ? 1 : 0;
     a.zpart = a.ypart + a.ld * (size_t)(a.pair ? a.ncol_tiles + a.nrow_tiles + 1 : a.ncol_tiles);
     return a;
 }
@@ -659,7 +660,7 @@ int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxi
 // FP64 operator (stored complex-FP64, else recomputed FP64 entries).
 int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
     const int op = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
-    const int true_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    const int true_op = fp64_operator(ctx);
     Workspace ws;
     int rc = prepare(ctx, opts->maxit, &ws);
     if (rc) return rc;

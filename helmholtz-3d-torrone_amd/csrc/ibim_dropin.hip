@@ -138,7 +138,7 @@ int assemble_fd(ibim_ctx* ctx, double sign, double diag, double thr2, int const_
                        ctx->mat.as<double2>(), ctx->ld);
     IBIM_HIP(hipGetLastError());
     ctx->mat_precision = IBIM_C128;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     return IBIM_OK;
 }
 

@@ -336,7 +336,7 @@ int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     const int count = ctx->count;
     const size_t ld = ctx->ld;
     const int inner = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
-    const int outer_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    const int outer_op = fp64_operator(ctx);
     if (inner == OP_DENSE && outer_op == OP_DENSE) return IBIM_ERR_ARG;  // unreachable: one precision stored
     if (ctx->refine.ensure(sizeof(double2) * ld * 4 + sizeof(double) * 1024)) return IBIM_ERR_NOMEM;
     double2* r = ctx->refine.as<double2>();

@@ -75,8 +75,9 @@ struct Context {
     // operator
     int mat_precision = -1;  // IBIM_C128 / IBIM_C64 (element type), -1 when not assembled
     bool mat_pair = false;   // pair storage (ibim_pair.hip) instead of dense rows
+    bool mat_pair64 = false; // complex64 pairs with a complex128 copy in mat64 (IBIM_MIXED_PAIR)
     size_t ld = 0;
-    DevBuf mat;
+    DevBuf mat, mat64;
     int regcount = 0;
     // pair storage: tile table (row block, tile), first tile of each row block,
     // point table, near-neighbour lists (offsets, indices, build cursors)
@@ -147,9 +148,12 @@ int launch_curvatures(Context* ctx);
 int launch_tube(Context* ctx, int* count_out);
 int launch_assemble(Context* ctx, int precision);
 // Pair storage of the traditional operator (precision IBIM_C128 / IBIM_C64 as the element type).
-int launch_assemble_pair(Context* ctx, int precision);
+// with64: keep a complex128 copy of the factors (the FP64 operator of the mixed solvers).
+int launch_assemble_pair(Context* ctx, int precision, bool with64);
 size_t pair_tile_count(int count);
-int sweep_pair_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done);
+// fp64_copy: sweep the complex128 copy (OP_PAIR64) instead of the primary factors.
+int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
+                      const int* done);
 // Dense rows of the pair operator into out_dev (nrows x count complex FP64).
 int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 int launch_rhs(Context* ctx, int kind, const double* x0, double amp, const double* dir, int col);
@@ -167,7 +171,14 @@ int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double*
 int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
 
 // Operator used by a sweep: the stored matrix, or entries recomputed on the fly.
-enum OpKind { OP_DENSE = 0, OP_FREE64 = 1, OP_FREE32 = 2 };
+// OP_PAIR64: the complex128 copy kept beside complex64 pair storage.
+enum OpKind { OP_DENSE = 0, OP_FREE64 = 1, OP_FREE32 = 2, OP_PAIR64 = 3 };
+// The FP64 operator the mixed solvers check against: the stored complex128
+// operator, the complex128 copy of mixed pair storage, else recomputed entries.
+inline int fp64_operator(const Context* ctx) {
+    if (ctx->mat_pair64) return OP_PAIR64;
+    return ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+}
 // Tile sizes shared by every sweep (the partial planes have these shapes).
 constexpr int kSweepCols = 1024;
 constexpr int kSweepRows = 512;

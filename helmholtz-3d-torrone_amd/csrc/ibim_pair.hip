@@ -117,11 +117,13 @@ __global__ void __launch_bounds__(256) k_pair_points(const double* __restrict__ 
     pts[p] = o;
 }
 
-// F of one tile, kPairAsmRows rows per workgroup; counts near pairs per point.
+// F of one tile, kPairAsmRows rows per workgroup (and, for the mixed storage,
+// the complex128 copy in F64); counts near pairs per point.
 template <class Store>
 __global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict__ tube, int count,
                                                        const int2* __restrict__ tiles, ibim_params prm,
-                                                       Store* __restrict__ F, int* __restrict__ ncnt) {
+                                                       Store* __restrict__ F, double2* __restrict__ F64,
+                                                       int* __restrict__ ncnt) {
     const int2 tl = tiles[blockIdx.x];
     const int r0 = tl.x * kPR, c0 = r0 + tl.y * kPC;
     const size_t cnt = (size_t)count;
@@ -132,7 +134,8 @@ __global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict_
         const int j = c0 + threadIdx.x + kPT * m;
         b[m] = j < count ? load_point(tube, cnt, j) : PointData{};
     }
-    Store* Fb = F + (size_t)blockIdx.x * kPR * kPC + threadIdx.x;
+    const size_t tb = (size_t)blockIdx.x * kPR * kPC + threadIdx.x;
+    Store* Fb = F + tb;
     const int rb = blockIdx.y * kPairAsmRows;
     for (int r = rb; r < rb + kPairAsmRows; ++r) {
         const int i = r0 + r;
@@ -154,6 +157,7 @@ __global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict_
                 }
             }
             store_pair<Store>(Fb + (size_t)r * kPC + kPT * m, fr, fi);
+            if (F64 != nullptr) F64[tb + (size_t)r * kPC + kPT * m] = make_double2(fr, fi);
         }
     }
 }
@@ -385,7 +389,7 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
 }
 
 template <class MT>
-int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles) {
+int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) {
     const int count = ctx->count;
     const unsigned nt = (unsigned)tiles.size();
     const double* tube = ctx->tube.as<double>();
@@ -398,7 +402,8 @@ int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles) {
     {
         TimedScope ts(ctx, "assemble");
         hipLaunchKernelGGL(k_pair_assemble<MT>, grid, dim3(kPT), 0, ctx->stream, tube, count,
-                           ctx->pair_tiles.as<int2>(), ctx->prm, ctx->mat.as<MT>(), cnt);
+                           ctx->pair_tiles.as<int2>(), ctx->prm, ctx->mat.as<MT>(),
+                           with64 ? ctx->mat64.as<double2>() : nullptr, cnt);
     }
     IBIM_HIP(hipGetLastError());
     // near-neighbour list offsets (exclusive scan on the host: count ints)
@@ -438,7 +443,7 @@ size_t pair_tile_count(int count) {
     return n;
 }
 
-int launch_assemble_pair(Context* ctx, int precision) {
+int launch_assemble_pair(Context* ctx, int precision, bool with64) {
     if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
         set_error("ibim_assemble: pair storage needs the traditional tube");
         return IBIM_ERR_ARG;
@@ -454,9 +459,11 @@ int launch_assemble_pair(Context* ctx, int precision) {
     blk.push_back((int)tiles.size());
     const size_t esz = precision == IBIM_C128 ? sizeof(double2) : sizeof(float2);
     ctx->mat_precision = -1;
-    ctx->mat_pair = false;
+    ctx->mat_pair = ctx->mat_pair64 = false;
     ctx->ld = ((size_t)count + 63) / 64 * 64;
-    if (ctx->mat.ensure(esz * tiles.size() * kPR * kPC) || ctx->pair_tiles.ensure(sizeof(int2) * tiles.size()) ||
+    if (ctx->mat.ensure(esz * tiles.size() * kPR * kPC) ||
+        (with64 && ctx->mat64.ensure(sizeof(double2) * tiles.size() * kPR * kPC)) ||
+        ctx->pair_tiles.ensure(sizeof(int2) * tiles.size()) ||
         ctx->pair_blk.ensure(sizeof(int) * blk.size()) ||
         ctx->pair_pts.ensure(sizeof(PairPoint) * ((size_t)count + kPC)) ||
         ctx->near_off.ensure(sizeof(int) * ((size_t)count + 1)) || ctx->near_cur.ensure(sizeof(int) * count))
@@ -465,16 +472,19 @@ int launch_assemble_pair(Context* ctx, int precision) {
                             ctx->stream));
     IBIM_HIP(hipMemcpyAsync(ctx->pair_blk.ptr, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice,
                             ctx->stream));
-    const int rc = precision == IBIM_C128 ? assemble_pair_as<double2>(ctx, tiles) : assemble_pair_as<float2>(ctx, tiles);
+    const int rc = precision == IBIM_C128 ? assemble_pair_as<double2>(ctx, tiles, false)
+                                          : assemble_pair_as<float2>(ctx, tiles, with64);
     if (rc) return rc;
     ctx->pair_ntiles = (int)tiles.size();
     ctx->mat_precision = precision;
     ctx->mat_pair = true;
+    ctx->mat_pair64 = with64 && precision != IBIM_C128;
     ctx->regcount = 0;
     return IBIM_OK;
 }
 
-int sweep_pair_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
+                      const int* done) {
     const int count = ctx->count;
     const size_t ld = ctx->ld;
     const int ncol = (count + kPC - 1) / kPC, nrow = (count + kPR - 1) / kPR;
@@ -486,11 +496,12 @@ int sweep_pair_launch(Context* ctx, const void* p, const void* q, void* ypart, v
     const int2* tiles = ctx->pair_tiles.as<int2>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
     {
-        TimedScope ts(ctx, "matvec");
-        if (ctx->mat_precision == IBIM_C128) {
+        TimedScope ts(ctx, fp64_copy ? "matvec64" : "matvec");
+        if (fp64_copy || ctx->mat_precision == IBIM_C128) {
             auto kern = dir ? k_pair_sweep<double2, true> : k_pair_sweep<double2, false>;
             hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
-                               ctx->mat.as<double2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done);
+                               (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(), tiles, pts, count, ld, ncol, P, Q,
+                               Y, Z, done);
         } else {
             auto kern = dir ? k_pair_sweep<float2, true> : k_pair_sweep<float2, false>;
             hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,

@@ -35,6 +35,8 @@ TUBE_TRADITIONAL, TUBE_COMBO = 0, 1
 C128, C64 = 0, 1
 # pair storage of the traditional operator (half the bytes per sweep)
 C128_PAIR, C64_PAIR = 2, 3
+# complex64 pairs + a complex128 copy for the FP64 residuals of the mixed solvers
+MIXED_PAIR = 4
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
 EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO = 0, 1, 2
 SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2

@@ -86,6 +86,11 @@ typedef struct ibim_params {
  * IBIM_C128 / IBIM_C64 per sweep. Not available to the batched solvers. */
 #define IBIM_C128_PAIR 2
 #define IBIM_C64_PAIR 3
+/* Complex64 pairs for the BiCG iterations with a complex128 copy of the same
+ * factors beside them: the FP64 operator of IBIM_SOLVE_RELIABLE's residual
+ * replacements, IBIM_SOLVE_REFINED's residuals and ibim_gradient's adjoint
+ * solve (1.5x the HBM of IBIM_C128_PAIR). */
+#define IBIM_MIXED_PAIR 4
 
 #define IBIM_RHS_POINT_SOURCE 0    /* Phi(x*, x0) = e^{ik|x*-x0|} / (4 pi |x*-x0|)        */
 #define IBIM_RHS_PLANE_NEUMANN 1   /* amp d/dn e^{ik dir.x} at x* (cal_partialu*)        */

@@ -90,19 +90,32 @@ def test_pair_bicg_matches_dense_solution(gpu):
     assert np.array_equal(xp[0], xp2[0]) and np.array_equal(xp[1], xp2[1])
 
 
-def test_pair_mixed_solvers_reach_the_fp64_solution(gpu):
+@pytest.mark.parametrize("storage", ["C64_PAIR", "MIXED_PAIR"])
+def test_pair_mixed_solvers_reach_the_fp64_solution(gpu, storage):
     pb = problem(gpu, 28, k=6.0)
     pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3])
     pb.assemble(gpu.C128)
     info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-12)
     x64 = pb.density()
     x64 = x64[0] + 1j * x64[1]
-    pb.assemble(gpu.C64_PAIR)  # inner: FP32 pairs; replacements / corrections: FP64 matrix-free
+    # inner: complex64 pairs; replacements / corrections: FP64 matrix-free
+    # (C64_PAIR) or the complex128 copy of the same factors (MIXED_PAIR)
+    pb.assemble(getattr(gpu, storage))
+    pb.reset_timers()
+    pb.set_profiling(True)
     info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=2000, tol=1e-10, inner_tol=1e-3, chunk=1)
+    pb.sync()
+    pb.set_profiling(False)
     x = pb.density()
     assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 1
     assert info.iterations <= 1.5 * info64.iterations
     assert relnorm(x[0] + 1j * x[1], x64) <= 1e-8
+    n64 = pb.kernel_time("matvec64")[1]
+    nfree = pb.kernel_time("sweep_free64")[1]
+    if storage == "MIXED_PAIR":
+        assert n64 == info.outer and nfree == 0
+    else:
+        assert n64 == 0 and nfree == info.outer
     info, _ = pb.solve(gpu.SOLVE_REFINED, maxit=2000, tol=1e-10, inner_tol=1e-4, max_outer=10)
     x = pb.density()
     assert info.status == 0 and info.rel_residual < 1e-10
@@ -142,17 +155,21 @@ def test_pair_gradient_matches_dense_gradient(gpu):
     pb.rhs(gpu.RHS_POINT_SOURCE, x0=(0.1, 0.2, 0.3))
     target = np.zeros(len(recv), complex) + 0.01
     out = {}
-    for prec in ("C128", "C128_PAIR"):
+    for prec in ("C128", "C128_PAIR", "MIXED_PAIR"):
         pb.assemble(getattr(gpu, prec))
-        info, _ = pb.solve(tol=1e-13, maxit=2000)
+        if prec == "MIXED_PAIR":  # forward: complex64 pairs + FP64 replacement; adjoint: the complex128 copy
+            info, _ = pb.solve(gpu.SOLVE_RELIABLE, tol=1e-13, maxit=2000, inner_tol=1e-3)
+        else:
+            info, _ = pb.solve(tol=1e-13, maxit=2000)
         assert info.status == 0
         pb.set_receivers(recv, target)
         out[prec] = pb.gradient(mode=gpu.EVAL_DOUBLE, tol=1e-13, maxit=2000)
     J0, g0, dk0 = out["C128"]
-    J1, g1, dk1 = out["C128_PAIR"]
-    assert abs(J1 - J0) <= 1e-10 * J0
-    assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max()
-    assert abs(dk1 - dk0) <= 1e-8 * abs(dk0)
+    for prec in ("C128_PAIR", "MIXED_PAIR"):
+        J1, g1, dk1 = out[prec]
+        assert abs(J1 - J0) <= 1e-10 * J0, prec
+        assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max(), prec
+        assert abs(dk1 - dk0) <= 1e-8 * abs(dk0), prec
 
 
 def test_pair_storage_rejects_what_it_cannot_do(gpu):
