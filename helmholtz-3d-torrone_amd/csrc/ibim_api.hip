@@ -527,7 +527,8 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
         return batched && (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C64)
                    ? run_refined_batched(ctx, opts, info)
                    : run_refined(ctx, opts, info, hist);
-    if (opts->method == IBIM_SOLVE_RELIABLE) return run_reliable(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_RELIABLE)
+        return batched ? run_reliable_batched(ctx, opts, info, hist) : run_reliable(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
 }
 

@@ -453,10 +453,13 @@ __global__ void __launch_bounds__(kBT) kb_beta(BArgs a, int nb, int it, double t
     }
 }
 
-// p = r + beta p, p~ = r~ + conj(beta) p~ (BICGSTAB.c:271-278)
-__global__ void __launch_bounds__(kBT) kb_update_p(BArgs a) {
+// p = r + beta p, p~ = r~ + conj(beta) p~ (BICGSTAB.c:271-278), also for a
+// column that stopped in this iteration `it`: its state is then complete, so
+// residual replacement can send it back to iterate.
+__global__ void __launch_bounds__(kBT) kb_update_p(BArgs a, int it) {
     const int r = threadIdx.x & 15;
-    if (a.st->done[r] != 0) return;  // per-thread: the column is finished (no barriers below)
+    const int d = a.st->done[r];
+    if (d != 0 && d != it) return;  // per-thread: the column is finished (no barriers below)
     const double be = a.st->alpha[r][0], bei = a.st->alpha[r][1];
     for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
         const size_t e = (size_t)j * kR + r;
@@ -593,10 +596,102 @@ static int batch_workspace(Context* ctx, int maxit, int extra, BArgs* a, unsigne
     return IBIM_OK;
 }
 
+struct ConvMask {
+    int c[kR];
+};
+
+// R = B - Y in the columns flagged in `mask` (the others keep their R), with
+// per-block partials of |B - Y|^2 (2 per column) for every column.
+__global__ void __launch_bounds__(kBT) kb_replace(const double2* __restrict__ B, const double2* __restrict__ Y,
+                                                  double2* __restrict__ R, int count, ConvMask mask,
+                                                  double* __restrict__ part) {
+    const int r = threadIdx.x & 15;
+    double v[2] = {0.0, 0.0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < count; j += gridDim.x * 16) {
+        const size_t e = (size_t)j * kR + r;
+        const double2 b = B[e], y = Y[e];
+        const double2 rr = make_double2(b.x - y.x, b.y - y.y);
+        if (mask.c[r]) R[e] = rr;
+        v[0] += rr.x * rr.x;
+        v[1] += rr.y * rr.y;
+    }
+    block_partials<2>(v, part + (size_t)blockIdx.x * kR * 8);
+}
+
+// Restart of the flagged columns from their (replaced) residual: r~ = conj(r),
+// p = r, p~ = r~ (BICGSTAB.c:104-117), partials of the new <r, r~>.
+__global__ void __launch_bounds__(kBT) kb_restart(BArgs a, ConvMask mask) {
+    const int r = threadIdx.x & 15;
+    double v[4] = {0, 0, 0, 0};
+    for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
+        if (!mask.c[r]) continue;
+        const size_t e = (size_t)j * kR + r;
+        const double2 rr = a.r[e];
+        const double2 c = make_double2(rr.x, -rr.y);
+        a.p[e] = rr;
+        a.rt[e] = c;
+        a.pt[e] = c;
+        v[0] += rr.x * c.x;
+        v[1] += rr.y * c.y;
+        v[2] += rr.y * c.x;
+        v[3] += rr.x * c.y;
+    }
+    block_partials<4>(v, a.part + (size_t)blockIdx.x * kR * 8);
+}
+
+// rho of the restarted columns, which run again (no previous error).
+__global__ void __launch_bounds__(kBT) kb_restart_finish(BArgs a, int nb, ConvMask mask) {
+    __shared__ double tot[kR][8];
+    reduce_all(a.part, nb, 4, tot);
+    if (threadIdx.x < kR && mask.c[threadIdx.x]) {
+        const int r = threadIdx.x;
+        BState* s = a.st;
+        s->rho[r][0] = tot[r][0] + tot[r][1];
+        s->rho[r][1] = tot[r][2] - tot[r][3];
+        s->err[r] = INFINITY;
+        s->done[r] = 0;
+        s->stop[r] = 0;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) a.st->all_done = 0;
+}
+
+// Per-column sums of the first nv partial values (host, fixed order).
+static int column_sums(Context* ctx, const double* part, unsigned nvb, int nv, std::vector<double>& out) {
+    std::vector<double> h((size_t)nvb * kR * 8);
+    IBIM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    out.assign((size_t)kR * nv, 0.0);
+    for (unsigned b = 0; b < nvb; ++b)
+        for (int r = 0; r < kR; ++r)
+            for (int q = 0; q < nv; ++q) out[(size_t)r * nv + q] += h[((size_t)b * kR + r) * 8 + q];
+    return IBIM_OK;
+}
+
+// FP64 residual replacement for the batched BiCG (van der Vorst and Ye, as
+// in the single-vector solver): at a host check, when a running column's
+// recursive residual has dropped by `factor` since its last replacement or a
+// column claims convergence, one batched pass of the FP64 operator over the
+// current solutions gives r = b - A64 x for those columns. A claim whose true
+// residual misses tol restarts that column's BiCG from the true residual: by
+// then its shadow residual and search directions have converged with the
+// recursive residual and cannot reduce the difference (on a 3,700-point
+// sphere the recursive residual reached 3.5e-11 while the true one was 8e-8,
+// where a resumed BiCG stayed for 100 iterations).
+struct BReplace {
+    int true_op;
+    double factor;
+    int replacements = 0;
+    double next[kR];       // replace when the recursive residual falls below
+    double true_err[kR];   // last true residual (-1 before the first)
+    bool verified[kR];     // a convergence claim confirmed by the true residual
+};
+
 // BiCG for the 16 interleaved right-hand sides in a.b (padding rows zero);
-// the solutions are left in a.x and the final state in *hs.
+// the solutions are left in a.x and the final state in *hs. With `rep`, FP64
+// residual replacement by rep->true_op.
 static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, int maxit, double tol, int chunk,
-                           BState* hs) {
+                           BState* hs, BReplace* rep = nullptr) {
     hipLaunchKernelGGL(kb_init, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
     hipLaunchKernelGGL(kb_init_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
     IBIM_HIP(hipGetLastError());
@@ -604,25 +699,79 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
     int it = 1;
     IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    while (!hs->all_done && it <= maxit) {
-        const int last = std::min(it + chunk - 1, maxit);
-        for (; it <= last; ++it) {
-            int rc = batch_apply(ctx, op, false, a.p, a.v, a.st);
-            if (!rc) rc = batch_apply(ctx, op, true, a.pt, a.w, a.st);
-            if (rc) return rc;
-            hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
-            hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
-            hipLaunchKernelGGL(kb_update_xr, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
-            hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, it, tol);
-            hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+    if (rep)
+        for (int r = 0; r < kR; ++r) {
+            rep->next[r] = rep->factor;
+            rep->true_err[r] = -1.0;
+            rep->verified[r] = false;
         }
-        IBIM_HIP(hipGetLastError());
-        IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
-        IBIM_HIP(hipStreamSynchronize(ctx->stream));
-        double worst = 0.0;
-        for (int r = 0; r < kR; ++r)
-            if (hs->done[r] == 0) worst = std::max(worst, hs->err[r]);
-        if (!hs->all_done) report_progress(ctx, "batched BiCG", it - 1, worst);
+    std::vector<double> sums;
+    while (it <= maxit) {
+        if (!hs->all_done) {
+            const int last = std::min(it + chunk - 1, maxit);
+            for (; it <= last; ++it) {
+                int rc = batch_apply(ctx, op, false, a.p, a.v, a.st);
+                if (!rc) rc = batch_apply(ctx, op, true, a.pt, a.w, a.st);
+                if (rc) return rc;
+                hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+                hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
+                hipLaunchKernelGGL(kb_update_xr, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+                hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, it, tol);
+                hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a, it);
+            }
+            IBIM_HIP(hipGetLastError());
+            IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
+            IBIM_HIP(hipStreamSynchronize(ctx->stream));
+            double worst = 0.0;
+            for (int r = 0; r < kR; ++r)
+                if (hs->done[r] == 0) worst = std::max(worst, hs->err[r]);
+            if (!hs->all_done) report_progress(ctx, "batched BiCG", it - 1, worst);
+        }
+        if (rep) {
+            ConvMask m{};
+            bool any = false;
+            for (int r = 0; r < kR; ++r) {
+                const bool claims = hs->done[r] > 0 && hs->stop[r] == 1 && !rep->verified[r];
+                const bool drop = hs->done[r] == 0 && hs->err[r] < rep->next[r];
+                m.c[r] = claims || drop;
+                any = any || m.c[r];
+            }
+            if (any) {
+                // a.v is free between iterations: the true products of all columns
+                int rc = batch_apply(ctx, rep->true_op, false, a.x, a.v, nullptr);
+                if (rc) return rc;
+                hipLaunchKernelGGL(kb_replace, dim3(nvb), dim3(kBT), 0, ctx->stream, a.b, a.v, a.r, a.count, m,
+                                   a.part);
+                IBIM_HIP(hipGetLastError());
+                rc = column_sums(ctx, a.part, nvb, 2, sums);
+                if (rc) return rc;
+                rep->replacements++;
+                ConvMask restart{};
+                bool any_restart = false;
+                for (int r = 0; r < kR; ++r) {
+                    if (!m.c[r]) continue;
+                    const double terr = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / hs->normb[r];
+                    rep->true_err[r] = terr;
+                    rep->next[r] = terr * rep->factor;
+                    if (hs->done[r] > 0) {  // a convergence claim
+                        if (terr < tol) {
+                            rep->verified[r] = true;
+                        } else {
+                            restart.c[r] = 1;
+                            any_restart = true;
+                        }
+                    }
+                }
+                if (any_restart) {
+                    hipLaunchKernelGGL(kb_restart, dim3(nvb), dim3(kBT), 0, ctx->stream, a, restart);
+                    hipLaunchKernelGGL(kb_restart_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, restart);
+                    IBIM_HIP(hipGetLastError());
+                    IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
+                    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+                }
+            }
+        }
+        if (hs->all_done) break;
     }
     return IBIM_OK;
 }
@@ -631,7 +780,7 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
 // densities into the same columns of ctx->density. Per-column iteration counts,
 // residuals and stop reasons go to the out arrays.
 static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
-                      double* errs, int* stops, double* hist) {
+                      double* errs, int* stops, double* hist, BReplace* rep = nullptr) {
     const int count = ctx->count;
     BArgs a;
     unsigned nvb;
@@ -641,7 +790,7 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
     hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol,
                        (double2*)a.b, ctx->ld);
     BState hs{};
-    rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs);
+    rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs, rep);
     if (rc) return rc;
     hipLaunchKernelGGL(kb_unpack, dim3((unsigned)(((size_t)count * kR + 255) / 256)), dim3(256), 0, ctx->stream, a.x,
                        count, c0, ncol, ctx->density.as<double2>());
@@ -655,7 +804,7 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
     for (int r = 0; r < ncol; ++r) {
         const int d = hs.done[r];
         iters[c0 + r] = d > 0 ? d : (d < 0 ? 0 : maxit);
-        errs[c0 + r] = d < 0 ? 0.0 : hs.err[r];
+        errs[c0 + r] = d < 0 ? 0.0 : (rep && rep->true_err[r] >= 0.0 ? rep->true_err[r] : hs.err[r]);
         stops[c0 + r] = d == 0 ? 1 : (hs.stop[r] == 1 ? 0 : hs.stop[r]);
         if (hist && c0 + r == 0)
             for (int i = 1; i <= iters[0]; ++i) hist[i - 1] = h[(size_t)i * kR];
@@ -674,10 +823,6 @@ __global__ void __launch_bounds__(kBT) kb_accumulate(double2* __restrict__ X, co
     }
 }
 
-struct ConvMask {
-    int c[kR];
-};
-
 __global__ void __launch_bounds__(kBT) kb_residual(const double2* __restrict__ B, const double2* __restrict__ Y,
                                                    double2* __restrict__ R, int count, ConvMask conv,
                                                    double* __restrict__ part) {
@@ -692,18 +837,6 @@ __global__ void __launch_bounds__(kBT) kb_residual(const double2* __restrict__ B
         v[1] += rr.y * rr.y;
     }
     block_partials<2>(v, part + (size_t)blockIdx.x * kR * 8);
-}
-
-// Per-column sums of the first nv partial values (host, fixed order).
-static int column_sums(Context* ctx, const double* part, unsigned nvb, int nv, std::vector<double>& out) {
-    std::vector<double> h((size_t)nvb * kR * 8);
-    IBIM_HIP(hipMemcpyAsync(h.data(), part, sizeof(double) * h.size(), hipMemcpyDeviceToHost, ctx->stream));
-    IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    out.assign((size_t)kR * nv, 0.0);
-    for (unsigned b = 0; b < nvb; ++b)
-        for (int r = 0; r < kR; ++r)
-            for (int q = 0; q < nv; ++q) out[(size_t)r * nv + q] += h[((size_t)b * kR + r) * 8 + q];
-    return IBIM_OK;
 }
 
 // FP64 iterative refinement around batched BiCG on the FP32 operator, 16
@@ -807,6 +940,41 @@ int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_in
     for (int c = 0; c < n; ++c) {
         worst = std::max(worst, errs[c]);
         outer = std::max(outer, outers[c]);
+        if (stops[c] && stops[c] != 4) status = stops[c];
+    }
+    info->iterations = total;
+    info->outer = outer;
+    info->rel_residual = worst;
+    info->status = status;
+    ctx->last_iters = total;
+    return IBIM_OK;
+}
+
+// BiCG on the FP32 operator with FP64 residual replacement (as run_reliable),
+// 16 right-hand sides per batch.
+int run_reliable_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    const int op = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
+    if (op == OP_FREE32 && ctx->kind != IBIM_TUBE_TRADITIONAL) return run_reliable(ctx, opts, info, hist);
+    const int n = ctx->nrhs;
+    std::vector<int> iters(n), stops(n);
+    std::vector<double> errs(n);
+    int outer = 0;
+    for (int c0 = 0; c0 < n; c0 += kR) {
+        BReplace rep{};
+        rep.true_op = fp64_operator(ctx);
+        rep.factor = opts->inner_tol > 0 && opts->inner_tol < 1 ? opts->inner_tol : 1e-2;
+        // replacements happen at host checks: every 2 iterations unless asked
+        const int chunk = opts->chunk > 0 ? opts->chunk : 2;
+        int rc = bicg_batch(ctx, op, c0, std::min(kR, n - c0), opts->maxit, opts->tol, chunk, iters.data(),
+                            errs.data(), stops.data(), c0 == 0 ? hist : nullptr, &rep);
+        if (rc) return rc;
+        outer += rep.replacements;
+    }
+    int total = 0, status = 0;
+    double worst = 0.0;
+    for (int c = 0; c < n; ++c) {
+        total += iters[c];
+        worst = std::max(worst, errs[c]);
         if (stops[c] && stops[c] != 4) status = stops[c];
     }
     info->iterations = total;

@@ -413,6 +413,43 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_replace(VecArgs a) {
     }
 }
 
+// Restart from the (replaced) residual: r~ = conj(r), p = r, p~ = r~
+// (BICGSTAB.c:104-117), with per-block partials of the new <r, r~>.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_restart(VecArgs a) {
+    __shared__ double sh[4 * 4];
+    double v[4] = {0, 0, 0, 0};
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads) {
+        const double2 r = a.r[j];
+        const double2 c = make_double2(r.x, (-1.0) * r.y);
+        a.p[j] = r;
+        a.rt[j] = c;
+        a.pt[j] = c;
+        v[0] += r.x * c.x;
+        v[1] += r.y * c.y;
+        v[2] += r.y * c.x;
+        v[3] += r.x * c.y;
+    }
+    block_reduce<4>(v, sh);
+    if (threadIdx.x == 0)
+        for (int q = 0; q < 4; ++q) a.part[(size_t)blockIdx.x * 4 + q] = v[q];
+}
+
+// The restarted state goes into record `at` (read as the previous iteration's
+// by the next one): rho = <r, r~>, no previous error, running again.
+__global__ void __launch_bounds__(kVecThreads) k_bicg_restart_finish(VecArgs a, int nblocks, int at) {
+    __shared__ double tot[4];
+    reduce_partials<4>(a.part, nblocks, tot);
+    if (threadIdx.x == 0) {
+        BicgScalars s = a.sc[at];
+        s.rho_re = tot[0] + tot[1];
+        s.rho_im = tot[2] - tot[3];
+        s.err = INFINITY;
+        s.stop = 0;
+        a.sc[at] = s;
+        *a.done = 0;
+    }
+}
+
 unsigned vec_blocks(int count) {
     unsigned nb = (unsigned)((count + kVecThreads - 1) / kVecThreads);
     return nb > (unsigned)kMaxVecBlocks ? (unsigned)kMaxVecBlocks : nb;
@@ -597,12 +634,17 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
         int rc = true_residual(ctx, a, nvb, rep, norms[0], &terr);
         if (rc) return rc;
         if (claims && terr < tol) break;
-        // carry on from the last real iteration with the true residual
+        // A convergence claim the true residual refutes: restart from the
+        // last real iteration with the true residual. By then the shadow
+        // residual and the search directions have converged with the
+        // recursive residual and cannot reduce the difference, so BiCG
+        // restarts (r~ = conj(r), p = r) instead of resuming.
         if (done) {
             it = done + 1;
-            cur.stop = 0;
-            IBIM_HIP(hipMemcpyAsync(a.sc + done, &cur, sizeof cur, hipMemcpyHostToDevice, ctx->stream));
-            IBIM_HIP(hipMemsetAsync(a.done, 0, sizeof(int), ctx->stream));
+            hipLaunchKernelGGL(k_bicg_restart, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a);
+            hipLaunchKernelGGL(k_bicg_restart_finish, dim3(1), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb,
+                               done);
+            IBIM_HIP(hipGetLastError());
             done = 0;
         }
         next_replace = terr * rep->factor;
@@ -670,8 +712,10 @@ int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* inf
         const double2* b = ctx->rhs.as<double2>() + (size_t)c * ctx->count;
         double2* x = ctx->density.as<double2>() + (size_t)c * ctx->count;
         Replace rep{true_op, opts->inner_tol > 0 && opts->inner_tol < 1 ? opts->inner_tol : 1e-2, 0, 0.0};
-        rc = bicg_device(ctx, op, ws, b, x, opts->maxit, opts->tol, opts->chunk, &last, c == 0 ? hist : nullptr,
-                         false, &rep);
+        // replacements happen at host checks: every 2 iterations unless asked
+        const int chunk = opts->chunk > 0 ? opts->chunk : 2;
+        rc = bicg_device(ctx, op, ws, b, x, opts->maxit, opts->tol, chunk, &last, c == 0 ? hist : nullptr, false,
+                         &rep);
         if (rc) return rc;
         total += last.iterations;
         outer += rep.replacements;

@@ -164,6 +164,7 @@ int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
 int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info);
+int run_reliable_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 // Y = op(A) P (adjoint: A^H) for 16 right-hand-side-interleaved vectors of ld rows.
 int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y, const void* state);
 int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double* pre, const double* pim, double* yre,

@@ -109,3 +109,32 @@ def test_batched_refinement_reaches_the_fp64_solution(gpu, store):
             assert not x.any()
         else:
             assert relnorm(x, r) < 1e-8, (c, relnorm(x, r))
+
+
+@pytest.mark.parametrize("store", ["c64", "free32"])
+def test_batched_residual_replacement_reaches_the_fp64_solution(gpu, store):
+    """Batched BiCG on the FP32 operator with FP64 residual replacement: every
+    column reaches the FP64 operator's solution with a true residual below tol
+    and about the FP64 iteration count (no restarts)."""
+    pb = problem(gpu)
+    n = 19
+    fill_rhs(gpu, pb, n)
+    pb.assemble(gpu.C128)
+    info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-13, batch=0)
+    ref = [pb.density(c) for c in range(n)]
+    if store == "c64":
+        pb.assemble(gpu.C64)  # inner: stored complex64; replacements: FP64 matrix-free
+    info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=500, tol=1e-10, inner_tol=1e-2, batch=0)
+    assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 2
+    for c in range(n):
+        x = pb.density(c)
+        x = x[0] + 1j * x[1]
+        r = ref[c][0] + 1j * ref[c][1]
+        if c == 3:
+            assert not x.any()
+        else:
+            assert relnorm(x, r) < 1e-8, (c, relnorm(x, r))
+    # the same solve one column at a time
+    info1, _ = pb.solve(gpu.SOLVE_RELIABLE, maxit=500, tol=1e-10, inner_tol=1e-2, batch=1)
+    assert info1.status == 0
+    assert info.iterations <= 1.3 * info1.iterations + n
