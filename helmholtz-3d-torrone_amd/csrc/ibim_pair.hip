@@ -78,11 +78,26 @@ __device__ __forceinline__ double pair_rho(const PointData& a, const PointData& 
 }
 
 // alpha_ij (factor of A_ij) and beta_ij (factor of A_ji) for row point R = i
-// and column point C = j.
-template <bool Dirichlet>
+// and column point C = j. Precise: from the separation r_ij = x_i - x_j (3 more
+// FP64 operations per pair), as the dense entries form r.n; else from the
+// per-point c, which cancels for nearly tangential pairs (|x| |m| / |r.m| ulps,
+// below the rounding of complex64 factors; 10x the dense operator's rounding
+// noise with complex128 factors).
+template <bool Dirichlet, bool Precise>
 __device__ __forceinline__ void pair_factors(const PairPoint& R, double cx, double cy, double cz, double cmx,
                                              double cmy, double cmz, double cc, double cw, double* alpha,
                                              double* beta) {
+    if (Precise) {
+        const double d1 = R.x - cx, d2 = R.y - cy, d3 = R.z - cz;
+        if (Dirichlet) {
+            *alpha = fma(d1, cmx, fma(d2, cmy, d3 * cmz));
+            *beta = -fma(d1, R.mx, fma(d2, R.my, d3 * R.mz));
+        } else {
+            *alpha = cw * fma(d1, R.mx, fma(d2, R.my, d3 * R.mz));
+            *beta = R.w * -fma(d1, cmx, fma(d2, cmy, d3 * cmz));
+        }
+        return;
+    }
     if (Dirichlet) {
         *alpha = fma(R.x, cmx, fma(R.y, cmy, fma(R.z, cmz, -cc)));
         *beta = fma(cx, R.mx, fma(cy, R.my, fma(cz, R.mz, -R.c)));
@@ -279,7 +294,8 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
 #pragma unroll
             for (int m = 0; m < kPCpt; ++m) {
                 double al, be;
-                pair_factors<Dirichlet>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m], cmz[m], cc[m], cw[m], &al, &be);
+                pair_factors<Dirichlet, sizeof(MT) == sizeof(double2)>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m],
+                                                                       cmz[m], cc[m], cw[m], &al, &be);
                 const double fr = (double)a[u][m].x, fi = (double)a[u][m].y;
                 const double ar = fr * al, ai = fi * al;  // A_ij
                 const double br = fr * be, bi = fi * be;  // A_ji
@@ -380,7 +396,8 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
         const MT f = F[tile * kPR * kPC + (size_t)(lo - I * kPR) * kPC + (hi - I * kPR - t * kPC)];
         const PairPoint L = pts[lo], H = pts[hi];
         double al, be;
-        pair_factors<Dirichlet>(L, H.x, H.y, H.z, H.mx, H.my, H.mz, H.c, H.w, &al, &be);
+        pair_factors<Dirichlet, sizeof(MT) == sizeof(double2)>(L, H.x, H.y, H.z, H.mx, H.my, H.mz, H.c, H.w, &al,
+                                                               &be);
         const double fac = i < j ? al : be;
         re = (double)f.x * fac;
         im = (double)f.y * fac;
