@@ -78,8 +78,8 @@ struct BState {
 // Row pass: a workgroup owns 64 rows (16 per wave) and steps over the columns
 // kKT at a time. The MFMA A operand (16 rows x 4 consecutive columns) is loaded
 // directly: four lanes read four adjacent entries of one row (64 B for
-// complex128, 32 B for complex64), and the next three k-steps read the rest of
-// those cache lines. The kKT x 16 block of P is staged once per workgroup.
+// complex128), and the next three k-steps read the rest of those cache lines.
+// The kKT x 16 block of P is staged once per workgroup.
 template <class MT>
 __global__ void __launch_bounds__(kBT) k_rows_stored(const MT* __restrict__ A, size_t ld, int count,
                                                      const double2* __restrict__ P, double2* __restrict__ Y,
@@ -107,6 +107,49 @@ __global__ void __launch_bounds__(kBT) k_rows_stored(const MT* __restrict__ A, s
                 ai = (double)a.y;
             }
             const double2 p = ps[4 * s + lk][li];
+            y.add(ar, ai, p.x, p.y);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = row0 + 4 * v + lk;
+        if (r < count) Y[(size_t)r * kR + li] = make_double2(y.re(v), y.im(v));
+    }
+}
+
+// Complex64 row pass: lane (li, lk) reads the 16 consecutive entries of row li
+// at columns k0 + 16 lk .. + 15 (128 B, eight 16-B loads), and k-step s of
+// the chunk sums column k0 + 16 lk + s in the MFMA k slot lk: every column
+// once per chunk, in a different order from the column index order. 1.2x the
+// one-entry-per-k-step pass (11.7 vs 13.7 ms on 64,676 points, 16 right-hand
+// sides); for complex128 the same runs were slower (18.9 vs 16.9 ms).
+__global__ void __launch_bounds__(kBT) k_rows_stored_c64(const float2* __restrict__ A, size_t ld, int count,
+                                                         const double2* __restrict__ P, double2* __restrict__ Y,
+                                                         const BState* __restrict__ st) {
+    if (st != nullptr && st->all_done) return;
+    __shared__ double2 ps[kKT][kR];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int row0 = (blockIdx.x * 4 + w) * 16;
+    const int row = row0 + li;
+    const bool rok = row < count;
+    // 16 B aligned: an even column of a row of ld = 64 m entries
+    const float4* Ar = (const float4*)(A + (size_t)(rok ? row : 0) * ld + 16 * lk);
+    Acc3 y;
+    for (int k0 = 0; k0 < count; k0 += kKT) {
+        // k0 + kKT <= ld: both are multiples of 64 and k0 < count <= ld
+        float4 a[8];
+#pragma unroll
+        for (int h = 0; h < 8; ++h) a[h] = Ar[(size_t)k0 / 2 + h];
+        __syncthreads();
+        for (int e = threadIdx.x; e < kKT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 16; ++s) {
+            const float4 v = a[s >> 1];
+            const double ar = rok ? (double)((s & 1) ? v.z : v.x) : 0.0;
+            const double ai = rok ? (double)((s & 1) ? v.w : v.y) : 0.0;
+            const double2 p = ps[16 * lk + s][li];
             y.add(ar, ai, p.x, p.y);
         }
     }
@@ -508,7 +551,7 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
                 hipLaunchKernelGGL(k_cols_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
                                    ctx->ld, count, P, Y, st);
             else
-                hipLaunchKernelGGL(k_rows_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
+                hipLaunchKernelGGL(k_rows_stored_c64, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
                                    ctx->ld, count, P, Y, st);
         }
     } else {
