@@ -82,7 +82,6 @@ struct Context {
     // pair storage: tile table (row block, tile), first tile of each row block,
     // point table, near-neighbour lists (offsets, indices, build cursors)
     int pair_ntiles = 0;
-    long long near_pairs = 0;
     DevBuf pair_tiles, pair_blk, pair_pts, near_off, near_idx, near_cur;
 
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
@@ -150,7 +149,6 @@ int launch_assemble(Context* ctx, int precision);
 // Pair storage of the traditional operator (precision IBIM_C128 / IBIM_C64 as the element type).
 // with64: keep a complex128 copy of the factors (the FP64 operator of the mixed solvers).
 int launch_assemble_pair(Context* ctx, int precision, bool with64);
-size_t pair_tile_count(int count);
 // fp64_copy: sweep the complex128 copy (OP_PAIR64) instead of the primary factors.
 int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
                       const int* done);

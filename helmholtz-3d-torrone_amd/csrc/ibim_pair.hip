@@ -448,17 +448,10 @@ int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) 
                        ctx->near_off.as<int>(), ctx->near_idx.as<int>(), count);
     IBIM_HIP(hipGetLastError());
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->near_pairs = run;
     return IBIM_OK;
 }
 
 }  // namespace
-
-size_t pair_tile_count(int count) {
-    size_t n = 0;
-    for (int I = 0; I * kPR < count; ++I) n += (size_t)((count - I * kPR + kPC - 1) / kPC);
-    return n;
-}
 
 int launch_assemble_pair(Context* ctx, int precision, bool with64) {
     if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
@@ -502,6 +495,10 @@ int launch_assemble_pair(Context* ctx, int precision, bool with64) {
 
 int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
                       const int* done) {
+    if (!ctx->mat_pair || (fp64_copy && !ctx->mat_pair64)) {
+        set_error("pair sweep without the pair storage it needs");
+        return IBIM_ERR_ARG;
+    }
     const int count = ctx->count;
     const size_t ld = ctx->ld;
     const int ncol = (count + kPC - 1) / kPC, nrow = (count + kPR - 1) / kPR;
@@ -529,8 +526,9 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;
         auto kern = dir ? k_pair_near<true> : k_pair_near<false>;
-        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream, ctx->tube.as<double>(), count, ctx->near_off.as<int>(),
-                           ctx->near_idx.as<int>(), P, Q, Y + np, Z + np, done);
+        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q,
+                           Y + np, Z + np, done);
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
