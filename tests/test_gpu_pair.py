@@ -144,7 +144,8 @@ def test_pair_several_right_hand_sides_solve_column_by_column(gpu):
             assert relnorm(x[0] + 1j * x[1], ref[c][0] + 1j * ref[c][1]) <= 1e-10
 
 
-def test_pair_gradient_matches_dense_gradient(gpu):
+@pytest.mark.parametrize("dirichlet,interior,mode", [(1, 0, "EVAL_DOUBLE"), (0, 1, "EVAL_SINGLE")])
+def test_pair_gradient_matches_dense_gradient(gpu, dirichlet, interior, mode):
     n, k = 24, 3.0
     h = 4.0 / n
     center = (0.0137, -0.0219, 0.0071)
@@ -154,7 +155,7 @@ def test_pair_gradient_matches_dense_gradient(gpu):
     pb = gpu.Problem()
     pb.set_level_shape(star(0.15), n)
     pb.geometry(5)
-    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=1))
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=dirichlet, interior=interior))
     pb.rhs(gpu.RHS_POINT_SOURCE, x0=(0.1, 0.2, 0.3))
     target = np.zeros(len(recv), complex) + 0.01
     out = {}
@@ -166,7 +167,7 @@ def test_pair_gradient_matches_dense_gradient(gpu):
             info, _ = pb.solve(tol=1e-13, maxit=2000)
         assert info.status == 0
         pb.set_receivers(recv, target)
-        out[prec] = pb.gradient(mode=gpu.EVAL_DOUBLE, tol=1e-13, maxit=2000)
+        out[prec] = pb.gradient(mode=getattr(gpu, mode), tol=1e-13, maxit=2000)
     J0, g0, dk0 = out["C128"]
     for prec in ("C128_PAIR", "MIXED_PAIR"):
         J1, g1, dk1 = out[prec]
