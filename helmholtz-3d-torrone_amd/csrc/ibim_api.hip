@@ -151,6 +151,7 @@ void ibim_destroy(ibim_ctx* ctx) {
                       &ctx->pair_tiles, &ctx->pair_blk, &ctx->pair_pts, &ctx->near_off, &ctx->near_idx,
                       &ctx->near_cur})
         b->release();
+    release_staging(ctx);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -316,44 +317,13 @@ int ibim_regcount(ibim_ctx* ctx) { return ctx ? ctx->regcount : 0; }
 int ibim_get_rows(ibim_ctx* ctx, int row0, int nrows, double* re, double* im) {
     CHECK(ctx);
     const int n = ctx->count;
-    if (ctx->mat_precision < 0 || row0 < 0 || nrows < 0 || row0 + nrows > n) return IBIM_ERR_ARG;
-    if (ctx->mat_pair) {  // rebuilt on the device, a few rows at a time
-        const int per = (int)((64u << 20) / (sizeof(double2) * (size_t)n)) + 1;
-        if (ctx->xfer.ensure(sizeof(double2) * (size_t)n * per)) return IBIM_ERR_NOMEM;
-        std::vector<double2> h((size_t)n * per);
-        for (int r0 = 0; r0 < nrows; r0 += per) {
-            const int nr = nrows - r0 < per ? nrows - r0 : per;
-            int rc = pair_rows(ctx, row0 + r0, nr, ctx->xfer.as<double2>());
-            if (rc) return rc;
-            IBIM_HIP(hipMemcpyAsync(h.data(), ctx->xfer.ptr, sizeof(double2) * (size_t)n * nr, hipMemcpyDeviceToHost,
-                                    ctx->stream));
-            IBIM_HIP(hipStreamSynchronize(ctx->stream));
-            for (size_t o = 0; o < (size_t)n * nr; ++o) {
-                re[(size_t)r0 * n + o] = h[o].x;
-                im[(size_t)r0 * n + o] = h[o].y;
-            }
-        }
-        return IBIM_OK;
-    }
-    const size_t esz = ctx->mat_precision == IBIM_C128 ? 16 : 8;
-    std::vector<char> buf(esz * ctx->ld * (size_t)nrows);
-    IBIM_HIP(hipMemcpyAsync(buf.data(), ctx->mat.as<char>() + esz * ctx->ld * (size_t)row0, buf.size(),
-                            hipMemcpyDeviceToHost, ctx->stream));
-    IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    for (int r = 0; r < nrows; ++r)
-        for (int j = 0; j < n; ++j) {
-            size_t o = (size_t)r * ctx->ld + j;
-            if (esz == 16) {
-                const double* v = (const double*)buf.data() + 2 * o;
-                re[(size_t)r * n + j] = v[0];
-                im[(size_t)r * n + j] = v[1];
-            } else {
-                const float* v = (const float*)buf.data() + 2 * o;
-                re[(size_t)r * n + j] = v[0];
-                im[(size_t)r * n + j] = v[1];
-            }
-        }
-    return IBIM_OK;
+    if (ctx->mat_precision < 0 || row0 < 0 || nrows < 0 || row0 + nrows > n || (nrows && (!re || !im)))
+        return IBIM_ERR_ARG;
+    HostRows dst;
+    dst.re = re;
+    dst.im = im;
+    dst.stride = (size_t)n;
+    return download_rows(ctx, row0, nrows, dst);
 }
 
 int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const double* im, int precision) {
@@ -363,35 +333,12 @@ int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const doubl
     ctx->ld = ((size_t)count + 63) / 64 * 64;
     const size_t esz = precision == IBIM_C128 ? 16 : 8;
     if (ctx->mat.ensure(esz * ctx->ld * (size_t)count)) return IBIM_ERR_NOMEM;
-    // interleave row blocks through a pinned staging buffer
-    const int rows_per = (int)((64u << 20) / (esz * ctx->ld)) + 1;
-    void* stage = nullptr;
-    size_t stage_bytes = esz * ctx->ld * (size_t)rows_per;
-    IBIM_HIP(hipHostMalloc(&stage, stage_bytes, hipHostMallocDefault));
-    for (int r0 = 0; r0 < count; r0 += rows_per) {
-        int nr = count - r0 < rows_per ? count - r0 : rows_per;
-        IBIM_HIP(hipStreamSynchronize(ctx->stream));
-        for (int r = 0; r < nr; ++r)
-            for (size_t j = 0; j < ctx->ld; ++j) {
-                size_t o = (size_t)r * ctx->ld + j;
-                double a = 0.0, b = 0.0;
-                if (j < (size_t)count) {
-                    a = re[(size_t)(r0 + r) * count + j];
-                    b = im[(size_t)(r0 + r) * count + j];
-                }
-                if (esz == 16) {
-                    ((double*)stage)[2 * o] = a;
-                    ((double*)stage)[2 * o + 1] = b;
-                } else {
-                    ((float*)stage)[2 * o] = (float)a;
-                    ((float*)stage)[2 * o + 1] = (float)b;
-                }
-            }
-        IBIM_HIP(hipMemcpyAsync(ctx->mat.as<char>() + esz * ctx->ld * (size_t)r0, stage, esz * ctx->ld * (size_t)nr,
-                                hipMemcpyHostToDevice, ctx->stream));
-    }
-    IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    (void)hipHostFree(stage);
+    HostRows src;
+    src.re = const_cast<double*>(re);
+    src.im = const_cast<double*>(im);
+    src.stride = (size_t)count;
+    ctx->mat_precision = -1;  // not a valid operator until the upload completes
+    if (int rc = upload_rows(ctx, src, precision)) return rc;
     ctx->mat_precision = precision;
     ctx->mat_pair = ctx->mat_pair64 = false;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);

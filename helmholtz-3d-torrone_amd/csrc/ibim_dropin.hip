@@ -65,18 +65,11 @@ void scatter(const double* flat, int S, double*** g) {
 
 // Copy rows of the assembled operator into a double** pair.
 int rows_out(ibim_ctx* ctx, int size, double** re, double** im) {
-    const int chunk = 256;
-    std::vector<double> a((size_t)chunk * size), b((size_t)chunk * size);
-    for (int r0 = 0; r0 < size; r0 += chunk) {
-        int nr = size - r0 < chunk ? size - r0 : chunk;
-        int rc = ibim_get_rows(ctx, r0, nr, a.data(), b.data());
-        if (rc) return rc;
-        for (int r = 0; r < nr; ++r) {
-            memcpy(re[r0 + r], a.data() + (size_t)r * size, sizeof(double) * size);
-            memcpy(im[r0 + r], b.data() + (size_t)r * size, sizeof(double) * size);
-        }
-    }
-    return IBIM_OK;
+    if (ctx->mat_precision < 0 || size != ctx->count) return IBIM_ERR_ARG;
+    HostRows dst;
+    dst.re_rows = re;
+    dst.im_rows = im;
+    return download_rows(ctx, 0, size, dst);
 }
 
 // N^2 fill with finite-difference gradients of the reference's Phi
@@ -210,12 +203,20 @@ int dropin_compbicg(int size, double* density, double* densityimg, double** K, d
                     int maxit) {
     ibim_ctx* ctx = dropin_ctx();
     if (!ctx) return IBIM_ERR_NODEVICE;
-    std::vector<double> re((size_t)size * size), im((size_t)size * size);
-    for (int r = 0; r < size; ++r) {
-        memcpy(re.data() + (size_t)r * size, K[r], sizeof(double) * size);
-        memcpy(im.data() + (size_t)r * size, Ki[r], sizeof(double) * size);
-    }
-    int rc = ibim_set_matrix_host(ctx, size, re.data(), im.data(), IBIM_C128);
+    if (size <= 0 || !K || !Ki) return IBIM_ERR_ARG;
+    // the rows are read in place (double**) and interleaved on the way to HBM
+    ctx->count = size;
+    ctx->ld = ((size_t)size + 63) / 64 * 64;
+    ctx->mat_precision = -1;
+    ctx->mat_pair = ctx->mat_pair64 = false;
+    if (ctx->mat.ensure(sizeof(double2) * ctx->ld * (size_t)size)) return IBIM_ERR_NOMEM;
+    HostRows src;
+    src.re_rows = K;
+    src.im_rows = Ki;
+    int rc = upload_rows(ctx, src, IBIM_C128);
+    if (rc) return rc;
+    ctx->mat_precision = IBIM_C128;
+    rc = ibim_set_nrhs(ctx, 1);
     if (!rc) rc = ibim_set_rhs_host(ctx, 0, b, bimg);
     if (rc) return rc;
     ibim_solve_opts o{};

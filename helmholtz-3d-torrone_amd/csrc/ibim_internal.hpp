@@ -117,6 +117,24 @@ struct Context {
 
     // scratch host copies
     std::vector<double> host_tmp;
+
+    // pinned host staging of operator transfers: two blocks in flight, each
+    // with the event of the copy that last used it
+    void* stage[2] = {nullptr, nullptr};
+    hipEvent_t stage_ev[2] = {nullptr, nullptr};
+    size_t stage_bytes = 0;
+};
+
+// Host rows of the operator, real and imaginary parts apart: row pointers
+// (the reference's double**) or contiguous rows at a stride.
+struct HostRows {
+    double* const* re_rows = nullptr;
+    double* const* im_rows = nullptr;
+    double* re = nullptr;
+    double* im = nullptr;
+    size_t stride = 0;
+    double* re_row(int r) const { return re_rows ? re_rows[r] : re + stride * (size_t)r; }
+    double* im_row(int r) const { return im_rows ? im_rows[r] : im + stride * (size_t)r; }
 };
 
 // Timed launch on ctx->stream: records hipEvents around the launch when
@@ -154,6 +172,14 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
                       const int* done);
 // Dense rows of the pair operator into out_dev (nrows x count complex FP64).
 int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
+
+// Operator rows between the host and HBM (ibim_hostio.hip). upload_rows fills
+// ctx->mat (count rows at ctx->ld, precision IBIM_C128 or IBIM_C64, allocated
+// by the caller); download_rows reads rows [row0, row0 + nrows) of the stored
+// operator, dense or pair. Both return when the host side is complete.
+int upload_rows(Context* ctx, const HostRows& src, int precision);
+int download_rows(Context* ctx, int row0, int nrows, const HostRows& dst);
+void release_staging(Context* ctx);
 int launch_rhs(Context* ctx, int kind, const double* x0, double amp, const double* dir, int col);
 int launch_eval(Context* ctx, int mode, int col, int nrecv, const double* xyz_dev, double* out_dev);
 int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, double* J, double* dlevel, double* dk);
