@@ -189,11 +189,12 @@ def run_batch(args):
     """Many right-hand sides and surfaces: the torus (128^3, k = 10) with 64 plane
     waves on a Fibonacci sphere, plus 8 spheres / ellipsoids with seed-random
     radii in [0.6, 1.2] on 48^3-96^3 grids with 16 plane waves each; sound-soft
-    data, tol 1e-7. FP64 path: batched BiCG on the stored complex128 operator.
-    Mixed path: batched BiCG on the stored complex64 operator with FP64
-    residual replacement (batched FP64 matrix-free passes). 16 right-hand sides
-    per pass over the operator on the FP64 matrix cores; surfaces one after the
-    other."""
+    data, tol 1e-7. FP64 path: batched BiCG on complex128 pair storage. Mixed
+    path: batched BiCG on complex64 pairs with FP64 residual replacement over
+    the complex128 copy of the same pairs (IBIM_MIXED_PAIR). 16 right-hand
+    sides per pass, A P and A^H Q fused into one pass over the pairs on the FP64
+    matrix cores (dense storage where the pairs would not fit); surfaces one
+    after the other."""
     tol, k = 1e-7, 10.0
     rng = np.random.default_rng(0)
     surfaces = [("torus128", ibim.shape(ibim.SHAPE_TORUS, a=1.0, b=0.4), 128, 64)]
@@ -221,7 +222,11 @@ def run_batch(args):
             t0 = time.perf_counter()
             pb.geometry(5)
             count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1))
-            pb.assemble(ibim.C128 if path == "fp64" else ibim.C64)
+            if path == "fp64":
+                pb.assemble(ibim.C128_PAIR if pair_bytes(count, 16) <= STORE_MAX_BYTES else ibim.C128)
+            else:
+                store = mixed_storage(count)
+                pb.assemble(store if store is not None else ibim.C64)
             pb.set_nrhs(m)
             for c in range(m):
                 pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
@@ -253,7 +258,8 @@ def run_batch(args):
     emit({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / totals["mixed"],
           "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": totals["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "complex64 operator, fp64 matrix-core products, fp64 residual replacement",
+          "dtype": "complex64 pair storage, fp64 matrix-core products, fp64 residual replacement over the "
+                   "complex128 copy of the pairs",
           "data": "synthetic: torus + 8 seed-random spheres/ellipsoids, Fibonacci plane waves",
           "config": {"workload": "batch: torus 128^3 x 64 plane waves + 8 surfaces on 48^3-96^3 x 16 plane waves, "
                                  "k=10, eps=tau=2h, tol %g" % tol, "rhs": n_rhs, "parallelism": "single GPU"},

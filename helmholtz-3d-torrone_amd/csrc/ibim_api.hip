@@ -380,10 +380,6 @@ int ibim_apply_batch(ibim_ctx* ctx, int op, int adjoint, int nvec, const double*
     if (!ctx->count || nvec < 1 || nvec > 16 || !pre || !yre || !yim || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 ||
         (op == IBIM_OP_STORED && ctx->mat_precision < 0) || (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL))
         return IBIM_ERR_ARG;
-    if (op == IBIM_OP_STORED && ctx->mat_pair) {
-        set_error("ibim_apply_batch: the batched products need dense or matrix-free operators, not pair storage");
-        return IBIM_ERR_ARG;
-    }
     const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
     return batch_apply_host(ctx, kinds[op], adjoint != 0, nvec, pre, pim, yre, yim);
 }
@@ -465,9 +461,7 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     CHECK(ctx);
     if (!opts || !info || opts->maxit <= 0 || !ctx->count) return IBIM_ERR_ARG;
     memset(info, 0, sizeof *info);
-    // the batched solvers need dense or matrix-free operators; pair storage
-    // solves one column at a time
-    const bool batched = ctx->nrhs > 1 && opts->batch == 0 && !ctx->mat_pair;
+    const bool batched = ctx->nrhs > 1 && opts->batch == 0;
     if (opts->method == IBIM_SOLVE_BICG)
         return batched ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_REFINED)

@@ -30,38 +30,16 @@
 
 #include "ibim_entries.hpp"
 #include "ibim_internal.hpp"
+#include "ibim_mfma.hpp"
 
 namespace ibim {
 
 namespace {
 
-constexpr int kR = 16;     // right-hand sides per batch: the MFMA N dimension
+constexpr int kR = kBatchRHS;  // right-hand sides per batch: the MFMA N dimension
 constexpr int kBT = 256;   // threads per workgroup (4 waves)
 constexpr int kKT = 64;    // columns (row pass) or rows (column pass) staged per step
 constexpr int kMaxVB = 1024;
-
-typedef double d4 __attribute__((ext_vector_type(4)));
-
-// D = A B + C for a 16x4 A, 4x16 B, 16x16 D. Lane l holds A[l % 16][l / 16],
-// B[l / 16][l % 16], and D[4 v + l / 16][l % 16] in component v.
-__device__ __forceinline__ d4 mfma(double a, double b, d4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-}
-
-// Complex accumulate with three real products (Gauss): for a = ar + i ai and
-// b = br + i bi, k1 += br (ar + ai), k2 += ar (bi - br), k3 += ai (br + bi);
-// then re = k1 - k3, im = k1 + k2. Three independent accumulators also keep
-// consecutive matrix-core instructions free of dependences.
-struct Acc3 {
-    d4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
-    __device__ __forceinline__ void add(double ar, double ai, double br, double bi) {
-        k1 = mfma(ar + ai, br, k1);
-        k2 = mfma(ar, bi - br, k2);
-        k3 = mfma(ai, br + bi, k3);
-    }
-    __device__ __forceinline__ double re(int v) const { return k1[v] - k3[v]; }
-    __device__ __forceinline__ double im(int v) const { return k1[v] + k2[v]; }
-};
 
 struct BState {
     double rho[kR][2];
@@ -537,6 +515,9 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
     const int count = ctx->count;
     const unsigned nb = (unsigned)((count + 63) / 64);  // 64 rows / columns per workgroup
     const BState* st = (const BState*)state;
+    if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
+        return adjoint ? pair_batch_apply(ctx, op == OP_PAIR64, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
+                       : pair_batch_apply(ctx, op == OP_PAIR64, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
     if (op == OP_DENSE) {
         TimedScope ts(ctx, adjoint ? "batch_cols" : "batch_rows");
         if (ctx->mat_precision == IBIM_C128) {
@@ -579,6 +560,17 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
+}
+
+// V = op(A) P and W = op(A)^H Q: one pass over pair storage, else two passes.
+int batch_apply2(Context* ctx, int op, const double2* P, const double2* Q, double2* V, double2* W,
+                 const void* state) {
+    const BState* st = (const BState*)state;
+    if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
+        return pair_batch_apply(ctx, op == OP_PAIR64, P, Q, V, W, st ? &st->all_done : nullptr);
+    int rc = batch_apply(ctx, op, false, P, V, state);
+    if (!rc) rc = batch_apply(ctx, op, true, Q, W, state);
+    return rc;
 }
 
 // Host entry of the batched products: nvec <= 16 column-major host vectors.
@@ -753,8 +745,7 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
         if (!hs->all_done) {
             const int last = std::min(it + chunk - 1, maxit);
             for (; it <= last; ++it) {
-                int rc = batch_apply(ctx, op, false, a.p, a.v, a.st);
-                if (!rc) rc = batch_apply(ctx, op, true, a.pt, a.w, a.st);
+                int rc = batch_apply2(ctx, op, a.p, a.pt, a.v, a.w, a.st);
                 if (rc) return rc;
                 hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
                 hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
@@ -968,7 +959,7 @@ static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol
 int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info) {
     if (ctx->kind != IBIM_TUBE_TRADITIONAL && ctx->mat_precision != IBIM_C64) return IBIM_ERR_ARG;
     const int inner = ctx->mat_precision == IBIM_C64 ? OP_DENSE : OP_FREE32;
-    const int outer_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+    const int outer_op = fp64_operator(ctx);
     const int n = ctx->nrhs;
     std::vector<int> iters(n), stops(n), outers(n);
     std::vector<double> errs(n);

@@ -172,6 +172,11 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
                       const int* done);
 // Dense rows of the pair operator into out_dev (nrows x count complex FP64).
 int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
+// Y = A P and / or Z = A^H Q for 16 RHS-interleaved vectors over the pair
+// storage (the complex128 copy when fp64_copy); a null Y or Z skips that
+// product; kernels return at once while *stop is set.
+int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const double2* Q, double2* Y, double2* Z,
+                     const int* stop);
 
 // Operator rows between the host and HBM (ibim_hostio.hip). upload_rows fills
 // ctx->mat (count rows at ctx->ld, precision IBIM_C128 or IBIM_C64, allocated

@@ -38,6 +38,7 @@
 
 #include "ibim_entries.hpp"
 #include "ibim_internal.hpp"
+#include "ibim_mfma.hpp"
 #include "ibim_wave.hpp"
 
 namespace ibim {
@@ -405,6 +406,187 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
     out[(size_t)r * count + j] = make_double2(re, im);
 }
 
+// ------------------------------------------------ 16 right-hand sides a pass --
+// Y = A P and Z = A^H Q for 16 RHS-interleaved vectors (element (j, r) at
+// j * 16 + r) in one pass over the pair storage, on the FP64 matrix cores. A
+// workgroup owns 64 rows of one row block (16 per wave) and walks every column
+// once: j < its rows from the column strips of the tiles (J, t_J), J <= I, that
+// hold the pairs (j, i), then j >= its rows from the row block's own tiles.
+// Each F feeds A_ij = F a(i, j) into Y and conj(A_ji) = conj(F) a(j, i) into
+// Z, with a(i, j), a(j, i) = pair_factors(i, j) -- the same FP64 operations
+// the single-vector sweep performs for that pair, so the entries are the same
+// bits. The rows are reduced inside the wave, so no partial planes are
+// written; both products read the pairs twice per pass (N^2 values), the bytes
+// of one dense pass for the matrix-core work of two.
+constexpr int kBatchRows = 64;  // rows per workgroup
+constexpr int kBatchK = 64;     // columns staged per step
+static_assert(kPR % kBatchRows == 0 && kPC % kBatchRows == 0, "a workgroup's rows share their tiles");
+
+template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+__global__ void __launch_bounds__(256) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
+                                                    const PairPoint* __restrict__ pts, int count,
+                                                    const double2* __restrict__ P, const double2* __restrict__ Q,
+                                                    double2* __restrict__ Y, double2* __restrict__ Z,
+                                                    const int* __restrict__ stop) {
+    constexpr bool Precise = sizeof(MT) == sizeof(double2);
+    constexpr int R = kBatchRHS;
+    if (stop != nullptr && *stop) return;
+    __shared__ double2 ps[WithY ? kBatchK : 1][R];
+    __shared__ double2 qs[WithZ ? kBatchK : 1][R];
+    __shared__ PairPoint sp[kBatchK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int rowW = blockIdx.x * kBatchRows;  // < count
+    const int I = rowW / kPR;
+    const int i0 = rowW + 16 * w;
+    const int i = i0 + li;  // < count + kPC: inside the padded point table
+    const PairPoint Ri = pts[i];
+    Acc3 y, z;
+
+    // P, Q rows j0 .. j0 + 63 (zero from count on) and their points
+    auto stage = [&](int j0) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kBatchK * R; e += 256) {
+            const int j = j0 + e / R;
+            if (WithY) ps[e / R][e % R] = j < count ? P[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
+            if (WithZ) qs[e / R][e % R] = j < count ? Q[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
+        }
+        if (threadIdx.x < kBatchK) sp[threadIdx.x] = pts[j0 + threadIdx.x];  // j0 + 63 < count + kPC
+        __syncthreads();
+    };
+    // one k-step: this lane's entry F (row i, column j = staged index jj)
+    auto apply = [&](const MT f, int jj) {
+        const PairPoint& C = sp[jj];
+        double al, be;
+        pair_factors<Dirichlet, Precise>(Ri, C.x, C.y, C.z, C.mx, C.my, C.mz, C.c, C.w, &al, &be);
+        const double fr = (double)f.x, fi = (double)f.y;
+        if (WithY) {
+            const double2 p = ps[jj][li];
+            y.add(fr * al, fi * al, p.x, p.y);  // A_ij p_j
+        }
+        if (WithZ) {
+            const double2 q = qs[jj][li];
+            z.add(fr * be, -(fi * be), q.x, q.y);  // conj(A_ji) q_j
+        }
+    };
+
+    // j < rows: pairs (j, i) in the column strip of tile (J, t_J); for J = I
+    // the strip's rows j >= i hold zeros (the diagonal block's lower half)
+    for (int J = 0; J <= I; ++J) {
+        const int cJ = rowW - J * kPR;  // column of rowW relative to row block J
+        const int t = cJ / kPC;
+        const MT* T = F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
+        for (int rj0 = 0; rj0 < kPR; rj0 += kBatchK) {
+            stage(J * kPR + rj0);
+#pragma unroll 8
+            for (int s = 0; s < kBatchK / 4; ++s) {
+                const int jj = 4 * s + lk;
+                apply(T[(size_t)(rj0 + jj) * kPC], jj);
+            }
+        }
+    }
+    // j >= rows: row i of the row block's tiles
+    const int nt = blk_start[I + 1] - blk_start[I];
+    const MT* Trow = F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
+    for (int t = 0; t < nt; ++t) {
+        const int c0 = I * kPR + t * kPC;
+        const MT* Tr = Trow + (size_t)t * kPR * kPC;
+        for (int cb = 0; cb < kPC; cb += kBatchK) {
+            stage(c0 + cb);
+#pragma unroll 8
+            for (int s = 0; s < kBatchK / 4; ++s) {
+                const int jj = 4 * s + lk;
+                apply(Tr[cb + jj], jj);
+            }
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = i0 + 4 * v + lk;
+        if (r < count) {
+            if (WithY) Y[(size_t)r * R + li] = make_double2(y.re(v), y.im(v));
+            if (WithZ) Z[(size_t)r * R + li] = make_double2(z.re(v), z.im(v));
+        }
+    }
+}
+
+// Near-field entries of the batched products (one thread per (row, RHS)),
+// added after the far part in list order.
+template <bool Dirichlet, bool WithY, bool WithZ>
+__global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restrict__ tube, int count,
+                                                         const int* __restrict__ off, const int* __restrict__ idx,
+                                                         const double2* __restrict__ P, const double2* __restrict__ Q,
+                                                         double2* __restrict__ Y, double2* __restrict__ Z,
+                                                         const int* __restrict__ stop) {
+    constexpr int R = kBatchRHS;
+    if (stop != nullptr && *stop) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    const int i = g / R, r = g % R;
+    if (i >= count) return;
+    const double* W = tube + (size_t)IBIM_F_W * count;
+    const double* G = tube + (size_t)IBIM_F_REG * count;
+    const double wi = W[i], gi = G[i];
+    const double aii = wi * gi + 0.5;
+    double2 yv = make_double2(0.0, 0.0), zv = make_double2(0.0, 0.0);
+    if (WithY) {
+        const double2 p = P[(size_t)i * R + r];
+        yv = make_double2(aii * p.x, aii * p.y);
+    }
+    if (WithZ) {
+        const double2 q = Q[(size_t)i * R + r];
+        zv = make_double2(aii * q.x, aii * q.y);
+    }
+    const int e = off[i + 1];
+    for (int s = off[i]; s < e; ++s) {
+        const int j = idx[s];
+        const double wj = W[j], gj = G[j];
+        if (WithY) {
+            const double aij = Dirichlet ? wj * gj : wj * gi;
+            const double2 p = P[(size_t)j * R + r];
+            yv.x += aij * p.x;
+            yv.y += aij * p.y;
+        }
+        if (WithZ) {
+            const double aji = Dirichlet ? wi * gi : wi * gj;
+            const double2 q = Q[(size_t)j * R + r];
+            zv.x += aji * q.x;
+            zv.y += aji * q.y;
+        }
+    }
+    if (WithY) {
+        double2& o = Y[(size_t)i * R + r];
+        o = make_double2(o.x + yv.x, o.y + yv.y);
+    }
+    if (WithZ) {
+        double2& o = Z[(size_t)i * R + r];
+        o = make_double2(o.x + zv.x, o.y + zv.y);
+    }
+}
+
+template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+void launch_pair_batch(Context* ctx, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
+                       const int* stop) {
+    const int count = ctx->count;
+    hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ>), dim3((unsigned)((count + kBatchRows - 1) / kBatchRows)),
+                       dim3(256), 0, ctx->stream, F, ctx->pair_blk.as<int>(), ctx->pair_pts.as<PairPoint>(), count, P,
+                       Q, Y, Z, stop);
+    hipLaunchKernelGGL((k_pair_near_batch<Dirichlet, WithY, WithZ>),
+                       dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
+                       stop);
+}
+
+template <class MT, bool Dirichlet>
+void launch_pair_batch_any(Context* ctx, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
+                           const int* stop) {
+    if (Y && Z)
+        launch_pair_batch<MT, Dirichlet, true, true>(ctx, F, P, Q, Y, Z, stop);
+    else if (Y)
+        launch_pair_batch<MT, Dirichlet, true, false>(ctx, F, P, Q, Y, Z, stop);
+    else
+        launch_pair_batch<MT, Dirichlet, false, true>(ctx, F, P, Q, Y, Z, stop);
+}
+
 template <class MT>
 int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) {
     const int count = ctx->count;
@@ -529,6 +711,31 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
         hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q,
                            Y + np, Z + np, done);
+    }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const double2* Q, double2* Y, double2* Z,
+                     const int* stop) {
+    if (!ctx->mat_pair || (fp64_copy && !ctx->mat_pair64) || (!Y && !Z) || (Y && !P) || (Z && !Q)) {
+        set_error("batched pair products without the pair storage or vectors they need");
+        return IBIM_ERR_ARG;
+    }
+    const bool dir = ctx->prm.dirichlet != 0;
+    TimedScope ts(ctx, fp64_copy ? "batch_pair64" : "batch_pair");
+    if (fp64_copy || ctx->mat_precision == IBIM_C128) {
+        const double2* F = (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>();
+        if (dir)
+            launch_pair_batch_any<double2, true>(ctx, F, P, Q, Y, Z, stop);
+        else
+            launch_pair_batch_any<double2, false>(ctx, F, P, Q, Y, Z, stop);
+    } else {
+        const float2* F = ctx->mat.as<float2>();
+        if (dir)
+            launch_pair_batch_any<float2, true>(ctx, F, P, Q, Y, Z, stop);
+        else
+            launch_pair_batch_any<float2, false>(ctx, F, P, Q, Y, Z, stop);
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;

@@ -83,7 +83,8 @@ typedef struct ibim_params {
  * unordered pair i < j stored once (complex FP64 / FP32), the entries
  * A_ij = F_ij w_j r_ij.n_q rebuilt from the points in every sweep, near-field
  * pairs (rho <= tau) in per-point neighbour lists: half the bytes of
- * IBIM_C128 / IBIM_C64 per sweep. Not available to the batched solvers. */
+ * IBIM_C128 / IBIM_C64 per sweep. The batched solvers and ibim_apply_batch
+ * apply it 16 right-hand sides at a time, A P and A^H Q in one pass. */
 #define IBIM_C128_PAIR 2
 #define IBIM_C64_PAIR 3
 /* Complex64 pairs for the BiCG iterations with a complex128 copy of the same

@@ -1,5 +1,6 @@
 """Batched right-hand sides: the 16-wide matrix-core products and the batched
-BiCG against dense numpy products and the one-column-at-a-time solver."""
+BiCG against dense numpy products and the one-column-at-a-time solver, on dense,
+pair (one fused pass for A P and A^H Q) and matrix-free operators."""
 import numpy as np
 import pytest
 
@@ -48,6 +49,27 @@ def test_batched_products_match_dense(gpu, nvec):
         assert relnorm(pb.apply_batch(P, gpu.OP_STORED, adjoint), ref) < 1e-13
 
 
+@pytest.mark.parametrize("dirichlet", [1, 0])
+def test_batched_pair_products_match_their_rows(gpu, dirichlet):
+    """Pair storage: A P and A^H P against numpy on the rows the storage
+    rebuilds (complex128 pairs: also against the dense complex128 operator)."""
+    pb = problem(gpu, dirichlet=dirichlet)
+    rng = np.random.default_rng(3 + dirichlet)
+    P = rng.standard_normal((pb.count, 16)) + 1j * rng.standard_normal((pb.count, 16))
+    pb.assemble(gpu.C128)
+    A = dense(pb)
+    for store in ("C128_PAIR", "C64_PAIR", "MIXED_PAIR"):
+        pb.assemble(getattr(gpu, store))
+        Ap = dense(pb)
+        if store == "C128_PAIR":
+            assert relnorm(Ap, A) < 1e-14
+        for adjoint in (False, True):
+            ref = (Ap.conj().T if adjoint else Ap) @ P
+            got = pb.apply_batch(P, gpu.OP_STORED, adjoint)
+            assert relnorm(got, ref) < 1e-13, (store, adjoint)
+            assert np.array_equal(got, pb.apply_batch(P, gpu.OP_STORED, adjoint))  # fixed order
+
+
 def fill_rhs(gpu, pb, n):
     pb.set_nrhs(n)
     rng = np.random.default_rng(7)
@@ -61,13 +83,15 @@ def fill_rhs(gpu, pb, n):
             pb.rhs(gpu.RHS_POINT_SOURCE, x0=0.3 * rng.standard_normal(3), col=c)
 
 
-@pytest.mark.parametrize("store", ["c128", "c64", "free"])
+STORES = {"c128": "C128", "c64": "C64", "c128_pair": "C128_PAIR", "c64_pair": "C64_PAIR",
+          "mixed_pair": "MIXED_PAIR"}
+
+
+@pytest.mark.parametrize("store", ["c128", "c64", "free", "c128_pair", "c64_pair"])
 def test_batched_bicg_matches_column_solves(gpu, store):
     pb = problem(gpu)
-    if store == "c128":
-        pb.assemble(gpu.C128)
-    elif store == "c64":
-        pb.assemble(gpu.C64)
+    if store in STORES:
+        pb.assemble(getattr(gpu, STORES[store]))
     n = 19  # one full batch of 16 and one of 3
     fill_rhs(gpu, pb, n)
     info_b, hist_b = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
@@ -89,7 +113,7 @@ def test_batched_bicg_matches_column_solves(gpu, store):
     np.testing.assert_allclose(hist_b[:m], hist_s[:m], rtol=1e-6)
 
 
-@pytest.mark.parametrize("store", ["c64", "free32"])
+@pytest.mark.parametrize("store", ["c64", "free32", "mixed_pair"])
 def test_batched_refinement_reaches_the_fp64_solution(gpu, store):
     pb = problem(gpu)
     n = 19
@@ -97,8 +121,10 @@ def test_batched_refinement_reaches_the_fp64_solution(gpu, store):
     pb.assemble(gpu.C128)
     info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-13, batch=0)
     ref = [pb.density(c) for c in range(n)]
-    if store == "c64":
-        pb.assemble(gpu.C64)  # inner: stored complex64; outer: FP64 matrix-free
+    if store in STORES:
+        # inner: stored complex64 (dense or pairs); outer: FP64 matrix-free or
+        # the complex128 copy of the pairs
+        pb.assemble(getattr(gpu, STORES[store]))
     info, _ = pb.solve(gpu.SOLVE_REFINED, maxit=500, tol=1e-10, inner_tol=1e-6, max_outer=10, batch=0)
     assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 2
     for c in range(n):
@@ -111,7 +137,7 @@ def test_batched_refinement_reaches_the_fp64_solution(gpu, store):
             assert relnorm(x, r) < 1e-8, (c, relnorm(x, r))
 
 
-@pytest.mark.parametrize("store", ["c64", "free32"])
+@pytest.mark.parametrize("store", ["c64", "free32", "c64_pair", "mixed_pair"])
 def test_batched_residual_replacement_reaches_the_fp64_solution(gpu, store):
     """Batched BiCG on the FP32 operator with FP64 residual replacement: every
     column reaches the FP64 operator's solution with a true residual below tol
@@ -122,10 +148,19 @@ def test_batched_residual_replacement_reaches_the_fp64_solution(gpu, store):
     pb.assemble(gpu.C128)
     info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-13, batch=0)
     ref = [pb.density(c) for c in range(n)]
-    if store == "c64":
-        pb.assemble(gpu.C64)  # inner: stored complex64; replacements: FP64 matrix-free
+    if store in STORES:
+        # inner: stored complex64; replacements: FP64 matrix-free, or the
+        # complex128 copy of the pairs (mixed_pair)
+        pb.assemble(getattr(gpu, STORES[store]))
+    pb.reset_timers()
+    pb.set_profiling(True)
     info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=500, tol=1e-10, inner_tol=1e-2, batch=0)
+    pb.sync()
+    pb.set_profiling(False)
     assert info.status == 0 and info.rel_residual < 1e-10 and info.outer >= 2
+    if store.endswith("pair"):  # one fused pass per iteration over the pairs
+        assert pb.kernel_time("batch_pair")[1] > 0 and pb.kernel_time("batch_rows")[1] == 0
+        assert (pb.kernel_time("batch_pair64")[1] > 0) == (store == "mixed_pair")
     for c in range(n):
         x = pb.density(c)
         x = x[0] + 1j * x[1]

@@ -125,7 +125,7 @@ def test_pair_mixed_solvers_reach_the_fp64_solution(gpu, storage):
     assert relnorm(x[0] + 1j * x[1], x64) <= 1e-8
 
 
-def test_pair_several_right_hand_sides_solve_column_by_column(gpu):
+def test_pair_several_right_hand_sides_solve_column_by_column_or_batched(gpu):
     pb = problem(gpu, 28)
     pb.set_nrhs(3)
     pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3], col=0)
@@ -134,14 +134,15 @@ def test_pair_several_right_hand_sides_solve_column_by_column(gpu):
     pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, batch=1)
     ref = [pb.density(c) for c in range(3)]
     pb.assemble(gpu.C128_PAIR)
-    info, _ = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, batch=0)  # pair storage: column by column
-    assert info.status == 0
-    for c in range(3):
-        x = pb.density(c)
-        if c == 1:
-            assert not x[0].any() and not x[1].any()
-        else:
-            assert relnorm(x[0] + 1j * x[1], ref[c][0] + 1j * ref[c][1]) <= 1e-10
+    for batch in (1, 0):  # column by column (pair sweeps), then batched (fused 16-column passes)
+        info, _ = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, batch=batch)
+        assert info.status == 0
+        for c in range(3):
+            x = pb.density(c)
+            if c == 1:
+                assert not x[0].any() and not x[1].any()
+            else:
+                assert relnorm(x[0] + 1j * x[1], ref[c][0] + 1j * ref[c][1]) <= 1e-10, (batch, c)
 
 
 @pytest.mark.parametrize("dirichlet,interior,mode", [(1, 0, "EVAL_DOUBLE"), (0, 1, "EVAL_SINGLE")])
@@ -183,8 +184,5 @@ def test_pair_storage_rejects_what_it_cannot_do(gpu):
     with pytest.raises(gpu.IbimError):
         pb.assemble(gpu.C128_PAIR)  # the combined-field kernel has no pair form
     pb2 = problem(gpu, 16)
-    pb2.assemble(gpu.C64_PAIR)
-    with pytest.raises(gpu.IbimError):
-        pb2.apply_batch(np.zeros((pb2.count, 2)), gpu.OP_STORED)
     with pytest.raises(gpu.IbimError):
         pb2.assemble(7)
