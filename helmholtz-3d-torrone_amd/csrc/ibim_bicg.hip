@@ -204,6 +204,7 @@ struct VecArgs {
 
 __device__ __forceinline__ double2 sum_planes(const double2* __restrict__ part, int nplanes, size_t ld, int j) {
     double sr = 0.0, si = 0.0;
+#pragma unroll 8
     for (int c = 0; c < nplanes; ++c) {
         const double2 y = part[(size_t)c * ld + j];
         sr += y.x;
@@ -217,11 +218,14 @@ __device__ __forceinline__ double2 sum_pair_planes(const double2* __restrict__ p
     const int blk = j / kSweepRows;
     const int nt = (count - blk * kSweepRows + kSweepCols - 1) / kSweepCols;
     double sr = 0.0, si = 0.0;
+    // unrolled for loads in flight; the sums keep the plane order
+#pragma unroll 8
     for (int c = 0; c < nt; ++c) {
         const double2 y = part[(size_t)c * ld + j];
         sr += y.x;
         si += y.y;
     }
+#pragma unroll 8
     for (int c = 0; c <= blk; ++c) {
         const double2 y = part[(size_t)(ncol + c) * ld + j];
         sr += y.x;

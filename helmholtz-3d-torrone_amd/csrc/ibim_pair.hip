@@ -356,6 +356,7 @@ __global__ void __launch_bounds__(256) k_pair_near(const double* __restrict__ tu
     const double2 pi = p[i], qi = q[i];
     double yr = aii * pi.x, yi = aii * pi.y, zr = aii * qi.x, zi = aii * qi.y;
     const int e = off[i + 1];
+#pragma unroll 4
     for (int s = off[i]; s < e; ++s) {
         const int j = idx[s];
         const double wj = W[j], gj = G[j];
