@@ -83,6 +83,9 @@ struct Context {
     // point table, near-neighbour lists (offsets, indices, build cursors)
     int pair_ntiles = 0;
     DevBuf pair_tiles, pair_blk, pair_pts, near_off, near_idx, near_cur;
+    // tuned workgroup shape (waves) of the 16-column pair passes per storage and
+    // product kind, 0 until timed; reset with every assembly
+    int batch_waves[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;

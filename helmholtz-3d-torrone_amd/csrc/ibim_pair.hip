@@ -34,6 +34,7 @@
 // ncol + nrow. Element j of block J = j / kSweepRows sums row-side planes
 // t < T_J, column-side planes I <= J and the near plane, in that order.
 #include <cmath>
+#include <cstdlib>
 #include <vector>
 
 #include "ibim_entries.hpp"
@@ -419,18 +420,22 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
 // bits. The rows are reduced inside the wave, so no partial planes are
 // written; both products read the pairs twice per pass (N^2 values), the bytes
 // of one dense pass for the matrix-core work of two.
-constexpr int kBatchRows = 64;  // rows per workgroup
-constexpr int kBatchK = 64;     // columns staged per step
-static_assert(kPR % kBatchRows == 0 && kPC % kBatchRows == 0, "a workgroup's rows share their tiles");
+// Waves rows-of-16 per workgroup (1, 2 or 4) is the tuned shape: more, smaller
+// workgroups fill the GPU on small operators; larger ones share each staged
+// chunk among more waves. Every wave owns its 16 rows and walks the columns in
+// the same order, so all shapes give the same bits.
+constexpr int kBatchK = 64;  // columns staged per step
+static_assert(kPR % 64 == 0 && kPC % 64 == 0, "a workgroup's rows share their tiles");
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ>
-__global__ void __launch_bounds__(256) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
-                                                    const PairPoint* __restrict__ pts, int count,
-                                                    const double2* __restrict__ P, const double2* __restrict__ Q,
-                                                    double2* __restrict__ Y, double2* __restrict__ Z,
-                                                    const int* __restrict__ stop) {
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves>
+__global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
+                                                           const PairPoint* __restrict__ pts, int count,
+                                                           const double2* __restrict__ P, const double2* __restrict__ Q,
+                                                           double2* __restrict__ Y, double2* __restrict__ Z,
+                                                           const int* __restrict__ stop) {
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
     constexpr int R = kBatchRHS;
+    constexpr int kBatchRows = 16 * Waves, kThreads = 64 * Waves;
     if (stop != nullptr && *stop) return;
     __shared__ double2 ps[WithY ? kBatchK : 1][R];
     __shared__ double2 qs[WithZ ? kBatchK : 1][R];
@@ -447,7 +452,7 @@ __global__ void __launch_bounds__(256) k_pair_batch(const MT* __restrict__ F, co
     // P, Q rows j0 .. j0 + 63 (zero from count on) and their points
     auto stage = [&](int j0) {
         __syncthreads();
-        for (int e = threadIdx.x; e < kBatchK * R; e += 256) {
+        for (int e = threadIdx.x; e < kBatchK * R; e += kThreads) {
             const int j = j0 + e / R;
             if (WithY) ps[e / R][e % R] = j < count ? P[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
             if (WithZ) qs[e / R][e % R] = j < count ? Q[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
@@ -565,27 +570,85 @@ __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restric
 }
 
 template <class MT, bool Dirichlet, bool WithY, bool WithZ>
-void launch_pair_batch(Context* ctx, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
-                       const int* stop) {
+void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* P, const double2* Q, double2* Y,
+                           double2* Z, const int* stop) {
     const int count = ctx->count;
-    hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ>), dim3((unsigned)((count + kBatchRows - 1) / kBatchRows)),
-                       dim3(256), 0, ctx->stream, F, ctx->pair_blk.as<int>(), ctx->pair_pts.as<PairPoint>(), count, P,
-                       Q, Y, Z, stop);
+    const int rows = 16 * waves;
+    const dim3 grid((unsigned)((count + rows - 1) / rows)), block((unsigned)(64 * waves));
+    const int* blk = ctx->pair_blk.as<int>();
+    const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
+    if (waves == 1)
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1>), grid, block, 0, ctx->stream, F, blk, pts,
+                           count, P, Q, Y, Z, stop);
+    else if (waves == 2)
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2>), grid, block, 0, ctx->stream, F, blk, pts,
+                           count, P, Q, Y, Z, stop);
+    else
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4>), grid, block, 0, ctx->stream, F, blk, pts,
+                           count, P, Q, Y, Z, stop);
+}
+
+// The workgroup shape of the far pass: IBIM_BATCH_WAVES (1, 2, 4) if set, else
+// the fastest of the three timed on this call's own operands (once per
+// context, storage and product kind; the outputs are the same bits).
+template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
+                const int* stop) {
+    if (const char* env = getenv("IBIM_BATCH_WAVES")) {
+        const int w = atoi(env);
+        if (w == 1 || w == 2 || w == 4) return w;
+    }
+    if (ctx->batch_waves[slot]) return ctx->batch_waves[slot];
+    const int cand[3] = {4, 2, 1};
+    float best = 0.0f;
+    int pick = 0;
+    for (int w : cand) {
+        hipEvent_t e0, e1;
+        if (hipEventCreate(&e0) != hipSuccess) return 4;
+        if (hipEventCreate(&e1) != hipSuccess) {
+            (void)hipEventDestroy(e0);
+            return 4;
+        }
+        (void)hipEventRecord(e0, ctx->stream);
+        launch_pair_batch_far<MT, Dirichlet, WithY, WithZ>(ctx, w, F, P, Q, Y, Z, stop);
+        (void)hipEventRecord(e1, ctx->stream);
+        float ms = 0.0f;
+        if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
+            (pick == 0 || ms < best)) {
+            best = ms;
+            pick = w;
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+    }
+    // a pass stopped by the flag times nothing: keep it untuned
+    if (pick && best > 0.02f) ctx->batch_waves[slot] = pick;
+    return pick ? pick : 4;
+}
+
+template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+void launch_pair_batch(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y,
+                       double2* Z, const int* stop) {
+    const int count = ctx->count;
+    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ>(ctx, slot, F, P, Q, Y, Z, stop);
+    TimedScope ts(ctx, slot >= 6 ? "batch_pair64" : "batch_pair");
+    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ>(ctx, waves, F, P, Q, Y, Z, stop);
     hipLaunchKernelGGL((k_pair_near_batch<Dirichlet, WithY, WithZ>),
                        dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
                        stop);
 }
 
+// slot: storage (0 complex64, 3 complex128, 6 complex128 copy) + product kind
 template <class MT, bool Dirichlet>
-void launch_pair_batch_any(Context* ctx, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
-                           const int* stop) {
+void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* P, const double2* Q, double2* Y,
+                           double2* Z, const int* stop) {
     if (Y && Z)
-        launch_pair_batch<MT, Dirichlet, true, true>(ctx, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, true>(ctx, slot0, F, P, Q, Y, Z, stop);
     else if (Y)
-        launch_pair_batch<MT, Dirichlet, true, false>(ctx, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, false>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
     else
-        launch_pair_batch<MT, Dirichlet, false, true>(ctx, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, false, true>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
 }
 
 template <class MT>
@@ -669,6 +732,7 @@ int launch_assemble_pair(Context* ctx, int precision, bool with64) {
                                           : assemble_pair_as<float2>(ctx, tiles, with64);
     if (rc) return rc;
     ctx->pair_ntiles = (int)tiles.size();
+    for (int& w : ctx->batch_waves) w = 0;  // a new operator is tuned afresh
     ctx->mat_precision = precision;
     ctx->mat_pair = true;
     ctx->mat_pair64 = with64 && precision != IBIM_C128;
@@ -724,19 +788,19 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
         return IBIM_ERR_ARG;
     }
     const bool dir = ctx->prm.dirichlet != 0;
-    TimedScope ts(ctx, fp64_copy ? "batch_pair64" : "batch_pair");
     if (fp64_copy || ctx->mat_precision == IBIM_C128) {
         const double2* F = (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>();
+        const int slot0 = fp64_copy ? 6 : 3;
         if (dir)
-            launch_pair_batch_any<double2, true>(ctx, F, P, Q, Y, Z, stop);
+            launch_pair_batch_any<double2, true>(ctx, slot0, F, P, Q, Y, Z, stop);
         else
-            launch_pair_batch_any<double2, false>(ctx, F, P, Q, Y, Z, stop);
+            launch_pair_batch_any<double2, false>(ctx, slot0, F, P, Q, Y, Z, stop);
     } else {
         const float2* F = ctx->mat.as<float2>();
         if (dir)
-            launch_pair_batch_any<float2, true>(ctx, F, P, Q, Y, Z, stop);
+            launch_pair_batch_any<float2, true>(ctx, 0, F, P, Q, Y, Z, stop);
         else
-            launch_pair_batch_any<float2, false>(ctx, F, P, Q, Y, Z, stop);
+            launch_pair_batch_any<float2, false>(ctx, 0, F, P, Q, Y, Z, stop);
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;

@@ -70,6 +70,34 @@ def test_batched_pair_products_match_their_rows(gpu, dirichlet):
             assert np.array_equal(got, pb.apply_batch(P, gpu.OP_STORED, adjoint))  # fixed order
 
 
+def test_batched_pair_pass_shapes_give_the_same_bits(gpu, monkeypatch):
+    """The fused pair pass runs 16, 32 or 64 rows per workgroup (IBIM_BATCH_WAVES
+    forces one, else the fastest is timed on the first call): every wave walks
+    its rows' columns in the same order, so products and solves are bitwise
+    the same for every shape."""
+    pb = problem(gpu)
+    rng = np.random.default_rng(11)
+    P = rng.standard_normal((pb.count, 16)) + 1j * rng.standard_normal((pb.count, 16))
+    pb.assemble(gpu.C64_PAIR)
+    fill_rhs(gpu, pb, 16)
+    runs = []
+    for w in ("1", "2", "4", None):
+        if w is None:
+            monkeypatch.delenv("IBIM_BATCH_WAVES", raising=False)
+        else:
+            monkeypatch.setenv("IBIM_BATCH_WAVES", w)
+        y = pb.apply_batch(P, gpu.OP_STORED, False)
+        z = pb.apply_batch(P, gpu.OP_STORED, True)
+        info, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+        x = [pb.density(c) for c in range(16)]
+        runs.append((w, y, z, info.iterations, x))
+    for w, y, z, iters, x in runs[1:]:
+        assert np.array_equal(y, runs[0][1]) and np.array_equal(z, runs[0][2]), w
+        assert iters == runs[0][3], w
+        for c in range(16):
+            assert np.array_equal(x[c][0], runs[0][4][c][0]) and np.array_equal(x[c][1], runs[0][4][c][1]), (w, c)
+
+
 def fill_rhs(gpu, pb, n):
     pb.set_nrhs(n)
     rng = np.random.default_rng(7)
