@@ -535,6 +535,12 @@ int ibim_set_profiling(ibim_ctx* ctx, int on) {
     return IBIM_OK;
 }
 
+int ibim_set_graphs(ibim_ctx* ctx, int on) {
+    CHECK(ctx);
+    ctx->use_graphs = on != 0;
+    return IBIM_OK;
+}
+
 int ibim_kernel_time(ibim_ctx* ctx, const char* name, double* ms_total, long* launches) {
     CHECK(ctx);
     collect_timers(ctx);

@@ -199,6 +199,10 @@ struct VecArgs {
     BicgScalars* sc;
     double* norms;  // [0] = ||b||, [1] = sum |re|+|im| of b
     int* done;
+    // the iteration the vector kernels work on, kept on the device so that a
+    // chunk of iterations is the same launches every time (HIP graph replay);
+    // k_bicg_dot advances it, null outside the BiCG loop
+    int* iter;
     int count;
 };
 
@@ -283,6 +287,7 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_init_finish(VecArgs a, int
         a.norms[0] = sqrt(tot[4] + tot[5]);
         a.norms[1] = tot[6];
         *a.done = 0;
+        if (a.iter) *a.iter = 0;
     }
 }
 
@@ -315,12 +320,15 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_dot(VecArgs a) {
     block_reduce<8>(v, sh);
     if (threadIdx.x == 0)
         for (int q = 0; q < 8; ++q) a.part[(size_t)blockIdx.x * 8 + q] = v[q];
+    // the update kernels after this one work on the next iteration
+    if (a.iter && blockIdx.x == 0 && threadIdx.x == 0) *a.iter += 1;
 }
 
 // alpha = <r, r~> / <v, p~> (BICGSTAB.c:217-229); x += alpha p, r -= alpha v,
 // r~ -= conj(alpha) w (231-242); partials of the new <r, r~> and ||r||^2.
-__global__ void __launch_bounds__(kVecThreads) k_bicg_update_xr(VecArgs a, int nblocks, int it, double* part_out) {
+__global__ void __launch_bounds__(kVecThreads) k_bicg_update_xr(VecArgs a, int nblocks, double* part_out) {
     if (*a.done) return;
+    const int it = *a.iter;
     __shared__ double tot[8];
     __shared__ double sh[6 * 4];
     reduce_partials<8>(a.part, nblocks, tot);
@@ -360,8 +368,9 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_update_xr(VecArgs a, int n
 // beta = <r, r~>_new / <r, r~>_old (BICGSTAB.c:246-259); p = r + beta p,
 // p~ = r~ + conj(beta) p~ (271-278); error and stop rules (289-306).
 __global__ void __launch_bounds__(kVecThreads) k_bicg_update_p(VecArgs a, const double* __restrict__ part_in,
-                                                               int nblocks, int it, double tol) {
+                                                               int nblocks, double tol) {
     if (*a.done) return;
+    const int it = *a.iter;
     __shared__ double tot[6];
     reduce_partials<6>(part_in, nblocks, tot);
     const BicgScalars old = a.sc[it - 1];
@@ -451,6 +460,7 @@ __global__ void __launch_bounds__(kVecThreads) k_bicg_restart_finish(VecArgs a, 
         s.stop = 0;
         a.sc[at] = s;
         *a.done = 0;
+        *a.iter = at;  // the next k_bicg_dot starts iteration at + 1
     }
 }
 
@@ -521,6 +531,7 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
     ws->part2 = tail + (size_t)nvb * 8;
     a.norms = ws->part2 + (size_t)nvb * 6;
     a.done = ctx->done.as<int>();
+    a.iter = nullptr;  // set by the BiCG loop only
     a.count = count;
     ws->nvb = nvb;
     return IBIM_OK;
@@ -575,6 +586,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
                 bool ref_guards = true) {
     VecArgs a = with_layout(ws.a, ctx, op);
     a.b = b;
+    a.iter = ctx->done.as<int>() + 1;
     // The operator's adjoint system A^H x = b runs the same recurrences: one
     // sweep with (p~, p) in place of (p, p~) gives A p~ and A^H p, so the dot
     // kernel reads the two partial planes the other way round.
@@ -606,17 +618,52 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     int it = 1, done = 0;
     if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
     double next_replace = rep ? rep->factor : 0.0;
+    // one iteration: the same launches every time (the kernels read the
+    // iteration from the device), so a chunk can be captured once and replayed
+    auto launch_iteration = [&]() -> int {
+        int rc = adjoint ? sweep_any(ctx, op, a.pt, a.p, a.ypart, a.zpart, a.done)
+                         : sweep_any(ctx, op, a.p, a.pt, a.ypart, a.zpart, a.done);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_bicg_dot, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, adjoint ? ad : a);
+        hipLaunchKernelGGL(k_bicg_update_xr, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb, ws.part2);
+        hipLaunchKernelGGL(k_bicg_update_p, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, ws.part2, (int)nvb,
+                           tol);
+        return IBIM_OK;
+    };
+    // HIP graph of `chunk` iterations (no memset or copy nodes: kernels only);
+    // not with profiling, whose timing events wrap the individual launches
+    struct GraphExec {
+        hipGraphExec_t g = nullptr;
+        ~GraphExec() {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } chunk_graph;
+    const bool graphs = ctx->use_graphs && !ctx->profile;
     while (it <= maxit && !done) {
         int last = it + chunk - 1 < maxit ? it + chunk - 1 : maxit;
-        for (; it <= last; ++it) {
-            int rc = adjoint ? sweep_any(ctx, op, a.pt, a.p, a.ypart, a.zpart, a.done)
-                             : sweep_any(ctx, op, a.p, a.pt, a.ypart, a.zpart, a.done);
-            if (rc) return rc;
-            hipLaunchKernelGGL(k_bicg_dot, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, adjoint ? ad : a);
-            hipLaunchKernelGGL(k_bicg_update_xr, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, (int)nvb, it,
-                               ws.part2);
-            hipLaunchKernelGGL(k_bicg_update_p, dim3(nvb), dim3(kVecThreads), 0, ctx->stream, a, ws.part2, (int)nvb,
-                               it, tol);
+        if (graphs && last - it + 1 == chunk) {
+            if (!chunk_graph.g) {
+                hipGraph_t g = nullptr;
+                IBIM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int rc = IBIM_OK;
+                for (int k = 0; k < chunk && !rc; ++k) rc = launch_iteration();
+                const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                IBIM_HIP(e);
+                const hipError_t ei = hipGraphInstantiate(&chunk_graph.g, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                IBIM_HIP(ei);
+            }
+            IBIM_HIP(hipGraphLaunch(chunk_graph.g, ctx->stream));
+            it = last + 1;
+        } else {
+            for (; it <= last; ++it) {
+                int rc = launch_iteration();
+                if (rc) return rc;
+            }
         }
         IBIM_HIP(hipGetLastError());
         IBIM_HIP(hipMemcpyAsync(&done, a.done, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));

@@ -114,6 +114,10 @@ struct Context {
     void* progress_user = nullptr;
     double progress_last = 0.0;
 
+    // single-vector BiCG: replay one captured chunk of iterations as a HIP graph
+    // (ibim_set_graphs; not while profiling)
+    bool use_graphs = false;
+
     // profiling
     bool profile = false;
     std::deque<KernelTimer> timers;  // stable addresses for open TimedScopes

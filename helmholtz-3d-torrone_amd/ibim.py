@@ -113,6 +113,7 @@ def lib():
             "ibim_set_receivers": ([vp, I, P, P, P], I),
             "ibim_gradient": ([vp, I, I, C.POINTER(SolveOpts), P, P, P], I),
             "ibim_set_profiling": ([vp, I], I),
+            "ibim_set_graphs": ([vp, I], I),
             "ibim_set_progress": ([vp, PROGRESS_FN, vp], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
             "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
@@ -352,6 +353,11 @@ class Problem:
 
     def set_profiling(self, on=True):
         _check(lib().ibim_set_profiling(self.h, 1 if on else 0), "ibim_set_profiling")
+
+    def set_graphs(self, on=True):
+        """Replay one captured chunk of BiCG iterations as a HIP graph."""
+        _check(lib().ibim_set_graphs(self.h, 1 if on else 0), "ibim_set_graphs")
+        return self
 
     def kernel_time(self, name):
         ms, n = C.c_double(0), C.c_long(0)

@@ -1,0 +1,85 @@
+"""HIP graph replay of BiCG iteration chunks (ibim_set_graphs): every solve
+captures one chunk of iterations and replays it for each full chunk. Checked
+bitwise against the same solves launched one kernel at a time, over several
+solves in a row with other device work between them, for plain BiCG on dense,
+pair and mixed pair storage, the residual-replacement and refinement solvers
+(their replacements, restarts and corrections run between replays) and the
+adjoint solve of the misfit gradient (J and dJ/dk bitwise; the level-set
+gradient, scattered with atomics, to 1e-12)."""
+import numpy as np
+import pytest
+
+import shapes
+
+pytestmark = pytest.mark.gpu
+
+
+def problem(gpu, n=28, k=5.0):
+    pb = gpu.Problem()
+    pb.set_level(shapes.sphere(n))
+    pb.geometry(5)
+    h = 4.0 / n
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=1))
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.1, 0.2, 0.3])
+    return pb
+
+
+def run(gpu, pb, method, chunk):
+    if method == "bicg":
+        info, hist = pb.solve(gpu.SOLVE_BICG, maxit=1000, tol=1e-12, chunk=chunk)
+    elif method == "reliable":
+        info, hist = pb.solve(gpu.SOLVE_RELIABLE, maxit=1000, tol=1e-11, inner_tol=1e-2, chunk=chunk)
+    else:
+        info, hist = pb.solve(gpu.SOLVE_REFINED, maxit=1000, tol=1e-11, inner_tol=1e-4, chunk=chunk)
+    x = pb.density()
+    return info.iterations, info.outer, info.status, np.asarray(hist), x[0], x[1]
+
+
+@pytest.mark.parametrize("store,method", [("C128", "bicg"), ("C128_PAIR", "bicg"), ("MIXED_PAIR", "bicg"),
+                                          ("MIXED_PAIR", "reliable"), ("C64_PAIR", "refined")])
+@pytest.mark.parametrize("chunk", [1, 3])
+def test_graph_replay_gives_the_eager_bits(gpu, store, method, chunk):
+    pb = problem(gpu)
+    pb.assemble(getattr(gpu, store))
+    eager = run(gpu, pb, method, chunk)
+    assert eager[2] == 0
+    rng = np.random.default_rng(chunk)
+    pb.set_graphs(True)
+    for rep in range(3):
+        got = run(gpu, pb, method, chunk)
+        assert got[:3] == eager[:3], (rep, got[:3], eager[:3])
+        assert np.array_equal(got[3], eager[3]), rep
+        assert np.array_equal(got[4], eager[4]) and np.array_equal(got[5], eager[5]), rep
+        # other device work between the solves
+        p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+        pb.apply(p, p.conj(), gpu.OP_STORED)
+    pb.set_graphs(False)
+    again = run(gpu, pb, method, chunk)
+    assert np.array_equal(again[4], eager[4]) and np.array_equal(again[5], eager[5])
+
+
+def test_graph_replay_adjoint_gradient(gpu):
+    n, k = 24, 3.0
+    h = 4.0 / n
+    star = gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.15, m=3, n=2, variant=0, center=(0.0137, -0.0219, 0.0071))
+    v = np.random.default_rng(0).standard_normal((40, 3))
+    recv = 2.5 * v / np.linalg.norm(v, axis=1, keepdims=True)
+    pb = gpu.Problem()
+    pb.set_level_shape(star, n)
+    pb.geometry(5)
+    pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=1))
+    pb.rhs(gpu.RHS_POINT_SOURCE, x0=(0.1, 0.2, 0.3))
+    pb.assemble(gpu.C128_PAIR)
+    out = []
+    for graphs in (False, True, True):
+        pb.set_graphs(graphs)
+        info, _ = pb.solve(tol=1e-13, maxit=2000, chunk=4)
+        assert info.status == 0
+        pb.set_receivers(recv, np.zeros(len(recv), complex) + 0.01)
+        out.append(pb.gradient(tol=1e-13, maxit=2000, chunk=4))
+    # the forward and adjoint solves give the same bits (J and dJ/dk depend on
+    # nothing else); the level-set gradient is scattered onto the grid with
+    # atomics, whose order varies from run to run with or without graphs
+    for J, g, dk in out[1:]:
+        assert J == out[0][0] and dk == out[0][2]
+        assert np.abs(g - out[0][1]).max() <= 1e-12 * np.abs(out[0][1]).max()
