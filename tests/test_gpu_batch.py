@@ -201,3 +201,32 @@ def test_batched_residual_replacement_reaches_the_fp64_solution(gpu, store):
     info1, _ = pb.solve(gpu.SOLVE_RELIABLE, maxit=500, tol=1e-10, inner_tol=1e-2, batch=1)
     assert info1.status == 0
     assert info.iterations <= 1.3 * info1.iterations + n
+
+
+def test_contexts_solve_concurrently_from_host_threads(gpu):
+    """Independent handles (one stream each) solved from several host threads
+    at once, as the batch workload runs its smaller surfaces: the same bits as
+    one after another."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    def solve(n):
+        pb = gpu.Problem()
+        pb.set_level(shapes.sphere(n))
+        pb.geometry(5)
+        h = 4.0 / n
+        pb.tube(gpu.params(4.0, 3 * h, 2 * h, dirichlet=1))
+        pb.assemble(gpu.C128_PAIR)
+        fill_rhs(gpu, pb, 16)
+        info, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+        x = [pb.density(c) for c in range(16)]
+        pb.close()
+        return info.iterations, x
+
+    sizes = (20, 24, 28)
+    seq = [solve(n) for n in sizes]
+    with ThreadPoolExecutor(max_workers=len(sizes)) as pool:
+        par = list(pool.map(solve, sizes))
+    for (i0, x0), (i1, x1) in zip(seq, par):
+        assert i0 == i1
+        for c in range(16):
+            assert np.array_equal(x0[c][0], x1[c][0]) and np.array_equal(x0[c][1], x1[c][1])
