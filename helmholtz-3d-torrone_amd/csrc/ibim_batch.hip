@@ -970,7 +970,7 @@ int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_in
     }
     int total = 0, status = 0, outer = 0;
     double worst = 0.0;
-    for (int c0 = 0; c0 < n; c0 += kR) total += iters[c0];
+    for (int c = 0; c < n; ++c) total += iters[c];  // summed over the columns, as the batched BiCG counts
     for (int c = 0; c < n; ++c) {
         worst = std::max(worst, errs[c]);
         outer = std::max(outer, outers[c]);

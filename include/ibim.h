@@ -120,7 +120,8 @@ typedef struct ibim_solve_opts {
 } ibim_solve_opts;
 
 typedef struct ibim_solve_info {
-    int iterations;      /* BiCG iterations (summed over refinement steps)   */
+    int iterations;      /* BiCG iterations, summed over refinement steps and,
+                          * with several right-hand sides, over the columns */
     int outer;           /* refinement steps                                 */
     double rel_residual; /* last ||r|| / ||b|| (recursive residual for BiCG, */
                          /* true FP64 residual for refinement)               */
