@@ -1,8 +1,9 @@
 /*
  * ibim_oracle.c -- CPU FP64 restatement of the reference IBIM hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker, never the product): only tests/,
- * __graft_entry__.smoke() and bench.py's cpu_baseline leg load this library.
+ * TEST INFRASTRUCTURE ONLY: it checks the HIP library's results and is never the
+ * measured or shipped path; only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg load it.
  *
  * Every function names the reference lines it restates. Where the reference's
  * operation order decides a discrete branch (limiter choices, min-abs ties) or

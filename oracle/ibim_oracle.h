@@ -2,7 +2,7 @@
  * ibim_oracle.h -- CPU FP64 restatement of the reference's implicit boundary
  * integral (IBIM) hot path.
  *
- * TEST INFRASTRUCTURE ONLY. This library is the checker for the HIP product
+ * TEST INFRASTRUCTURE ONLY. This library checks the results of the HIP
  * library: tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may
  * load it; the product path never does.
  *

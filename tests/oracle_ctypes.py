@@ -1,4 +1,4 @@
-"""ctypes binding of the CPU oracle (oracle/liboracle.so) -- the checker.
+"""ctypes binding of the CPU oracle (oracle/liboracle.so), which checks the HIP library.
 
 Only tests, __graft_entry__.smoke() and bench.py's cpu_baseline leg import this.
 """
