@@ -50,6 +50,7 @@ struct BState {
     int done[kR];  // 0 running; else the iteration it stopped at (-1: zero data)
     int stop[kR];  // stop rule that fired (1 tol, 2 NaN, 3 stagnation, 4 zero data)
     int all_done;
+    int iter;  // the iteration the vector kernels work on (kb_dot advances it)
 };
 
 // ------------------------------------------------------------ stored passes --
@@ -364,12 +365,18 @@ __global__ void __launch_bounds__(kBT) kb_init_finish(BArgs a, int nb) {
         int all = 1;
         for (int r = 0; r < kR; ++r) all &= a.st->done[r] != 0;
         a.st->all_done = all;
+        a.st->iter = 0;
     }
 }
 
 // partials of <r, r~> and <v, p~> per column
 __global__ void __launch_bounds__(kBT) kb_dot(BArgs a) {
-    if (a.st->all_done) return;
+    if (a.st->all_done) {
+        // the counter moves on with every launched iteration, as the columns'
+        // stop iterations are compared with it
+        if (blockIdx.x == 0 && threadIdx.x == 0) a.st->iter += 1;
+        return;
+    }
     const int r = threadIdx.x & 15;
     double v[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = blockIdx.x * 16 + (threadIdx.x >> 4); j < a.count; j += gridDim.x * 16) {
@@ -385,6 +392,8 @@ __global__ void __launch_bounds__(kBT) kb_dot(BArgs a) {
         v[7] += y.x * pt.y;
     }
     block_partials<8>(v, a.part + (size_t)blockIdx.x * kR * 8);
+    // the kernels after this one work on the next iteration
+    if (blockIdx.x == 0 && threadIdx.x == 0) a.st->iter += 1;
 }
 
 // alpha = <r, r~> / <v, p~> per running column (BICGSTAB.c:217-229)
@@ -436,7 +445,8 @@ __global__ void __launch_bounds__(kBT) kb_update_xr(BArgs a) {
 
 // beta = <r, r~>_new / <r, r~>_old (BICGSTAB.c:246-259), errors and stop rules
 // (289-306) per running column; beta goes to alpha's slot for kb_update_p.
-__global__ void __launch_bounds__(kBT) kb_beta(BArgs a, int nb, int it, double tol) {
+__global__ void __launch_bounds__(kBT) kb_beta(BArgs a, int nb, double tol) {
+    const int it = a.st->iter;
     if (a.st->all_done) return;
     __shared__ double tot[kR][8];
     reduce_all(a.part, nb, 6, tot);
@@ -477,7 +487,8 @@ __global__ void __launch_bounds__(kBT) kb_beta(BArgs a, int nb, int it, double t
 // p = r + beta p, p~ = r~ + conj(beta) p~ (BICGSTAB.c:271-278), also for a
 // column that stopped in this iteration `it`: its state is then complete, so
 // residual replacement can send it back to iterate.
-__global__ void __launch_bounds__(kBT) kb_update_p(BArgs a, int it) {
+__global__ void __launch_bounds__(kBT) kb_update_p(BArgs a) {
+    const int it = a.st->iter;
     const int r = threadIdx.x & 15;
     const int d = a.st->done[r];
     if (d != 0 && d != it) return;  // per-thread: the column is finished (no barriers below)
@@ -741,17 +752,53 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
             rep->verified[r] = false;
         }
     std::vector<double> sums;
+    // one iteration: the same launches every time (the kernels read the
+    // iteration from the state), so a chunk can be captured once and replayed
+    auto launch_iteration = [&]() -> int {
+        int rc = batch_apply2(ctx, op, a.p, a.pt, a.v, a.w, a.st);
+        if (rc) return rc;
+        hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+        hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
+        hipLaunchKernelGGL(kb_update_xr, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+        hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, tol);
+        hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
+        return IBIM_OK;
+    };
+    // HIP graph of `chunk` iterations (kernels only), captured after the first
+    // chunk ran eagerly (the pair passes time their workgroup shapes there)
+    struct GraphExec {
+        hipGraphExec_t g = nullptr;
+        ~GraphExec() {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } chunk_graph;
+    const bool graphs = ctx->use_graphs && !ctx->profile;
     while (it <= maxit) {
         if (!hs->all_done) {
             const int last = std::min(it + chunk - 1, maxit);
-            for (; it <= last; ++it) {
-                int rc = batch_apply2(ctx, op, a.p, a.pt, a.v, a.w, a.st);
-                if (rc) return rc;
-                hipLaunchKernelGGL(kb_dot, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
-                hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb);
-                hipLaunchKernelGGL(kb_update_xr, dim3(nvb), dim3(kBT), 0, ctx->stream, a);
-                hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, it, tol);
-                hipLaunchKernelGGL(kb_update_p, dim3(nvb), dim3(kBT), 0, ctx->stream, a, it);
+            if (graphs && it > 1 && last - it + 1 == chunk) {
+                if (!chunk_graph.g) {
+                    hipGraph_t g = nullptr;
+                    IBIM_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    int rc = IBIM_OK;
+                    for (int k = 0; k < chunk && !rc; ++k) rc = launch_iteration();
+                    const hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+                    if (rc) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc;
+                    }
+                    IBIM_HIP(e);
+                    const hipError_t ei = hipGraphInstantiate(&chunk_graph.g, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    IBIM_HIP(ei);
+                }
+                IBIM_HIP(hipGraphLaunch(chunk_graph.g, ctx->stream));
+                it = last + 1;
+            } else {
+                for (; it <= last; ++it) {
+                    int rc = launch_iteration();
+                    if (rc) return rc;
+                }
             }
             IBIM_HIP(hipGetLastError());
             IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));

@@ -243,11 +243,12 @@ int ibim_gradient(ibim_ctx *ctx, int col, int mode, const ibim_solve_opts *opts,
 /* ------------------------------------------------------------- profiling --- */
 /* When on, kernels are bracketed by hipEvents on the handle's stream. */
 int ibim_set_profiling(ibim_ctx *ctx, int on);
-/* When on (default off), single-vector BiCG solves capture one chunk of
- * iterations (ibim_solve_opts.chunk: the sweep, inner products and vector
- * updates) as a HIP graph once per solve and replay it for every full chunk;
- * the results are the same bits as the launches one by one. Ignored while
- * profiling is on. No counterpart in the reference (launch overhead). */
+/* When on (default off), BiCG solves, single-vector and batched, capture one
+ * chunk of iterations (ibim_solve_opts.chunk: the products, inner products and
+ * vector updates) as a HIP graph once per solve and replay it for every full
+ * chunk (batched: from the second chunk on); the results are the same bits as
+ * the launches one by one. Ignored while profiling is on. No counterpart in
+ * the reference (launch overhead). */
 int ibim_set_graphs(ibim_ctx *ctx, int on);
 /* Total milliseconds and launch count of a kernel class ("matvec", "assemble", ...). */
 int ibim_kernel_time(ibim_ctx *ctx, const char *name, double *ms_total, long *launches);

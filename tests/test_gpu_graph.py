@@ -1,5 +1,6 @@
 """HIP graph replay of BiCG iteration chunks (ibim_set_graphs): every solve
-captures one chunk of iterations and replays it for each full chunk. Checked
+captures one chunk of iterations and replays it for each full chunk (batched
+solves: after a first eager chunk). Checked
 bitwise against the same solves launched one kernel at a time, over several
 solves in a row with other device work between them, for plain BiCG on dense,
 pair and mixed pair storage, the residual-replacement and refinement solvers
@@ -83,3 +84,42 @@ def test_graph_replay_adjoint_gradient(gpu):
     for J, g, dk in out[1:]:
         assert J == out[0][0] and dk == out[0][2]
         assert np.abs(g - out[0][1]).max() <= 1e-12 * np.abs(out[0][1]).max()
+
+
+def fill_rhs(gpu, pb, n):
+    pb.set_nrhs(n)
+    rng = np.random.default_rng(7)
+    for c in range(n):
+        d = rng.standard_normal(3)
+        pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=d / np.linalg.norm(d), col=c)
+
+
+@pytest.mark.parametrize("store,method", [("C128", "bicg"), ("C128_PAIR", "bicg"), ("MIXED_PAIR", "reliable"),
+                                          ("C64", "refined")])
+@pytest.mark.parametrize("chunk", [2, 5])
+def test_batched_graph_replay_gives_the_eager_bits(gpu, store, method, chunk):
+    """16 right-hand sides per pass (19 columns: a batch of 16 and one of 3):
+    the first chunk runs eagerly, later full chunks replay one captured graph."""
+    pb = problem(gpu)
+    pb.assemble(getattr(gpu, store))
+    fill_rhs(gpu, pb, 19)
+
+    def run_batched():
+        if method == "bicg":
+            info, _ = pb.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, chunk=chunk, batch=0)
+        elif method == "reliable":
+            info, _ = pb.solve(gpu.SOLVE_RELIABLE, maxit=500, tol=1e-11, inner_tol=1e-2, chunk=chunk, batch=0)
+        else:
+            info, _ = pb.solve(gpu.SOLVE_REFINED, maxit=500, tol=1e-11, inner_tol=1e-4, chunk=chunk, batch=0)
+        return info.iterations, info.outer, info.status, [pb.density(c) for c in range(19)]
+
+    eager = run_batched()
+    assert eager[2] == 0
+    pb.set_graphs(True)
+    for rep in range(2):
+        got = run_batched()
+        assert got[:3] == eager[:3], (rep, got[:3], eager[:3])
+        for c in range(19):
+            assert np.array_equal(got[3][c][0], eager[3][c][0]) and np.array_equal(got[3][c][1], eager[3][c][1])
+        pb.apply_batch(np.ones((pb.count, 3)), gpu.OP_STORED)  # other device work between the solves
+    pb.set_graphs(False)
