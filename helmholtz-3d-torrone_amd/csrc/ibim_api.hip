@@ -177,7 +177,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->weno_order = 0;
     ctx->count = 0;
     ctx->mat_precision = -1;
-    ctx->mat_pair = ctx->mat_pair64 = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
     return IBIM_OK;
 }
 
@@ -275,7 +275,7 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
-    ctx->mat_pair = ctx->mat_pair64 = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
     ctx->count = 0;  // a failed build leaves no stale tube behind
     int c = 0;
     int rc = launch_tube(ctx, &c);
@@ -340,7 +340,7 @@ int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const doubl
     ctx->mat_precision = -1;  // not a valid operator until the upload completes
     if (int rc = upload_rows(ctx, src, precision)) return rc;
     ctx->mat_precision = precision;
-    ctx->mat_pair = ctx->mat_pair64 = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
 }
 

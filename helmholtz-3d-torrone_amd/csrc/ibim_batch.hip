@@ -529,6 +529,14 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
     if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
         return adjoint ? pair_batch_apply(ctx, op == OP_PAIR64, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
                        : pair_batch_apply(ctx, op == OP_PAIR64, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
+    if (op == OP_FREE64) {
+        if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
+            set_error("batched matrix-free products need the traditional tube");
+            return IBIM_ERR_ARG;
+        }
+        return adjoint ? pair_free_batch_apply(ctx, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
+                       : pair_free_batch_apply(ctx, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
+    }
     if (op == OP_DENSE) {
         TimedScope ts(ctx, adjoint ? "batch_cols" : "batch_rows");
         if (ctx->mat_precision == IBIM_C128) {
@@ -553,21 +561,12 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
         }
         TimedScope ts(ctx, adjoint ? "batch_cols_free" : "batch_rows_free");
         const double* tube = ctx->tube.as<double>();
-        if (op == OP_FREE32) {
-            if (adjoint)
-                hipLaunchKernelGGL(k_cols_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
-                                   P, Y, st);
-            else
-                hipLaunchKernelGGL(k_rows_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
-                                   P, Y, st);
-        } else {
-            if (adjoint)
-                hipLaunchKernelGGL(k_cols_free<OP_FREE64>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
-                                   P, Y, st);
-            else
-                hipLaunchKernelGGL(k_rows_free<OP_FREE64>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm,
-                                   P, Y, st);
-        }
+        if (adjoint)
+            hipLaunchKernelGGL(k_cols_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm, P,
+                               Y, st);
+        else
+            hipLaunchKernelGGL(k_rows_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm, P,
+                               Y, st);
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
@@ -579,6 +578,8 @@ int batch_apply2(Context* ctx, int op, const double2* P, const double2* Q, doubl
     const BState* st = (const BState*)state;
     if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
         return pair_batch_apply(ctx, op == OP_PAIR64, P, Q, V, W, st ? &st->all_done : nullptr);
+    if (op == OP_FREE64 && ctx->kind == IBIM_TUBE_TRADITIONAL)
+        return pair_free_batch_apply(ctx, P, Q, V, W, st ? &st->all_done : nullptr);
     int rc = batch_apply(ctx, op, false, P, V, state);
     if (!rc) rc = batch_apply(ctx, op, true, Q, W, state);
     return rc;

@@ -83,9 +83,13 @@ struct Context {
     // point table, near-neighbour lists (offsets, indices, build cursors)
     int pair_ntiles = 0;
     DevBuf pair_tiles, pair_blk, pair_pts, near_off, near_idx, near_cur;
-    // tuned workgroup shape (waves) of the 16-column pair passes per storage and
-    // product kind, 0 until timed; reset with every assembly
-    int batch_waves[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    // the five tables above match the current tube and parameters (built by a
+    // pair assembly, or alone for the matrix-free pair pass)
+    bool pair_aux = false;
+    // tuned workgroup shape (waves) of the 16-column pair passes per storage
+    // (complex64, complex128, complex128 copy, computed) and product kind, 0
+    // until timed; reset with every assembly
+    int batch_waves[12] = {};
 
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;
@@ -184,6 +188,12 @@ int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 // product; kernels return at once while *stop is set.
 int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const double2* Q, double2* Y, double2* Z,
                      const int* stop);
+// The same products of the FP64 matrix-free operator (traditional tube): each
+// factor computed where the complex128 pair storage would read it, so the
+// results are that storage's bits. Builds the point table and near lists on
+// first use (host synchronisation: not inside a stream capture).
+int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
+                          const int* stop);
 
 // Operator rows between the host and HBM (ibim_hostio.hip). upload_rows fills
 // ctx->mat (count rows at ctx->ld, precision IBIM_C128 or IBIM_C64, allocated

@@ -213,6 +213,36 @@ __global__ void __launch_bounds__(kPT) k_near_fill(const double* __restrict__ tu
     }
 }
 
+// Near pairs per point without an assembly (the matrix-free pair operator):
+// k_pair_assemble's count, the same decision on the same FP64 distance.
+__global__ void __launch_bounds__(kPT) k_near_count(const double* __restrict__ tube, int count,
+                                                    const int2* __restrict__ tiles, double tau,
+                                                    int* __restrict__ ncnt) {
+    const int2 tl = tiles[blockIdx.x];
+    const int r0 = tl.x * kPR, c0 = r0 + tl.y * kPC;
+    const size_t cnt = (size_t)count;
+    PointData b[kPCpt];
+#pragma unroll
+    for (int m = 0; m < kPCpt; ++m) {
+        const int j = c0 + threadIdx.x + kPT * m;
+        b[m] = j < count ? load_point(tube, cnt, j) : PointData{};
+    }
+    const int rb = blockIdx.y * kPairAsmRows;
+    for (int r = rb; r < rb + kPairAsmRows; ++r) {
+        const int i = r0 + r;
+        if (i >= count) break;
+        const PointData a = load_point(tube, cnt, i);
+#pragma unroll
+        for (int m = 0; m < kPCpt; ++m) {
+            const int j = c0 + threadIdx.x + kPT * m;
+            if (i < j && j < count && !(pair_rho(a, b[m]) > tau)) {
+                atomicAdd(&ncnt[i], 1);
+                atomicAdd(&ncnt[j], 1);
+            }
+        }
+    }
+}
+
 __global__ void __launch_bounds__(256) k_near_sort(const int* __restrict__ off, int* __restrict__ idx, int count) {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
@@ -427,12 +457,16 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
 constexpr int kBatchK = 64;  // columns staged per step
 static_assert(kPR % 64 == 0 && kPC % 64 == 0, "a workgroup's rows share their tiles");
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves>
+// Computed: no stored factors (F unused); every visit evaluates F of the pair
+// it would have read -- with the assembly's operations, so the products are
+// the complex128 pair storage's bits -- the FP64 matrix-free operator.
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves, bool Computed>
 __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
                                                            const PairPoint* __restrict__ pts, int count,
-                                                           const double2* __restrict__ P, const double2* __restrict__ Q,
-                                                           double2* __restrict__ Y, double2* __restrict__ Z,
-                                                           const int* __restrict__ stop) {
+                                                           ibim_params prm, const double2* __restrict__ P,
+                                                           const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                           double2* __restrict__ Z, const int* __restrict__ stop) {
+    static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
     constexpr int R = kBatchRHS;
     constexpr int kBatchRows = 16 * Waves, kThreads = 64 * Waves;
@@ -460,6 +494,26 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
         if (threadIdx.x < kBatchK) sp[threadIdx.x] = pts[j0 + threadIdx.x];  // j0 + 63 < count + kPC
         __syncthreads();
     };
+    // F of pair (i, j) as k_pair_assemble stores it (0 when not `keep`: the
+    // visit's zero pattern, the near field and everything past count)
+    const double sigma = prm.interior ? -1.0 : 1.0;
+    auto factor = [&](int jj, bool keep) -> MT {
+        MT f;
+        f.x = 0.0;
+        f.y = 0.0;
+        if (keep) {
+            const PairPoint& C = sp[jj];
+            const double d1 = Ri.x - C.x, d2 = Ri.y - C.y, d3 = Ri.z - C.z;
+            const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+            if (rho > prm.tau) {
+                double c1, c2;
+                grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                f.x = sigma * c1;
+                f.y = sigma * c2;
+            }
+        }
+        return f;
+    };
     // one k-step: this lane's entry F (row i, column j = staged index jj)
     auto apply = [&](const MT f, int jj) {
         const PairPoint& C = sp[jj];
@@ -481,28 +535,37 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
     for (int J = 0; J <= I; ++J) {
         const int cJ = rowW - J * kPR;  // column of rowW relative to row block J
         const int t = cJ / kPC;
-        const MT* T = F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
+        const MT* T = Computed ? nullptr
+                               : F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
         for (int rj0 = 0; rj0 < kPR; rj0 += kBatchK) {
             stage(J * kPR + rj0);
 #pragma unroll 8
             for (int s = 0; s < kBatchK / 4; ++s) {
                 const int jj = 4 * s + lk;
-                apply(T[(size_t)(rj0 + jj) * kPC], jj);
+                if (Computed)
+                    apply(factor(jj, J * kPR + rj0 + jj < i && i < count), jj);
+                else
+                    apply(T[(size_t)(rj0 + jj) * kPC], jj);
             }
         }
     }
     // j >= rows: row i of the row block's tiles
     const int nt = blk_start[I + 1] - blk_start[I];
-    const MT* Trow = F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
+    const MT* Trow = Computed ? nullptr : F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
     for (int t = 0; t < nt; ++t) {
         const int c0 = I * kPR + t * kPC;
-        const MT* Tr = Trow + (size_t)t * kPR * kPC;
+        const MT* Tr = Computed ? nullptr : Trow + (size_t)t * kPR * kPC;
         for (int cb = 0; cb < kPC; cb += kBatchK) {
             stage(c0 + cb);
 #pragma unroll 8
             for (int s = 0; s < kBatchK / 4; ++s) {
                 const int jj = 4 * s + lk;
-                apply(Tr[cb + jj], jj);
+                if (Computed) {
+                    const int j = c0 + cb + jj;
+                    apply(factor(jj, i < j && j < count), jj);
+                } else {
+                    apply(Tr[cb + jj], jj);
+                }
             }
         }
     }
@@ -569,7 +632,7 @@ __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restric
     }
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
 void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     const int count = ctx->count;
@@ -578,20 +641,20 @@ void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* 
     const int* blk = ctx->pair_blk.as<int>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
     if (waves == 1)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1>), grid, block, 0, ctx->stream, F, blk, pts,
-                           count, P, Q, Y, Z, stop);
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1, Computed>), grid, block, 0, ctx->stream, F,
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else if (waves == 2)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2>), grid, block, 0, ctx->stream, F, blk, pts,
-                           count, P, Q, Y, Z, stop);
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2, Computed>), grid, block, 0, ctx->stream, F,
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4>), grid, block, 0, ctx->stream, F, blk, pts,
-                           count, P, Q, Y, Z, stop);
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4, Computed>), grid, block, 0, ctx->stream, F,
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
 }
 
 // The workgroup shape of the far pass: IBIM_BATCH_WAVES (1, 2, 4) if set, else
 // the fastest of the three timed on this call's own operands (once per
 // context, storage and product kind; the outputs are the same bits).
-template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
 int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
                 const int* stop) {
     if (const char* env = getenv("IBIM_BATCH_WAVES")) {
@@ -599,6 +662,10 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
         if (w == 1 || w == 2 || w == 4) return w;
     }
     if (ctx->batch_waves[slot]) return ctx->batch_waves[slot];
+    // no timing inside a stream capture (events would be captured): any shape
+    // gives the same bits
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(ctx->stream, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return 4;
     // two interleaved rounds, each shape's faster time: other streams sharing
     // the GPU disturb single timings
     const int cand[3] = {4, 2, 1};
@@ -612,7 +679,7 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
     for (int round = 0; round < 2; ++round)
         for (int c = 0; c < 3; ++c) {
             (void)hipEventRecord(e0, ctx->stream);
-            launch_pair_batch_far<MT, Dirichlet, WithY, WithZ>(ctx, cand[c], F, P, Q, Y, Z, stop);
+            launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Computed>(ctx, cand[c], F, P, Q, Y, Z, stop);
             (void)hipEventRecord(e1, ctx->stream);
             float ms = 0.0f;
             if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
@@ -630,30 +697,33 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
     return cand[pick - 1];
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
 void launch_pair_batch(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y,
                        double2* Z, const int* stop) {
     const int count = ctx->count;
-    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ>(ctx, slot, F, P, Q, Y, Z, stop);
-    TimedScope ts(ctx, slot >= 6 ? "batch_pair64" : "batch_pair");
-    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ>(ctx, waves, F, P, Q, Y, Z, stop);
+    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ, Computed>(ctx, slot, F, P, Q, Y, Z, stop);
+    TimedScope ts(ctx, Computed ? "batch_free64" : slot >= 6 ? "batch_pair64" : "batch_pair");
+    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Computed>(ctx, waves, F, P, Q, Y, Z, stop);
     hipLaunchKernelGGL((k_pair_near_batch<Dirichlet, WithY, WithZ>),
                        dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
                        stop);
 }
 
-// slot: storage (0 complex64, 3 complex128, 6 complex128 copy) + product kind
-template <class MT, bool Dirichlet>
+// slot: storage (0 complex64, 3 complex128, 6 complex128 copy, 9 computed) +
+// product kind
+template <class MT, bool Dirichlet, bool Computed = false>
 void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     if (Y && Z)
-        launch_pair_batch<MT, Dirichlet, true, true>(ctx, slot0, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, true, Computed>(ctx, slot0, F, P, Q, Y, Z, stop);
     else if (Y)
-        launch_pair_batch<MT, Dirichlet, true, false>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, false, Computed>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
     else
-        launch_pair_batch<MT, Dirichlet, false, true>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, false, true, Computed>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
 }
+
+int near_lists(Context* ctx, dim3 grid);
 
 template <class MT>
 int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) {
@@ -673,7 +743,15 @@ int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) 
                            with64 ? ctx->mat64.as<double2>() : nullptr, cnt);
     }
     IBIM_HIP(hipGetLastError());
-    // near-neighbour list offsets (exclusive scan on the host: count ints)
+    return near_lists(ctx, grid);
+}
+
+// The near-neighbour lists from the per-point counts in near_cur.
+int near_lists(Context* ctx, dim3 grid) {
+    const int count = ctx->count;
+    const double* tube = ctx->tube.as<double>();
+    int* cnt = ctx->near_cur.as<int>();
+    // offsets: exclusive scan on the host (count ints)
     std::vector<int> h((size_t)count + 1, 0);
     IBIM_HIP(hipMemcpyAsync(h.data(), cnt, sizeof(int) * count, hipMemcpyDeviceToHost, ctx->stream));
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
@@ -701,6 +779,30 @@ int assemble_pair_as(Context* ctx, const std::vector<int2>& tiles, bool with64) 
     return IBIM_OK;
 }
 
+// Tiles (I, t) of every row block, each block's first tile, and the buffers of
+// the point table and near lists; uploads the tile lists.
+int pair_layout(Context* ctx, std::vector<int2>& tiles) {
+    const int count = ctx->count;
+    std::vector<int> blk;
+    tiles.clear();
+    for (int I = 0; I * kPR < count; ++I) {
+        blk.push_back((int)tiles.size());
+        const int T = (count - I * kPR + kPC - 1) / kPC;
+        for (int t = 0; t < T; ++t) tiles.push_back(make_int2(I, t));
+    }
+    blk.push_back((int)tiles.size());
+    if (ctx->pair_tiles.ensure(sizeof(int2) * tiles.size()) || ctx->pair_blk.ensure(sizeof(int) * blk.size()) ||
+        ctx->pair_pts.ensure(sizeof(PairPoint) * ((size_t)count + kPC)) ||
+        ctx->near_off.ensure(sizeof(int) * ((size_t)count + 1)) || ctx->near_cur.ensure(sizeof(int) * count))
+        return IBIM_ERR_NOMEM;
+    IBIM_HIP(hipMemcpyAsync(ctx->pair_tiles.ptr, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    IBIM_HIP(hipMemcpyAsync(ctx->pair_blk.ptr, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice,
+                            ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));  // the host vectors go out of scope
+    return IBIM_OK;
+}
+
 }  // namespace
 
 int launch_assemble_pair(Context* ctx, int precision, bool with64) {
@@ -709,38 +811,53 @@ int launch_assemble_pair(Context* ctx, int precision, bool with64) {
         return IBIM_ERR_ARG;
     }
     const int count = ctx->count;
-    std::vector<int2> tiles;
-    std::vector<int> blk;
-    for (int I = 0; I * kPR < count; ++I) {
-        blk.push_back((int)tiles.size());
-        const int T = (count - I * kPR + kPC - 1) / kPC;
-        for (int t = 0; t < T; ++t) tiles.push_back(make_int2(I, t));
-    }
-    blk.push_back((int)tiles.size());
     const size_t esz = precision == IBIM_C128 ? sizeof(double2) : sizeof(float2);
     ctx->mat_precision = -1;
     ctx->mat_pair = ctx->mat_pair64 = false;
+    ctx->pair_aux = false;
     ctx->ld = ((size_t)count + 63) / 64 * 64;
+    std::vector<int2> tiles;
+    if (int rc = pair_layout(ctx, tiles)) return rc;
     if (ctx->mat.ensure(esz * tiles.size() * kPR * kPC) ||
-        (with64 && ctx->mat64.ensure(sizeof(double2) * tiles.size() * kPR * kPC)) ||
-        ctx->pair_tiles.ensure(sizeof(int2) * tiles.size()) ||
-        ctx->pair_blk.ensure(sizeof(int) * blk.size()) ||
-        ctx->pair_pts.ensure(sizeof(PairPoint) * ((size_t)count + kPC)) ||
-        ctx->near_off.ensure(sizeof(int) * ((size_t)count + 1)) || ctx->near_cur.ensure(sizeof(int) * count))
+        (with64 && ctx->mat64.ensure(sizeof(double2) * tiles.size() * kPR * kPC)))
         return IBIM_ERR_NOMEM;
-    IBIM_HIP(hipMemcpyAsync(ctx->pair_tiles.ptr, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice,
-                            ctx->stream));
-    IBIM_HIP(hipMemcpyAsync(ctx->pair_blk.ptr, blk.data(), sizeof(int) * blk.size(), hipMemcpyHostToDevice,
-                            ctx->stream));
     const int rc = precision == IBIM_C128 ? assemble_pair_as<double2>(ctx, tiles, false)
                                           : assemble_pair_as<float2>(ctx, tiles, with64);
     if (rc) return rc;
     ctx->pair_ntiles = (int)tiles.size();
     for (int& w : ctx->batch_waves) w = 0;  // a new operator is tuned afresh
+    ctx->pair_aux = true;
     ctx->mat_precision = precision;
     ctx->mat_pair = true;
     ctx->mat_pair64 = with64 && precision != IBIM_C128;
     ctx->regcount = 0;
+    return IBIM_OK;
+}
+
+// The pair layout, point table and near lists of the current tube without the
+// factors (kept until the tube changes; a pair assembly builds the same).
+int prepare_pair_free(Context* ctx) {
+    if (ctx->pair_aux) return IBIM_OK;
+    if (ctx->kind != IBIM_TUBE_TRADITIONAL || ctx->count <= 0) {
+        set_error("matrix-free pair products need the traditional tube");
+        return IBIM_ERR_ARG;
+    }
+    const int count = ctx->count;
+    std::vector<int2> tiles;
+    if (int rc = pair_layout(ctx, tiles)) return rc;
+    const double* tube = ctx->tube.as<double>();
+    IBIM_HIP(hipMemsetAsync(ctx->near_cur.ptr, 0, sizeof(int) * count, ctx->stream));
+    const int npad = count + kPC;
+    hipLaunchKernelGGL(k_pair_points, dim3((unsigned)((npad + 255) / 256)), dim3(256), 0, ctx->stream, tube, count,
+                       ctx->prm.dirichlet != 0, ctx->pair_pts.as<PairPoint>(), npad);
+    const dim3 grid((unsigned)tiles.size(), kPR / kPairAsmRows);
+    hipLaunchKernelGGL(k_near_count, grid, dim3(kPT), 0, ctx->stream, tube, count, ctx->pair_tiles.as<int2>(),
+                       ctx->prm.tau, ctx->near_cur.as<int>());
+    IBIM_HIP(hipGetLastError());
+    if (int rc = near_lists(ctx, grid)) return rc;
+    ctx->pair_ntiles = (int)tiles.size();
+    for (int w = 9; w < 12; ++w) ctx->batch_waves[w] = 0;
+    ctx->pair_aux = true;
     return IBIM_OK;
 }
 
@@ -806,6 +923,23 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
         else
             launch_pair_batch_any<float2, false>(ctx, 0, F, P, Q, Y, Z, stop);
     }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+// The FP64 matrix-free batched products: the fused pair pass evaluating each
+// factor where the complex128 pair storage would read it (the same bits).
+int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
+                          const int* stop) {
+    if ((!Y && !Z) || (Y && !P) || (Z && !Q)) {
+        set_error("batched matrix-free pair products without the vectors they need");
+        return IBIM_ERR_ARG;
+    }
+    if (int rc = prepare_pair_free(ctx)) return rc;
+    if (ctx->prm.dirichlet != 0)
+        launch_pair_batch_any<double2, true, true>(ctx, 9, nullptr, P, Q, Y, Z, stop);
+    else
+        launch_pair_batch_any<double2, false, true>(ctx, 9, nullptr, P, Q, Y, Z, stop);
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

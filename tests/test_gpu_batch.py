@@ -98,6 +98,47 @@ def test_batched_pair_pass_shapes_give_the_same_bits(gpu, monkeypatch):
             assert np.array_equal(x[c][0], runs[0][4][c][0]) and np.array_equal(x[c][1], runs[0][4][c][1]), (w, c)
 
 
+
+@pytest.mark.parametrize("dirichlet,interior", [(1, 0), (0, 0), (1, 1)])
+def test_matrix_free_batched_pass_gives_the_pair_storage_bits(gpu, dirichlet, interior):
+    """The FP64 matrix-free batched products evaluate each pair's factor where
+    the complex128 pair storage reads it, with the assembly's operations, so
+    products and batched solves are that storage's bits. The matrix-free
+    problem builds its point table and near lists alone, and again when its
+    tube changes (a wider near field)."""
+    def make(tau_h):
+        pb = gpu.Problem()
+        pb.set_level(shapes.sphere(N))
+        pb.geometry(5)
+        h = 4.0 / N
+        pb.tube(gpu.params(4.0, 3 * h, tau_h * h, dirichlet=dirichlet, interior=interior))
+        return pb
+
+    rng = np.random.default_rng(21 + dirichlet + 2 * interior)
+    free = make(2.0)
+    for tau_h in (2.0, 2.5):
+        if tau_h != 2.0:
+            h = 4.0 / N
+            free.tube(gpu.params(4.0, 3 * h, tau_h * h, dirichlet=dirichlet, interior=interior))
+        stored = make(tau_h)
+        stored.assemble(gpu.C128_PAIR)
+        assert free.count == stored.count
+        P = rng.standard_normal((free.count, 16)) + 1j * rng.standard_normal((free.count, 16))
+        for adjoint in (False, True):
+            got = free.apply_batch(P, gpu.OP_FREE64, adjoint)
+            assert np.array_equal(got, stored.apply_batch(P, gpu.OP_STORED, adjoint)), (tau_h, adjoint)
+            assert np.array_equal(got, free.apply_batch(P, gpu.OP_FREE64, adjoint))  # fixed order
+        fill_rhs(gpu, free, 19)
+        fill_rhs(gpu, stored, 19)
+        ia, _ = free.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+        ib, _ = stored.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+        assert ia.status == 0 and (ia.iterations, ia.status) == (ib.iterations, ib.status), tau_h
+        for c in range(19):
+            a, b = free.density(c), stored.density(c)
+            assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), (tau_h, c)
+        stored.close()
+    free.close()
+
 def fill_rhs(gpu, pb, n):
     pb.set_nrhs(n)
     rng = np.random.default_rng(7)
