@@ -377,9 +377,16 @@ int ibim_apply(ibim_ctx* ctx, int op, const double* pre, const double* pim, cons
 int ibim_apply_batch(ibim_ctx* ctx, int op, int adjoint, int nvec, const double* pre, const double* pim, double* yre,
                      double* yim) {
     CHECK(ctx);
-    if (!ctx->count || nvec < 1 || nvec > 16 || !pre || !yre || !yim || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 ||
-        (op == IBIM_OP_STORED && ctx->mat_precision < 0) || (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL))
+    if (!ctx->count || nvec < 1 || nvec > 16 || !pre || !yre || !yim || op < IBIM_OP_STORED || op > IBIM_OP_FREE32)
         return IBIM_ERR_ARG;
+    if (op == IBIM_OP_STORED && ctx->mat_precision < 0) {
+        set_error("ibim_apply_batch: no stored operator (assemble one, or use a matrix-free op)");
+        return IBIM_ERR_ARG;
+    }
+    if (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL) {
+        set_error("ibim_apply_batch: matrix-free products need the traditional tube");
+        return IBIM_ERR_ARG;
+    }
     const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
     return batch_apply_host(ctx, kinds[op], adjoint != 0, nvec, pre, pim, yre, yim);
 }

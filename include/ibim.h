@@ -200,8 +200,12 @@ int ibim_apply(ibim_ctx *ctx, int op, const double *pre, const double *pim, cons
                double *yre, double *yim, double *zre, double *zim);
 /* Y = A P (adjoint = 0) or Y = A^H P (adjoint = 1) for nvec <= 16 vectors at
  * once (host arrays, column-major, count per column, split re / im), on the
- * FP64 matrix cores: the product the batched BiCG applies per pass. No
- * counterpart in the reference (it solves one right-hand side at a time). */
+ * FP64 matrix cores: the product the batched BiCG applies per pass. Matrix-free
+ * ops need the traditional tube; IBIM_OP_FREE64 computes each factor where
+ * complex128 pair storage would read it (the same bits as that storage) after
+ * building a point table and near lists on first use, kept until the tube
+ * changes. No counterpart in the reference (it solves one right-hand side at a
+ * time). */
 int ibim_apply_batch(ibim_ctx *ctx, int op, int adjoint, int nvec, const double *pre, const double *pim, double *yre,
                      double *yim);
 

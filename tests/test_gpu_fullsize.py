@@ -7,6 +7,8 @@ properties that hold at any size, between kernels that share no code path:
   (to 1e-14 and 1e-7 of the row's largest entry);
 * <A p, q> = <p, A^H q> for the fused sweeps of every storage, and the 16-column
   fused pair pass against the single-vector sweep, column by column;
+* the FP64 matrix-free 16-column pass (factors computed, tables built without an
+  assembly) against the complex128 pair storage's pass, bit for bit;
 * the mixed-precision solve's density against the dense FP64 operator: its true
   residual ||b - A x|| / ||b|| is below the requested tolerance.
 """
@@ -20,14 +22,19 @@ pytestmark = pytest.mark.gpu
 N, K = 128, 10.0
 
 
-@pytest.fixture(scope="module")
-def torus(gpu):
+def make_torus(gpu):
     pb = gpu.Problem()
     pb.set_level(shapes.torus(N))
     pb.geometry(5)
     h = 4.0 / N
     count = pb.tube(gpu.params(K, 2 * h, 2 * h, dirichlet=1))
     assert count == 64676
+    return pb
+
+
+@pytest.fixture(scope="module")
+def torus(gpu):
+    pb = make_torus(gpu)
     pb.rhs(gpu.RHS_POINT_SOURCE, x0=[1.0, 0.0, 0.05])
     yield pb
     pb.close()
@@ -87,6 +94,17 @@ def test_fused_batched_pass_matches_single_sweeps_at_full_size(gpu, torus):
         y, z = pb.apply(P[:, c], P[:, c], gpu.OP_STORED)
         assert relnorm(Y[:, c], y) <= 1e-13, c
         assert relnorm(Z[:, c], z) <= 1e-13, c
+
+
+def test_matrix_free_batched_pass_is_the_pair_storage_at_full_size(gpu, torus):
+    free = make_torus(gpu)  # never assembled: its own point table and near lists
+    torus.assemble(gpu.C128_PAIR)
+    rng = np.random.default_rng(8)
+    P = rng.standard_normal((free.count, 16)) + 1j * rng.standard_normal((free.count, 16))
+    for adjoint in (False, True):
+        got = free.apply_batch(P, gpu.OP_FREE64, adjoint)
+        assert np.array_equal(got, torus.apply_batch(P, gpu.OP_STORED, adjoint)), adjoint
+    free.close()
 
 
 def test_mixed_solve_meets_its_tolerance_on_the_fp64_operator(gpu, torus):
