@@ -475,6 +475,7 @@ unsigned vec_blocks(int count) {
 static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
                      const int* done) {
     if (op == OP_PAIR64) return sweep_pair_launch(ctx, true, p, q, ypart, zpart, done);
+    if (free_pair_op(ctx, op)) return sweep_pair_free_launch(ctx, p, q, ypart, zpart, done);
     if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
     if (ctx->mat_pair) return sweep_pair_launch(ctx, false, p, q, ypart, zpart, done);
     const int count = ctx->count;
@@ -504,8 +505,10 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
     const int nct = (count + kMvCols - 1) / kMvCols, nrt = (count + kMvRows - 1) / kMvRows;
     const unsigned nvb = vec_blocks(count);
     const size_t vbytes = sizeof(double2) * ld;
-    // pair storage: row-side, column-side and near planes for v and w alike
-    const size_t nplanes = ctx->mat_pair ? 2 * (size_t)(nct + nrt + 1) : (size_t)(nct + nrt);
+    // pair storage and the FP64 matrix-free operator of a traditional tube:
+    // row-side, column-side and near planes for v and w alike
+    const bool pairs = ctx->mat_pair || ctx->kind == IBIM_TUBE_TRADITIONAL;
+    const size_t nplanes = pairs ? 2 * (size_t)(nct + nrt + 1) : (size_t)(nct + nrt);
     if (ctx->work.ensure(vbytes * 7) || ctx->partial.ensure(sizeof(double2) * ld * nplanes) ||
         ctx->scal.ensure(sizeof(BicgScalars) * (size_t)(maxit + 1) + sizeof(double) * (size_t)nvb * 16 + 64) ||
         ctx->done.ensure(sizeof(int) * 4))
@@ -540,7 +543,7 @@ int prepare(Context* ctx, int maxit, Workspace* ws) {
 // The workspace's vector arguments with the partial-plane layout of operator op.
 VecArgs with_layout(const VecArgs& base, const Context* ctx, int op) {
     VecArgs a = base;
-    a.pair = (op == OP_DENSE && ctx->mat_pair) || op == OP_PAIR64 ? 1 : 0;
+    a.pair = pair_layout_op(ctx, op) ? 1 : 0;
     a.zpart = a.ypart + a.ld * (size_t)(a.pair ? a.ncol_tiles + a.nrow_tiles + 1 : a.ncol_tiles);
     return a;
 }
@@ -584,6 +587,9 @@ int true_residual(Context* ctx, const VecArgs& base, unsigned nvb, Replace* rep,
 int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* x_out, int maxit, double tol,
                 int chunk, ibim_solve_info* info, double* hist, bool adjoint = false, Replace* rep = nullptr,
                 bool ref_guards = true) {
+    // the matrix-free pair tables are built here, before any graph capture
+    if (free_pair_op(ctx, op) || (rep && free_pair_op(ctx, rep->true_op)))
+        if (int rc = prepare_pair_free(ctx)) return rc;
     VecArgs a = with_layout(ws.a, ctx, op);
     a.b = b;
     a.iter = ctx->done.as<int>() + 1;

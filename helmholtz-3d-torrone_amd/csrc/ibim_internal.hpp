@@ -194,6 +194,12 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
 // first use (host synchronisation: not inside a stream capture).
 int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
                           const int* stop);
+// The single-vector sweep of the same operator into the pair planes (one
+// factor evaluation per pair; the complex128 pair sweep's bits).
+int sweep_pair_free_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done);
+// Point table and near lists of the matrix-free pair operator (built once per
+// tube; host synchronisation, so before any stream capture).
+int prepare_pair_free(Context* ctx);
 
 // Operator rows between the host and HBM (ibim_hostio.hip). upload_rows fills
 // ctx->mat (count rows at ctx->ld, precision IBIM_C128 or IBIM_C64, allocated
@@ -225,6 +231,15 @@ enum OpKind { OP_DENSE = 0, OP_FREE64 = 1, OP_FREE32 = 2, OP_PAIR64 = 3 };
 inline int fp64_operator(const Context* ctx) {
     if (ctx->mat_pair64) return OP_PAIR64;
     return ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
+}
+// The FP64 matrix-free operator of a traditional tube sweeps as pair storage
+// with computed factors (sweep_pair_free_launch): the pair plane layout.
+inline bool free_pair_op(const Context* ctx, int op) {
+    return op == OP_FREE64 && ctx->kind == IBIM_TUBE_TRADITIONAL;
+}
+// Sweeps of op write the pair storage's partial-plane layout.
+inline bool pair_layout_op(const Context* ctx, int op) {
+    return (op == OP_DENSE && ctx->mat_pair) || op == OP_PAIR64 || free_pair_op(ctx, op);
 }
 // Tile sizes shared by every sweep (the partial planes have these shapes).
 constexpr int kSweepCols = 1024;

@@ -259,12 +259,18 @@ __global__ void __launch_bounds__(256) k_near_sort(const int* __restrict__ off, 
 }
 
 // One tile of the pair sweep (see the file comment for the planes written).
-template <class MT, bool Dirichlet>
+// Computed: no stored factors (F unused); each pair's F is evaluated where it
+// would be read, with k_pair_assemble's operations, so the sweep is the
+// complex128 pair storage's bits -- the FP64 matrix-free operator, one factor
+// evaluation per pair.
+template <class MT, bool Dirichlet, bool Computed = false>
 __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
                                                     const PairPoint* __restrict__ pts, int count, size_t ld,
                                                     int ncol, const double2* __restrict__ p,
                                                     const double2* __restrict__ q, double2* __restrict__ ypart,
-                                                    double2* __restrict__ zpart, const int* __restrict__ done) {
+                                                    double2* __restrict__ zpart, const int* __restrict__ done,
+                                                    ibim_params prm) {
+    static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     if (done != nullptr && *done) return;
     __shared__ double sh[4][kPT / 64][kPR];  // (v re, v im, w re, w im) x wave x row
     const int2 tl = tiles[blockIdx.x];
@@ -296,20 +302,25 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
         qim[m] = qv.y;
         vre[m] = vim[m] = wre[m] = wim[m] = 0.0;
     }
-    const MT* Fb = F + (size_t)blockIdx.x * kPR * kPC + t;
+    const MT* Fb = Computed ? nullptr : F + (size_t)blockIdx.x * kPR * kPC + t;
     // the next two rows' values are loaded while the current two are applied
     MT nxt[2][kPCpt];
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-#pragma unroll
-        for (int m = 0; m < kPCpt; ++m) nxt[u][m] = load_stream(Fb + (size_t)u * kPC + kPT * m);
-    for (int rr = 0; rr < kPR; rr += 2) {
-        MT a[2][kPCpt];
+    if (!Computed) {
 #pragma unroll
         for (int u = 0; u < 2; ++u)
 #pragma unroll
-            for (int m = 0; m < kPCpt; ++m) a[u][m] = nxt[u][m];
-        if (rr + 2 < kPR) {
+            for (int m = 0; m < kPCpt; ++m) nxt[u][m] = load_stream(Fb + (size_t)u * kPC + kPT * m);
+    }
+    const double sigma = prm.interior ? -1.0 : 1.0;
+    for (int rr = 0; rr < kPR; rr += 2) {
+        MT a[2][kPCpt];
+        if (!Computed) {
+#pragma unroll
+            for (int u = 0; u < 2; ++u)
+#pragma unroll
+                for (int m = 0; m < kPCpt; ++m) a[u][m] = nxt[u][m];
+        }
+        if (!Computed && rr + 2 < kPR) {
 #pragma unroll
             for (int u = 0; u < 2; ++u)
 #pragma unroll
@@ -325,6 +336,23 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
             double svr = 0.0, svi = 0.0, swr = 0.0, swi = 0.0;
 #pragma unroll
             for (int m = 0; m < kPCpt; ++m) {
+                if (Computed) {
+                    // F of pair (i, j), 0 where the storage holds 0 (j <= i,
+                    // past count, the near field)
+                    const int j = c0 + t + kPT * m;
+                    a[u][m].x = 0.0;
+                    a[u][m].y = 0.0;
+                    if (i < j && j < count) {
+                        const double d1 = R.x - cx[m], d2 = R.y - cy[m], d3 = R.z - cz[m];
+                        const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
+                        if (rho > prm.tau) {
+                            double c1, c2;
+                            grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                            a[u][m].x = sigma * c1;
+                            a[u][m].y = sigma * c2;
+                        }
+                    }
+                }
                 double al, be;
                 pair_factors<Dirichlet, sizeof(MT) == sizeof(double2)>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m],
                                                                        cmz[m], cc[m], cw[m], &al, &be);
@@ -891,12 +919,41 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
             auto kern = dir ? k_pair_sweep<double2, true> : k_pair_sweep<double2, false>;
             hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
                                (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(), tiles, pts, count, ld, ncol, P, Q,
-                               Y, Z, done);
+                               Y, Z, done, ctx->prm);
         } else {
             auto kern = dir ? k_pair_sweep<float2, true> : k_pair_sweep<float2, false>;
             hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
-                               ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done);
+                               ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
         }
+    }
+    {
+        TimedScope ts(ctx, "near");
+        const size_t np = (size_t)(ncol + nrow) * ld;
+        auto kern = dir ? k_pair_near<true> : k_pair_near<false>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+                           ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q,
+                           Y + np, Z + np, done);
+    }
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int sweep_pair_free_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+    if (int rc = prepare_pair_free(ctx)) return rc;
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const int ncol = (count + kPC - 1) / kPC, nrow = (count + kPR - 1) / kPR;
+    auto P = (const double2*)p;
+    auto Q = (const double2*)q;
+    auto Y = (double2*)ypart;
+    auto Z = (double2*)zpart;
+    const bool dir = ctx->prm.dirichlet != 0;
+    {
+        TimedScope ts(ctx, "sweep_free64");
+        auto kern = dir ? k_pair_sweep<double2, true, true> : k_pair_sweep<double2, false, true>;
+        hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream, nullptr,
+                           ctx->pair_tiles.as<int2>(), ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z,
+                           done, ctx->prm);
     }
     {
         TimedScope ts(ctx, "near");

@@ -7,7 +7,9 @@ reference (test_gpu_parity): rebuilt rows entry by entry to 1e-14 relative
 than w_j (c_1 r.n_j)) and to 1e-7 of the largest entry (complex FP32 pairs,
 whose point factors use the cheaper x_i.m_j - c_j); sweeps to 1e-13 in norm
 against the rebuilt rows; solutions, residual-replacement solves and the misfit
-gradient against the dense operator's.
+gradient against the dense operator's. The FP64 matrix-free operator, which
+sweeps as complex128 pairs with computed factors, against that storage bit for
+bit.
 """
 import numpy as np
 import pytest
@@ -176,6 +178,53 @@ def test_pair_gradient_matches_dense_gradient(gpu, dirichlet, interior, mode):
         assert np.abs(g1 - g0).max() <= 1e-8 * np.abs(g0).max(), prec
         assert abs(dk1 - dk0) <= 1e-8 * abs(dk0), prec
 
+
+
+@pytest.mark.parametrize("dirichlet,interior", [(1, 0), (0, 0), (1, 1)])
+def test_matrix_free_sweep_is_the_pair_storage(gpu, dirichlet, interior):
+    """The FP64 matrix-free operator of a traditional tube sweeps as complex128
+    pair storage with each factor computed where it would be read (once per
+    pair): its products, BiCG and residual-replacement solves (FP32 inner
+    sweeps, FP64 replacements) and the adjoint gradient are that storage's
+    bits -- the gradient's grid scatter, made with atomics, to 1e-12."""
+    n, k = 24, 3.0
+    h = 4.0 / n
+    center = (0.0137, -0.0219, 0.0071)
+    star = gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.15, m=3, n=2, variant=0, center=center)
+    v = np.random.default_rng(0).standard_normal((40, 3))
+    recv = 2.5 * v / np.linalg.norm(v, axis=1, keepdims=True)
+    target = np.zeros(len(recv), complex) + 0.01
+    out = []
+    for stored in (False, True):
+        pb = gpu.Problem()
+        pb.set_level_shape(star, n)
+        pb.geometry(5)
+        pb.tube(gpu.params(k, 3 * h, 2 * h, dirichlet=dirichlet, interior=interior))
+        pb.rhs(gpu.RHS_POINT_SOURCE, x0=(0.1, 0.2, 0.3))
+        if stored:
+            pb.assemble(gpu.C128_PAIR)
+        rng = np.random.default_rng(4)
+        p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+        q = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+        y, z = pb.apply(p, q, gpu.OP_STORED if stored else gpu.OP_FREE64)
+        rinfo, rhist = pb.solve(gpu.SOLVE_RELIABLE, tol=1e-11, maxit=2000, inner_tol=1e-3)
+        xr = pb.density()
+        info, hist = pb.solve(tol=1e-13, maxit=2000)
+        x = pb.density()
+        assert info.status == 0 and rinfo.status == 0
+        pb.set_receivers(recv, target)
+        J, g, dk = pb.gradient(tol=1e-13, maxit=2000)
+        out.append(dict(y=y, z=z, its=(info.iterations, rinfo.iterations, rinfo.outer), hist=np.asarray(hist),
+                        rhist=np.asarray(rhist), x=x, xr=xr, J=J, g=g, dk=dk))
+        pb.close()
+    free, pair = out
+    assert np.array_equal(free["y"], pair["y"]) and np.array_equal(free["z"], pair["z"])
+    assert free["its"] == pair["its"]
+    assert np.array_equal(free["hist"], pair["hist"]) and np.array_equal(free["rhist"], pair["rhist"])
+    for key in ("x", "xr"):
+        assert np.array_equal(free[key][0], pair[key][0]) and np.array_equal(free[key][1], pair[key][1]), key
+    assert free["J"] == pair["J"] and free["dk"] == pair["dk"]
+    assert np.abs(free["g"] - pair["g"]).max() <= 1e-12 * np.abs(pair["g"]).max()
 
 def test_pair_storage_rejects_what_it_cannot_do(gpu):
     pb = gpu.Problem().set_level(shapes.sphere(16)).geometry(5)
