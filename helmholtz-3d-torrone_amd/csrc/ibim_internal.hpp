@@ -195,8 +195,11 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
 int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
                           const int* stop);
 // The single-vector sweep of the same operator into the pair planes (one
-// factor evaluation per pair; the complex128 pair sweep's bits).
-int sweep_pair_free_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done);
+// factor evaluation per pair; the complex128 pair sweep's bits), or with fast
+// the mixed-precision matrix-free operator: FP32 sine and cosine of an FP64
+// phase in the factors, FP64 everywhere else.
+int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q, void* ypart, void* zpart,
+                           const int* done);
 // Point table and near lists of the matrix-free pair operator (built once per
 // tube; host synchronisation, so before any stream capture).
 int prepare_pair_free(Context* ctx);
@@ -232,10 +235,10 @@ inline int fp64_operator(const Context* ctx) {
     if (ctx->mat_pair64) return OP_PAIR64;
     return ctx->mat_precision == IBIM_C128 ? OP_DENSE : OP_FREE64;
 }
-// The FP64 matrix-free operator of a traditional tube sweeps as pair storage
-// with computed factors (sweep_pair_free_launch): the pair plane layout.
+// The matrix-free operators of a traditional tube sweep as pair storage with
+// computed factors (sweep_pair_free_launch): the pair plane layout.
 inline bool free_pair_op(const Context* ctx, int op) {
-    return op == OP_FREE64 && ctx->kind == IBIM_TUBE_TRADITIONAL;
+    return (op == OP_FREE64 || op == OP_FREE32) && ctx->kind == IBIM_TUBE_TRADITIONAL;
 }
 // Sweeps of op write the pair storage's partial-plane layout.
 inline bool pair_layout_op(const Context* ctx, int op) {

@@ -258,18 +258,38 @@ __global__ void __launch_bounds__(256) k_near_sort(const int* __restrict__ off, 
     }
 }
 
+// F = sigma (c1 + i c2)(rho) of grad_phi_coeffs with the phase's sine and
+// cosine from the FP32 polynomial (phase reduced in FP64, in revolutions) and
+// 1/rho from the FP32 reciprocal refined by one FP64 Newton step: the radial
+// factors of the mixed-precision operator (about one FP32 ulp of sin / cos).
+__device__ __forceinline__ void grad_phi_coeffs_fast(double k, double kappa, double rho, double* c1, double* c2) {
+    const double i0 = (double)(1.0f / (float)rho);
+    const double ir = fma(i0, fma(-rho, i0, 1.0), i0);
+    const double t = rho * kappa;
+    float s32, c32;
+    sincos_rev_f32(t - rint(t), &s32, &c32);
+    const double sv = (double)s32, cv = (double)c32;
+    const double den = (1.0 / (4.0 * kPi)) * ir * ir;
+    *c1 = -1.0 * (k * sv + cv * ir) * den;
+    *c2 = (k * cv - sv * ir) * den;
+}
+
+// Where a pair sweep takes its factors F: the stored tiles, or evaluated where
+// they would be read -- with k_pair_assemble's operations (the complex128 pair
+// storage's bits: the FP64 matrix-free operator) or with
+// grad_phi_coeffs_fast (the mixed-precision matrix-free operator); one
+// evaluation per pair either way.
+enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2 };
+
 // One tile of the pair sweep (see the file comment for the planes written).
-// Computed: no stored factors (F unused); each pair's F is evaluated where it
-// would be read, with k_pair_assemble's operations, so the sweep is the
-// complex128 pair storage's bits -- the FP64 matrix-free operator, one factor
-// evaluation per pair.
-template <class MT, bool Dirichlet, bool Computed = false>
+template <class MT, bool Dirichlet, int Factors = kStoredFactors>
 __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
                                                     const PairPoint* __restrict__ pts, int count, size_t ld,
                                                     int ncol, const double2* __restrict__ p,
                                                     const double2* __restrict__ q, double2* __restrict__ ypart,
                                                     double2* __restrict__ zpart, const int* __restrict__ done,
                                                     ibim_params prm) {
+    constexpr bool Computed = Factors != kStoredFactors;
     static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     if (done != nullptr && *done) return;
     __shared__ double sh[4][kPT / 64][kPR];  // (v re, v im, w re, w im) x wave x row
@@ -312,6 +332,7 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
             for (int m = 0; m < kPCpt; ++m) nxt[u][m] = load_stream(Fb + (size_t)u * kPC + kPT * m);
     }
     const double sigma = prm.interior ? -1.0 : 1.0;
+    const double kappa = prm.k / (2.0 * kPi);
     for (int rr = 0; rr < kPR; rr += 2) {
         MT a[2][kPCpt];
         if (!Computed) {
@@ -347,7 +368,10 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
                         const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
                         if (rho > prm.tau) {
                             double c1, c2;
-                            grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                            if (Factors == kExactFactors)
+                                grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                            else
+                                grad_phi_coeffs_fast(prm.k, kappa, rho, &c1, &c2);
                             a[u][m].x = sigma * c1;
                             a[u][m].y = sigma * c2;
                         }
@@ -938,7 +962,8 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
     return IBIM_OK;
 }
 
-int sweep_pair_free_launch(Context* ctx, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q, void* ypart, void* zpart,
+                           const int* done) {
     if (int rc = prepare_pair_free(ctx)) return rc;
     const int count = ctx->count;
     const size_t ld = ctx->ld;
@@ -949,8 +974,10 @@ int sweep_pair_free_launch(Context* ctx, const void* p, const void* q, void* ypa
     auto Z = (double2*)zpart;
     const bool dir = ctx->prm.dirichlet != 0;
     {
-        TimedScope ts(ctx, "sweep_free64");
-        auto kern = dir ? k_pair_sweep<double2, true, true> : k_pair_sweep<double2, false, true>;
+        TimedScope ts(ctx, fast ? "sweep_free32" : "sweep_free64");
+        auto kern = fast ? (dir ? k_pair_sweep<double2, true, kFastFactors> : k_pair_sweep<double2, false, kFastFactors>)
+                         : (dir ? k_pair_sweep<double2, true, kExactFactors>
+                                : k_pair_sweep<double2, false, kExactFactors>);
         hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream, nullptr,
                            ctx->pair_tiles.as<int2>(), ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z,
                            done, ctx->prm);
