@@ -15,9 +15,9 @@
 //                          loaded directly.
 // The vector operand (a step x 16 block of P or Q) is staged in LDS once per
 // workgroup and shared by its four waves.
-// Matrix-free variants evaluate the entry in the same layout instead of
-// loading it (traditional tube; FP64 library entries or the fast ones); the points of the
-// streamed index are staged in LDS per workgroup.
+// Pair storage and the matrix-free operators (traditional tube) run instead
+// the fused 16-column pair pass of ibim_pair.hip: one pass for both products,
+// the matrix-free ones evaluating each factor where the pairs would be read.
 //
 // Vectors are RHS-interleaved: element (point j, right-hand side r) at
 // [j * 16 + r], with ld rows so padding rows read as zero. Each column keeps
@@ -170,96 +170,6 @@ __global__ void __launch_bounds__(kBT) k_cols_stored(const MT* __restrict__ A, s
                 ar = (double)a.x;
                 ai = (double)a.y;
             }
-            z.add(ar, -ai, q.x, q.y);  // conj(a) q
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int c = c0 + 4 * v + lk;
-        if (c < count) Z[(size_t)c * kR + li] = make_double2(z.re(v), z.im(v));
-    }
-}
-
-// ------------------------------------------------------- matrix-free passes --
-template <int Op>
-__device__ __forceinline__ void free_entry(const PointData& a, const PointData& b, bool diag, const ibim_params& prm,
-                                           double kappa, double tau2, double* re, double* im) {
-    if (Op == OP_FREE32)
-        trad_entry_fast(a, b, diag, prm.k, kappa, tau2, prm.dirichlet, prm.interior, re, im);
-    else
-        trad_entry(a, b, diag, prm.k, prm.tau, prm.dirichlet, prm.interior, re, im);
-}
-
-template <int Op>
-__global__ void __launch_bounds__(kBT) k_rows_free(const double* __restrict__ tube, int count, ibim_params prm,
-                                                   const double2* __restrict__ P, double2* __restrict__ Y,
-                                                   const BState* __restrict__ st) {
-    if (st != nullptr && st->all_done) return;
-    __shared__ PointData cp[kKT];
-    __shared__ double2 ps[kKT][kR];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int row0 = (blockIdx.x * 4 + w) * 16;
-    const size_t cnt = (size_t)count;
-    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
-    const int row = row0 + li;
-    const bool rok = row < count;
-    const PointData a = load_point(tube, cnt, rok ? row : 0);
-    Acc3 y;
-    for (int k0 = 0; k0 < count; k0 += kKT) {
-        __syncthreads();
-        if (threadIdx.x < kKT) {
-            const int j = k0 + threadIdx.x;
-            cp[threadIdx.x] = load_point(tube, cnt, j < count ? j : 0);
-        }
-        for (int e = threadIdx.x; e < kKT * kR; e += kBT) ps[e / kR][e % kR] = P[(size_t)k0 * kR + e];  // rows < ld
-        __syncthreads();
-#pragma unroll 2
-        for (int s = 0; s < kKT / 4; ++s) {
-            const int j = k0 + 4 * s + lk;
-            double er = 0.0, ei = 0.0;
-            if (rok && j < count) free_entry<Op>(a, cp[4 * s + lk], row == j, prm, kappa, tau2, &er, &ei);
-            const double2 p = ps[4 * s + lk][li];
-            y.add(er, ei, p.x, p.y);
-        }
-    }
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-        const int r = row0 + 4 * v + lk;
-        if (r < count) Y[(size_t)r * kR + li] = make_double2(y.re(v), y.im(v));
-    }
-}
-
-template <int Op>
-__global__ void __launch_bounds__(kBT) k_cols_free(const double* __restrict__ tube, int count, ibim_params prm,
-                                                   const double2* __restrict__ Q, double2* __restrict__ Z,
-                                                   const BState* __restrict__ st) {
-    if (st != nullptr && st->all_done) return;
-    __shared__ PointData rp[kKT];
-    __shared__ double2 qs[kKT][kR];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int li = lane & 15, lk = lane >> 4;
-    const int c0 = (blockIdx.x * 4 + w) * 16;
-    const size_t cnt = (size_t)count;
-    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau;
-    const int col = c0 + li;
-    const bool cok = col < count;
-    const PointData b = load_point(tube, cnt, cok ? col : 0);
-    Acc3 z;
-    for (int i0 = 0; i0 < count; i0 += kKT) {
-        __syncthreads();
-        if (threadIdx.x < kKT) {
-            const int i = i0 + threadIdx.x;
-            rp[threadIdx.x] = load_point(tube, cnt, i < count ? i : 0);
-        }
-        for (int e = threadIdx.x; e < kKT * kR; e += kBT) qs[e / kR][e % kR] = Q[(size_t)i0 * kR + e];  // rows < ld
-        __syncthreads();
-#pragma unroll 2
-        for (int s = 0; s < kKT / 4; ++s) {
-            const int i = i0 + 4 * s + lk;
-            double ar = 0.0, ai = 0.0;
-            if (cok && i < count) free_entry<Op>(rp[4 * s + lk], b, i == col, prm, kappa, tau2, &ar, &ai);
-            const double2 q = qs[4 * s + lk][li];
             z.add(ar, -ai, q.x, q.y);  // conj(a) q
         }
     }
@@ -529,44 +439,34 @@ int batch_apply(Context* ctx, int op, bool adjoint, const double2* P, double2* Y
     if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
         return adjoint ? pair_batch_apply(ctx, op == OP_PAIR64, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
                        : pair_batch_apply(ctx, op == OP_PAIR64, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
-    if (op == OP_FREE64) {
+    if (op == OP_FREE64 || op == OP_FREE32) {
         if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
             set_error("batched matrix-free products need the traditional tube");
             return IBIM_ERR_ARG;
         }
-        return adjoint ? pair_free_batch_apply(ctx, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
-                       : pair_free_batch_apply(ctx, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
+        const bool fast = op == OP_FREE32;
+        return adjoint ? pair_free_batch_apply(ctx, fast, nullptr, P, nullptr, Y, st ? &st->all_done : nullptr)
+                       : pair_free_batch_apply(ctx, fast, P, nullptr, Y, nullptr, st ? &st->all_done : nullptr);
     }
-    if (op == OP_DENSE) {
-        TimedScope ts(ctx, adjoint ? "batch_cols" : "batch_rows");
-        if (ctx->mat_precision == IBIM_C128) {
-            if (adjoint)
-                hipLaunchKernelGGL(k_cols_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
-                                   ctx->ld, count, P, Y, st);
-            else
-                hipLaunchKernelGGL(k_rows_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
-                                   ctx->ld, count, P, Y, st);
-        } else {
-            if (adjoint)
-                hipLaunchKernelGGL(k_cols_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
-                                   ctx->ld, count, P, Y, st);
-            else
-                hipLaunchKernelGGL(k_rows_stored_c64, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
-                                   ctx->ld, count, P, Y, st);
-        }
-    } else {
-        if (ctx->kind != IBIM_TUBE_TRADITIONAL) {
-            set_error("batched matrix-free products need the traditional tube");
-            return IBIM_ERR_ARG;
-        }
-        TimedScope ts(ctx, adjoint ? "batch_cols_free" : "batch_rows_free");
-        const double* tube = ctx->tube.as<double>();
+    if (op != OP_DENSE) {
+        set_error("batched products: unknown operator %d", op);
+        return IBIM_ERR_ARG;
+    }
+    TimedScope ts(ctx, adjoint ? "batch_cols" : "batch_rows");
+    if (ctx->mat_precision == IBIM_C128) {
         if (adjoint)
-            hipLaunchKernelGGL(k_cols_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm, P,
-                               Y, st);
+            hipLaunchKernelGGL(k_cols_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
+                               ctx->ld, count, P, Y, st);
         else
-            hipLaunchKernelGGL(k_rows_free<OP_FREE32>, dim3(nb), dim3(kBT), 0, ctx->stream, tube, count, ctx->prm, P,
-                               Y, st);
+            hipLaunchKernelGGL(k_rows_stored<double2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<double2>(),
+                               ctx->ld, count, P, Y, st);
+    } else {
+        if (adjoint)
+            hipLaunchKernelGGL(k_cols_stored<float2>, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
+                               ctx->ld, count, P, Y, st);
+        else
+            hipLaunchKernelGGL(k_rows_stored_c64, dim3(nb), dim3(kBT), 0, ctx->stream, ctx->mat.as<float2>(),
+                               ctx->ld, count, P, Y, st);
     }
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
@@ -578,8 +478,8 @@ int batch_apply2(Context* ctx, int op, const double2* P, const double2* Q, doubl
     const BState* st = (const BState*)state;
     if (op == OP_PAIR64 || (op == OP_DENSE && ctx->mat_pair))
         return pair_batch_apply(ctx, op == OP_PAIR64, P, Q, V, W, st ? &st->all_done : nullptr);
-    if (op == OP_FREE64 && ctx->kind == IBIM_TUBE_TRADITIONAL)
-        return pair_free_batch_apply(ctx, P, Q, V, W, st ? &st->all_done : nullptr);
+    if ((op == OP_FREE64 || op == OP_FREE32) && ctx->kind == IBIM_TUBE_TRADITIONAL)
+        return pair_free_batch_apply(ctx, op == OP_FREE32, P, Q, V, W, st ? &st->all_done : nullptr);
     int rc = batch_apply(ctx, op, false, P, V, state);
     if (!rc) rc = batch_apply(ctx, op, true, Q, W, state);
     return rc;

@@ -104,7 +104,6 @@ __device__ __forceinline__ int combo_entry(const PointData& a, const PointData& 
     return near;
 }
 
-// Traditional entry with FP32 arithmetic around an FP64-reduced phase.
 // sin and cos of 2 pi f for a reduced phase f in [-1/2, 1/2] revolutions, in
 // FP32: quadrant q = rint(4 f) and x = 2 pi (f - q / 4) in [-pi/4, pi/4] are
 // formed in FP64 and x is rounded once; Taylor polynomials in x^2 (sin to x^9,
@@ -128,51 +127,6 @@ __device__ __forceinline__ void sincos_rev_f32(double f, float* s, float* c) {
     const int quad = ((int)q) & 3;               // q in [-2, 2]
     *s = quad == 0 ? sx : quad == 1 ? cx : quad == 2 ? -sx : -cx;
     *c = quad == 0 ? cx : quad == 1 ? -sx : quad == 2 ? -cx : sx;
-}
-
-// The traditional entry of the mixed-precision ("fast") operator. FP64 where
-// rounding would make the operator's error systematic (an error that adds up
-// coherently on smooth densities): the separation, rho (FP32 sqrt seed + one
-// FP64 Newton step), 1/rho (FP32 seed + Newton), the phase reduction and the
-// radial factors c1, c2 of grad Phi. FP32 for sin / cos (sincos_rev_f32), r.n
-// and the weight products. The operator's error on the solution density is then
-// that of rounding the FP64 entries to complex64 (8.7e-8 vs 2.2e-7 with FP32 c1,
-// c2, and 1.4e-7 with the hardware sin / cos units on a 64,676-point torus).
-// kappa = k / (2 pi).
-// PA / PB: any point type with sx, sy, sz (FP64) and nx, ny, nz, w, reg
-// (read as FP32), e.g. PointData.
-template <class PA, class PB>
-__device__ __forceinline__ void trad_entry_fast(const PA& a, const PB& b, bool diag, double k, double kappa,
-                                                double tau2, int dirichlet, int interior, double* re, double* im) {
-    const double d1 = a.sx - b.sx, d2 = a.sy - b.sy, d3 = a.sz - b.sz;
-    const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
-    float pr, pim;
-    if (rho2 > tau2) {
-        const float r32 = sqrtf((float)rho2);
-        const float ir32 = 1.0f / r32;
-        const double rd = (double)r32;
-        const double rho = fma(fma(-rd, rd, rho2), 0.5 * (double)ir32, rd);
-        const double i0 = (double)ir32;
-        const double ir = fma(i0, fma(-rho, i0, 1.0), i0);
-        const double t = rho * kappa;
-        float s32, c32;
-        sincos_rev_f32(t - rint(t), &s32, &c32);
-        const double sv = (double)s32, cv = (double)c32;
-        const double den = (1.0 / (4.0 * kPi)) * ir * ir;
-        const float c1 = (float)(-(k * sv + cv * ir) * den);
-        const float c2 = (float)((k * cv - sv * ir) * den);
-        const float ip = dirichlet ? (float)d1 * (float)b.nx + (float)d2 * (float)b.ny + (float)d3 * (float)b.nz
-                                   : (float)d1 * (float)a.nx + (float)d2 * (float)a.ny + (float)d3 * (float)a.nz;
-        pr = interior ? -c1 * ip : c1 * ip;
-        pim = interior ? -c2 * ip : c2 * ip;
-    } else {
-        pr = (float)(dirichlet ? b.reg : a.reg);
-        pim = 0.0f;
-    }
-    float er = (float)b.w * pr;
-    if (diag) er += 0.5f;
-    *re = (double)er;
-    *im = (double)((float)b.w * pim);
 }
 
 }  // namespace ibim

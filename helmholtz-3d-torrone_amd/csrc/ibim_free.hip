@@ -1,17 +1,15 @@
-// ibim_free.hip -- matrix-free fused sweeps and FP64 iterative refinement.
+// ibim_free.hip -- the matrix-free sweep of the combined-field tube and FP64
+// iterative refinement.
 //
 // When the count x count operator does not fit (or should not be stored), each
 // BiCG iteration recomputes the entries of a kSweepRows x kSweepCols tile in
 // registers and applies them to p (row sums) and p~ (column sums) from that one
 // evaluation -- the same tile / partial-plane structure as the dense sweep, so
-// the BiCG vector kernels are shared.
-//
-// OP_FREE64 evaluates the reference's FP64 entries (ibim_entries.hpp).
-// OP_FREE32 (the mixed-precision operator) evaluates the traditional entry
-// with FP32 sin / cos, r.n and weight products and FP64 for the separation,
-// rho, 1/rho, the phase reduction and the radial factors (trad_entry_fast,
-// ibim_entries.hpp), and applies it to the FP64 vectors in FP64: one fixed
-// matrix whose error is that of the complex64-rounded FP64 operator.
+// the BiCG vector kernels are shared. That is the combined-field tube's
+// operator (the reference's FP64 entries, ibim_entries.hpp), which has no pair
+// factorisation; the traditional tube's matrix-free operators (OP_FREE64 and
+// the mixed-precision OP_FREE32) sweep as pair storage with computed factors,
+// one evaluation per pair (ibim_pair.hip).
 //
 // Refinement: x = 0, r = b; repeat { solve A32 d = r by BiCG to inner_tol;
 // x += d; r = b - A64 x with the FP64 operator } until ||r|| / ||b|| < tol.
@@ -44,8 +42,7 @@ __device__ __forceinline__ T wsum(T v) {
     return v;
 }
 
-// The FP64 operator (the reference's entries, either tube kind) recomputed per
-// sweep: column points in registers, row points staged in LDS, one wavefront
+// The FP64 operator (the reference's entries) recomputed per sweep: column points in registers, row points staged in LDS, one wavefront
 // reduction per row (the FP64 entry evaluation dominates this kernel).
 template <int Kind>
 __global__ void __launch_bounds__(kFT) k_sweep_free64(const double* __restrict__ tube, int count, ibim_params prm,
@@ -148,134 +145,6 @@ __global__ void __launch_bounds__(kFT) k_sweep_free64(const double* __restrict__
         if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2((double)zre[m], (double)zim[m]);
 }
 
-// FP32 operator sweep (traditional kernel). Column points keep FP64 positions
-// and FP32 normal / weight / near-field value in registers; row points are
-// staged in LDS the same way. Dirichlet / interior are compile-time switches.
-constexpr int kRowBatch = 4;  // rows per transposed reduction (4 rows x re, im)
-
-// The fast operator for the traditional tube, specialised on the
-// Dirichlet / interior switches: entries by trad_entry_fast, applied to the
-// FP64 vectors in FP64 (products and sums), so the recomputed operator is one
-// fixed matrix for BiCG. Column points stay in registers, row points are
-// staged in LDS; rows go through the transposed 8-value wave reduction four at
-// a time.
-// A column point as the mixed-precision entry reads it (FP32 where the entry
-// rounds to FP32 anyway): fewer registers per column.
-struct ColPoint {
-    double sx, sy, sz;
-    float nx, ny, nz, w, reg;
-};
-
-template <int Dirichlet, int Interior>
-__global__ void __launch_bounds__(kFT) k_sweep_free32(const double* __restrict__ tube, int count, ibim_params prm,
-                                                      const double2* __restrict__ p, const double2* __restrict__ q,
-                                                      double2* __restrict__ ypart, double2* __restrict__ zpart,
-                                                      size_t pld, const int* __restrict__ done) {
-    if (done != nullptr && *done) return;
-    __shared__ RowStage st;
-    __shared__ double ysh[2][kFT / 64][kFRows];
-    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int c0 = blockIdx.x * kSweepCols;
-    const int r0 = blockIdx.y * kFRows;
-    const int nrow = min(kFRows, count - r0);
-    const size_t cnt = (size_t)count;
-    const double kappa = prm.k / (2.0 * kPi), tau2 = prm.tau * prm.tau, k = prm.k;
-    const int my_val = wave_sum8_index(lane);
-
-    ColPoint col[kFCpt];
-    double pre[kFCpt], pim[kFCpt], zre[kFCpt], zim[kFCpt];
-    int cj[kFCpt];
-#pragma unroll
-    for (int m = 0; m < kFCpt; ++m) {
-        cj[m] = c0 + t + kFT * m;
-        const bool ok = cj[m] < count;
-        const int j = ok ? cj[m] : 0;
-        col[m].sx = tube[IBIM_F_SX * cnt + j];
-        col[m].sy = tube[IBIM_F_SY * cnt + j];
-        col[m].sz = tube[IBIM_F_SZ * cnt + j];
-        col[m].nx = (float)tube[IBIM_F_NX * cnt + j];
-        col[m].ny = (float)tube[IBIM_F_NY * cnt + j];
-        col[m].nz = (float)tube[IBIM_F_NZ * cnt + j];
-        col[m].w = ok ? (float)tube[IBIM_F_W * cnt + j] : 0.0f;  // silences padding columns
-        col[m].reg = (float)tube[IBIM_F_REG * cnt + j];
-        const double2 pv = ok ? p[cj[m]] : make_double2(0.0, 0.0);
-        pre[m] = pv.x;
-        pim[m] = pv.y;
-        zre[m] = zim[m] = 0.0;
-    }
-    for (int s0 = 0; s0 < nrow; s0 += kStage) {
-        const int ns = min(kStage, nrow - s0);
-        __syncthreads();
-        if (t < kStage) {
-            // rows past the tile repeat the tile's first row with q = 0: finite
-            // entries times zero add nothing to the column sums, and their row
-            // sums are dropped
-            const bool ok = t < ns;
-            const int i = r0 + s0 + (ok ? t : 0);
-            st.sx[t] = tube[IBIM_F_SX * cnt + i];
-            st.sy[t] = tube[IBIM_F_SY * cnt + i];
-            st.sz[t] = tube[IBIM_F_SZ * cnt + i];
-            st.nx[t] = tube[IBIM_F_NX * cnt + i];
-            st.ny[t] = tube[IBIM_F_NY * cnt + i];
-            st.nz[t] = tube[IBIM_F_NZ * cnt + i];
-            st.w[t] = tube[IBIM_F_W * cnt + i];
-            st.reg[t] = tube[IBIM_F_REG * cnt + i];
-            st.dist[t] = 0.0;
-            const double2 qv = ok ? q[i] : make_double2(0.0, 0.0);
-            st.qre[t] = qv.x;
-            st.qim[t] = qv.y;
-        }
-        __syncthreads();
-        for (int sb = 0; sb < ns; sb += kRowBatch) {
-            double rows[2 * kRowBatch];
-#pragma unroll
-            for (int u = 0; u < kRowBatch; ++u) {
-                const int s = sb + u;  // < kStage; rows >= ns carry q = 0 and their sums are dropped
-                const int i = r0 + s0 + s;
-                PointData a;
-                a.sx = st.sx[s];
-                a.sy = st.sy[s];
-                a.sz = st.sz[s];
-                a.nx = st.nx[s];
-                a.ny = st.ny[s];
-                a.nz = st.nz[s];
-                a.w = st.w[s];
-                a.reg = st.reg[s];
-                a.dist = 0.0;
-                const double qre = st.qre[s], qim = st.qim[s];
-                double sre = 0.0, sim = 0.0;
-#pragma unroll
-                for (int m = 0; m < kFCpt; ++m) {
-                    double er, ei;
-                    trad_entry_fast(a, col[m], i == cj[m], k, kappa, tau2, Dirichlet, Interior, &er, &ei);
-                    sre += er * pre[m] - ei * pim[m];
-                    sim += er * pim[m] + ei * pre[m];
-                    zre[m] += er * qre + ei * qim;
-                    zim[m] += er * qim - ei * qre;
-                }
-                rows[2 * u] = sre;
-                rows[2 * u + 1] = sim;
-            }
-            const double tot = wave_sum8_transposed(rows, lane);
-            const int s = sb + (my_val >> 1);
-            if ((lane & 7) == 0 && s < ns) ysh[my_val & 1][wave][s0 + s] = tot;
-        }
-    }
-    __syncthreads();
-    for (int row = t; row < nrow; row += kFT) {
-        double a = 0.0, b = 0.0;
-#pragma unroll
-        for (int w = 0; w < kFT / 64; ++w) {
-            a += ysh[0][w][row];
-            b += ysh[1][w][row];
-        }
-        ypart[(size_t)blockIdx.x * pld + r0 + row] = make_double2(a, b);
-    }
-#pragma unroll
-    for (int m = 0; m < kFCpt; ++m)
-        if (cj[m] < count) zpart[(size_t)blockIdx.y * pld + cj[m]] = make_double2(zre[m], zim[m]);
-}
-
 // r = b - y, partial sums of |r|^2 per block.
 __global__ void __launch_bounds__(256) k_residual(const double2* __restrict__ b, const double2* __restrict__ y,
                                                   double2* __restrict__ r, int count, double* __restrict__ part) {
@@ -307,27 +176,19 @@ unsigned nblocks(int count) {
 
 }  // namespace
 
+// The combined-field tube's matrix-free sweep: FP64 entries for either op (the
+// mixed-precision entries are the traditional kernel's).
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+    if (ctx->kind != IBIM_TUBE_COMBO || (op != OP_FREE64 && op != OP_FREE32)) {
+        set_error("matrix-free sweep: the traditional tube sweeps as pairs");
+        return IBIM_ERR_ARG;
+    }
     const int count = ctx->count;
     dim3 grid((unsigned)((count + kSweepCols - 1) / kSweepCols), (unsigned)((count + kFRows - 1) / kFRows));
-    const double* tube = ctx->tube.as<double>();
-    auto P = (const double2*)p;
-    auto Q = (const double2*)q;
-    auto Y = (double2*)ypart;
-    auto Z = (double2*)zpart;
-    if (op == OP_FREE32 && ctx->kind != IBIM_TUBE_TRADITIONAL) op = OP_FREE64;  // FP32 entries: traditional only
-    TimedScope ts(ctx, op == OP_FREE32 ? "sweep_free32" : "sweep_free64");
-    if (op == OP_FREE32) {
-        const int dir = ctx->prm.dirichlet != 0, in = ctx->prm.interior != 0;
-        auto kern = dir ? (in ? k_sweep_free32<1, 1> : k_sweep_free32<1, 0>)
-                        : (in ? k_sweep_free32<0, 1> : k_sweep_free32<0, 0>);
-        hipLaunchKernelGGL(kern, grid, dim3(kFT), 0, ctx->stream, tube, count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
-    } else if (ctx->kind == IBIM_TUBE_TRADITIONAL)
-        hipLaunchKernelGGL((k_sweep_free64<IBIM_TUBE_TRADITIONAL>), grid, dim3(kFT), 0, ctx->stream, tube,
-                           count, ctx->prm, P, Q, Y, Z, ctx->ld, done);
-    else
-        hipLaunchKernelGGL((k_sweep_free64<IBIM_TUBE_COMBO>), grid, dim3(kFT), 0, ctx->stream, tube, count,
-                           ctx->prm, P, Q, Y, Z, ctx->ld, done);
+    TimedScope ts(ctx, "sweep_free64");
+    hipLaunchKernelGGL((k_sweep_free64<IBIM_TUBE_COMBO>), grid, dim3(kFT), 0, ctx->stream, ctx->tube.as<double>(),
+                       count, ctx->prm, (const double2*)p, (const double2*)q, (double2*)ypart, (double2*)zpart,
+                       ctx->ld, done);
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

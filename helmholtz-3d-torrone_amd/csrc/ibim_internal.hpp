@@ -86,10 +86,10 @@ struct Context {
     // the five tables above match the current tube and parameters (built by a
     // pair assembly, or alone for the matrix-free pair pass)
     bool pair_aux = false;
-    // tuned workgroup shape (waves) of the 16-column pair passes per storage
-    // (complex64, complex128, complex128 copy, computed) and product kind, 0
-    // until timed; reset with every assembly
-    int batch_waves[12] = {};
+    // tuned workgroup shape (waves) of the 16-column pair passes per factor
+    // source (complex64, complex128, complex128 copy, computed FP64, computed
+    // mixed) and product kind, 0 until timed; reset with every assembly
+    int batch_waves[15] = {};
 
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;
@@ -188,11 +188,12 @@ int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 // product; kernels return at once while *stop is set.
 int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const double2* Q, double2* Y, double2* Z,
                      const int* stop);
-// The same products of the FP64 matrix-free operator (traditional tube): each
-// factor computed where the complex128 pair storage would read it, so the
-// results are that storage's bits. Builds the point table and near lists on
-// first use (host synchronisation: not inside a stream capture).
-int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
+// The same products of the matrix-free operators (traditional tube): each
+// factor computed where the complex128 pair storage would read it -- FP64
+// (that storage's bits) or, with fast, the mixed-precision factors. Builds the
+// point table and near lists on first use (host synchronisation: not inside a
+// stream capture).
+int pair_free_batch_apply(Context* ctx, bool fast, const double2* P, const double2* Q, double2* Y, double2* Z,
                           const int* stop);
 // The single-vector sweep of the same operator into the pair planes (one
 // factor evaluation per pair; the complex128 pair sweep's bits), or with fast

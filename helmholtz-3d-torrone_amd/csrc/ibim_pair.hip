@@ -509,15 +509,16 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
 constexpr int kBatchK = 64;  // columns staged per step
 static_assert(kPR % 64 == 0 && kPC % 64 == 0, "a workgroup's rows share their tiles");
 
-// Computed: no stored factors (F unused); every visit evaluates F of the pair
-// it would have read -- with the assembly's operations, so the products are
-// the complex128 pair storage's bits -- the FP64 matrix-free operator.
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves, bool Computed>
+// Factors other than kStoredFactors: F unused; every visit evaluates F of the
+// pair it would have read (kExactFactors: the complex128 pair storage's bits,
+// the FP64 matrix-free operator; kFastFactors: the mixed-precision one).
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves, int Factors>
 __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
                                                            const PairPoint* __restrict__ pts, int count,
                                                            ibim_params prm, const double2* __restrict__ P,
                                                            const double2* __restrict__ Q, double2* __restrict__ Y,
                                                            double2* __restrict__ Z, const int* __restrict__ stop) {
+    constexpr bool Computed = Factors != kStoredFactors;
     static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
     constexpr int R = kBatchRHS;
@@ -549,6 +550,7 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
     // F of pair (i, j) as k_pair_assemble stores it (0 when not `keep`: the
     // visit's zero pattern, the near field and everything past count)
     const double sigma = prm.interior ? -1.0 : 1.0;
+    const double kappa = prm.k / (2.0 * kPi);
     auto factor = [&](int jj, bool keep) -> MT {
         MT f;
         f.x = 0.0;
@@ -559,7 +561,10 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
             const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
             if (rho > prm.tau) {
                 double c1, c2;
-                grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                if (Factors == kExactFactors)
+                    grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                else
+                    grad_phi_coeffs_fast(prm.k, kappa, rho, &c1, &c2);
                 f.x = sigma * c1;
                 f.y = sigma * c2;
             }
@@ -684,7 +689,7 @@ __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restric
     }
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
 void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     const int count = ctx->count;
@@ -693,20 +698,20 @@ void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* 
     const int* blk = ctx->pair_blk.as<int>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
     if (waves == 1)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1, Computed>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else if (waves == 2)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2, Computed>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4, Computed>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
 }
 
 // The workgroup shape of the far pass: IBIM_BATCH_WAVES (1, 2, 4) if set, else
 // the fastest of the three timed on this call's own operands (once per
 // context, storage and product kind; the outputs are the same bits).
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
 int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
                 const int* stop) {
     if (const char* env = getenv("IBIM_BATCH_WAVES")) {
@@ -738,7 +743,7 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
         for (int c = 0; c < 3; ++c) {
             if (long_pass && c == 2 && best_of[1] >= best_of[0]) break;
             (void)hipEventRecord(e0, ctx->stream);
-            launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Computed>(ctx, cand[c], F, P, Q, Y, Z, stop);
+            launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Factors>(ctx, cand[c], F, P, Q, Y, Z, stop);
             (void)hipEventRecord(e1, ctx->stream);
             float ms = 0.0f;
             if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
@@ -757,30 +762,33 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
     return cand[pick - 1];
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, bool Computed>
+template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
 void launch_pair_batch(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y,
                        double2* Z, const int* stop) {
     const int count = ctx->count;
-    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ, Computed>(ctx, slot, F, P, Q, Y, Z, stop);
-    TimedScope ts(ctx, Computed ? "batch_free64" : slot >= 6 ? "batch_pair64" : "batch_pair");
-    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Computed>(ctx, waves, F, P, Q, Y, Z, stop);
+    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ, Factors>(ctx, slot, F, P, Q, Y, Z, stop);
+    TimedScope ts(ctx, Factors == kExactFactors  ? "batch_free64"
+                       : Factors == kFastFactors ? "batch_free32"
+                       : slot >= 6               ? "batch_pair64"
+                                                 : "batch_pair");
+    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Factors>(ctx, waves, F, P, Q, Y, Z, stop);
     hipLaunchKernelGGL((k_pair_near_batch<Dirichlet, WithY, WithZ>),
                        dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
                        stop);
 }
 
-// slot: storage (0 complex64, 3 complex128, 6 complex128 copy, 9 computed) +
-// product kind
-template <class MT, bool Dirichlet, bool Computed = false>
+// slot: factors (0 complex64, 3 complex128, 6 complex128 copy, 9 exact and 12
+// fast computed) + product kind
+template <class MT, bool Dirichlet, int Factors = kStoredFactors>
 void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     if (Y && Z)
-        launch_pair_batch<MT, Dirichlet, true, true, Computed>(ctx, slot0, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, true, Factors>(ctx, slot0, F, P, Q, Y, Z, stop);
     else if (Y)
-        launch_pair_batch<MT, Dirichlet, true, false, Computed>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, true, false, Factors>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
     else
-        launch_pair_batch<MT, Dirichlet, false, true, Computed>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Dirichlet, false, true, Factors>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
 }
 
 int near_lists(Context* ctx, dim3 grid);
@@ -916,7 +924,7 @@ int prepare_pair_free(Context* ctx) {
     IBIM_HIP(hipGetLastError());
     if (int rc = near_lists(ctx, grid)) return rc;
     ctx->pair_ntiles = (int)tiles.size();
-    for (int w = 9; w < 12; ++w) ctx->batch_waves[w] = 0;
+    for (int w = 9; w < 15; ++w) ctx->batch_waves[w] = 0;
     ctx->pair_aux = true;
     return IBIM_OK;
 }
@@ -1019,19 +1027,25 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
     return IBIM_OK;
 }
 
-// The FP64 matrix-free batched products: the fused pair pass evaluating each
-// factor where the complex128 pair storage would read it (the same bits).
-int pair_free_batch_apply(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z,
+// The matrix-free batched products: the fused pair pass evaluating each factor
+// where the complex128 pair storage would read it (FP64: the same bits; fast:
+// the mixed-precision factors).
+int pair_free_batch_apply(Context* ctx, bool fast, const double2* P, const double2* Q, double2* Y, double2* Z,
                           const int* stop) {
     if ((!Y && !Z) || (Y && !P) || (Z && !Q)) {
         set_error("batched matrix-free pair products without the vectors they need");
         return IBIM_ERR_ARG;
     }
     if (int rc = prepare_pair_free(ctx)) return rc;
-    if (ctx->prm.dirichlet != 0)
-        launch_pair_batch_any<double2, true, true>(ctx, 9, nullptr, P, Q, Y, Z, stop);
+    const bool dir = ctx->prm.dirichlet != 0;
+    if (fast && dir)
+        launch_pair_batch_any<double2, true, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
+    else if (fast)
+        launch_pair_batch_any<double2, false, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
+    else if (dir)
+        launch_pair_batch_any<double2, true, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
     else
-        launch_pair_batch_any<double2, false, true>(ctx, 9, nullptr, P, Q, Y, Z, stop);
+        launch_pair_batch_any<double2, false, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

@@ -191,8 +191,11 @@ int ibim_regcount(ibim_ctx *ctx);
 
 /* One application y = A p, z = A^H q (host vectors of length count, split
  * re / im) of the stored operator (op IBIM_OP_STORED) or of the entries
- * recomputed on the fly in FP64 (IBIM_OP_FREE64) or in FP32 around an FP64
- * phase (IBIM_OP_FREE32, traditional kernel only). */
+ * recomputed on the fly in FP64 (IBIM_OP_FREE64) or with FP32 sine and cosine
+ * of an FP64 phase (IBIM_OP_FREE32, traditional kernel only; the combined-field
+ * kernel falls back to FP64). On the traditional tube both recompute one
+ * radial factor per unordered pair, IBIM_OP_FREE64 with the operations of the
+ * complex128 pair storage (the same bits). */
 #define IBIM_OP_STORED 0
 #define IBIM_OP_FREE64 1
 #define IBIM_OP_FREE32 2

@@ -128,6 +128,13 @@ def test_matrix_free_batched_pass_gives_the_pair_storage_bits(gpu, dirichlet, in
             got = free.apply_batch(P, gpu.OP_FREE64, adjoint)
             assert np.array_equal(got, stored.apply_batch(P, gpu.OP_STORED, adjoint)), (tau_h, adjoint)
             assert np.array_equal(got, free.apply_batch(P, gpu.OP_FREE64, adjoint))  # fixed order
+            # the mixed-precision factors (FP32 sine / cosine): the same pass,
+            # and the single-vector sweep's operator
+            fast = free.apply_batch(P, gpu.OP_FREE32, adjoint)
+            assert relnorm(fast, got) < 1e-7, (tau_h, adjoint, relnorm(fast, got))
+            for c in (0, 9):
+                y, z = free.apply(P[:, c], P[:, c], gpu.OP_FREE32)
+                assert relnorm(fast[:, c], z if adjoint else y) < 1e-13, (tau_h, adjoint, c)
         fill_rhs(gpu, free, 19)
         fill_rhs(gpu, stored, 19)
         ia, _ = free.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
