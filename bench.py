@@ -9,18 +9,20 @@ Two paths are timed on the same workload:
   mixed  complex BiCG on the FP32 operator -- its radial factors stored once per
          unordered pair as complex64, with a complex128 copy for the FP64
          residual replacements (IBIM_MIXED_PAIR), while that fits in HBM, else
-         FP32 entries recomputed per sweep with an FP64 range-reduced phase --
-         with FP64 residual replacement by the FP64 operator (method
-         IBIM_SOLVE_RELIABLE);
+         the factors recomputed per pair and sweep with FP32 sine and cosine
+         of an FP64 phase -- with FP64 residual replacement by the FP64
+         operator (method IBIM_SOLVE_RELIABLE);
   fp64   the FP64 GPU path: the operator's radial factors stored once per pair
          as complex128 (IBIM_C128_PAIR), the reference's complex BiCG.
+--matrix-free stores nothing on either path: both recompute the factors per
+pair and sweep (FP64 path: bitwise the complex128-pair operator).
 Both run with one host check per iteration (chunk 1), so every sweep launch in
 the timed region does a full pass.
 `value` is the mixed path's solves/s; both paths' densities are checked against
 an FP64 solve to tol 1e-11 (rel err must be <= 1e-6) and the field against the
 analytic point-source field.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus128]
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload torus128] [--matrix-free]
 """
 import argparse
 import json
@@ -59,6 +61,7 @@ REPLACE = 1e-2      # residual drop between FP64 residual replacements (torus128
 BATCH_REPLACE = 1e-3  # the same for the batched solver (at 1e-1 its torus128 columns stagnated)
 NRECV = 500
 STORE_MAX_BYTES = 200e9  # store an operator in HBM up to this size (288 GB per GPU)
+MATRIX_FREE = False  # --matrix-free: both timed paths recompute the operator per sweep
 STAR_STORE_MAX_BYTES = 260e9  # the 256^3 star's complex64 pairs (~251 GB) alone beside its vectors
 PAIR_ROWS, PAIR_COLS = 512, 1024  # tile shape of the pair storage
 
@@ -86,12 +89,14 @@ def params_of(w):
 
 
 def mixed_stores(count):
-    return pair_bytes(count, 8) <= STORE_MAX_BYTES
+    return not MATRIX_FREE and pair_bytes(count, 8) <= STORE_MAX_BYTES
 
 
 def mixed_storage(count):
     """complex64 pairs + their complex128 copy when both fit, else complex64 pairs
     (FP64 replacements recomputed), else nothing stored."""
+    if MATRIX_FREE:
+        return None
     if pair_bytes(count, 8) + pair_bytes(count, 16) <= STORE_MAX_BYTES:
         return ibim.MIXED_PAIR
     return ibim.C64_PAIR if mixed_stores(count) else None
@@ -100,9 +105,9 @@ def mixed_storage(count):
 def step(pb, w, recv, mixed):
     pb.geometry(5)
     count = pb.tube(params_of(w), ibim.TUBE_TRADITIONAL)
-    if not mixed:
+    if not mixed and not MATRIX_FREE:
         pb.assemble(ibim.C128_PAIR if pair_bytes(count, 16) <= STORE_MAX_BYTES else ibim.C128)
-    elif mixed_storage(count) is not None:
+    elif mixed and mixed_storage(count) is not None:
         pb.assemble(mixed_storage(count))
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if mixed:
@@ -491,7 +496,11 @@ def main():
     ap.add_argument("--star-k", type=float, default=40.0, help="wavenumber of the star256 workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
+    ap.add_argument("--matrix-free", action="store_true",
+                    help="store no operator: both paths recompute the radial factors per pair and sweep")
     args = ap.parse_args()
+    global MATRIX_FREE
+    MATRIX_FREE = args.matrix_free
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
@@ -527,7 +536,7 @@ def main():
           (count, info_ref.iterations, info_ref.status), file=sys.stderr, flush=True)
 
     t64, info64, x64, u64 = timed(p64, w, recv, False, args.steps, args.warmup)
-    ms_mv, n_mv = p64.kernel_time("matvec")
+    ms_mv, n_mv = p64.kernel_time("sweep_free64" if MATRIX_FREE else "matvec")
     ms_asm, n_asm = p64.kernel_time("assemble")
     print("[bench] fp64 path: %.3f s/solve, %d iterations" % (t64, info64.iterations), file=sys.stderr, flush=True)
     p64.close()
@@ -572,7 +581,7 @@ def main():
                           (C_EVAL + 48, count, count - 1)}
     # the FP64 path's fused sweep (HBM-bound): one pass over the complex128 pairs
     avg_mv = ms_mv / max(n_mv, 1) * 1e-3
-    mv_gbs = (npairs * 16 + vec_bytes) / avg_mv / 1e9 if n_mv else 0.0
+    mv_gbs = (npairs * 16 + vec_bytes) / avg_mv / 1e9 if n_mv and not MATRIX_FREE else None
     avg_dense = ms_dense / max(n_dense, 1) * 1e-3
 
     line = {
@@ -586,23 +595,28 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "complex64 radial factors per pair (fp64 point factors, products and sums) + fp64 residual "
-                 "replacement",
+        "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors + fp64 residual replacement"
+                  if MATRIX_FREE else
+                  "complex64 radial factors per pair (fp64 point factors, products and sums) + fp64 residual "
+                  "replacement"),
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
-        "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g" % (
-            args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"]),
+        "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g%s" % (
+            args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"],
+            ", matrix-free" if MATRIX_FREE else ""),
             "tube_points": count, "parallelism": "single GPU",
             "mixed_operator": {ibim.MIXED_PAIR: "complex64 pair storage + complex128 copy for the FP64 residuals",
                                ibim.C64_PAIR: "complex64 pair storage"}.get(mixed_storage(count),
-                                                                            "recomputed FP32 entries"),
-            "fp64_operator": "complex128 pair storage" if pair_bytes(count, 16) <= STORE_MAX_BYTES else "dense complex128"},
+                                                                            "radial factors recomputed per pair"),
+            "fp64_operator": ("FP64 radial factors recomputed per pair" if MATRIX_FREE else
+                              "complex128 pair storage" if pair_bytes(count, 16) <= STORE_MAX_BYTES else
+                              "dense complex128")},
         "mixed": {"solves_per_s": 1.0 / tm, "iterations": infom.iterations, "replacements": infom.outer,
                   "rel_err_vs_fp64": err_mixed, "field_rel_err_vs_analytic": field_err,
                   "sweep_ms": avg_dom * 1e3, "replacement_sweep_ms": ms_rep / max(n_rep, 1)},
         "fp64": {"solves_per_s": 1.0 / t64, "iterations": info64.iterations, "rel_err_vs_fp64": err_fp64,
                  "assemble_ms": ms_asm / max(n_asm, 1), "matvec_ms": avg_mv * 1e3,
-                 "matvec_GBs": mv_gbs, "matvec_frac_of_peak": mv_gbs / HBM_PEAK_GBS,
-                 "matvec_traffic": measured_traffic("k_pair_sweep<double2>", args.workload)},
+                 "matvec_GBs": mv_gbs, "matvec_frac_of_peak": mv_gbs / HBM_PEAK_GBS if mv_gbs else None,
+                 "matvec_traffic": None if MATRIX_FREE else measured_traffic("k_pair_sweep<double2>", args.workload)},
         "dense_c128_reference": {"iterations": info_ref.iterations, "tol": REF_TOL, "sweep_ms": avg_dense * 1e3,
                                  "sweep_GBs": (n2 * 16 + vec_bytes) / avg_dense / 1e9 if n_dense else 0.0},
         "speedup_mixed_vs_fp64": t64 / tm,
