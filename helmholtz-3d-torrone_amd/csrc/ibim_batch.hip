@@ -884,7 +884,7 @@ static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol
             if (res[r] < opts->tol) conv.c[r] = 1;
             all = all && conv.c[r];
         }
-        // columns that converged now get a zero right-hand side next round
+        // columns that converged now get a zero right-hand side in the next refinement step
         hipLaunchKernelGGL(kb_residual, dim3(nvb), dim3(kBT), 0, ctx->stream, B, a.v, Rb, count, conv, a.part);
         IBIM_HIP(hipGetLastError());
         if (all) break;
