@@ -258,13 +258,15 @@ __global__ void __launch_bounds__(256) k_near_sort(const int* __restrict__ off, 
     }
 }
 
-// F = sigma (c1 + i c2)(rho) of grad_phi_coeffs with the phase's sine and
-// cosine from the FP32 polynomial (phase reduced in FP64, in revolutions) and
-// 1/rho from the FP32 reciprocal refined by one FP64 Newton step: the radial
-// factors of the mixed-precision operator (about one FP32 ulp of sin / cos).
-__device__ __forceinline__ void grad_phi_coeffs_fast(double k, double kappa, double rho, double* c1, double* c2) {
-    const double i0 = (double)(1.0f / (float)rho);
-    const double ir = fma(i0, fma(-rho, i0, 1.0), i0);
+// c1, c2 of grad_phi_coeffs from rho^2 for the mixed-precision operator: 1/rho
+// from the FP32 reciprocal square root refined by one FP64 Newton step
+// (relative error ~3e-14), rho = rho^2 / rho, the phase kappa rho reduced in
+// FP64 (revolutions) and its sine and cosine from the FP32 polynomial (about
+// one FP32 ulp); c1, c2 in FP64.
+__device__ __forceinline__ void grad_phi_coeffs_fast(double k, double kappa, double rho2, double* c1, double* c2) {
+    const double i0 = (double)__builtin_amdgcn_rsqf((float)rho2);
+    const double ir = i0 * fma(-0.5 * rho2, i0 * i0, 1.5);
+    const double rho = rho2 * ir;
     const double t = rho * kappa;
     float s32, c32;
     sincos_rev_f32(t - rint(t), &s32, &c32);
@@ -272,6 +274,15 @@ __device__ __forceinline__ void grad_phi_coeffs_fast(double k, double kappa, dou
     const double den = (1.0 / (4.0 * kPi)) * ir * ir;
     *c1 = -1.0 * (k * sv + cv * ir) * den;
     *c2 = (k * cv - sv * ir) * den;
+}
+
+// sqrt(rho2) > tau, the near lists' far-field rule, without the FP64 square
+// root unless rho2 lies within a relative 1e-9 of tau^2 (tau2_lo, tau2_hi),
+// where the rounding of the square root could decide.
+__device__ __forceinline__ bool far_pair(double rho2, double tau, double tau2_lo, double tau2_hi) {
+    if (rho2 > tau2_hi) return true;
+    if (rho2 < tau2_lo) return false;
+    return sqrt(rho2) > tau;
 }
 
 // Where a pair sweep takes its factors F: the stored tiles, or evaluated where
@@ -333,6 +344,7 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
     }
     const double sigma = prm.interior ? -1.0 : 1.0;
     const double kappa = prm.k / (2.0 * kPi);
+    const double tau2_lo = prm.tau * prm.tau * (1.0 - 1e-9), tau2_hi = prm.tau * prm.tau * (1.0 + 1e-9);
     for (int rr = 0; rr < kPR; rr += 2) {
         MT a[2][kPCpt];
         if (!Computed) {
@@ -365,13 +377,17 @@ __global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, co
                     a[u][m].y = 0.0;
                     if (i < j && j < count) {
                         const double d1 = R.x - cx[m], d2 = R.y - cy[m], d3 = R.z - cz[m];
-                        const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-                        if (rho > prm.tau) {
-                            double c1, c2;
-                            if (Factors == kExactFactors)
+                        const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+                        double c1, c2;
+                        if (Factors == kExactFactors) {
+                            const double rho = sqrt(rho2);
+                            if (rho > prm.tau) {
                                 grad_phi_coeffs(prm.k, rho, &c1, &c2);
-                            else
-                                grad_phi_coeffs_fast(prm.k, kappa, rho, &c1, &c2);
+                                a[u][m].x = sigma * c1;
+                                a[u][m].y = sigma * c2;
+                            }
+                        } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
+                            grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
                             a[u][m].x = sigma * c1;
                             a[u][m].y = sigma * c2;
                         }
@@ -551,6 +567,7 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
     // visit's zero pattern, the near field and everything past count)
     const double sigma = prm.interior ? -1.0 : 1.0;
     const double kappa = prm.k / (2.0 * kPi);
+    const double tau2_lo = prm.tau * prm.tau * (1.0 - 1e-9), tau2_hi = prm.tau * prm.tau * (1.0 + 1e-9);
     auto factor = [&](int jj, bool keep) -> MT {
         MT f;
         f.x = 0.0;
@@ -558,13 +575,17 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
         if (keep) {
             const PairPoint& C = sp[jj];
             const double d1 = Ri.x - C.x, d2 = Ri.y - C.y, d3 = Ri.z - C.z;
-            const double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
-            if (rho > prm.tau) {
-                double c1, c2;
-                if (Factors == kExactFactors)
+            const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+            double c1, c2;
+            if (Factors == kExactFactors) {
+                const double rho = sqrt(rho2);
+                if (rho > prm.tau) {
                     grad_phi_coeffs(prm.k, rho, &c1, &c2);
-                else
-                    grad_phi_coeffs_fast(prm.k, kappa, rho, &c1, &c2);
+                    f.x = sigma * c1;
+                    f.y = sigma * c2;
+                }
+            } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
+                grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
                 f.x = sigma * c1;
                 f.y = sigma * c2;
             }
