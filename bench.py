@@ -47,8 +47,11 @@ WORKLOADS = {
     # Torus (R=1, r=0.4) on 128^3 over [-2,2]^3, eps = tau = 2h, k = 10, interior
     # point source at (1, 0, 0.05): exterior Dirichlet problem, double layer.
     "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05), tol=1e-8),
-    # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10.
-    "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-10),
+    # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10;
+    # 11 iterations: replacements every 1e-3 of residual drop (3 instead of 4 at 1e-2,
+    # 9.8 vs 10.5 ms per solve, the same iterations)
+    "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-10,
+                     replace=1e-3),
     # small case for quick checks
     "sphere32": dict(shape="sphere", n=32, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-8),
 }
@@ -111,7 +114,8 @@ def step(pb, w, recv, mixed):
         pb.assemble(mixed_storage(count))
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if mixed:
-        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"], inner_tol=REPLACE, chunk=1)
+        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"], inner_tol=w.get("replace", REPLACE),
+                           chunk=1)
     else:
         info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
     u = pb.evaluate(recv, ibim.EVAL_DOUBLE)
