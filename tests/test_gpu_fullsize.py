@@ -7,8 +7,9 @@ properties that hold at any size, between kernels that share no code path:
   (to 1e-14 and 1e-7 of the row's largest entry);
 * <A p, q> = <p, A^H q> for the fused sweeps of every storage, and the 16-column
   fused pair pass against the single-vector sweep, column by column;
-* the FP64 matrix-free 16-column pass (factors computed, tables built without an
-  assembly) against the complex128 pair storage's pass, bit for bit;
+* the FP64 matrix-free sweep and 16-column pass (factors computed, tables built
+  without an assembly) against the complex128 pair storage's, bit for bit, and the
+  mixed-precision matrix-free ones against them (1e-7) and each other (1e-13);
 * the mixed-precision solve's density against the dense FP64 operator: its true
   residual ||b - A x|| / ||b|| is below the requested tolerance.
 """
@@ -96,7 +97,7 @@ def test_fused_batched_pass_matches_single_sweeps_at_full_size(gpu, torus):
         assert relnorm(Z[:, c], z) <= 1e-13, c
 
 
-def test_matrix_free_batched_pass_is_the_pair_storage_at_full_size(gpu, torus):
+def test_matrix_free_passes_are_the_pair_storage_at_full_size(gpu, torus):
     free = make_torus(gpu)  # never assembled: its own point table and near lists
     torus.assemble(gpu.C128_PAIR)
     rng = np.random.default_rng(8)
@@ -104,6 +105,18 @@ def test_matrix_free_batched_pass_is_the_pair_storage_at_full_size(gpu, torus):
     for adjoint in (False, True):
         got = free.apply_batch(P, gpu.OP_FREE64, adjoint)
         assert np.array_equal(got, torus.apply_batch(P, gpu.OP_STORED, adjoint)), adjoint
+    # the single-vector sweeps: FP64 bitwise; the mixed-precision factors within
+    # their FP32 sine and cosine of the FP64 ones, and their batched pass equal
+    # to their sweep up to summation order
+    p, q = P[:, 3], P[:, 11]
+    y, z = free.apply(p, q, gpu.OP_FREE64)
+    ys, zs = torus.apply(p, q, gpu.OP_STORED)
+    assert np.array_equal(y, ys) and np.array_equal(z, zs)
+    y32, z32 = free.apply(p, q, gpu.OP_FREE32)
+    assert relnorm(y32, y) <= 1e-7 and relnorm(z32, z) <= 1e-7
+    Y32 = free.apply_batch(P, gpu.OP_FREE32, False)
+    Z32 = free.apply_batch(P, gpu.OP_FREE32, True)
+    assert relnorm(Y32[:, 3], y32) <= 1e-13 and relnorm(Z32[:, 11], z32) <= 1e-13
     free.close()
 
 
