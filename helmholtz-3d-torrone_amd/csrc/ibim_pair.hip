@@ -1,5 +1,7 @@
 // ibim_pair.hip -- the traditional operator stored once per unordered pair
-// (IBIM_C128_PAIR / IBIM_C64_PAIR).
+// (IBIM_C128_PAIR / IBIM_C64_PAIR), and its matrix-free forms (OP_FREE64,
+// OP_FREE32), which run the same sweeps and 16-column passes with each F
+// computed where it would be read instead of loaded.
 //
 // For rho_ij > tau the traditional entry (large3Dexample2.c:4463-4564) factors
 // into a part symmetric in (i, j) and two real point factors:
