@@ -282,6 +282,7 @@ __device__ __forceinline__ void grad_phi_coeffs_fast(double k, double kappa, dou
 // root unless rho2 lies within a relative 1e-9 of tau^2 (tau2_lo, tau2_hi),
 // where the rounding of the square root could decide.
 __device__ __forceinline__ bool far_pair(double rho2, double tau, double tau2_lo, double tau2_hi) {
+    if (!(tau > 0.0)) return sqrt(rho2) > tau;  // tau <= 0 (or NaN): the rule itself
     if (rho2 > tau2_hi) return true;
     if (rho2 < tau2_lo) return false;
     return sqrt(rho2) > tau;

@@ -226,6 +226,27 @@ def test_matrix_free_sweep_is_the_pair_storage(gpu, dirichlet, interior):
     assert free["J"] == pair["J"] and free["dk"] == pair["dk"]
     assert np.abs(free["g"] - pair["g"]).max() <= 1e-12 * np.abs(pair["g"]).max()
 
+
+@pytest.mark.parametrize("tau_h", [2.0, 0.75])
+def test_mixed_matrix_free_factors_follow_the_fp64_operator(gpu, tau_h):
+    """The mixed-precision matrix-free factors take the near/far decision from
+    rho^2 away from tau^2 and the FP64 rule next to it: for a wide and a narrow
+    near field the operator stays within its FP32 sine and cosine of the FP64
+    one, with no pair lost or counted twice."""
+    n = 24
+    h = 4.0 / n
+    pb = gpu.Problem()
+    pb.set_level(shapes.sphere(n))
+    pb.geometry(5)
+    pb.tube(gpu.params(4.0, 3 * h, tau_h * h, dirichlet=1))
+    rng = np.random.default_rng(9)
+    p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    q = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    y64, z64 = pb.apply(p, q, gpu.OP_FREE64)
+    y32, z32 = pb.apply(p, q, gpu.OP_FREE32)
+    assert relnorm(y32, y64) <= 1e-7 and relnorm(z32, z64) <= 1e-7, (relnorm(y32, y64), relnorm(z32, z64))
+    pb.close()
+
 def test_pair_storage_rejects_what_it_cannot_do(gpu):
     pb = gpu.Problem().set_level(shapes.sphere(16)).geometry(5)
     h = 4.0 / 16
