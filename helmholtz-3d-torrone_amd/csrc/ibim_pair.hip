@@ -296,8 +296,11 @@ __device__ __forceinline__ bool far_pair(double rho2, double tau, double tau2_lo
 enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2 };
 
 // One tile of the pair sweep (see the file comment for the planes written).
+// Matrix-free sweeps are held to 2 waves per SIMD: the FP64 factor's registers
+// otherwise leave the Neumann variant at one (14.8 vs 10.1 ms on 64,676 points).
 template <class MT, bool Dirichlet, int Factors = kStoredFactors>
-__global__ void __launch_bounds__(kPT) k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
+__global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
+    k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
                                                     const PairPoint* __restrict__ pts, int count, size_t ld,
                                                     int ncol, const double2* __restrict__ p,
                                                     const double2* __restrict__ q, double2* __restrict__ ypart,
