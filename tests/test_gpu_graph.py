@@ -3,7 +3,8 @@ captures one chunk of iterations and replays it for each full chunk (batched
 solves: after a first eager chunk). Checked
 bitwise against the same solves launched one kernel at a time, over several
 solves in a row with other device work between them, for plain BiCG on dense,
-pair and mixed pair storage, the residual-replacement and refinement solvers
+pair and mixed pair storage and the matrix-free pair sweeps, the
+residual-replacement and refinement solvers
 (their replacements, restarts and corrections run between replays) and the
 adjoint solve of the misfit gradient (J and dJ/dk bitwise; the level-set
 gradient, scattered with atomics, to 1e-12)."""
@@ -37,11 +38,13 @@ def run(gpu, pb, method, chunk):
 
 
 @pytest.mark.parametrize("store,method", [("C128", "bicg"), ("C128_PAIR", "bicg"), ("MIXED_PAIR", "bicg"),
-                                          ("MIXED_PAIR", "reliable"), ("C64_PAIR", "refined")])
+                                          ("MIXED_PAIR", "reliable"), ("C64_PAIR", "refined"),
+                                          ("matrix-free", "bicg"), ("matrix-free", "reliable")])
 @pytest.mark.parametrize("chunk", [1, 3])
 def test_graph_replay_gives_the_eager_bits(gpu, store, method, chunk):
     pb = problem(gpu)
-    pb.assemble(getattr(gpu, store))
+    if store != "matrix-free":  # else the pair sweeps computing their factors
+        pb.assemble(getattr(gpu, store))
     eager = run(gpu, pb, method, chunk)
     assert eager[2] == 0
     rng = np.random.default_rng(chunk)
@@ -53,7 +56,7 @@ def test_graph_replay_gives_the_eager_bits(gpu, store, method, chunk):
         assert np.array_equal(got[4], eager[4]) and np.array_equal(got[5], eager[5]), rep
         # other device work between the solves
         p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
-        pb.apply(p, p.conj(), gpu.OP_STORED)
+        pb.apply(p, p.conj(), gpu.OP_FREE64 if store == "matrix-free" else gpu.OP_STORED)
     pb.set_graphs(False)
     again = run(gpu, pb, method, chunk)
     assert np.array_equal(again[4], eager[4]) and np.array_equal(again[5], eager[5])
