@@ -139,6 +139,19 @@ def timed(pb, w, recv, mixed, steps, warmup):
     return t / steps, info, x, u
 
 
+def cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), or the platform's processor string."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    import platform
+    return platform.processor() or "unknown"
+
+
 def cpu_baseline(w, count, iters, budget_s=20.0):
     """The oracle (FP64 C + OpenMP, the reference's loop nests) on a bounded sample:
     assembly of a block of rows and reference-style sweeps over it, extrapolated
@@ -169,7 +182,7 @@ def cpu_baseline(w, count, iters, budget_s=20.0):
     t_solve = count * per_row_asm + iters * count * per_row_sweep
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {
-        "value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port",
+        "value": 1.0 / t_solve, "unit": "solves/s", "cores": threads, "kind": "port", "cpu_model": cpu_model(),
         "sample": "oracle FP64 assembly of %d of %d rows (%.2f s) and %d reference-order sweeps over them "
                   "(%.2f s), extrapolated to one solve = full assembly + %d BiCG sweeps" %
                   (nrows, count, t_asm, sweeps, t_sw, iters),
@@ -282,7 +295,7 @@ def run_batch(args):
                      "mixed_rel_err_vs_fp64": err})
         print("[bench] %s: N_t=%d, %d rhs, fp64 %.2f s, mixed %.2f s, rel err %.1e" %
               (name, f[3], m, f[0], mx[0], err), file=sys.stderr, flush=True)
-    emit({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / totals["mixed"],
+    return ({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / totals["mixed"],
           "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": totals["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "complex64 pair storage, fp64 matrix-core products, fp64 residual replacement over the "
@@ -365,7 +378,7 @@ def run_adjoint(args):
     fk = (J_of(level, k + 1e-6) - J_of(level, k - 1e-6)) / 2e-6
     print("[bench] adjoint96: N_t=%d forward %.3f s gradient %.3f s; %d/50 entries resolved, worst rel err %.1e" %
           (count, t_fwd, t_adj, resolved, max(errs) if errs else float("nan")), file=sys.stderr, flush=True)
-    emit({"metric": "misfit gradients/s (forward solve + adjoint)", "value": 1.0 / (t_fwd + t_adj),
+    return ({"metric": "misfit gradients/s (forward solve + adjoint)", "value": 1.0 / (t_fwd + t_adj),
           "unit": "gradients/s", "n_gpus": 1, "steps": 1, "warmup": 1, "ms_per_step": (t_fwd + t_adj) * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp64",
           "data": "synthetic: perturbed-sphere signed distance, point source, 200 receivers",
@@ -377,22 +390,28 @@ def run_adjoint(args):
 
 
 def run_star(args):
-    """The 256^3 case: star r = 1 + 0.2 cos4t sin3p, k = 40 (or --star-k),
-    eps = tau = 2h, 16 plane waves (Fibonacci), tol 1e-6. FP64 path: batched
-    BiCG on the FP64 matrix-free operator, the fused pair pass computing each
-    factor where complex128 pairs (which would need ~500 GB) are read.
-    Mixed path: batched FP64 refinement around batched BiCG on complex64 pair
-    storage (~251 GB, assembled inside the timed region) when it fits, else on
-    the mixed-precision matrix-free operator; corrections on the FP64
-    matrix-free operator. Each pass applies the operator to all 16 right-hand
-    sides on the matrix cores, with an iteration budget (--star-iterations;
-    BiCG does not converge within 300 iterations at k = 40). Reports solves/s
-    when every column converged, else batched iterations/s, and the residuals
-    reached."""
+    """The 256^3 case: star r = 1 + 0.2 cos4t sin3p, k = 40 (--star-k), eps = tau
+    = 2h, 16 plane waves (Fibonacci), sound-soft data, tol 1e-6 (the relative
+    residual of the FP64 operator). Formulation: the combined-field operator
+    1/2 I + K - i eta S with eta = k (--star-formulation double: the
+    reference's double layer 1/2 I + K alone, singular near the interior
+    Neumann eigenvalues, which lie ~0.003 apart in k here; BiCG does not
+    converge on it at k = 40).
+    FP64 path: batched BiCG on the FP64 matrix-free operator, the fused pair
+    pass computing each factor where complex128 pairs (which would need
+    ~490 GB) would be read. Mixed path: batched BiCG on complex64 pair storage
+    (~246 GB, assembled inside the timed region) when it fits, else on the
+    mixed-precision matrix-free operator, with FP64 residual replacement by the
+    FP64 matrix-free operator. Each pass applies the operator to all 16
+    right-hand sides on the matrix cores. Reports solves/s for both paths when
+    every column converged (else batched iterations/s and the residuals
+    reached) and the mixed densities' largest relative difference from the
+    FP64 ones."""
     n, k, m, its = 256, args.star_k, 16, args.star_iterations
+    combined = args.star_formulation == "combined"
     h = 4.0 / n
     dirs = fibonacci_directions(m)
-    out = {}
+    out, dens = {}, {}
     for path in ("mixed", "fp64"):
         pb = ibim.Problem()
         progress(pb)
@@ -400,12 +419,14 @@ def run_star(args):
         pb.sync()
         t0 = time.perf_counter()
         pb.geometry(5)
-        count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1))
+        count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1, combined=1 if combined else 0, eta=k))
         pb.set_nrhs(m)
         for c in range(m):
             pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
         pb.sync()
         t_setup = time.perf_counter() - t0
+        pb.reset_timers()
+        pb.set_profiling(True)
         t0 = time.perf_counter()
         if path == "mixed":
             store = "matrix-free"
@@ -417,43 +438,65 @@ def run_star(args):
                     print("[bench] star256: complex64 pairs not stored (%s); matrix-free" % e, file=sys.stderr,
                           flush=True)
             out["mixed_operator"] = store
-            info, _ = pb.solve(ibim.SOLVE_REFINED, maxit=its, tol=1e-6, inner_tol=1e-6, max_outer=3, batch=0)
+            info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=1e-6, inner_tol=BATCH_REPLACE, batch=0)
         else:
             info, _ = pb.solve(ibim.SOLVE_BICG, maxit=its, tol=1e-6, batch=0)
         pb.sync()
-        out[path] = (time.perf_counter() - t0, info, count, t_setup)
-        print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations (budget %d) %.1f s, residual %.2e, status %d" %
-              (path, count, t_setup, info.iterations, its, out[path][0], info.rel_residual, info.status),
-              file=sys.stderr, flush=True)
+        el = time.perf_counter() - t0
+        pb.set_profiling(False)
+        kern = {"mixed": "batch_pair" if out["mixed_operator"] != "matrix-free" else "batch_free32",
+                "fp64": "batch_free64"}[path]
+        ms, nl = pb.kernel_time(kern)
+        ms_asm, _ = pb.kernel_time("assemble")
+        ms_rep, n_rep = pb.kernel_time("batch_free64") if path == "mixed" else (0.0, 0)
+        d = [pb.density(c) for c in range(m)]
+        dens[path] = [a + 1j * b for a, b in d]
+        out[path] = dict(seconds=el, info=info, count=count, setup=t_setup, pass_ms=ms / max(nl, 1), passes=nl,
+                         assemble_ms=ms_asm, replacement_passes=n_rep)
+        print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations (budget %d) %.1f s, residual %.2e, status %d, "
+              "%d passes of %.1f ms" % (path, count, t_setup, info.iterations, its, el, info.rel_residual,
+                                         info.status, nl, ms / max(nl, 1)), file=sys.stderr, flush=True)
         pb.close()
     mixed_operator = out.pop("mixed_operator")
-    converged = {p: out[p][1].status == 0 for p in out}
+    converged = {p: out[p]["info"].status == 0 for p in out}
+    rel_err = float(max(np.linalg.norm(dens["mixed"][c] - dens["fp64"][c]) / np.linalg.norm(dens["fp64"][c])
+                        for c in range(m)))
     if all(converged.values()):
-        # every column converged: whole solves
-        metric, unit, value = "solves/s (16 right-hand sides)", "solves/s", m / out["mixed"][0]
-        ms = out["mixed"][0] * 1e3
-        fp64_rate = m / out["fp64"][0]
+        metric, unit, value = "solves/s (16 right-hand sides)", "solves/s", m / out["mixed"]["seconds"]
+        ms = out["mixed"]["seconds"] * 1e3
+        fp64_rate = m / out["fp64"]["seconds"]
     else:
-        # iteration budget exhausted: batched iterations run per second (the
-        # refinement runs its budget once per outer step)
-        rate = {p: out[p][1].iterations / m / out[p][0] for p in out}
+        # iteration budget exhausted: batched iterations run per second
+        rate = {p: out[p]["info"].iterations / m / out[p]["seconds"] for p in out}
         metric, unit, value = "batched BiCG iterations/s (16 right-hand sides)", "iterations/s", rate["mixed"]
         ms = 1e3 / rate["mixed"]
         fp64_rate = rate["fp64"]
-    emit({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
+    npairs = float(out["mixed"]["count"]) * (out["mixed"]["count"] - 1) / 2
+    # complex products per pass: A P and A^H Q, 16 columns, every pair visited
+    # from both of its rows (2 N^2 entries), 8 real flops per complex multiply-add
+    pass_flops = 2.0 * 2 * npairs * 16 * 8
+    return ({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "mixed: complex64 pair storage (else fp32 entries with fp64 phase), fp64 matrix-core products; "
-                   "fp64: fp64 matrix-free factors, fp64 matrix-core products",
+          "dtype": "mixed: complex64 pair storage (else fp32 sine / cosine of an fp64 phase), fp64 matrix-core "
+                   "products, fp64 residual replacement; fp64: fp64 matrix-free factors, fp64 matrix-core products",
           "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
-          "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, "
-                                 "%d-iteration budget" % (k, its),
-                     "tube_points": out["mixed"][2], "parallelism": "single GPU",
-                     "mixed_operator": mixed_operator, "fp64_operator": "matrix-free (fused pair pass, factors computed)"},
+          "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, tol 1e-6, "
+                                 "%s, %d-iteration budget" % (
+                                     k, "combined field 1/2 I + K - i eta S, eta = k" if combined else
+                                     "double layer 1/2 I + K", its),
+                     "tube_points": out["mixed"]["count"], "parallelism": "single GPU",
+                     "mixed_operator": mixed_operator,
+                     "fp64_operator": "matrix-free (fused pair pass, factors computed)"},
           "fp64_rate": fp64_rate, "speedup_mixed_vs_fp64": value / fp64_rate,
-          "seconds": {p: out[p][0] for p in out},
-          "iterations_summed_over_columns": {p: out[p][1].iterations for p in out},
-          "residual": {p: out[p][1].rel_residual for p in out}, "converged": converged,
-          "setup_s": out["mixed"][3]})
+          "mixed_rel_err_vs_fp64_density": rel_err,
+          "seconds": {p: out[p]["seconds"] for p in out},
+          "iterations_summed_over_columns": {p: out[p]["info"].iterations for p in out},
+          "residual": {p: out[p]["info"].rel_residual for p in out}, "converged": converged,
+          "pass_ms": {p: out[p]["pass_ms"] for p in out},
+          "pass_tflops": {p: pass_flops / (out[p]["pass_ms"] * 1e-3) / 1e12 for p in out if out[p]["pass_ms"] > 0},
+          "mixed_assemble_ms": out["mixed"]["assemble_ms"],
+          "mixed_replacement_passes": out["mixed"]["replacement_passes"],
+          "setup_s": out["mixed"]["setup"]})
 
 
 def measured_traffic(kernel, workload):
@@ -496,7 +539,12 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS) + list(EXTRA))
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--star-iterations", type=int, default=60, help="iteration budget of the star256 workload")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="default run: only the torus128 line, without the other workloads' records")
+    ap.add_argument("--no-star", action="store_true", help="default run: leave out the star256 record")
+    ap.add_argument("--star-iterations", type=int, default=1000, help="iteration budget of the star256 workload")
+    ap.add_argument("--star-formulation", choices=("combined", "double"), default="combined",
+                    help="star256: combined-field operator (eta = k) or the double layer alone")
     ap.add_argument("--star-k", type=float, default=40.0, help="wavenumber of the star256 workload")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
@@ -513,13 +561,23 @@ def main():
         calibrate_copy()
         return
     if args.workload in EXTRA:
-        {"batch": run_batch, "adjoint96": run_adjoint, "star256": run_star}[args.workload](args)
+        emit({"batch": run_batch, "adjoint96": run_adjoint, "star256": run_star}[args.workload](args))
         return
-    w = WORKLOADS[args.workload]
+    line = run_standard(args, args.workload)
+    if args.workload == "torus128" and not args.matrix_free and not args.no_configs:
+        line["configs"] = other_configs(args)
+    print(json.dumps(line), flush=True)
+
+
+def run_standard(args, workload):
+    """One of the single-right-hand-side workloads (torus128, sphere64, ...):
+    the FP64 and mixed paths timed, checked against an FP64 solve and the
+    analytic field. Returns the JSON record."""
+    w = WORKLOADS[workload]
     level = level_of(w)
     recv = shapes.shell_receivers(NRECV, 1.5, 3.0, seed=0)
     exact = shapes.point_source(recv, w["x0"], w["k"])
-    print("[bench] workload %s: uploading level set" % args.workload, file=sys.stderr, flush=True)
+    print("[bench] workload %s: uploading level set" % workload, file=sys.stderr, flush=True)
 
     # FP64 reference density (untimed): the dense complex128 operator (the
     # reference's entries), BiCG to REF_TOL; its sweep time is reported beside
@@ -569,7 +627,7 @@ def main():
         dom_bytes = npairs * 8 + vec_bytes
         achieved = dom_bytes / avg_dom / 1e9 if n_dom else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_pair_sweep<float2>", args.workload),
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_pair_sweep<float2>", workload),
                 "kernel": "k_pair_sweep<float2>: 8 B/pair x %d*%d/2 pairs + 64 B/point vectors per launch" %
                           (count, count - 1)}
     else:
@@ -605,7 +663,7 @@ def main():
                   "replacement"),
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
         "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g%s" % (
-            args.workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"],
+            workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"],
             ", matrix-free" if MATRIX_FREE else ""),
             "tube_points": count, "parallelism": "single GPU",
             "mixed_operator": {ibim.MIXED_PAIR: "complex64 pair storage + complex128 copy for the FP64 residuals",
@@ -620,7 +678,7 @@ def main():
         "fp64": {"solves_per_s": 1.0 / t64, "iterations": info64.iterations, "rel_err_vs_fp64": err_fp64,
                  "assemble_ms": ms_asm / max(n_asm, 1), "matvec_ms": avg_mv * 1e3,
                  "matvec_GBs": mv_gbs, "matvec_frac_of_peak": mv_gbs / HBM_PEAK_GBS if mv_gbs else None,
-                 "matvec_traffic": None if MATRIX_FREE else measured_traffic("k_pair_sweep<double2>", args.workload)},
+                 "matvec_traffic": None if MATRIX_FREE else measured_traffic("k_pair_sweep<double2>", workload)},
         "dense_c128_reference": {"iterations": info_ref.iterations, "tol": REF_TOL, "sweep_ms": avg_dense * 1e3,
                                  "sweep_GBs": (n2 * 16 + vec_bytes) / avg_dense / 1e9 if n_dense else 0.0},
         "speedup_mixed_vs_fp64": t64 / tm,
@@ -629,7 +687,70 @@ def main():
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(w, count, info64.iterations)
-    print(json.dumps(line))
+    return line
+
+
+def other_configs(args):
+    """The other workloads of the default run, one compact record each: solves
+    (or gradients) per second of both paths, the mixed path's error against
+    the FP64 one, and the checks each workload carries."""
+    global MATRIX_FREE
+    out = {}
+    sub = argparse.Namespace(**vars(args))
+    sub.no_cpu = True
+    sub.steps, sub.warmup = min(args.steps, 5), min(args.warmup, 1)
+    for name, fn in (("sphere64", lambda: run_standard(sub, "sphere64")),
+                     ("torus128_matrix_free", None),
+                     ("batch", lambda: run_batch(sub)),
+                     ("adjoint96", lambda: run_adjoint(sub)),
+                     ("star256", lambda: run_star(sub))):
+        if name == "star256" and args.no_star:
+            continue
+        t0 = time.perf_counter()
+        if name == "torus128_matrix_free":
+            MATRIX_FREE = True
+            try:
+                rec = run_standard(sub, "torus128")
+            finally:
+                MATRIX_FREE = False
+        else:
+            rec = fn()
+        out[name] = compact(name, rec)
+        out[name]["wall_s"] = time.perf_counter() - t0
+        print("[bench] %s: %s" % (name, json.dumps(out[name])), file=sys.stderr, flush=True)
+    return out
+
+
+def compact(name, r):
+    """The figures of a workload record that the default run reports."""
+    if name in ("sphere64", "torus128_matrix_free"):
+        return {"workload": r["config"]["workload"], "tube_points": r["config"]["tube_points"],
+                "solves_per_s": r["mixed"]["solves_per_s"], "fp64_solves_per_s": r["fp64"]["solves_per_s"],
+                "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
+                "rel_err_vs_fp64": {"mixed": r["mixed"]["rel_err_vs_fp64"], "fp64": r["fp64"]["rel_err_vs_fp64"]},
+                "field_rel_err_vs_analytic": r["mixed"]["field_rel_err_vs_analytic"],
+                "iterations": {"mixed": r["mixed"]["iterations"], "fp64": r["fp64"]["iterations"]},
+                "mixed_operator": r["config"]["mixed_operator"], "fp64_operator": r["config"]["fp64_operator"],
+                "roofline": r["roofline"]}
+    if name == "batch":
+        return {"workload": r["config"]["workload"], "rhs": r["config"]["rhs"], "solves_per_s": r["value"],
+                "fp64_solves_per_s": r["fp64_solves_per_s"], "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
+                "worst_rel_err_mixed_vs_fp64": r["worst_rel_err_mixed_vs_fp64"],
+                "schedule": r["config"]["schedule"]}
+    if name == "adjoint96":
+        keys = ("value", "unit", "forward_s", "gradient_s", "forward_iterations", "fd_entries_resolved",
+                "fd_worst_rel_err", "dk_rel_err")
+        d = {k: r[k] for k in keys}
+        d["workload"] = r["config"]["workload"]
+        d["tube_points"] = r["config"]["tube_points"]
+        return d
+    keys = ("metric", "value", "unit", "fp64_rate", "speedup_mixed_vs_fp64", "mixed_rel_err_vs_fp64_density",
+            "seconds", "iterations_summed_over_columns", "residual", "converged", "pass_ms", "pass_tflops")
+    d = {k: r[k] for k in keys}
+    d["workload"] = r["config"]["workload"]
+    d["tube_points"] = r["config"]["tube_points"]
+    d["mixed_operator"] = r["config"]["mixed_operator"]
+    return d
 
 
 if __name__ == "__main__":

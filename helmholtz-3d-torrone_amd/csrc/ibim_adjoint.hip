@@ -507,6 +507,14 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
     ibim_solve_info info{};
     rc = bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true, false);
     if (rc) return rc;
+    // an unconverged lambda gives a wrong gradient: fail rather than return it
+    // (status 4, a zero E^H e, has the exact zero lambda)
+    if (info.status != 0 && info.status != 4) {
+        set_error("ibim_gradient: the adjoint solve A^H lambda = E^H e did not converge (status %d after %d "
+                  "iterations, residual %.3e, tol %.3e)",
+                  info.status, info.iterations, info.rel_residual, opts->tol);
+        return IBIM_ERR_NOCONV;
+    }
 
     hipLaunchKernelGGL(k_adj_operator<1>, dim3(nb), dim3(kT), 0, ctx->stream, ctx->tube.as<double>(), count,
                        ctx->prm, phi, lam, adj);

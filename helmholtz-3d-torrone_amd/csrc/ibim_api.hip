@@ -176,6 +176,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->have_geometry = false;
     ctx->weno_order = 0;
     ctx->count = 0;
+    ctx->have_tube = false;
     ctx->mat_precision = -1;
     ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
     return IBIM_OK;
@@ -272,15 +273,21 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
         set_error("ibim_tube: run ibim_geometry first");
         return IBIM_ERR_ARG;
     }
+    if (prm->combined && (kind != IBIM_TUBE_TRADITIONAL || !prm->dirichlet || prm->interior)) {
+        set_error("ibim_tube: the combined-field operator is the exterior Dirichlet problem on the traditional tube");
+        return IBIM_ERR_ARG;
+    }
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
     ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
     ctx->count = 0;  // a failed build leaves no stale tube behind
+    ctx->have_tube = false;
     int c = 0;
     int rc = launch_tube(ctx, &c);
     if (rc) return rc;
     ctx->count = c;
+    ctx->have_tube = true;
     ctx->ld = ((size_t)c + 63) / 64 * 64;
     *count = c;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
@@ -329,6 +336,9 @@ int ibim_get_rows(ibim_ctx* ctx, int row0, int nrows, double* re, double* im) {
 int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const double* im, int precision) {
     CHECK(ctx);
     if (count <= 0 || !re || !im || (precision != IBIM_C128 && precision != IBIM_C64)) return IBIM_ERR_ARG;
+    // the operator no longer belongs to a tube: matrix-free operators, the
+    // FP64 operators of the mixed solvers and the evaluation need one
+    ctx->have_tube = false;
     ctx->count = count;
     ctx->ld = ((size_t)count + 63) / 64 * 64;
     const size_t esz = precision == IBIM_C128 ? 16 : 8;
@@ -350,6 +360,10 @@ int ibim_apply(ibim_ctx* ctx, int op, const double* pre, const double* pim, cons
     const int n = ctx->count;
     if (!n || op < IBIM_OP_STORED || op > IBIM_OP_FREE32 || (op == IBIM_OP_STORED && ctx->mat_precision < 0))
         return IBIM_ERR_ARG;
+    if (op != IBIM_OP_STORED && !ctx->have_tube) {
+        set_error("ibim_apply: matrix-free operators need a tube (the operator was set from the host)");
+        return IBIM_ERR_ARG;
+    }
     if (ctx->xfer.ensure(sizeof(double2) * 4 * (size_t)n)) return IBIM_ERR_NOMEM;
     double2* d = ctx->xfer.as<double2>();
     std::vector<double> t(4 * (size_t)n);
@@ -383,7 +397,7 @@ int ibim_apply_batch(ibim_ctx* ctx, int op, int adjoint, int nvec, const double*
         set_error("ibim_apply_batch: no stored operator (assemble one, or use a matrix-free op)");
         return IBIM_ERR_ARG;
     }
-    if (op != IBIM_OP_STORED && ctx->kind != IBIM_TUBE_TRADITIONAL) {
+    if (op != IBIM_OP_STORED && (ctx->kind != IBIM_TUBE_TRADITIONAL || !ctx->have_tube)) {
         set_error("ibim_apply_batch: matrix-free products need the traditional tube");
         return IBIM_ERR_ARG;
     }
@@ -405,6 +419,10 @@ int ibim_set_nrhs(ibim_ctx* ctx, int nrhs) {
 int ibim_rhs(ibim_ctx* ctx, int kind, const double* x0, double amp, const double* dir, int col) {
     CHECK(ctx);
     if (!ctx->count || col < 0 || col >= ctx->nrhs || kind < 0 || kind > 2) return IBIM_ERR_ARG;
+    if (!ctx->have_tube) {
+        set_error("ibim_rhs: no tube for the current operator (set with ibim_set_matrix_host)");
+        return IBIM_ERR_ARG;
+    }
     ctx->rhs_desc.resize(ctx->nrhs);
     auto& d = ctx->rhs_desc[col];
     d.kind = kind;
@@ -468,6 +486,12 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     CHECK(ctx);
     if (!opts || !info || opts->maxit <= 0 || !ctx->count) return IBIM_ERR_ARG;
     memset(info, 0, sizeof *info);
+    // without a tube only the stored operator exists: BiCG on it (the mixed
+    // solvers recompute their FP32 or FP64 operator from the points)
+    if (!ctx->have_tube && (ctx->mat_precision < 0 || opts->method != IBIM_SOLVE_BICG)) {
+        set_error("ibim_solve: the operator was set from the host; without a tube only BiCG on it is possible");
+        return IBIM_ERR_ARG;
+    }
     const bool batched = ctx->nrhs > 1 && opts->batch == 0;
     if (opts->method == IBIM_SOLVE_BICG)
         return batched ? run_bicg_batched(ctx, opts, info, hist) : run_bicg(ctx, opts, info, hist);
@@ -482,8 +506,12 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
 
 int ibim_evaluate(ibim_ctx* ctx, int mode, int col, int nrecv, const double* xyz, double* ure, double* uim) {
     CHECK(ctx);
-    if (!ctx->count || nrecv <= 0 || !xyz || col < 0 || col >= ctx->nrhs || mode < 0 || mode > 2)
+    if (!ctx->count || nrecv <= 0 || !xyz || col < 0 || col >= ctx->nrhs || mode < 0 || mode > IBIM_EVAL_COMBINED)
         return IBIM_ERR_ARG;
+    if (!ctx->have_tube) {
+        set_error("ibim_evaluate: no tube for the current operator (set with ibim_set_matrix_host)");
+        return IBIM_ERR_ARG;
+    }
     if (ctx->xfer.ensure(sizeof(double) * 5 * (size_t)nrecv)) return IBIM_ERR_NOMEM;
     double* dxyz = ctx->xfer.as<double>();
     double* dout = dxyz + 3 * (size_t)nrecv;
@@ -520,9 +548,10 @@ int ibim_gradient(ibim_ctx* ctx, int col, int mode, const ibim_solve_opts* opts,
                   double* dk) {
     CHECK(ctx);
     if (!ctx->count || col < 0 || col >= ctx->nrhs || !ctx->nrecv || !opts || (mode != IBIM_EVAL_SINGLE &&
-        mode != IBIM_EVAL_DOUBLE) || ctx->kind != IBIM_TUBE_TRADITIONAL || !ctx->weno_order) {
-        set_error("ibim_gradient: needs a solved traditional problem on WENO geometry, receivers and a "
-                  "single/double-layer mode");
+        mode != IBIM_EVAL_DOUBLE) || ctx->kind != IBIM_TUBE_TRADITIONAL || !ctx->weno_order || !ctx->have_tube ||
+        ctx->prm.combined) {
+        set_error("ibim_gradient: needs a solved traditional problem (not the combined-field operator) on WENO "
+                  "geometry, receivers and a single/double-layer mode");
         return IBIM_ERR_ARG;
     }
     return run_gradient(ctx, col, mode, opts, J, dlevel, dk);

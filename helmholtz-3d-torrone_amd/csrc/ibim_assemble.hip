@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(kAssembleCols) k_assemble(const double* __rest
         if (j < count) {
             PointData a = load_point(tube, c, i);
             if (Kind == IBIM_TUBE_TRADITIONAL)
-                trad_entry(a, b, i == j, prm.k, prm.tau, prm.dirichlet, prm.interior, &re, &im);
+                trad_entry(a, b, i == j, prm.k, prm.tau, prm.dirichlet, prm.interior, prm.combined, prm.eta, &re,
+                           &im);
             else
                 near += combo_entry(a, b, i == j, prm.k, prm.tau, prm.eta, &re, &im);
         }

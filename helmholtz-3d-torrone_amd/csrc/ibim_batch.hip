@@ -857,7 +857,8 @@ static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol
     }
     const double inner_tol = opts->inner_tol > 0 ? opts->inner_tol : 1e-4;
     const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
-    int outer = 0, total = 0;
+    int outer = 0;
+    int col_its[kR] = {};  // each column's own inner iterations, summed over the steps
     for (outer = 1; outer <= max_outer; ++outer) {
         double worst = 0.0;
         for (int r = 0; r < ncol; ++r)
@@ -867,9 +868,8 @@ static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol
         BState hs{};
         rc = bicg_batch_core(ctx, inner, a, nvb, opts->maxit, itol, opts->chunk, &hs);
         if (rc) return rc;
-        int most = 0;
-        for (int r = 0; r < kR; ++r) most = std::max(most, hs.done[r] > 0 ? hs.done[r] : (hs.done[r] < 0 ? 0 : opts->maxit));
-        total += most;
+        // a converged column has a zero right-hand side: zero data, no iterations
+        for (int r = 0; r < kR; ++r) col_its[r] += hs.done[r] > 0 ? hs.done[r] : (hs.done[r] < 0 ? 0 : opts->maxit);
         hipLaunchKernelGGL(kb_accumulate, dim3(nvb), dim3(kBT), 0, ctx->stream, X, a.x, count);
         rc = batch_apply(ctx, outer_op, false, X, a.v, nullptr);
         if (rc) return rc;
@@ -895,7 +895,7 @@ static int refined_batch(Context* ctx, int inner, int outer_op, int c0, int ncol
     IBIM_HIP(hipGetLastError());
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
     for (int r = 0; r < ncol; ++r) {
-        iters[c0 + r] = total;
+        iters[c0 + r] = col_its[r];
         errs[c0 + r] = res[r];
         stops[c0 + r] = normb[r] == 0.0 ? 4 : (res[r] < opts->tol ? 0 : 1);
         outers[c0 + r] = outer;

@@ -206,6 +206,7 @@ int dropin_compbicg(int size, double* density, double* densityimg, double** K, d
     if (size <= 0 || !K || !Ki) return IBIM_ERR_ARG;
     // the rows are read in place (double**) and interleaved on the way to HBM
     ctx->count = size;
+    ctx->have_tube = false;  // a host operator without points
     ctx->ld = ((size_t)size + 63) / 64 * 64;
     ctx->mat_precision = -1;
     ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
@@ -378,6 +379,7 @@ int dropin_new_kernel(int size, double delta, double epsilon, double* zxl, doubl
         }
     }
     ctx->count = count;
+    ctx->have_tube = false;  // the sphere's analytic points serve only this dense operator
     ctx->prm = ibim_params{};
     ctx->prm.k = WAVE;
     if (ctx->tube.ensure(sizeof(double) * t.size())) return IBIM_ERR_NOMEM;

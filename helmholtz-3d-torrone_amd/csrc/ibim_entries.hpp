@@ -32,9 +32,12 @@ __device__ __forceinline__ PointData load_point(const double* __restrict__ tube,
 // A_ij = w_j grad_x Phi(x*_i, y*_j) . n_q (+1/2 on the diagonal) for rho > tau,
 // q = i (Neumann) or j (Dirichlet); w_j * reg_q in the near field (the
 // reference indexes reg with a stale loop variable there; q is the intended
-// point).
+// point). combined (exterior Dirichlet only) adds the combined-field coupling
+// -i eta w_j Phi(x*_i, y*_j), in the near field -i eta w_j times the disc mean
+// of Phi (single_layer_near): A = 1/2 I + K - i eta S.
 __device__ __forceinline__ void trad_entry(const PointData& a, const PointData& b, bool diag, double k, double tau,
-                                           int dirichlet, int interior, double* re, double* im) {
+                                           int dirichlet, int interior, int combined, double eta, double* re,
+                                           double* im) {
     double d1 = a.sx - b.sx, d2 = a.sy - b.sy, d3 = a.sz - b.sz;
     double rho = sqrt(d1 * d1 + d2 * d2 + d3 * d3);
     double pr, pim;
@@ -49,9 +52,21 @@ __device__ __forceinline__ void trad_entry(const PointData& a, const PointData& 
             pr = c1 * ip;
             pim = c2 * ip;
         }
+        if (combined) {
+            double gr, gi;
+            phi_value(k, rho, &gr, &gi);
+            pr = pr + eta * gi;
+            pim = pim - eta * gr;
+        }
     } else {
         pr = dirichlet ? b.reg : a.reg;
         pim = 0.0;
+        if (combined) {
+            double nr, ni;
+            combined_near(k, tau, eta, &nr, &ni);
+            pr = pr + nr;
+            pim = ni;
+        }
     }
     *re = b.w * pr;
     *im = b.w * pim;

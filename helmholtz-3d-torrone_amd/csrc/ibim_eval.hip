@@ -47,7 +47,8 @@ __global__ void __launch_bounds__(kThreads) k_rhs(const double* __restrict__ tub
 
 // Field at receiver blockIdx.x. mode IBIM_EVAL_SINGLE: NeumannTest
 // (large3Dexample2.c:1810-1824); IBIM_EVAL_DOUBLE: the layer of the Dirichlet
-// kernel, w_j grad_x Phi(x, y*_j) . n_j; IBIM_EVAL_COMBO: Combo_NeumannTest's
+// kernel, w_j grad_x Phi(x, y*_j) . n_j; IBIM_EVAL_COMBINED: that layer minus
+// i eta w_j Phi(x, y*_j), the combined-field potential; IBIM_EVAL_COMBO: Combo_NeumannTest's
 // sample kernel (large3Dexample2.c:1988-2058) with the near field from reg_j.
 __global__ void __launch_bounds__(kThreads) k_eval(const double* __restrict__ tube, int count, ibim_params prm,
                                                    int mode, const double2* __restrict__ phi,
@@ -68,13 +69,19 @@ __global__ void __launch_bounds__(kThreads) k_eval(const double* __restrict__ tu
                 kr = cv / (-4. * kPi * rho);
                 ki = sv / (-4. * kPi * rho);
             }
-        } else if (mode == IBIM_EVAL_DOUBLE) {
+        } else if (mode == IBIM_EVAL_DOUBLE || mode == IBIM_EVAL_COMBINED) {
             if (rho > prm.tau) {
                 double c1, c2;
                 grad_phi_coeffs(k, rho, &c1, &c2);
                 double ip = a * tube[IBIM_F_NX * c + j] + b * tube[IBIM_F_NY * c + j] + cc * tube[IBIM_F_NZ * c + j];
                 kr = c1 * ip;
                 ki = c2 * ip;
+                if (mode == IBIM_EVAL_COMBINED) {  // - i eta Phi
+                    double gr, gi;
+                    phi_value(k, rho, &gr, &gi);
+                    kr = kr + prm.eta * gi;
+                    ki = ki - prm.eta * gr;
+                }
             }
         } else {
             double sv, cv;

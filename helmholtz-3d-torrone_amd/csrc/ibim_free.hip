@@ -113,7 +113,8 @@ __global__ void __launch_bounds__(kFT) k_sweep_free64(const double* __restrict__
                 double er = 0.0, ei = 0.0;
                 if (cj[m] < count) {
                     if (Kind == IBIM_TUBE_TRADITIONAL)
-                        trad_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.dirichlet, prm.interior, &er, &ei);
+                        trad_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.dirichlet, prm.interior, prm.combined,
+                                   prm.eta, &er, &ei);
                     else
                         combo_entry(a, col[m], i == cj[m], prm.k, prm.tau, prm.eta, &er, &ei);
                 }
@@ -179,6 +180,10 @@ unsigned nblocks(int count) {
 // The combined-field tube's matrix-free sweep: FP64 entries for either op (the
 // mixed-precision entries are the traditional kernel's).
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done) {
+    if (!ctx->have_tube) {
+        set_error("matrix-free sweep: no tube for the current operator (set from the host)");
+        return IBIM_ERR_ARG;
+    }
     if (ctx->kind != IBIM_TUBE_COMBO || (op != OP_FREE64 && op != OP_FREE32)) {
         set_error("matrix-free sweep: the traditional tube sweeps as pairs");
         return IBIM_ERR_ARG;

@@ -70,6 +70,10 @@ struct Context {
     ibim_params prm{};
     int kind = 0;
     int count = 0;
+    // count and the tube fields describe the current operator (false after
+    // ibim_set_matrix_host or a drop-in solve on a host matrix, which give an
+    // operator without points)
+    bool have_tube = false;
     DevBuf node, tube, blockcnt, err_flag;
 
     // operator

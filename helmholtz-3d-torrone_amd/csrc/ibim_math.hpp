@@ -435,4 +435,35 @@ IBIM_HD void grad_phi_coeffs(double k, double rho, double* c1, double* c2) {
     *c2 = (k * cv - esv) * den;
 }
 
+// Phi(x, y) = e^{ik rho} / (4 pi rho), the single-layer kernel of the
+// combined-field Dirichlet operator.
+IBIM_HD void phi_value(double k, double rho, double* re, double* im) {
+    const double rfp = 1. / (4. * kPi);
+    double rr = 1. / rho, sv, cv;
+    sincos(k * rho, &sv, &cv);
+    *re = cv * rr * rfp;
+    *im = sv * rr * rfp;
+}
+
+// Near-field value of the single layer: the mean of Phi over a flat disc of
+// radius tau about the point, (e^{ik tau} - 1) / (2 pi i k tau^2)
+// = [sin(k tau) + i (1 - cos(k tau))] / (2 pi k tau^2), written with sinc
+// factors so that k = 0 gives the Laplace value 1 / (2 pi tau).
+IBIM_HD void single_layer_near(double k, double tau, double* re, double* im) {
+    const double x = k * tau, hx = 0.5 * x;
+    const double sx = x != 0.0 ? sin(x) / x : 1.0;
+    const double sh = hx != 0.0 ? sin(hx) / hx : 1.0;
+    *re = sx / (2. * kPi * tau);
+    *im = k * sh * sh / (4. * kPi);
+}
+
+// The combined-field coupling's near-field term -i eta (S_re + i S_im) added
+// to the double layer's real near value: (eta S_im, -eta S_re).
+IBIM_HD void combined_near(double k, double tau, double eta, double* re, double* im) {
+    double sr, si;
+    single_layer_near(k, tau, &sr, &si);
+    *re = eta * si;
+    *im = -1. * eta * sr;
+}
+
 }  // namespace ibim

@@ -23,6 +23,15 @@
 // near neighbours (sorted, so the sums have a fixed order) in a small kernel
 // that writes one more partial plane.
 //
+// The combined-field Dirichlet operator (ibim_params.combined, kernel
+// kCombined) adds -i eta w_j Phi_ij with Phi = e^{ik rho} / (4 pi rho), also
+// symmetric in (i, j). Its pairs store G_ij = Phi(rho_ij) instead of F: with
+// u = 1 / rho, F = G (i k u - u^2), so
+//   A_ij = G [alpha_ij (i k u - u^2) - i eta w_j],  A_ji = G [beta_ij (i k u - u^2) - i eta w_i],
+// one stored value per pair as before, u rebuilt from the points in the sweep.
+// Its near entries are w_j (reg_j - i eta S_near) with S_near the disc mean of
+// Phi (single_layer_near).
+//
 // Storage: row blocks of kSweepRows rows; row block I holds tiles t = 0 ..
 // T_I - 1 of kSweepRows x kSweepCols values covering columns
 // I kSweepRows + t kSweepCols .. + kSweepCols - 1 (the first kSweepRows of them
@@ -37,6 +46,7 @@
 // t < T_J, column-side planes I <= J and the near plane, in that order.
 #include <cmath>
 #include <cstdlib>
+#include <type_traits>
 #include <vector>
 
 #include "ibim_entries.hpp"
@@ -136,8 +146,9 @@ __global__ void __launch_bounds__(256) k_pair_points(const double* __restrict__ 
     pts[p] = o;
 }
 
-// F of one tile, kPairAsmRows rows per workgroup (and, for the mixed storage,
-// the complex128 copy in F64); counts near pairs per point.
+// F (G for the combined operator) of one tile, kPairAsmRows rows per
+// workgroup (and, for the mixed storage, the complex128 copy in F64); counts
+// near pairs per point.
 template <class Store>
 __global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict__ tube, int count,
                                                        const int2* __restrict__ tiles, ibim_params prm,
@@ -166,10 +177,14 @@ __global__ void __launch_bounds__(kPT) k_pair_assemble(const double* __restrict_
             if (i < j && j < count) {
                 const double rho = pair_rho(a, b[m]);
                 if (rho > prm.tau) {
-                    double c1, c2;
-                    grad_phi_coeffs(prm.k, rho, &c1, &c2);
-                    fr = sigma * c1;
-                    fi = sigma * c2;
+                    if (prm.combined) {
+                        phi_value(prm.k, rho, &fr, &fi);  // G_ij (exterior only)
+                    } else {
+                        double c1, c2;
+                        grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                        fr = sigma * c1;
+                        fi = sigma * c2;
+                    }
                 } else {
                     atomicAdd(&ncnt[i], 1);
                     atomicAdd(&ncnt[j], 1);
@@ -288,6 +303,57 @@ __device__ __forceinline__ bool far_pair(double rho2, double tau, double tau2_lo
     return sqrt(rho2) > tau;
 }
 
+// Which operator the pair kernels build: the traditional kernel with the
+// normal of the row point (Neumann) or of the column point (Dirichlet), or the
+// combined-field Dirichlet operator.
+enum PairKernel { kNeumann = 0, kDirichlet = 1, kCombined = 2 };
+
+int pair_kernel(const Context* ctx) {
+    if (ctx->prm.dirichlet == 0) return kNeumann;
+    return ctx->prm.combined ? kCombined : kDirichlet;
+}
+
+// 1 / rho from rho^2: the FP32 reciprocal square root refined by Steps FP64
+// Newton steps (two: within a few ulp, the FP64 operators; one: ~3e-14, the
+// mixed ones). rho^2 is floored at 1e-30 so that the zero factors of
+// coincident or padding points stay zero in the products.
+template <int Steps>
+__device__ __forceinline__ double inv_rho(double rho2) {
+    const double r2 = fmax(rho2, 1e-30);
+    double ir = (double)__builtin_amdgcn_rsqf((float)r2);
+#pragma unroll
+    for (int s = 0; s < Steps; ++s) ir = ir * fma(-0.5 * r2, ir * ir, 1.5);
+    return ir;
+}
+
+// G = Phi(rho) for the mixed-precision combined operator (the phase reduced in
+// FP64, FP32 sine and cosine, as grad_phi_coeffs_fast), and u = 1 / rho.
+__device__ __forceinline__ void phi_fast(double kappa, double rho2, double* gr, double* gi, double* u) {
+    const double ir = inv_rho<1>(rho2);
+    const double t = (rho2 * ir) * kappa;
+    float s32, c32;
+    sincos_rev_f32(t - rint(t), &s32, &c32);
+    const double sc = ir * (1.0 / (4.0 * kPi));
+    *gr = (double)c32 * sc;
+    *gi = (double)s32 * sc;
+    *u = ir;
+}
+
+// The combined-field entries of a pair from G = Phi(rho), u = 1 / rho and the
+// point factors: A_ij = G [alpha (i k u - u^2) - i eta w_j] and A_ji likewise
+// with beta and w_i.
+__device__ __forceinline__ void combined_entries(double gr, double gi, double u, double k, double eta, double al,
+                                                 double be, double wi, double wj, double* ar, double* ai, double* br,
+                                                 double* bi) {
+    const double u2 = u * u, ku = k * u;
+    const double xr = -u2 * al, xi = fma(ku, al, -eta * wj);
+    const double yr = -u2 * be, yi = fma(ku, be, -eta * wi);
+    *ar = fma(gr, xr, -gi * xi);
+    *ai = fma(gr, xi, gi * xr);
+    *br = fma(gr, yr, -gi * yi);
+    *bi = fma(gr, yi, gi * yr);
+}
+
 // Where a pair sweep takes its factors F: the stored tiles, or evaluated where
 // they would be read -- with k_pair_assemble's operations (the complex128 pair
 // storage's bits: the FP64 matrix-free operator) or with
@@ -298,7 +364,7 @@ enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2 };
 // One tile of the pair sweep (see the file comment for the planes written).
 // Matrix-free sweeps are held to 2 waves per SIMD: the FP64 factor's registers
 // otherwise leave the Neumann variant at one (14.8 vs 10.1 ms on 64,676 points).
-template <class MT, bool Dirichlet, int Factors = kStoredFactors>
+template <class MT, int Kern, int Factors = kStoredFactors>
 __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
     k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
                                                     const PairPoint* __restrict__ pts, int count, size_t ld,
@@ -307,7 +373,9 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
                                                     double2* __restrict__ zpart, const int* __restrict__ done,
                                                     ibim_params prm) {
     constexpr bool Computed = Factors != kStoredFactors;
-    static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
+    constexpr bool Precise = sizeof(MT) == sizeof(double2);
+    static_assert(!Computed || Precise, "computed factors are complex128");
     if (done != nullptr && *done) return;
     __shared__ double sh[4][kPT / 64][kPR];  // (v re, v im, w re, w im) x wave x row
     const int2 tl = tiles[blockIdx.x];
@@ -375,36 +443,59 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
             double svr = 0.0, svi = 0.0, swr = 0.0, swi = 0.0;
 #pragma unroll
             for (int m = 0; m < kPCpt; ++m) {
+                double rho2 = 0.0, uinv = 0.0;
+                if (Computed || Combined) {
+                    const double d1 = R.x - cx[m], d2 = R.y - cy[m], d3 = R.z - cz[m];
+                    rho2 = d1 * d1 + d2 * d2 + d3 * d3;
+                }
                 if (Computed) {
-                    // F of pair (i, j), 0 where the storage holds 0 (j <= i,
+                    // F (G) of pair (i, j), 0 where the storage holds 0 (j <= i,
                     // past count, the near field)
                     const int j = c0 + t + kPT * m;
                     a[u][m].x = 0.0;
                     a[u][m].y = 0.0;
                     if (i < j && j < count) {
-                        const double d1 = R.x - cx[m], d2 = R.y - cy[m], d3 = R.z - cz[m];
-                        const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
                         double c1, c2;
                         if (Factors == kExactFactors) {
                             const double rho = sqrt(rho2);
                             if (rho > prm.tau) {
-                                grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                                if (Combined) {
+                                    phi_value(prm.k, rho, &c1, &c2);
+                                    a[u][m].x = c1;
+                                    a[u][m].y = c2;
+                                } else {
+                                    grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                                    a[u][m].x = sigma * c1;
+                                    a[u][m].y = sigma * c2;
+                                }
+                            }
+                        } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
+                            if (Combined) {
+                                phi_fast(kappa, rho2, &c1, &c2, &uinv);
+                                a[u][m].x = c1;
+                                a[u][m].y = c2;
+                            } else {
+                                grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
                                 a[u][m].x = sigma * c1;
                                 a[u][m].y = sigma * c2;
                             }
-                        } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
-                            grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
-                            a[u][m].x = sigma * c1;
-                            a[u][m].y = sigma * c2;
                         }
                     }
                 }
                 double al, be;
-                pair_factors<Dirichlet, sizeof(MT) == sizeof(double2)>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m],
-                                                                       cmz[m], cc[m], cw[m], &al, &be);
+                pair_factors<Dirichlet, Precise>(R, cx[m], cy[m], cz[m], cmx[m], cmy[m], cmz[m], cc[m], cw[m], &al,
+                                                 &be);
                 const double fr = (double)a[u][m].x, fi = (double)a[u][m].y;
-                const double ar = fr * al, ai = fi * al;  // A_ij
-                const double br = fr * be, bi = fi * be;  // A_ji
+                double ar, ai, br, bi;  // A_ij, A_ji
+                if (Combined) {
+                    if (Factors != kFastFactors) uinv = inv_rho<Precise ? 2 : 1>(rho2);
+                    combined_entries(fr, fi, uinv, prm.k, prm.eta, al, be, R.w, cw[m], &ar, &ai, &br, &bi);
+                } else {
+                    ar = fr * al;
+                    ai = fi * al;
+                    br = fr * be;
+                    bi = fi * be;
+                }
                 svr = fma(ar, pre[m], fma(-ai, pim[m], svr));  // A_ij p_j
                 svi = fma(ar, pim[m], fma(ai, pre[m], svi));
                 swr = fma(br, qre[m], fma(bi, qim[m], swr));  // conj(A_ji) q_j
@@ -444,34 +535,58 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
 }
 
 // Near-field entries (w_j reg_q, +1/2 on the diagonal; q = j for Dirichlet,
-// i for Neumann) applied to p and, conjugate-transposed (they are real), to q.
-template <bool Dirichlet>
+// i for Neumann) applied to p and, conjugate-transposed, to q. The combined
+// operator adds w_j (nr + i ni), its single layer's disc mean times -i eta
+// (combined_near), to every near entry.
+template <int Kern>
 __global__ void __launch_bounds__(256) k_pair_near(const double* __restrict__ tube, int count,
                                                    const int* __restrict__ off, const int* __restrict__ idx,
                                                    const double2* __restrict__ p, const double2* __restrict__ q,
                                                    double2* __restrict__ ynear, double2* __restrict__ znear,
-                                                   const int* __restrict__ done) {
+                                                   const int* __restrict__ done, double nr, double ni) {
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     if (done != nullptr && *done) return;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= count) return;
     const double* W = tube + (size_t)IBIM_F_W * count;
     const double* G = tube + (size_t)IBIM_F_REG * count;
     const double wi = W[i], gi = G[i];
-    const double aii = wi * gi + 0.5;
     const double2 pi = p[i], qi = q[i];
-    double yr = aii * pi.x, yi = aii * pi.y, zr = aii * qi.x, zi = aii * qi.y;
+    double yr, yi, zr, zi;
+    if (Combined) {
+        const double ar = wi * (gi + nr) + 0.5, ai = wi * ni;
+        yr = ar * pi.x - ai * pi.y;
+        yi = ar * pi.y + ai * pi.x;
+        zr = ar * qi.x + ai * qi.y;
+        zi = ar * qi.y - ai * qi.x;
+    } else {
+        const double aii = wi * gi + 0.5;
+        yr = aii * pi.x;
+        yi = aii * pi.y;
+        zr = aii * qi.x;
+        zi = aii * qi.y;
+    }
     const int e = off[i + 1];
 #pragma unroll 4
     for (int s = off[i]; s < e; ++s) {
         const int j = idx[s];
         const double wj = W[j], gj = G[j];
-        const double aij = Dirichlet ? wj * gj : wj * gi;
-        const double aji = Dirichlet ? wi * gi : wi * gj;
         const double2 pj = p[j], qj = q[j];
-        yr += aij * pj.x;
-        yi += aij * pj.y;
-        zr += aji * qj.x;
-        zi += aji * qj.y;
+        if (Combined) {
+            const double ar = wj * (gj + nr), ai = wj * ni;  // A_ij
+            const double br = wi * (gi + nr), bi = wi * ni;  // A_ji
+            yr += ar * pj.x - ai * pj.y;
+            yi += ar * pj.y + ai * pj.x;
+            zr += br * qj.x + bi * qj.y;
+            zi += br * qj.y - bi * qj.x;
+        } else {
+            const double aij = Dirichlet ? wj * gj : wj * gi;
+            const double aji = Dirichlet ? wi * gi : wi * gj;
+            yr += aij * pj.x;
+            yi += aij * pj.y;
+            zr += aji * qj.x;
+            zi += aji * qj.y;
+        }
     }
     ynear[i] = make_double2(yr, yi);
     znear[i] = make_double2(zr, zi);
@@ -479,11 +594,13 @@ __global__ void __launch_bounds__(256) k_pair_near(const double* __restrict__ tu
 
 // Dense rows row0 .. row0 + nrows - 1 of the pair operator (the sweep's
 // arithmetic), for inspection and tests.
-template <class MT, bool Dirichlet>
+template <class MT, int Kern>
 __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, const int* __restrict__ blk_start,
                                                    const PairPoint* __restrict__ pts, const double* __restrict__ tube,
                                                    int count, ibim_params prm, int row0, int nrows,
                                                    double2* __restrict__ out) {
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
+    constexpr bool Precise = sizeof(MT) == sizeof(double2);
     const int j = blockIdx.x * 256 + threadIdx.x;
     const int r = blockIdx.y;
     if (j >= count || r >= nrows) return;
@@ -492,8 +609,15 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
     double re, im;
     const PointData a = load_point(tube, cnt, i), b = load_point(tube, cnt, j);
     if (!(pair_rho(a, b) > prm.tau)) {
-        re = b.w * (Dirichlet ? b.reg : a.reg);
-        im = 0.0;
+        if (Combined) {
+            double nr, ni;
+            combined_near(prm.k, prm.tau, prm.eta, &nr, &ni);
+            re = b.w * (b.reg + nr);
+            im = b.w * ni;
+        } else {
+            re = b.w * (Dirichlet ? b.reg : a.reg);
+            im = 0.0;
+        }
         if (i == j) re += 0.5;
     } else {
         const int lo = i < j ? i : j, hi = i < j ? j : i;
@@ -503,11 +627,19 @@ __global__ void __launch_bounds__(256) k_pair_rows(const MT* __restrict__ F, con
         const MT f = F[tile * kPR * kPC + (size_t)(lo - I * kPR) * kPC + (hi - I * kPR - t * kPC)];
         const PairPoint L = pts[lo], H = pts[hi];
         double al, be;
-        pair_factors<Dirichlet, sizeof(MT) == sizeof(double2)>(L, H.x, H.y, H.z, H.mx, H.my, H.mz, H.c, H.w, &al,
-                                                               &be);
-        const double fac = i < j ? al : be;
-        re = (double)f.x * fac;
-        im = (double)f.y * fac;
+        pair_factors<Dirichlet, Precise>(L, H.x, H.y, H.z, H.mx, H.my, H.mz, H.c, H.w, &al, &be);
+        if (Combined) {
+            const double d1 = L.x - H.x, d2 = L.y - H.y, d3 = L.z - H.z;
+            const double u = inv_rho<Precise ? 2 : 1>(d1 * d1 + d2 * d2 + d3 * d3);
+            double ar, ai, br, bi;
+            combined_entries((double)f.x, (double)f.y, u, prm.k, prm.eta, al, be, L.w, H.w, &ar, &ai, &br, &bi);
+            re = i < j ? ar : br;
+            im = i < j ? ai : bi;
+        } else {
+            const double fac = i < j ? al : be;
+            re = (double)f.x * fac;
+            im = (double)f.y * fac;
+        }
     }
     out[(size_t)r * count + j] = make_double2(re, im);
 }
@@ -534,13 +666,14 @@ static_assert(kPR % 64 == 0 && kPC % 64 == 0, "a workgroup's rows share their ti
 // Factors other than kStoredFactors: F unused; every visit evaluates F of the
 // pair it would have read (kExactFactors: the complex128 pair storage's bits,
 // the FP64 matrix-free operator; kFastFactors: the mixed-precision one).
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Waves, int Factors>
+template <class MT, int Kern, bool WithY, bool WithZ, int Waves, int Factors>
 __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
                                                            const PairPoint* __restrict__ pts, int count,
                                                            ibim_params prm, const double2* __restrict__ P,
                                                            const double2* __restrict__ Q, double2* __restrict__ Y,
                                                            double2* __restrict__ Z, const int* __restrict__ stop) {
     constexpr bool Computed = Factors != kStoredFactors;
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
     constexpr int R = kBatchRHS;
@@ -574,10 +707,12 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
     const double sigma = prm.interior ? -1.0 : 1.0;
     const double kappa = prm.k / (2.0 * kPi);
     const double tau2_lo = prm.tau * prm.tau * (1.0 - 1e-9), tau2_hi = prm.tau * prm.tau * (1.0 + 1e-9);
-    auto factor = [&](int jj, bool keep) -> MT {
+    // u: 1 / rho of the pair where the mixed combined factor computed it
+    auto factor = [&](int jj, bool keep, double* u) -> MT {
         MT f;
         f.x = 0.0;
         f.y = 0.0;
+        *u = 0.0;
         if (keep) {
             const PairPoint& C = sp[jj];
             const double d1 = Ri.x - C.x, d2 = Ri.y - C.y, d3 = Ri.z - C.z;
@@ -586,31 +721,56 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
             if (Factors == kExactFactors) {
                 const double rho = sqrt(rho2);
                 if (rho > prm.tau) {
-                    grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                    if (Combined) {
+                        phi_value(prm.k, rho, &c1, &c2);
+                        f.x = c1;
+                        f.y = c2;
+                    } else {
+                        grad_phi_coeffs(prm.k, rho, &c1, &c2);
+                        f.x = sigma * c1;
+                        f.y = sigma * c2;
+                    }
+                }
+            } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
+                if (Combined) {
+                    phi_fast(kappa, rho2, &c1, &c2, u);
+                    f.x = c1;
+                    f.y = c2;
+                } else {
+                    grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
                     f.x = sigma * c1;
                     f.y = sigma * c2;
                 }
-            } else if (far_pair(rho2, prm.tau, tau2_lo, tau2_hi)) {
-                grad_phi_coeffs_fast(prm.k, kappa, rho2, &c1, &c2);
-                f.x = sigma * c1;
-                f.y = sigma * c2;
             }
         }
         return f;
     };
     // one k-step: this lane's entry F (row i, column j = staged index jj)
-    auto apply = [&](const MT f, int jj) {
+    auto apply = [&](const MT f, double u, int jj) {
         const PairPoint& C = sp[jj];
         double al, be;
         pair_factors<Dirichlet, Precise>(Ri, C.x, C.y, C.z, C.mx, C.my, C.mz, C.c, C.w, &al, &be);
         const double fr = (double)f.x, fi = (double)f.y;
+        double ar, ai, br, bi;  // A_ij, A_ji
+        if (Combined) {
+            if (Factors != kFastFactors) {
+                const double d1 = Ri.x - C.x, d2 = Ri.y - C.y, d3 = Ri.z - C.z;
+                u = inv_rho<Precise ? 2 : 1>(d1 * d1 + d2 * d2 + d3 * d3);
+            }
+            combined_entries(fr, fi, u, prm.k, prm.eta, al, be, Ri.w, C.w, &ar, &ai, &br, &bi);
+        } else {
+            ar = fr * al;
+            ai = fi * al;
+            br = fr * be;
+            bi = fi * be;
+        }
         if (WithY) {
             const double2 p = ps[jj][li];
-            y.add(fr * al, fi * al, p.x, p.y);  // A_ij p_j
+            y.add(ar, ai, p.x, p.y);  // A_ij p_j
         }
         if (WithZ) {
             const double2 q = qs[jj][li];
-            z.add(fr * be, -(fi * be), q.x, q.y);  // conj(A_ji) q_j
+            z.add(br, -bi, q.x, q.y);  // conj(A_ji) q_j
         }
     };
 
@@ -626,10 +786,13 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
 #pragma unroll 8
             for (int s = 0; s < kBatchK / 4; ++s) {
                 const int jj = 4 * s + lk;
-                if (Computed)
-                    apply(factor(jj, J * kPR + rj0 + jj < i && i < count), jj);
-                else
-                    apply(T[(size_t)(rj0 + jj) * kPC], jj);
+                if (Computed) {
+                    double u;
+                    const MT f = factor(jj, J * kPR + rj0 + jj < i && i < count, &u);
+                    apply(f, u, jj);
+                } else {
+                    apply(T[(size_t)(rj0 + jj) * kPC], 0.0, jj);
+                }
             }
         }
     }
@@ -646,9 +809,11 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
                 const int jj = 4 * s + lk;
                 if (Computed) {
                     const int j = c0 + cb + jj;
-                    apply(factor(jj, i < j && j < count), jj);
+                    double u;
+                    const MT f = factor(jj, i < j && j < count, &u);
+                    apply(f, u, jj);
                 } else {
-                    apply(Tr[cb + jj], jj);
+                    apply(Tr[cb + jj], 0.0, jj);
                 }
             }
         }
@@ -664,13 +829,15 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
 }
 
 // Near-field entries of the batched products (one thread per (row, RHS)),
-// added after the far part in list order.
-template <bool Dirichlet, bool WithY, bool WithZ>
+// added after the far part in list order; the combined operator's single-layer
+// term (nr, ni) as in k_pair_near.
+template <int Kern, bool WithY, bool WithZ>
 __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restrict__ tube, int count,
                                                          const int* __restrict__ off, const int* __restrict__ idx,
                                                          const double2* __restrict__ P, const double2* __restrict__ Q,
                                                          double2* __restrict__ Y, double2* __restrict__ Z,
-                                                         const int* __restrict__ stop) {
+                                                         const int* __restrict__ stop, double nr, double ni) {
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     constexpr int R = kBatchRHS;
     if (stop != nullptr && *stop) return;
     const int g = blockIdx.x * 256 + threadIdx.x;
@@ -679,31 +846,47 @@ __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restric
     const double* W = tube + (size_t)IBIM_F_W * count;
     const double* G = tube + (size_t)IBIM_F_REG * count;
     const double wi = W[i], gi = G[i];
-    const double aii = wi * gi + 0.5;
+    // the diagonal entry (re, im)
+    const double aii = Combined ? wi * (gi + nr) + 0.5 : wi * gi + 0.5;
+    const double aiii = Combined ? wi * ni : 0.0;
     double2 yv = make_double2(0.0, 0.0), zv = make_double2(0.0, 0.0);
     if (WithY) {
         const double2 p = P[(size_t)i * R + r];
-        yv = make_double2(aii * p.x, aii * p.y);
+        yv = Combined ? make_double2(aii * p.x - aiii * p.y, aii * p.y + aiii * p.x)
+                      : make_double2(aii * p.x, aii * p.y);
     }
     if (WithZ) {
         const double2 q = Q[(size_t)i * R + r];
-        zv = make_double2(aii * q.x, aii * q.y);
+        zv = Combined ? make_double2(aii * q.x + aiii * q.y, aii * q.y - aiii * q.x)
+                      : make_double2(aii * q.x, aii * q.y);
     }
     const int e = off[i + 1];
     for (int s = off[i]; s < e; ++s) {
         const int j = idx[s];
         const double wj = W[j], gj = G[j];
         if (WithY) {
-            const double aij = Dirichlet ? wj * gj : wj * gi;
             const double2 p = P[(size_t)j * R + r];
-            yv.x += aij * p.x;
-            yv.y += aij * p.y;
+            if (Combined) {
+                const double ar = wj * (gj + nr), ai = wj * ni;
+                yv.x += ar * p.x - ai * p.y;
+                yv.y += ar * p.y + ai * p.x;
+            } else {
+                const double aij = Dirichlet ? wj * gj : wj * gi;
+                yv.x += aij * p.x;
+                yv.y += aij * p.y;
+            }
         }
         if (WithZ) {
-            const double aji = Dirichlet ? wi * gi : wi * gj;
             const double2 q = Q[(size_t)j * R + r];
-            zv.x += aji * q.x;
-            zv.y += aji * q.y;
+            if (Combined) {
+                const double br = wi * (gi + nr), bi = wi * ni;
+                zv.x += br * q.x + bi * q.y;
+                zv.y += br * q.y - bi * q.x;
+            } else {
+                const double aji = Dirichlet ? wi * gi : wi * gj;
+                zv.x += aji * q.x;
+                zv.y += aji * q.y;
+            }
         }
     }
     if (WithY) {
@@ -716,7 +899,7 @@ __global__ void __launch_bounds__(256) k_pair_near_batch(const double* __restric
     }
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
+template <class MT, int Kern, bool WithY, bool WithZ, int Factors>
 void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     const int count = ctx->count;
@@ -725,20 +908,20 @@ void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* 
     const int* blk = ctx->pair_blk.as<int>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
     if (waves == 1)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 1, Factors>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 1, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else if (waves == 2)
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 2, Factors>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 2, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
     else
-        hipLaunchKernelGGL((k_pair_batch<MT, Dirichlet, WithY, WithZ, 4, Factors>), grid, block, 0, ctx->stream, F,
+        hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 4, Factors>), grid, block, 0, ctx->stream, F,
                            blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
 }
 
 // The workgroup shape of the far pass: IBIM_BATCH_WAVES (1, 2, 4) if set, else
 // the fastest of the three timed on this call's own operands (once per
 // context, storage and product kind; the outputs are the same bits).
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
+template <class MT, int Kern, bool WithY, bool WithZ, int Factors>
 int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y, double2* Z,
                 const int* stop) {
     if (const char* env = getenv("IBIM_BATCH_WAVES")) {
@@ -770,7 +953,7 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
         for (int c = 0; c < 3; ++c) {
             if (long_pass && c == 2 && best_of[1] >= best_of[0]) break;
             (void)hipEventRecord(e0, ctx->stream);
-            launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Factors>(ctx, cand[c], F, P, Q, Y, Z, stop);
+            launch_pair_batch_far<MT, Kern, WithY, WithZ, Factors>(ctx, cand[c], F, P, Q, Y, Z, stop);
             (void)hipEventRecord(e1, ctx->stream);
             float ms = 0.0f;
             if (hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess &&
@@ -789,33 +972,57 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
     return cand[pick - 1];
 }
 
-template <class MT, bool Dirichlet, bool WithY, bool WithZ, int Factors>
+template <class MT, int Kern, bool WithY, bool WithZ, int Factors>
 void launch_pair_batch(Context* ctx, int slot, const MT* F, const double2* P, const double2* Q, double2* Y,
                        double2* Z, const int* stop) {
     const int count = ctx->count;
-    const int waves = tuned_waves<MT, Dirichlet, WithY, WithZ, Factors>(ctx, slot, F, P, Q, Y, Z, stop);
+    const int waves = tuned_waves<MT, Kern, WithY, WithZ, Factors>(ctx, slot, F, P, Q, Y, Z, stop);
     TimedScope ts(ctx, Factors == kExactFactors  ? "batch_free64"
                        : Factors == kFastFactors ? "batch_free32"
                        : slot >= 6               ? "batch_pair64"
                                                  : "batch_pair");
-    launch_pair_batch_far<MT, Dirichlet, WithY, WithZ, Factors>(ctx, waves, F, P, Q, Y, Z, stop);
-    hipLaunchKernelGGL((k_pair_near_batch<Dirichlet, WithY, WithZ>),
+    launch_pair_batch_far<MT, Kern, WithY, WithZ, Factors>(ctx, waves, F, P, Q, Y, Z, stop);
+    double nr = 0.0, ni = 0.0;
+    if (Kern == kCombined) combined_near(ctx->prm.k, ctx->prm.tau, ctx->prm.eta, &nr, &ni);
+    hipLaunchKernelGGL((k_pair_near_batch<Kern, WithY, WithZ>),
                        dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
                        ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
-                       stop);
+                       stop, nr, ni);
 }
 
 // slot: factors (0 complex64, 3 complex128, 6 complex128 copy, 9 exact and 12
 // fast computed) + product kind
-template <class MT, bool Dirichlet, int Factors = kStoredFactors>
+template <class MT, int Kern, int Factors = kStoredFactors>
 void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* P, const double2* Q, double2* Y,
                            double2* Z, const int* stop) {
     if (Y && Z)
-        launch_pair_batch<MT, Dirichlet, true, true, Factors>(ctx, slot0, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Kern, true, true, Factors>(ctx, slot0, F, P, Q, Y, Z, stop);
     else if (Y)
-        launch_pair_batch<MT, Dirichlet, true, false, Factors>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Kern, true, false, Factors>(ctx, slot0 + 1, F, P, Q, Y, Z, stop);
     else
-        launch_pair_batch<MT, Dirichlet, false, true, Factors>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
+        launch_pair_batch<MT, Kern, false, true, Factors>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
+}
+
+// Calls fn(std::integral_constant<int, K>) with the context's pair kernel K.
+template <class Fn>
+void with_kernel(const Context* ctx, Fn&& fn) {
+    switch (pair_kernel(ctx)) {
+        case kNeumann:
+            fn(std::integral_constant<int, kNeumann>{});
+            break;
+        case kDirichlet:
+            fn(std::integral_constant<int, kDirichlet>{});
+            break;
+        default:
+            fn(std::integral_constant<int, kCombined>{});
+            break;
+    }
+}
+
+// The combined operator's near-field single-layer term (0 otherwise).
+void near_term(const Context* ctx, double* nr, double* ni) {
+    *nr = *ni = 0.0;
+    if (pair_kernel(ctx) == kCombined) combined_near(ctx->prm.k, ctx->prm.tau, ctx->prm.eta, nr, ni);
 }
 
 int near_lists(Context* ctx, dim3 grid);
@@ -933,8 +1140,8 @@ int launch_assemble_pair(Context* ctx, int precision, bool with64) {
 // factors (kept until the tube changes; a pair assembly builds the same).
 int prepare_pair_free(Context* ctx) {
     if (ctx->pair_aux) return IBIM_OK;
-    if (ctx->kind != IBIM_TUBE_TRADITIONAL || ctx->count <= 0) {
-        set_error("matrix-free pair products need the traditional tube");
+    if (ctx->kind != IBIM_TUBE_TRADITIONAL || ctx->count <= 0 || !ctx->have_tube) {
+        set_error("matrix-free pair products need the traditional tube of the current operator");
         return IBIM_ERR_ARG;
     }
     const int count = ctx->count;
@@ -969,30 +1176,30 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
     auto Q = (const double2*)q;
     auto Y = (double2*)ypart;
     auto Z = (double2*)zpart;
-    const bool dir = ctx->prm.dirichlet != 0;
     const int2* tiles = ctx->pair_tiles.as<int2>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
-    {
-        TimedScope ts(ctx, fp64_copy ? "matvec64" : "matvec");
-        if (fp64_copy || ctx->mat_precision == IBIM_C128) {
-            auto kern = dir ? k_pair_sweep<double2, true> : k_pair_sweep<double2, false>;
-            hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
-                               (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(), tiles, pts, count, ld, ncol, P, Q,
-                               Y, Z, done, ctx->prm);
-        } else {
-            auto kern = dir ? k_pair_sweep<float2, true> : k_pair_sweep<float2, false>;
-            hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream,
-                               ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+    const bool wide = fp64_copy || ctx->mat_precision == IBIM_C128;
+    double nr, ni;
+    near_term(ctx, &nr, &ni);
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        {
+            TimedScope ts(ctx, fp64_copy ? "matvec64" : "matvec");
+            if (wide)
+                hipLaunchKernelGGL((k_pair_sweep<double2, Kern>), dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0,
+                                   ctx->stream, (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(), tiles, pts, count,
+                                   ld, ncol, P, Q, Y, Z, done, ctx->prm);
+            else
+                hipLaunchKernelGGL((k_pair_sweep<float2, Kern>), dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0,
+                                   ctx->stream, ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done,
+                                   ctx->prm);
         }
-    }
-    {
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;
-        auto kern = dir ? k_pair_near<true> : k_pair_near<false>;
-        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_pair_near<Kern>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q,
-                           Y + np, Z + np, done);
-    }
+                           Y + np, Z + np, done, nr, ni);
+    });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }
@@ -1007,24 +1214,27 @@ int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q
     auto Q = (const double2*)q;
     auto Y = (double2*)ypart;
     auto Z = (double2*)zpart;
-    const bool dir = ctx->prm.dirichlet != 0;
-    {
-        TimedScope ts(ctx, fast ? "sweep_free32" : "sweep_free64");
-        auto kern = fast ? (dir ? k_pair_sweep<double2, true, kFastFactors> : k_pair_sweep<double2, false, kFastFactors>)
-                         : (dir ? k_pair_sweep<double2, true, kExactFactors>
-                                : k_pair_sweep<double2, false, kExactFactors>);
-        hipLaunchKernelGGL(kern, dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream, nullptr,
-                           ctx->pair_tiles.as<int2>(), ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z,
-                           done, ctx->prm);
-    }
-    {
+    double nr, ni;
+    near_term(ctx, &nr, &ni);
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        {
+            TimedScope ts(ctx, fast ? "sweep_free32" : "sweep_free64");
+            if (fast)
+                hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kFastFactors>), dim3((unsigned)ctx->pair_ntiles),
+                                   dim3(kPT), 0, ctx->stream, nullptr, ctx->pair_tiles.as<int2>(),
+                                   ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+            else
+                hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kExactFactors>), dim3((unsigned)ctx->pair_ntiles),
+                                   dim3(kPT), 0, ctx->stream, nullptr, ctx->pair_tiles.as<int2>(),
+                                   ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+        }
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;
-        auto kern = dir ? k_pair_near<true> : k_pair_near<false>;
-        hipLaunchKernelGGL(kern, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
+        hipLaunchKernelGGL((k_pair_near<Kern>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,
                            ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q,
-                           Y + np, Z + np, done);
-    }
+                           Y + np, Z + np, done, nr, ni);
+    });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }
@@ -1035,21 +1245,14 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
         set_error("batched pair products without the pair storage or vectors they need");
         return IBIM_ERR_ARG;
     }
-    const bool dir = ctx->prm.dirichlet != 0;
-    if (fp64_copy || ctx->mat_precision == IBIM_C128) {
-        const double2* F = (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>();
-        const int slot0 = fp64_copy ? 6 : 3;
-        if (dir)
-            launch_pair_batch_any<double2, true>(ctx, slot0, F, P, Q, Y, Z, stop);
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        if (fp64_copy || ctx->mat_precision == IBIM_C128)
+            launch_pair_batch_any<double2, Kern>(ctx, fp64_copy ? 6 : 3, (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(),
+                                                 P, Q, Y, Z, stop);
         else
-            launch_pair_batch_any<double2, false>(ctx, slot0, F, P, Q, Y, Z, stop);
-    } else {
-        const float2* F = ctx->mat.as<float2>();
-        if (dir)
-            launch_pair_batch_any<float2, true>(ctx, 0, F, P, Q, Y, Z, stop);
-        else
-            launch_pair_batch_any<float2, false>(ctx, 0, F, P, Q, Y, Z, stop);
-    }
+            launch_pair_batch_any<float2, Kern>(ctx, 0, ctx->mat.as<float2>(), P, Q, Y, Z, stop);
+    });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }
@@ -1064,34 +1267,31 @@ int pair_free_batch_apply(Context* ctx, bool fast, const double2* P, const doubl
         return IBIM_ERR_ARG;
     }
     if (int rc = prepare_pair_free(ctx)) return rc;
-    const bool dir = ctx->prm.dirichlet != 0;
-    if (fast && dir)
-        launch_pair_batch_any<double2, true, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
-    else if (fast)
-        launch_pair_batch_any<double2, false, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
-    else if (dir)
-        launch_pair_batch_any<double2, true, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
-    else
-        launch_pair_batch_any<double2, false, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        if (fast)
+            launch_pair_batch_any<double2, Kern, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
+        else
+            launch_pair_batch_any<double2, Kern, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
+    });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }
 
 int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev) {
-    const bool dir = ctx->prm.dirichlet != 0;
     const dim3 grid((unsigned)((ctx->count + 255) / 256), (unsigned)nrows);
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
     const double* tube = ctx->tube.as<double>();
     const int* blk = ctx->pair_blk.as<int>();
-    if (ctx->mat_precision == IBIM_C128) {
-        auto kern = dir ? k_pair_rows<double2, true> : k_pair_rows<double2, false>;
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, ctx->mat.as<double2>(), blk, pts, tube, ctx->count,
-                           ctx->prm, row0, nrows, out_dev);
-    } else {
-        auto kern = dir ? k_pair_rows<float2, true> : k_pair_rows<float2, false>;
-        hipLaunchKernelGGL(kern, grid, dim3(256), 0, ctx->stream, ctx->mat.as<float2>(), blk, pts, tube, ctx->count,
-                           ctx->prm, row0, nrows, out_dev);
-    }
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        if (ctx->mat_precision == IBIM_C128)
+            hipLaunchKernelGGL((k_pair_rows<double2, Kern>), grid, dim3(256), 0, ctx->stream, ctx->mat.as<double2>(),
+                               blk, pts, tube, ctx->count, ctx->prm, row0, nrows, out_dev);
+        else
+            hipLaunchKernelGGL((k_pair_rows<float2, Kern>), grid, dim3(256), 0, ctx->stream, ctx->mat.as<float2>(),
+                               blk, pts, tube, ctx->count, ctx->prm, row0, nrows, out_dev);
+    });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

@@ -38,7 +38,7 @@ C128_PAIR, C64_PAIR = 2, 3
 # complex64 pairs + a complex128 copy for the FP64 residuals of the mixed solvers
 MIXED_PAIR = 4
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
-EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO = 0, 1, 2
+EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO, EVAL_COMBINED = 0, 1, 2, 3
 SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2
 OP_STORED, OP_FREE64, OP_FREE32 = 0, 1, 2
 
@@ -53,7 +53,7 @@ class IbimError(RuntimeError):
 class Params(C.Structure):
     """ibim_params: run-time form of the reference's PDE.h switches."""
     _fields_ = [("k", D), ("eps", D), ("tau", D), ("delta_in", D), ("eta", D), ("sine_root", D),
-                ("interior", I), ("dirichlet", I), ("one_sided", I)]
+                ("interior", I), ("dirichlet", I), ("one_sided", I), ("combined", I)]
 
 
 SHAPE_SPHERE, SHAPE_TORUS, SHAPE_ELLIPSOID, SHAPE_STAR = 0, 1, 2, 3
@@ -77,8 +77,10 @@ class SolveInfo(C.Structure):
     _fields_ = [("iterations", I), ("outer", I), ("rel_residual", D), ("status", I)]
 
 
-def params(k, eps, tau, delta_in=0.0, eta=-1.0, sine_root=0.1, interior=0, dirichlet=0, one_sided=0):
-    return Params(k, eps, tau, delta_in, eta, sine_root, interior, dirichlet, one_sided)
+def params(k, eps, tau, delta_in=0.0, eta=-1.0, sine_root=0.1, interior=0, dirichlet=0, one_sided=0, combined=0):
+    """combined=1: the combined-field exterior Dirichlet operator 1/2 I + K - i eta S
+    (dirichlet=1, interior=0, traditional tube); eta = k is the usual coupling."""
+    return Params(k, eps, tau, delta_in, eta, sine_root, interior, dirichlet, one_sided, combined)
 
 
 _lib = None

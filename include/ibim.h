@@ -44,6 +44,8 @@ extern "C" {
                                     (interpolate3D.c:31-36, 81-85)                 */
 #define IBIM_ERR_NOMEM (-5)      /* device allocation failed                       */
 #define IBIM_ERR_NODEVICE (-6)   /* no HIP device                                  */
+#define IBIM_ERR_NOCONV (-7)     /* a solve inside the call (ibim_gradient's       */
+                                 /* adjoint solve) did not converge                */
 
 /* ------------------------------------------------------------ data model --- */
 /* Tube fields, field-major (field f of point p at fields[f * count + p]). */
@@ -72,6 +74,11 @@ typedef struct ibim_params {
     int interior;      /* INTERIOR_MODE                                       */
     int dirichlet;     /* DIRICHLET: double-layer normal at the source point  */
     int one_sided;     /* TRAD_ONESIDED / COMBO_ONESIDED                      */
+    int combined;      /* traditional tube, exterior Dirichlet only: the      */
+                       /* combined-field potential u = (D - i eta S) phi,     */
+                       /* A = 1/2 I + K - i eta S (Brakhage-Werner), with the */
+                       /* coupling eta above; 0 = the reference's double      */
+                       /* layer 1/2 I + K                                     */
 } ibim_params;
 
 #define IBIM_TUBE_TRADITIONAL 0 /* |d| <= eps, cosine delta (large3Dexample2.c:4340-4452) */
@@ -100,6 +107,8 @@ typedef struct ibim_params {
 #define IBIM_EVAL_SINGLE 0 /* sum_j -Phi(x,y*_j) phi_j w_j   (NeumannTest, large3Dexample2.c:1810-1824) */
 #define IBIM_EVAL_DOUBLE 1 /* sum_j w_j grad_x Phi(x,y*_j).n_j phi_j (the Dirichlet kernel's layer)        */
 #define IBIM_EVAL_COMBO 2  /* Combo_NeumannTest sample kernel (large3Dexample2.c:1980-2058)                  */
+#define IBIM_EVAL_COMBINED 3 /* sum_j w_j [grad_x Phi(x,y*_j).n_j - i eta Phi(x,y*_j)] phi_j: the layer of the
+                                combined-field Dirichlet kernel (ibim_params.combined)                          */
 
 #define IBIM_SOLVE_BICG 0    /* the reference's complex BiCG on the stored operator      */
 #define IBIM_SOLVE_REFINED 1 /* FP64 iterative refinement around a complex-FP32 BiCG      */

@@ -16,6 +16,7 @@
  */
 #include "ibim_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stddef.h>
 #include <stdlib.h>
@@ -455,10 +456,24 @@ int ibo_tube_build(const double *u, const double *gx, const double *gy, const do
 /* ------------------------------------------------------------------------- */
 #define T(f, p) tube[(size_t)(f) * (size_t)count + (size_t)(p)]
 
+/* Near-field value of the single layer of the combined-field operator: the
+ * mean of e^{ik r} / (4 pi r) over a flat disc of radius tau,
+ * (1 / (pi tau^2)) int_0^tau e^{ik r} / (4 pi r) 2 pi r dr
+ * = (e^{ik tau} - 1) / (2 pi i k tau^2), and 1 / (2 pi tau) at k = 0. */
+static double complex single_layer_disc_mean(double k, double tau)
+{
+    if (k == 0.0) return 1.0 / (2.0 * IBO_PI * tau);
+    return (cexp(I * k * tau) - 1.0) / (2.0 * IBO_PI * I * k * tau * tau);
+}
+
 /* Traditional kernel: omp_get_Ccomp entry block (large3Dexample2.c:4463-4564).
  * A_ij = w_j grad_x Phi(x*_i, y*_j) . n_(i or j) (+1/2 on the diagonal) for
  * rho > tau, else w_j reg (reg of row i for Neumann, of column j for
- * Dirichlet -- the reference indexes this with a stale loop variable). */
+ * Dirichlet -- the reference indexes this with a stale loop variable).
+ * combined (no counterpart in the reference, whose combined field is the
+ * Neumann combo kernel, HelmholtzKernel.c:509-521): the exterior Dirichlet
+ * operator 1/2 I + K - i eta S of the potential u = (D - i eta S) phi, with the
+ * single layer's near field taken as its disc mean. */
 static void trad_entry(int count, const double *tube, const ibo_params *prm, int pi, int pj, double *re, double *im)
 {
     const double rfp = 1. / (4. * IBO_PI);
@@ -476,6 +491,13 @@ static void trad_entry(int count, const double *tube, const ibo_params *prm, int
     } else {
         pr = T(IBO_REG, prm->dirichlet ? pj : pi);
         pim = 0.0;
+    }
+    if (prm->combined) {
+        double complex s = rho > prm->tau ? cexp(I * prm->k * rho) / (4. * IBO_PI * rho)
+                                          : single_layer_disc_mean(prm->k, prm->tau);
+        double complex t = -I * prm->eta * s;
+        pr += creal(t);
+        pim += cimag(t);
     }
     *re = T(IBO_W, pj) * pr;
     *im = T(IBO_W, pj) * pim;
@@ -812,13 +834,18 @@ void ibo_eval(int count, const double *tube, const ibo_params *prm, const double
                     kr = 0.;
                     ki = 0.;
                 }
-            } else if (mode == 1) {
+            } else if (mode == 1 || mode == 3) {
                 if (rho > prm->tau) {
                     double sv = sin(k * rho), cv = cos(k * rho), den = 1. / (4. * IBO_PI * rho * rho);
                     double c1 = -(k * sv + cv / rho) * den, c2 = (k * cv - sv / rho) * den;
                     double ip = a * T(IBO_NX, j) + b * T(IBO_NY, j) + c * T(IBO_NZ, j);
                     kr = c1 * ip;
                     ki = c2 * ip;
+                    if (mode == 3) {
+                        double complex t = -I * prm->eta * cexp(I * k * rho) / (4. * IBO_PI * rho);
+                        kr += creal(t);
+                        ki += cimag(t);
+                    }
                 } else {
                     kr = 0.;
                     ki = 0.;

@@ -45,6 +45,7 @@ typedef struct {
     int interior;      /* INTERIOR_MODE                                      */
     int dirichlet;     /* DIRICHLET                                          */
     int one_sided;     /* TRAD_ONESIDED / COMBO_ONESIDED                     */
+    int combined;      /* exterior Dirichlet: add -i eta S (combined field)  */
 } ibo_params;
 
 /* L2: WENO and Godunov gradient, limited second derivatives, curvatures. */
@@ -102,7 +103,8 @@ void ibo_rhs_plane_neumann(int count, const double *tube, const double *interp_n
 void ibo_rhs_plane_dirichlet(int count, const double *tube, double k, double amp, const double *dir, double *bre,
                              double *bim);
 
-/* Field evaluation at receivers (mode 0 single layer, 1 double layer, 2 combo). */
+/* Field evaluation at receivers (mode 0 single layer, 1 double layer, 2 combo,
+ * 3 double layer - i eta single layer: the combined-field potential). */
 void ibo_eval(int count, const double *tube, const ibo_params *prm, const double *phre, const double *phim, int nrecv,
               const double *xyz, int mode, double *ure, double *uim);
 

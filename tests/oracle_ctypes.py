@@ -23,11 +23,12 @@ NF = len(FIELDS)
 
 class Params(C.Structure):
     _fields_ = [("k", D), ("eps", D), ("tau", D), ("delta_in", D), ("eta", D), ("sine_root", D),
-                ("interior", I), ("dirichlet", I), ("one_sided", I)]
+                ("interior", I), ("dirichlet", I), ("one_sided", I), ("combined", I)]
 
 
-def make_params(k, eps, tau, delta_in=0.0, eta=-1.0, sine_root=0.1, interior=0, dirichlet=0, one_sided=0):
-    return Params(k, eps, tau, delta_in, eta, sine_root, interior, dirichlet, one_sided)
+def make_params(k, eps, tau, delta_in=0.0, eta=-1.0, sine_root=0.1, interior=0, dirichlet=0, one_sided=0,
+                combined=0):
+    return Params(k, eps, tau, delta_in, eta, sine_root, interior, dirichlet, one_sided, combined)
 
 
 def _p(a):

@@ -73,3 +73,22 @@ def test_caller_globals_are_weak_definitions():
 def test_library_is_built_for_gfx950():
     out = subprocess.run(["strings", LIB], capture_output=True, text=True).stdout
     assert "gfx950" in out
+
+
+def test_cxx_caller_references_the_mangled_names_and_defines_the_globals():
+    """tests/cxx/reference_caller declares the reference's prototypes without
+    extern "C": the symbols it imports from libibim.so are the mangled twins,
+    and it defines the driver's globals itself (the GPU test runs it)."""
+    caller = os.path.join(REPO, "tests", "cxx", "reference_caller")
+    if not os.path.exists(caller):
+        subprocess.run(["make", "-s", "-C", os.path.dirname(caller)], check=True)
+    undef = subprocess.run(["nm", "-u", caller], capture_output=True, text=True).stdout.split()
+    for name in ("_Z8CompBiCGiPdS_PS_S0_S_S_", "_Z15ASSIGN_GRADIENTidPPPdS1_S1_S1_i",
+                 "_Z24new_HelmholtzKernelComboiddPPPdS1_S1_S1_S1_S1_S_S_S_S_S_S_S_S_S_S_S_S_S_S_S_S_S_S0_S0_S0_S0_dd"):
+        assert name in undef, name
+    assert not any(u in ("CompBiCG", "ASSIGN_GRADIENT", "new_HelmholtzKernelCombo") for u in undef)
+    defined = subprocess.run(["nm", caller], capture_output=True, text=True).stdout
+    for g in ("N", "SIZE", "H", "HCUBED", "WAVE"):
+        assert re.search(r"\s[BD]\s%s$" % g, defined, flags=re.M), g
+    linked = subprocess.run(["ldd", caller], capture_output=True, text=True).stdout
+    assert "libibim.so" in linked

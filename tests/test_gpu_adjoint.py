@@ -69,12 +69,13 @@ class Pipeline:
     def J(self, level, k, target):
         return float(np.sum(np.abs(self.field(level, k) - target) ** 2))
 
-    def gradient(self, level, k, target):
+    def gradient(self, level, k, target, maxit=2000):
+        """maxit: the adjoint solve's iteration budget."""
         self.prepare(level, k)
         info, _ = self.pb.solve(tol=1e-13, maxit=2000)
         assert info.status == 0
         self.pb.set_receivers(self.recv, target)
-        return self.pb.gradient(mode=self.mode, tol=1e-13, maxit=2000)
+        return self.pb.gradient(mode=self.mode, tol=1e-13, maxit=maxit)
 
 
 CASES = [
@@ -122,6 +123,17 @@ def test_gradient_matches_central_differences(gpu, dirichlet, interior, mode, k)
     fk1 = (P.J(level, k + 1e-6, target) - P.J(level, k - 1e-6, target)) / 2e-6
     fk2 = (P.J(level, k + 5e-7, target) - P.J(level, k - 5e-7, target)) / 1e-6
     assert abs((4 * fk2 - fk1) / 3 - dk) <= 1e-5 * abs(dk), (fk1, fk2, dk)
+
+
+def test_unconverged_adjoint_solve_is_an_error(gpu):
+    # a 2-iteration budget cannot solve A^H lambda = E^H e: the gradient call
+    # reports that instead of returning a gradient built from a wrong lambda
+    P = Pipeline(gpu, 1, 0, 1, 3.0)
+    target = P.field(P.star(0.10), 3.0)
+    level = P.star(0.15)
+    P.gradient(level, 3.0, target)  # converges with the normal budget
+    with pytest.raises(gpu.IbimError, match="adjoint"):
+        P.gradient(level, 3.0, target, maxit=2)
 
 
 def test_nodes_outside_every_stencil_have_zero_gradient(gpu):

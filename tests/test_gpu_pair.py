@@ -228,23 +228,35 @@ def test_matrix_free_sweep_is_the_pair_storage(gpu, dirichlet, interior):
 
 
 @pytest.mark.parametrize("tau_h", [2.0, 0.75])
-def test_mixed_matrix_free_factors_follow_the_fp64_operator(gpu, tau_h):
+@pytest.mark.parametrize("dirichlet,interior", [(1, 0), (0, 0), (1, 1), (0, 1)])
+def test_mixed_matrix_free_factors_follow_the_fp64_operator(gpu, tau_h, dirichlet, interior):
     """The mixed-precision matrix-free factors take the near/far decision from
     rho^2 away from tau^2 and the FP64 rule next to it: for a wide and a narrow
-    near field the operator stays within its FP32 sine and cosine of the FP64
-    one, with no pair lost or counted twice."""
+    near field, Dirichlet and Neumann, exterior and interior, the operator stays
+    within its FP32 sine and cosine of the FP64 one, with no pair lost or
+    counted twice, single-vector and 16 columns at a time."""
     n = 24
     h = 4.0 / n
     pb = gpu.Problem()
     pb.set_level(shapes.sphere(n))
     pb.geometry(5)
-    pb.tube(gpu.params(4.0, 3 * h, tau_h * h, dirichlet=1))
+    pb.tube(gpu.params(4.0, 3 * h, tau_h * h, dirichlet=dirichlet, interior=interior))
     rng = np.random.default_rng(9)
     p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
     q = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
     y64, z64 = pb.apply(p, q, gpu.OP_FREE64)
     y32, z32 = pb.apply(p, q, gpu.OP_FREE32)
     assert relnorm(y32, y64) <= 1e-7 and relnorm(z32, z64) <= 1e-7, (relnorm(y32, y64), relnorm(z32, z64))
+    # the batched passes of the same operators: FP64 against FP64 to rounding,
+    # mixed against its own single-vector sweep to rounding
+    P = np.stack([p, q] + [rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count) for _ in range(14)], 1)
+    # column 0 of A P is A p, column 1 of A^H P is A^H q
+    for adjoint, col, single64, single32 in ((False, 0, y64, y32), (True, 1, z64, z32)):
+        Y64 = pb.apply_batch(P, gpu.OP_FREE64, adjoint=adjoint)
+        Y32 = pb.apply_batch(P, gpu.OP_FREE32, adjoint=adjoint)
+        assert relnorm(Y32, Y64) <= 1e-7
+        assert relnorm(Y64[:, col], single64) <= 1e-13
+        assert relnorm(Y32[:, col], single32) <= 1e-13
     pb.close()
 
 def test_pair_storage_rejects_what_it_cannot_do(gpu):

@@ -296,6 +296,43 @@ def test_degenerate_inputs_are_errors_not_faults(gpu):
         pb2.apply_batch(np.zeros((cnt, 17)), gpu.OP_STORED)  # at most 16 vectors
 
 
+def test_host_matrix_without_a_tube_refuses_point_operators(gpu):
+    """An operator uploaded from the host has no points: the matrix-free
+    operators, the mixed solvers' FP64 operator, right-hand-side generators and
+    the evaluation all need a tube, so they are errors before any launch --
+    on a fresh handle and after an earlier, shorter tube alike."""
+    rng = np.random.default_rng(2)
+    m = 600
+    A = np.eye(m) * 2 + 0.01 * (rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m)))
+    b = rng.standard_normal(m) + 0j
+    fresh = gpu.Problem()
+    h = 4.0 / 12
+    earlier = gpu.Problem().set_level(shapes.sphere(12)).geometry(5)
+    cnt = earlier.tube(gpu.params(k=2.0, eps=1.5 * h, tau=1.5 * h, dirichlet=1))
+    earlier.assemble(gpu.C128_PAIR)
+    assert 0 < cnt < m
+    for pb in (fresh, earlier):
+        pb.set_matrix(A.real.copy(), A.imag.copy(), gpu.C128)
+        pb.set_rhs(b.real.copy(), b.imag.copy())
+        for op in (gpu.OP_FREE64, gpu.OP_FREE32):
+            with pytest.raises(gpu.IbimError):
+                pb.apply(b, b, op)
+            with pytest.raises(gpu.IbimError):
+                pb.apply_batch(np.ones((m, 2)), op)
+        for method in (gpu.SOLVE_RELIABLE, gpu.SOLVE_REFINED):
+            with pytest.raises(gpu.IbimError):
+                pb.solve(method, maxit=50, tol=1e-10)
+        with pytest.raises(gpu.IbimError):
+            pb.rhs(gpu.RHS_POINT_SOURCE, x0=[0.0, 0.0, 0.0])
+        with pytest.raises(gpu.IbimError):
+            pb.evaluate(np.array([[2.0, 0.0, 0.0]]), gpu.EVAL_DOUBLE)
+        # BiCG on the stored operator itself still works
+        info, _ = pb.solve(gpu.SOLVE_BICG, maxit=200, tol=1e-12)
+        x = pb.density()
+        assert info.status == 0
+        assert relnorm(x[0] + 1j * x[1], np.linalg.solve(A, b)) < 1e-10
+
+
 # ------------------------------------------------------------------ drop-ins --
 def test_compbicg_dropin_matches_reference(gpu):
     b = golden("bicg.npz")

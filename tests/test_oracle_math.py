@@ -280,3 +280,91 @@ def test_combo_evaluator_is_double_minus_i_eta_single():
             dn = (phi(x, s[j] + e * nrm[j], k) - phi(x, s[j] - e * nrm[j], k)) / (2 * e)
             tot += (dn - 1j * eta * phi(x, s[j], k)) * ph[j] * F["w"][j]
         assert abs(ure[q] + 1j * uim[q] - tot) < 1e-7 * abs(tot)
+
+
+# ------------------------------------------- combined-field Dirichlet operator --
+def _combined_pair(n, k, eta):
+    """Oracle rows of the double-layer Dirichlet operator and of the combined
+    one on the unit sphere, with the tube fields."""
+    prm0, flat, cnt = _sphere(n, k, dirichlet=1)
+    prm1 = O.make_params(k=k, eps=prm0.eps, tau=prm0.tau, dirichlet=1, combined=1, eta=eta)
+    a0 = O.kernel_rows(flat, cnt, prm0)
+    a1 = O.kernel_rows(flat, cnt, prm1)
+    return prm0, prm1, flat, cnt, a0[0] + 1j * a0[1], a1[0] + 1j * a1[1]
+
+
+def test_combined_single_layer_of_constant_density_on_sphere():
+    # the coupling term is -i eta S: its rows applied to 1 give the single layer
+    # of a constant density on the unit sphere, S[1] = R sin(kR) e^{ikR} / k at
+    # |x| = R, here with the near field as the disc mean of Phi (without it
+    # the sums would miss ~tau / 2, 18% at 48^3)
+    k, eta, R = 2.0, 2.0, 1.0
+    exact = R * np.sin(k * R) * np.exp(1j * k * R) / k
+    errs = []
+    for n in (24, 48):
+        _, _, _, _, A0, A1 = _combined_pair(n, k, eta)
+        s1 = (1j * (A1 - A0) / eta).sum(axis=1)
+        errs.append(np.abs(s1 - exact).max() / abs(exact))
+    assert errs[1] < 0.015 and errs[1] < errs[0] / 1.8, errs
+
+
+def test_combined_near_field_is_the_disc_mean_of_phi():
+    # near entries (rho <= tau): -i eta w_j (1 / (pi tau^2)) int_disc Phi dA, the
+    # integral by quadrature in polar coordinates
+    k, eta = 5.0, 3.0
+    prm0, prm1, flat, cnt, A0, A1 = _combined_pair(24, k, eta)
+    F = O.tube_fields(flat, cnt)
+    tau = prm0.tau
+    g = lambda r: np.exp(1j * k * r) / 2  # Phi(r) 2 pi r
+    mean = (quad(lambda r: g(r).real, 0, tau)[0] + 1j * quad(lambda r: g(r).imag, 0, tau)[0]) / (np.pi * tau ** 2)
+    s = np.stack([F["sx"], F["sy"], F["sz"]], 1)
+    for i in (0, cnt // 3, cnt - 1):
+        near = np.linalg.norm(s - s[i], axis=1) <= tau
+        assert near.sum() > 10
+        np.testing.assert_allclose((A1 - A0)[i, near], -1j * eta * F["w"][near] * mean, rtol=1e-13)
+
+
+def test_combined_far_entries_are_double_layer_minus_i_eta_phi():
+    k, eta = 4.0, 4.0
+    prm0, prm1, flat, cnt, A0, A1 = _combined_pair(24, k, eta)
+    F = O.tube_fields(flat, cnt)
+    s = np.stack([F["sx"], F["sy"], F["sz"]], 1)
+    rng = np.random.default_rng(11)
+    for i in rng.integers(0, cnt, 20):
+        for j in rng.integers(0, cnt, 20):
+            if np.linalg.norm(s[i] - s[j]) <= prm0.tau:
+                continue
+            want = -1j * eta * F["w"][j] * phi(s[i], s[j], k)
+            assert abs((A1 - A0)[i, j] - want) <= 1e-13 * abs(want)
+
+
+def test_combined_field_solves_the_point_source_problem_well_conditioned():
+    # k = 2 lies near the unit sphere's interior Neumann eigenvalue 2.0816, where
+    # the double layer alone is ill-conditioned; the combined operator is not,
+    # and its potential (mode 3) reproduces the exterior point-source field
+    k, eta, x0 = 2.0, 2.0, [0.1, 0.2, 0.3]
+    rec = shapes.shell_receivers(40, 1.5, 3.0, seed=6)
+    exact = shapes.point_source(rec, x0, k)
+    errs = []
+    for n in (24, 48):
+        prm0, prm1, flat, cnt, A0, A1 = _combined_pair(n, k, eta)
+        bre, bim = O.rhs_point_source(flat, cnt, k, x0)
+        x = np.linalg.solve(A1, bre + 1j * bim)
+        ure, uim = O.evaluate(flat, cnt, prm1, x.real.copy(), x.imag.copy(), rec, 3)
+        errs.append(relnorm(ure + 1j * uim, exact))
+        if n == 24:
+            assert np.linalg.cond(A1) < 0.25 * np.linalg.cond(A0)
+    assert errs[1] < 0.005 and errs[1] < errs[0] / 2, errs
+
+
+def test_combined_evaluator_is_double_minus_i_eta_single():
+    k, eta = 3.0, 3.0
+    prm0, prm1, flat, cnt, _, _ = _combined_pair(16, k, eta)
+    rng = np.random.default_rng(12)
+    ph = rng.standard_normal(cnt) + 1j * rng.standard_normal(cnt)
+    rec = shapes.shell_receivers(6, 1.5, 2.5, seed=12)
+    u3 = O.evaluate(flat, cnt, prm1, ph.real.copy(), ph.imag.copy(), rec, 3)
+    u1 = O.evaluate(flat, cnt, prm1, ph.real.copy(), ph.imag.copy(), rec, 1)
+    u0 = O.evaluate(flat, cnt, prm1, ph.real.copy(), ph.imag.copy(), rec, 0)  # sums -Phi
+    want = (u1[0] + 1j * u1[1]) + 1j * eta * (u0[0] + 1j * u0[1])
+    assert relnorm(u3[0] + 1j * u3[1], want) < 1e-13
