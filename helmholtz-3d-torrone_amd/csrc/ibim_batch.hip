@@ -774,6 +774,29 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
     BState hs{};
     rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs, rep);
     if (rc) return rc;
+    if (rep) {
+        // columns that stopped without a verified convergence claim
+        // (stagnation, NaN, the iteration cap) report the true FP64 residual
+        // of their final iterate, not that of their last replacement
+        ConvMask m{};
+        bool any = false;
+        for (int r = 0; r < ncol; ++r) {
+            m.c[r] = hs.done[r] >= 0 && !rep->verified[r];
+            any = any || m.c[r];
+        }
+        if (any) {
+            rc = batch_apply(ctx, rep->true_op, false, a.x, a.v, nullptr);
+            if (rc) return rc;
+            hipLaunchKernelGGL(kb_replace, dim3(nvb), dim3(kBT), 0, ctx->stream, a.b, a.v, a.r, a.count, m, a.part);
+            IBIM_HIP(hipGetLastError());
+            std::vector<double> sums;
+            rc = column_sums(ctx, a.part, nvb, 2, sums);
+            if (rc) return rc;
+            rep->replacements++;
+            for (int r = 0; r < ncol; ++r)
+                if (m.c[r]) rep->true_err[r] = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / hs.normb[r];
+        }
+    }
     hipLaunchKernelGGL(kb_unpack, dim3((unsigned)(((size_t)count * kR + 255) / 256)), dim3(256), 0, ctx->stream, a.x,
                        count, c0, ncol, ctx->density.as<double2>());
     IBIM_HIP(hipGetLastError());

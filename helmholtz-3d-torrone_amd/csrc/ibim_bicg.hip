@@ -622,6 +622,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
         return IBIM_OK;
     }
     int it = 1, done = 0;
+    bool verified = false;  // the last convergence claim was confirmed by the true residual
     if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
     double next_replace = rep ? rep->factor : 0.0;
     // one iteration: the same launches every time (the kernels read the
@@ -690,7 +691,10 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
         double terr = 0.0;
         int rc = true_residual(ctx, a, nvb, rep, norms[0], &terr);
         if (rc) return rc;
-        if (claims && terr < tol) break;
+        if (claims && terr < tol) {
+            verified = true;
+            break;
+        }
         // A convergence claim the true residual refutes: restart from the
         // last real iteration with the true residual. By then the shadow
         // residual and the search directions have converged with the
@@ -705,6 +709,12 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
             done = 0;
         }
         next_replace = terr * rep->factor;
+    }
+    // a solve that stopped without a confirmed claim (stagnation, NaN, the
+    // iteration cap) reports the true residual of its final iterate
+    if (rep && !verified) {
+        double terr = 0.0;
+        if (int rc = true_residual(ctx, a, nvb, rep, norms[0], &terr)) return rc;
     }
     const int iters = done ? done : maxit;
     std::vector<BicgScalars> sc((size_t)iters + 1);

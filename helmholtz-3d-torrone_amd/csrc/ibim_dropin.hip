@@ -38,21 +38,6 @@ ibim_ctx* dropin_ctx() {
     return g_ctx;
 }
 
-// double*** accessors for the shared stencil math.
-struct PtrLine {
-    double*** u;
-    int i, j, k, axis;  // axis 0 = x (j), 1 = y (i), 2 = z (k)
-    double operator()(int m) const { return axis == 0 ? u[i][m][k] : (axis == 1 ? u[m][j][k] : u[i][j][m]); }
-};
-
-struct PtrPlane {
-    double*** u;
-    int i, j, k, pair;  // 0 = xy (i, j), 1 = xz (j, k), 2 = yz (i, k)
-    double operator()(int p, int q) const {
-        return pair == 0 ? u[i + p][j + q][k] : (pair == 1 ? u[i][j + p][k + q] : u[i + p][j][k + q]);
-    }
-};
-
 void gather(double*** g, int S, double* flat) {
     for (int i = 0; i < S; ++i)
         for (int j = 0; j < S; ++j) memcpy(flat + ((size_t)i * S + j) * S, g[i][j], sizeof(double) * S);
@@ -158,25 +143,7 @@ void report(const char* who, int rc) {
 
 }  // namespace
 
-// ----------------------------------------------------------------- weno.h ---
-void dropin_weno(double* dfp, double* dfn, double* fi, double dx, int order) { weno_1d(fi, dx, order, dfp, dfn); }
-
-void dropin_weno_axis(int axis, double*** u, int i, int j, int k, int n, double h, double* up, double* un, int order) {
-    const int c = axis == 0 ? j : (axis == 1 ? i : k);
-    weno_line(PtrLine{u, i, j, k, axis}, c, n, h, order, up, un);
-}
-
-// --------------------------------------------------- distance_application3D.h
-void dropin_curvatures(const double* d6, double* mean, double* gauss) { curvatures_from(d6, mean, gauss); }
-
-void dropin_second(double*** u, int i, int j, int k, int n, double h, double* o6) {
-    o6[0] = d2_line(PtrLine{u, i, j, k, 0}, j, n, h, 0);
-    o6[1] = d2_line(PtrLine{u, i, j, k, 1}, i, n, h, 1);
-    o6[2] = d2_line(PtrLine{u, i, j, k, 2}, k, n, h, 1);
-    o6[3] = dmixed(PtrPlane{u, i, j, k, 0}, i, j, n, h);
-    o6[4] = dmixed(PtrPlane{u, i, j, k, 1}, j, k, n, h);
-    o6[5] = dmixed(PtrPlane{u, i, j, k, 2}, i, k, n, h);
-}
+// The host scalar entries (weno, WENO_D*, *Curv) are in ibim_stencil_host.cpp.
 
 int dropin_assign_gradient(int n, double h, double*** dist, double*** gx, double*** gy, double*** gz, int mode) {
     ibim_ctx* ctx = dropin_ctx();

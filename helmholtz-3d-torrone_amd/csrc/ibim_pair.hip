@@ -929,6 +929,11 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
         if (w == 1 || w == 2 || w == 4) return w;
     }
     if (ctx->batch_waves[slot]) return ctx->batch_waves[slot];
+    // Operators of 2^17 points and more (the 256^3 surfaces: 0.3-0.5 s per
+    // pass) take 64 rows per workgroup without timing: that shape won at
+    // every size measured alone, and each timed shape costs a whole pass of
+    // the caller's solve.
+    if (ctx->count >= (1 << 17)) return ctx->batch_waves[slot] = 4;
     // no timing inside a stream capture (events would be captured): any shape
     // gives the same bits
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;

@@ -10,6 +10,10 @@ import numpy as np
 
 REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_PATH = os.path.join(REPO, "oracle", "liboracle.so")
+# IBIM_ORACLE_LIB: load another build of the same restatement (the sanitizer
+# build, oracle/liboracle_asan.so, with libasan preloaded)
+if os.environ.get("IBIM_ORACLE_LIB"):
+    LIB_PATH = os.environ["IBIM_ORACLE_LIB"]
 
 D = C.c_double
 I = C.c_int
