@@ -131,6 +131,10 @@ int ibim_create(ibim_ctx** out) {
     ibim_ctx* ctx = new (std::nothrow) ibim_ctx();
     if (!ctx) return IBIM_ERR_NOMEM;
     IBIM_HIP(hipGetDevice(&ctx->device));
+    // resolve the no-allocation entry point's scratch array on this device now
+    // (a failure resurfaces, with its message, from ibim_apply_noalloc)
+    void* scratch_ptr = nullptr;
+    (void)noalloc_scratch(ctx->device, &scratch_ptr);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         set_error("hipStreamCreate failed");
         delete ctx;

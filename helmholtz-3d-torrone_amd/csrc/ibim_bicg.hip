@@ -751,6 +751,64 @@ int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* 
     return IBIM_OK;
 }
 
+namespace {
+
+__global__ void k_clear_flag(int* flag) { *flag = 0; }
+
+// The layout of a single product's workspace carved from one buffer: partial
+// planes, per-block partials of the plane-summing kernel, its stop flag.
+struct ScratchLayout {
+    size_t planes, part, flag, total;
+};
+
+ScratchLayout scratch_layout(const Context* ctx, int op) {
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const size_t nct = (size_t)(count + kMvCols - 1) / kMvCols, nrt = (size_t)(count + kMvRows - 1) / kMvRows;
+    const size_t nplanes = pair_layout_op(ctx, op) ? 2 * (nct + nrt + 1) : nct + nrt;
+    ScratchLayout L;
+    L.planes = 0;
+    L.part = (sizeof(double2) * ld * nplanes + 255) / 256 * 256;
+    L.flag = L.part + (sizeof(double) * (size_t)vec_blocks(count) * 8 + 255) / 256 * 256;
+    L.total = L.flag + 256;
+    return L;
+}
+
+}  // namespace
+
+size_t matvec_scratch_bytes(const Context* ctx, int op) { return scratch_layout(ctx, op).total; }
+
+// matvec_op with its workspace in `scratch` (matvec_scratch_bytes(ctx, op)
+// bytes of device memory): no allocation, no memset (the flag is cleared by a
+// kernel), no timing events.
+int matvec_op_scratch(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev,
+                      void* scratch) {
+    const ScratchLayout L = scratch_layout(ctx, op);
+    char* base = (char*)scratch;
+    const int count = ctx->count;
+    VecArgs a{};
+    a.ld = ctx->ld;
+    a.ncol_tiles = (count + kMvCols - 1) / kMvCols;
+    a.nrow_tiles = (count + kMvRows - 1) / kMvRows;
+    a.ypart = (double2*)(base + L.planes);
+    a.part = (double*)(base + L.part);
+    a.done = (int*)(base + L.flag);
+    a.iter = nullptr;
+    a.count = count;
+    a = with_layout(a, ctx, op);
+    int rc = sweep_any(ctx, op, (const double2*)p_dev, (const double2*)q_dev, a.ypart, a.zpart, nullptr);
+    if (rc) return rc;
+    a.v = (double2*)y_dev;
+    a.w = (double2*)z_dev;
+    a.r = (double2*)p_dev;
+    a.rt = (double2*)q_dev;
+    a.pt = (double2*)q_dev;
+    hipLaunchKernelGGL(k_clear_flag, dim3(1), dim3(1), 0, ctx->stream, a.done);
+    hipLaunchKernelGGL(k_bicg_dot, dim3(vec_blocks(count)), dim3(kVecThreads), 0, ctx->stream, a);
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
 int matvec_dense(Context* ctx, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev) {
     return matvec_op(ctx, OP_DENSE, p_dev, q_dev, y_dev, z_dev);
 }

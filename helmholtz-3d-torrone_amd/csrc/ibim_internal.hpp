@@ -254,6 +254,15 @@ constexpr int kSweepCols = 1024;
 constexpr int kSweepRows = 512;
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done);
 int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
+// The same product with its workspace (partial planes, block partials, a flag)
+// in caller-provided device memory of matvec_scratch_bytes(ctx, op) bytes:
+// allocates nothing.
+size_t matvec_scratch_bytes(const Context* ctx, int op);
+int matvec_op_scratch(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev,
+                      void* scratch);
+// Device address of the no-allocation entry point's scratch array on `device`
+// (resolved once per device; ibim_noalloc.hip).
+int noalloc_scratch(int device, void** ptr);
 // ref_guards = false for correction / adjoint solves (only exact zeros are zero data).
 int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
                   ibim_solve_info* info, double* hist, bool adjoint = false, bool ref_guards = true);

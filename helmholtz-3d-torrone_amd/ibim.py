@@ -107,6 +107,7 @@ def lib():
             "ibim_set_matrix_host": ([vp, I, P, P, I], I), "ibim_regcount": ([vp], I),
             "ibim_apply": ([vp, I, P, P, P, P, P, P, P, P], I),
             "ibim_apply_batch": ([vp, I, I, I, P, P, P, P], I),
+            "ibim_apply_noalloc": ([vp, I, vp, vp, vp, vp], I),
             "ibim_set_nrhs": ([vp, I], I), "ibim_rhs": ([vp, I, P, D, P, I], I),
             "ibim_set_rhs_host": ([vp, I, P, P], I), "ibim_get_rhs": ([vp, I, P, P], I),
             "ibim_solve": ([vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), P], I),
@@ -260,6 +261,12 @@ class Problem:
         out = [np.zeros(self.count) for _ in range(4)]
         _check(lib().ibim_apply(self.h, op, *[_p(a) for a in arrs], *[_p(a) for a in out]), "ibim_apply")
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
+
+    def apply_noalloc(self, p_dev, q_dev, y_dev, z_dev, op=0):
+        """ibim_apply_noalloc: y = A p, z = A^H q on device buffers (addresses of
+        count complex128 values on the handle's device); allocates nothing."""
+        _check(lib().ibim_apply_noalloc(self.h, op, C.c_void_p(p_dev), C.c_void_p(q_dev), C.c_void_p(y_dev),
+                                        C.c_void_p(z_dev)), "ibim_apply_noalloc")
 
     def apply_batch(self, P, op=0, adjoint=False):
         """op(A) P for a (count, m <= 16) complex array, on the matrix cores."""

@@ -220,6 +220,16 @@ int ibim_apply(ibim_ctx *ctx, int op, const double *pre, const double *pim, cons
  * time). */
 int ibim_apply_batch(ibim_ctx *ctx, int op, int adjoint, int nvec, const double *pre, const double *pim, double *yre,
                      double *yim);
+/* ibim_apply on device vectors (count complex FP64 values each, interleaved
+ * re / im, on the handle's device) that allocates nothing: its partial planes
+ * and flags live in a 64 MiB device array per GPU that the library holds until
+ * the process ends, which bounds count at about 26,000 points (IBIM_ERR_SIZE
+ * beyond). The matrix-free ops need their tables built beforehand (one
+ * ibim_apply, or a pair assembly). Returns after the product is complete. All
+ * calls on one GPU share the array: calls that overlap on a GPU race. No
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (HIP graph capture, tight serving loops). */
+int ibim_apply_noalloc(ibim_ctx *ctx, int op, const void *p_dev, const void *q_dev, void *y_dev, void *z_dev);
 
 /* Progress of long solves: called from ibim_solve / ibim_gradient at host
  * checks, at most every 5 s, with the iteration and the largest running
