@@ -395,28 +395,27 @@ def run_adjoint(args):
 
 def run_star(args):
     """The 256^3 case: star r = 1 + 0.2 cos4t sin3p, k = 40 (--star-k), eps = tau
-    = 2h, 16 plane waves (Fibonacci), sound-soft data, tol 1e-6 (the relative
-    residual of the FP64 operator). Formulation: the combined-field operator
-    1/2 I + K - i eta S with eta = k (--star-formulation double: the
-    reference's double layer 1/2 I + K alone, singular near the interior
-    Neumann eigenvalues, which lie ~0.003 apart in k here; BiCG does not
-    converge on it at k = 40).
+    = 2h, 16 plane waves (Fibonacci), sound-soft data, BiCG to tol 1e-7 (the
+    relative residual; the mixed path's is that of the FP64 operator).
+    Formulation: the combined-field operator 1/2 I + K - i eta S with eta = k
+    (--star-formulation double: the reference's double layer 1/2 I + K alone,
+    singular near the interior Neumann eigenvalues, which lie ~0.003 apart in k
+    here; BiCG does not converge on it at k = 40).
     FP64 path: batched BiCG on the FP64 matrix-free operator, the fused pair
     pass computing each factor where complex128 pairs (which would need
     ~490 GB) would be read. Mixed path: batched BiCG on complex64 pair storage
-    (~246 GB, assembled inside the timed region) when it fits, else on the
-    mixed-precision matrix-free operator, with FP64 residual replacement by the
-    FP64 matrix-free operator. Each pass applies the operator to all 16
-    right-hand sides on the matrix cores. Reports solves/s for both paths when
-    every column converged (else batched iterations/s and the residuals
-    reached) and the mixed densities' largest relative difference from the
-    FP64 ones."""
+    (~246 GB, assembled inside the timed region into memory allocated before
+    it) with its 16-column passes on the FP32 matrix cores
+    (--star-fp64-products: on the FP64 ones), FP64 residual replacement by the
+    FP64 matrix-free operator. Reports solves/s for both paths when every
+    column converged (else batched iterations/s and the residuals reached) and
+    the mixed densities' largest relative difference from the FP64 ones."""
     n, k, m, its = 256, args.star_k, 16, args.star_iterations
     combined = args.star_formulation == "combined"
     h = 4.0 / n
     dirs = fibonacci_directions(m)
     out, dens = {}, {}
-    for path in ("mixed", "fp64"):
+    for path in ("mixed",) if args.star_mixed_only else ("mixed", "fp64"):
         pb = ibim.Problem()
         progress(pb)
         pb.set_level_shape(ibim.shape(ibim.SHAPE_STAR, a=1.0, b=0.2, m=4, n=3, variant=1), n)
@@ -429,26 +428,36 @@ def run_star(args):
             pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
         pb.sync()
         t_setup = time.perf_counter() - t0
+        store = "matrix-free"
+        if path == "mixed" and pair_bytes(count, 8) <= STAR_STORE_MAX_BYTES:
+            # the 246 GB allocation is made once, untimed (a fresh allocation of
+            # that size right after another process freed one took seconds);
+            # the timed region assembles again into it
+            try:
+                pb.assemble(ibim.C64_PAIR)
+                store = "complex64 pairs (%.0f GB)" % (pair_bytes(count, 8) / 1e9)
+            except ibim.IbimError as e:
+                print("[bench] star256: complex64 pairs not stored (%s); matrix-free" % e, file=sys.stderr,
+                      flush=True)
+        pb.sync()
         pb.reset_timers()
         pb.set_profiling(True)
         t0 = time.perf_counter()
         if path == "mixed":
-            store = "matrix-free"
-            if pair_bytes(count, 8) <= STAR_STORE_MAX_BYTES:
-                try:
-                    pb.assemble(ibim.C64_PAIR)
-                    store = "complex64 pairs (%.0f GB)" % (pair_bytes(count, 8) / 1e9)
-                except ibim.IbimError as e:
-                    print("[bench] star256: complex64 pairs not stored (%s); matrix-free" % e, file=sys.stderr,
-                          flush=True)
+            if store != "matrix-free":
+                pb.assemble(ibim.C64_PAIR)
             out["mixed_operator"] = store
-            info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=STAR_TOL, inner_tol=BATCH_REPLACE, batch=0)
+            if not args.star_fp64_products:
+                pb.set_fp32_products(True)
+                out["mixed_operator"] += ", FP32 matrix-core products"
+            info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=STAR_TOL, inner_tol=args.star_replace, batch=0)
         else:
             info, _ = pb.solve(ibim.SOLVE_BICG, maxit=its, tol=STAR_TOL, batch=0)
         pb.sync()
         el = time.perf_counter() - t0
         pb.set_profiling(False)
-        kern = {"mixed": "batch_pair" if out["mixed_operator"] != "matrix-free" else "batch_free32",
+        kern = {"mixed": ("batch_pair" if args.star_fp64_products else "batch_pair32")
+                         if not out["mixed_operator"].startswith("matrix-free") else "batch_free32",
                 "fp64": "batch_free64"}[path]
         ms, nl = pb.kernel_time(kern)
         ms_asm, _ = pb.kernel_time("assemble")
@@ -462,6 +471,12 @@ def run_star(args):
                                          info.status, nl, ms / max(nl, 1)), file=sys.stderr, flush=True)
         pb.close()
     mixed_operator = out.pop("mixed_operator")
+    if args.star_mixed_only:
+        r = out["mixed"]
+        return {"workload": "star256 mixed path only (k=%g)" % k, "mixed_operator": mixed_operator,
+                "seconds": r["seconds"], "iterations_summed_over_columns": r["info"].iterations,
+                "residual": r["info"].rel_residual, "status": r["info"].status, "pass_ms": r["pass_ms"],
+                "passes": r["passes"], "replacement_passes": r["replacement_passes"]}
     converged = {p: out[p]["info"].status == 0 for p in out}
     rel_err = float(max(np.linalg.norm(dens["mixed"][c] - dens["fp64"][c]) / np.linalg.norm(dens["fp64"][c])
                         for c in range(m)))
@@ -481,8 +496,9 @@ def run_star(args):
     pass_flops = 2.0 * 2 * npairs * 16 * 8
     return ({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "mixed: complex64 pair storage (else fp32 sine / cosine of an fp64 phase), fp64 matrix-core "
-                   "products, fp64 residual replacement; fp64: fp64 matrix-free factors, fp64 matrix-core products",
+          "dtype": "mixed: complex64 pair storage (else fp32 sine / cosine of an fp64 phase), entries and products "
+                   "on the fp32 matrix cores summed in fp64 per 64 columns, fp64 residual replacement; fp64: fp64 "
+                   "matrix-free factors, fp64 matrix-core products",
           "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
           "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, tol 1e-7, "
                                  "%s, %d-iteration budget" % (
@@ -550,6 +566,12 @@ def main():
     ap.add_argument("--star-formulation", choices=("combined", "double"), default="combined",
                     help="star256: combined-field operator (eta = k) or the double layer alone")
     ap.add_argument("--star-k", type=float, default=40.0, help="wavenumber of the star256 workload")
+    ap.add_argument("--star-mixed-only", action="store_true",
+                    help="star256: time the mixed path alone (no FP64 path, no comparison)")
+    ap.add_argument("--star-fp64-products", action="store_true",
+                    help="star256 mixed path: 16-column passes on the FP64 matrix cores (default: FP32)")
+    ap.add_argument("--star-replace", type=float, default=BATCH_REPLACE,
+                    help="star256 mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     ap.add_argument("--matrix-free", action="store_true",

@@ -575,6 +575,12 @@ int ibim_set_profiling(ibim_ctx* ctx, int on) {
     return IBIM_OK;
 }
 
+int ibim_set_fp32_products(ibim_ctx* ctx, int on) {
+    CHECK(ctx);
+    ctx->fp32_products = on != 0;
+    return IBIM_OK;
+}
+
 int ibim_set_graphs(ibim_ctx* ctx, int on) {
     CHECK(ctx);
     ctx->use_graphs = on != 0;

@@ -632,7 +632,12 @@ struct BReplace {
     double next[kR];       // replace when the recursive residual falls below
     double true_err[kR];   // last true residual (-1 before the first)
     bool verified[kR];     // a convergence claim confirmed by the true residual
+    int stalls[kR];        // stagnation stops checked and restarted so far
 };
+// Stagnation stops of a column that are checked against the true residual
+// and restarted when it misses tol (the recursive residual of the FP32
+// operator can stall above the FP64 one's level).
+constexpr int kMaxStallRestarts = 3;
 
 // BiCG for the 16 interleaved right-hand sides in a.b (padding rows zero);
 // the solutions are left in a.x and the final state in *hs. With `rep`, FP64
@@ -651,6 +656,7 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
             rep->next[r] = rep->factor;
             rep->true_err[r] = -1.0;
             rep->verified[r] = false;
+            rep->stalls[r] = 0;
         }
     std::vector<double> sums;
     // one iteration: the same launches every time (the kernels read the
@@ -714,10 +720,22 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
             bool any = false;
             for (int r = 0; r < kR; ++r) {
                 const bool claims = hs->done[r] > 0 && hs->stop[r] == 1 && !rep->verified[r];
+                const bool stalled = hs->done[r] > 0 && hs->stop[r] == 3 && !rep->verified[r] &&
+                                     rep->stalls[r] < kMaxStallRestarts;
                 const bool drop = hs->done[r] == 0 && hs->err[r] < rep->next[r];
-                m.c[r] = claims || drop;
+                if (stalled) rep->stalls[r]++;
+                m.c[r] = claims || stalled || drop;
                 any = any || m.c[r];
             }
+            // the pass computes every column: running columns that have come
+            // at least half a replacement interval (sqrt(factor)) down since
+            // their last replacement are replaced with the others, so the
+            // columns stay in step and share the passes
+            if (any)
+                for (int r = 0; r < kR; ++r) {
+                    const double base = rep->true_err[r] >= 0.0 ? rep->true_err[r] : 1.0;
+                    if (hs->done[r] == 0 && hs->err[r] < base * sqrt(rep->factor)) m.c[r] = 1;
+                }
             if (any) {
                 // a.v is free between iterations: the true products of all columns
                 int rc = batch_apply(ctx, rep->true_op, false, a.x, a.v, nullptr);
@@ -735,7 +753,7 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
                     const double terr = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / hs->normb[r];
                     rep->true_err[r] = terr;
                     rep->next[r] = terr * rep->factor;
-                    if (hs->done[r] > 0) {  // a convergence claim
+                    if (hs->done[r] > 0) {  // a convergence claim or a checked stagnation stop
                         if (terr < tol) {
                             rep->verified[r] = true;
                         } else {
@@ -810,7 +828,7 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
         const int d = hs.done[r];
         iters[c0 + r] = d > 0 ? d : (d < 0 ? 0 : maxit);
         errs[c0 + r] = d < 0 ? 0.0 : (rep && rep->true_err[r] >= 0.0 ? rep->true_err[r] : hs.err[r]);
-        stops[c0 + r] = d == 0 ? 1 : (hs.stop[r] == 1 ? 0 : hs.stop[r]);
+        stops[c0 + r] = rep && rep->verified[r] ? 0 : d == 0 ? 1 : (hs.stop[r] == 1 ? 0 : hs.stop[r]);
         if (hist && c0 + r == 0)
             for (int i = 1; i <= iters[0]; ++i) hist[i - 1] = h[(size_t)i * kR];
     }

@@ -561,6 +561,7 @@ struct Replace {
     int replacements;
     double true_err;
 };
+constexpr int kMaxStallRestarts = 3;
 
 // r = b - A_true x into the BiCG workspace; returns ||r|| / ||b||.
 int true_residual(Context* ctx, const VecArgs& base, unsigned nvb, Replace* rep, double normb, double* err) {
@@ -623,6 +624,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     }
     int it = 1, done = 0;
     bool verified = false;  // the last convergence claim was confirmed by the true residual
+    int stalls = 0;         // stagnation stops checked against the true residual
     if (chunk <= 0) chunk = ctx->count > 20000 ? 2 : 16;
     double next_replace = rep ? rep->factor : 0.0;
     // one iteration: the same launches every time (the kernels read the
@@ -685,8 +687,13 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
             IBIM_HIP(hipMemcpyAsync(&cur, a.sc + done, sizeof cur, hipMemcpyDeviceToHost, ctx->stream));
             IBIM_HIP(hipStreamSynchronize(ctx->stream));
         }
-        const bool claims = done && cur.stop == 1;
-        if (done && !claims) break;  // NaN or stagnation: no replacement helps
+        // a stagnation stop is checked like a claim (the FP32 operator's
+        // recursive residual can stall above the FP64 level) up to
+        // kMaxStallRestarts times; NaN ends the solve
+        const bool stalled = done && cur.stop == 3 && stalls < kMaxStallRestarts;
+        if (stalled) ++stalls;
+        const bool claims = done && (cur.stop == 1 || stalled);
+        if (done && !claims) break;
         if (!claims && !(cur.err < next_replace)) continue;
         double terr = 0.0;
         int rc = true_residual(ctx, a, nvb, rep, norms[0], &terr);
@@ -727,6 +734,7 @@ int bicg_device(Context* ctx, int op, Workspace& ws, const double2* b, double2* 
     info->iterations = iters;
     info->rel_residual = rep && rep->replacements ? rep->true_err : sc[iters].err;
     info->status = done ? (sc[iters].stop == 1 ? 0 : sc[iters].stop) : 1;
+    if (verified) info->status = 0;  // the true residual met tol (also after a checked stagnation stop)
     return IBIM_OK;
 }
 

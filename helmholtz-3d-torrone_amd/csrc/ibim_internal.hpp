@@ -95,6 +95,10 @@ struct Context {
     // mixed) and product kind, 0 until timed; reset with every assembly
     int batch_waves[15] = {};
 
+    // 16-column passes over complex64 pairs multiply on the FP32 matrix cores
+    // (ibim_set_fp32_products)
+    bool fp32_products = false;
+
     // right-hand sides and densities: nrhs columns of length count (complex FP64)
     int nrhs = 0;
     DevBuf rhs, density;

@@ -1008,6 +1008,186 @@ void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* 
         launch_pair_batch<MT, Kern, false, true, Factors>(ctx, slot0 + 2, F, P, Q, Y, Z, stop);
 }
 
+// ------------------------------------------- the same pass on FP32 matrix cores --
+// Complex64 pair storage only (the mixed solvers' inner operator): the walk of
+// k_pair_batch, with the entries formed in FP32 -- the separation r = x_i - x_j
+// in FP64, rounded once; alpha, beta, rho^2, 1 / rho (the FP32 reciprocal square
+// root) and the combined-field entries in FP32 -- and multiplied with FP32 copies
+// of P and Q on v_mfma_f32_16x16x4f32 (twice the FP64 rate), the products summed
+// in FP32 over each staged chunk of kBatchK columns and the chunk sums in FP64.
+// Entry errors stay at the level of the complex64 factors (relative to the
+// largest entries); the products add FP32 rounding that depends on the vector,
+// which the mixed solvers' FP64 residuals absorb.
+typedef float f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// Gauss's three products in FP32, as Acc3 does in FP64. D = A B + C for a
+// 16x4 A, 4x16 B: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
+// D[4 (l / 16) + v][l % 16] in component v.
+struct Acc3f {
+    f4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
+    __device__ __forceinline__ void add(float ar, float ai, float br, float bi) {
+        k1 = mfma32(ar + ai, br, k1);
+        k2 = mfma32(ar, bi - br, k2);
+        k3 = mfma32(ai, br + bi, k3);
+    }
+    // chunk sums into the FP64 accumulators, then restart from zero
+    __device__ __forceinline__ void flush(Acc3& acc) {
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            acc.k1[v] += (double)k1[v];
+            acc.k2[v] += (double)k2[v];
+            acc.k3[v] += (double)k3[v];
+        }
+        k1 = f4{0, 0, 0, 0};
+        k2 = f4{0, 0, 0, 0};
+        k3 = f4{0, 0, 0, 0};
+    }
+};
+
+// Point data of a staged column in FP32: m (= w n for Dirichlet, n for
+// Neumann) and w; the position stays FP64 for the separation.
+struct alignas(16) PairPointF {
+    float mx, my, mz, w;
+};
+
+template <int Kern, bool WithY, bool WithZ, int Waves>
+__global__ void __launch_bounds__(64 * Waves) k_pair_batch32(const float2* __restrict__ F,
+                                                             const int* __restrict__ blk_start,
+                                                             const PairPoint* __restrict__ pts, int count,
+                                                             ibim_params prm, const double2* __restrict__ P,
+                                                             const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                             double2* __restrict__ Z, const int* __restrict__ stop) {
+    constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
+    constexpr int R = kBatchRHS;
+    constexpr int kBatchRows = 16 * Waves, kThreads = 64 * Waves;
+    if (stop != nullptr && *stop) return;
+    __shared__ float2 ps[WithY ? kBatchK : 1][R];
+    __shared__ float2 qs[WithZ ? kBatchK : 1][R];
+    __shared__ double3 sx[kBatchK];
+    __shared__ PairPointF sf[kBatchK];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const int rowW = blockIdx.x * kBatchRows;  // < count
+    const int I = rowW / kPR;
+    const int i0 = rowW + 16 * w;
+    const int i = i0 + li;  // < count + kPC: inside the padded point table
+    const PairPoint Ri = pts[i];
+    const float rmx = (float)Ri.mx, rmy = (float)Ri.my, rmz = (float)Ri.mz, rw = (float)Ri.w;
+    const float k = (float)prm.k, eta = (float)prm.eta;
+    Acc3 y, z;
+    Acc3f y32, z32;
+
+    auto stage = [&](int j0) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < kBatchK * R; e += kThreads) {
+            const int j = j0 + e / R;
+            if (WithY) {
+                const double2 v = j < count ? P[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
+                ps[e / R][e % R] = make_float2((float)v.x, (float)v.y);
+            }
+            if (WithZ) {
+                const double2 v = j < count ? Q[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
+                qs[e / R][e % R] = make_float2((float)v.x, (float)v.y);
+            }
+        }
+        if (threadIdx.x < kBatchK) {
+            const PairPoint C = pts[j0 + threadIdx.x];  // j0 + 63 < count + kPC
+            sx[threadIdx.x] = make_double3(C.x, C.y, C.z);
+            sf[threadIdx.x] = PairPointF{(float)C.mx, (float)C.my, (float)C.mz, (float)C.w};
+        }
+        __syncthreads();
+    };
+    auto apply = [&](const float2 f, int jj) {
+        const double3 cx = sx[jj];
+        const PairPointF C = sf[jj];
+        const float d1 = (float)(Ri.x - cx.x), d2 = (float)(Ri.y - cx.y), d3 = (float)(Ri.z - cx.z);
+        float al, be;  // alpha_ij, beta_ij from the separation
+        if (Dirichlet) {
+            al = fmaf(d1, C.mx, fmaf(d2, C.my, d3 * C.mz));
+            be = -fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
+        } else {
+            al = C.w * fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
+            be = rw * -fmaf(d1, C.mx, fmaf(d2, C.my, d3 * C.mz));
+        }
+        float ar, ai, br, bi;
+        if (Combined) {
+            const float r2 = fmaxf(fmaf(d1, d1, fmaf(d2, d2, d3 * d3)), 1e-30f);
+            const float u = __builtin_amdgcn_rsqf(r2);
+            const float u2 = u * u, ku = k * u;
+            const float xr = -u2 * al, xi = fmaf(ku, al, -eta * C.w);
+            const float yr = -u2 * be, yi = fmaf(ku, be, -eta * rw);
+            ar = fmaf(f.x, xr, -f.y * xi);
+            ai = fmaf(f.x, xi, f.y * xr);
+            br = fmaf(f.x, yr, -f.y * yi);
+            bi = fmaf(f.x, yi, f.y * yr);
+        } else {
+            ar = f.x * al;
+            ai = f.y * al;
+            br = f.x * be;
+            bi = f.y * be;
+        }
+        if (WithY) {
+            const float2 p = ps[jj][li];
+            y32.add(ar, ai, p.x, p.y);  // A_ij p_j
+        }
+        if (WithZ) {
+            const float2 q = qs[jj][li];
+            z32.add(br, -bi, q.x, q.y);  // conj(A_ji) q_j
+        }
+    };
+    auto flush = [&]() {
+        if (WithY) y32.flush(y);
+        if (WithZ) z32.flush(z);
+    };
+
+    // j < rows: the column strips of tiles (J, t_J), J <= I, as k_pair_batch
+    for (int J = 0; J <= I; ++J) {
+        const int cJ = rowW - J * kPR;
+        const int t = cJ / kPC;
+        const float2* T = F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
+        for (int rj0 = 0; rj0 < kPR; rj0 += kBatchK) {
+            stage(J * kPR + rj0);
+#pragma unroll 8
+            for (int s = 0; s < kBatchK / 4; ++s) {
+                const int jj = 4 * s + lk;
+                apply(T[(size_t)(rj0 + jj) * kPC], jj);
+            }
+            flush();
+        }
+    }
+    // j >= rows: row i of the row block's tiles
+    const int nt = blk_start[I + 1] - blk_start[I];
+    const float2* Trow = F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
+    for (int t = 0; t < nt; ++t) {
+        const int c0 = I * kPR + t * kPC;
+        const float2* Tr = Trow + (size_t)t * kPR * kPC;
+        for (int cb = 0; cb < kPC; cb += kBatchK) {
+            stage(c0 + cb);
+#pragma unroll 8
+            for (int s = 0; s < kBatchK / 4; ++s) {
+                const int jj = 4 * s + lk;
+                apply(Tr[cb + jj], jj);
+            }
+            flush();
+        }
+    }
+    // the FP32 instruction's D layout: lane (li, lk) holds rows 4 lk + v of
+    // column li (the FP64 one holds rows 4 v + lk); the FP64 chunk sums
+    // follow it
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int r = i0 + 4 * lk + v;
+        if (r < count) {
+            if (WithY) Y[(size_t)r * R + li] = make_double2(y.re(v), y.im(v));
+            if (WithZ) Z[(size_t)r * R + li] = make_double2(z.re(v), z.im(v));
+        }
+    }
+}
+
 // Calls fn(std::integral_constant<int, K>) with the context's pair kernel K.
 template <class Fn>
 void with_kernel(const Context* ctx, Fn&& fn) {
@@ -1028,6 +1208,35 @@ void with_kernel(const Context* ctx, Fn&& fn) {
 void near_term(const Context* ctx, double* nr, double* ni) {
     *nr = *ni = 0.0;
     if (pair_kernel(ctx) == kCombined) combined_near(ctx->prm.k, ctx->prm.tau, ctx->prm.eta, nr, ni);
+}
+
+// The FP32 matrix-core pass over complex64 pairs (64 rows per workgroup) and
+// the FP64 near field after it.
+template <int Kern, bool WithY, bool WithZ>
+void launch_pair_batch32(Context* ctx, const float2* F, const double2* P, const double2* Q, double2* Y, double2* Z,
+                         const int* stop) {
+    const int count = ctx->count;
+    TimedScope ts(ctx, "batch_pair32");
+    hipLaunchKernelGGL((k_pair_batch32<Kern, WithY, WithZ, 4>), dim3((unsigned)((count + 63) / 64)), dim3(256), 0,
+                       ctx->stream, F, ctx->pair_blk.as<int>(), ctx->pair_pts.as<PairPoint>(), count, ctx->prm, P, Q,
+                       Y, Z, stop);
+    double nr = 0.0, ni = 0.0;
+    if (Kern == kCombined) combined_near(ctx->prm.k, ctx->prm.tau, ctx->prm.eta, &nr, &ni);
+    hipLaunchKernelGGL((k_pair_near_batch<Kern, WithY, WithZ>),
+                       dim3((unsigned)(((size_t)count * kBatchRHS + 255) / 256)), dim3(256), 0, ctx->stream,
+                       ctx->tube.as<double>(), count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(), P, Q, Y, Z,
+                       stop, nr, ni);
+}
+
+template <int Kern>
+void launch_pair_batch32_any(Context* ctx, const float2* F, const double2* P, const double2* Q, double2* Y,
+                             double2* Z, const int* stop) {
+    if (Y && Z)
+        launch_pair_batch32<Kern, true, true>(ctx, F, P, Q, Y, Z, stop);
+    else if (Y)
+        launch_pair_batch32<Kern, true, false>(ctx, F, P, Q, Y, Z, stop);
+    else
+        launch_pair_batch32<Kern, false, true>(ctx, F, P, Q, Y, Z, stop);
 }
 
 int near_lists(Context* ctx, dim3 grid);
@@ -1255,6 +1464,8 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
         if (fp64_copy || ctx->mat_precision == IBIM_C128)
             launch_pair_batch_any<double2, Kern>(ctx, fp64_copy ? 6 : 3, (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(),
                                                  P, Q, Y, Z, stop);
+        else if (ctx->fp32_products)
+            launch_pair_batch32_any<Kern>(ctx, ctx->mat.as<float2>(), P, Q, Y, Z, stop);
         else
             launch_pair_batch_any<float2, Kern>(ctx, 0, ctx->mat.as<float2>(), P, Q, Y, Z, stop);
     });

@@ -117,6 +117,7 @@ def lib():
             "ibim_gradient": ([vp, I, I, C.POINTER(SolveOpts), P, P, P], I),
             "ibim_set_profiling": ([vp, I], I),
             "ibim_set_graphs": ([vp, I], I),
+            "ibim_set_fp32_products": ([vp, I], I),
             "ibim_set_progress": ([vp, PROGRESS_FN, vp], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
             "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
@@ -362,6 +363,10 @@ class Problem:
 
     def set_profiling(self, on=True):
         _check(lib().ibim_set_profiling(self.h, 1 if on else 0), "ibim_set_profiling")
+
+    def set_fp32_products(self, on=True):
+        """16-column passes over complex64 pairs on the FP32 matrix cores."""
+        _check(lib().ibim_set_fp32_products(self.h, 1 if on else 0), "ibim_set_fp32_products")
 
     def set_graphs(self, on=True):
         """Replay one captured chunk of BiCG iterations as a HIP graph."""

@@ -276,6 +276,15 @@ int ibim_set_profiling(ibim_ctx *ctx, int on);
  * the launches one by one. Ignored while profiling is on. No counterpart in
  * the reference (launch overhead). */
 int ibim_set_graphs(ibim_ctx *ctx, int on);
+/* When on (default off), the 16-column passes over complex64 pair storage
+ * (IBIM_C64_PAIR, IBIM_MIXED_PAIR: the mixed solvers' inner operator, and
+ * ibim_apply_batch on it) multiply on the FP32 matrix cores: entries formed in
+ * FP32 from the stored factors (the separation of the points in FP64), FP32
+ * copies of the vectors, products summed in FP32 over 64 columns and in FP64
+ * beyond. The operator then depends on the vector at the FP32 rounding level,
+ * which the mixed solvers' FP64 residuals correct. No counterpart in the
+ * reference. */
+int ibim_set_fp32_products(ibim_ctx *ctx, int on);
 /* Total milliseconds and launch count of a kernel class ("matvec", "assemble", ...). */
 int ibim_kernel_time(ibim_ctx *ctx, const char *name, double *ms_total, long *launches);
 int ibim_reset_timers(ibim_ctx *ctx);

@@ -221,3 +221,41 @@ def test_combined_needs_the_exterior_dirichlet_problem(gpu):
             pb.tube(gpu.params(3.0, 1.5 * h, 1.5 * h, combined=1, eta=3.0, **kw))
     with pytest.raises(gpu.IbimError):
         pb.tube(gpu.params(3.0, 1.5 * h, 1.5 * h, combined=1, eta=3.0, dirichlet=1), gpu.TUBE_COMBO)
+
+
+@pytest.mark.parametrize("combined", [0, 1])
+def test_fp32_matrix_core_pass_is_the_complex64_operator_to_fp32_rounding(gpu, combined):
+    # ibim_set_fp32_products: the 16-column pass over complex64 pairs on the FP32
+    # matrix cores against the same pass with FP64 products (both products,
+    # each alone), and the mixed reliable solver on it against the FP64 solution
+    n, k = 28, 6.0
+    h = 4.0 / n
+    pb = gpu.Problem()
+    pb.set_level_shape(gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.2, m=4, n=3, variant=1), n)
+    pb.geometry(5)
+    pb.tube(gpu.params(k, 2 * h, 2 * h, dirichlet=1, combined=combined, eta=k))
+    pb.assemble(gpu.C64_PAIR)
+    rng = np.random.default_rng(31 + combined)
+    P = rng.standard_normal((pb.count, 16)) + 1j * rng.standard_normal((pb.count, 16))
+    Y64 = pb.apply_batch(P, gpu.OP_STORED)
+    Z64 = pb.apply_batch(P, gpu.OP_STORED, adjoint=True)
+    pb.set_fp32_products(True)
+    Y32 = pb.apply_batch(P, gpu.OP_STORED)
+    Z32 = pb.apply_batch(P, gpu.OP_STORED, adjoint=True)
+    assert relnorm(Y32, Y64) <= 3e-6 and relnorm(Z32, Z64) <= 3e-6, (relnorm(Y32, Y64), relnorm(Z32, Z64))
+    assert relnorm(Y32, Y64) > 0  # really a different product
+    m = 16
+    pb.set_nrhs(m)
+    dirs = rng.standard_normal((m, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    for c in range(m):
+        pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+    info32, _ = pb.solve(gpu.SOLVE_RELIABLE, maxit=600, tol=1e-9, inner_tol=1e-3, batch=0)
+    x32 = [pb.density(c) for c in range(m)]
+    pb.set_fp32_products(False)
+    pb.assemble(gpu.C128_PAIR)
+    info64, _ = pb.solve(gpu.SOLVE_BICG, maxit=600, tol=1e-11, batch=0)
+    assert info32.status == 0 and info64.status == 0 and info32.rel_residual < 1e-9
+    for c in range(m):
+        x64 = pb.density(c)
+        assert relnorm(x32[c][0] + 1j * x32[c][1], x64[0] + 1j * x64[1]) < 1e-7
