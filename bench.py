@@ -329,15 +329,18 @@ def run_adjoint(args):
     pb = ibim.Problem()
     progress(pb)
 
-    def prepare(level, kk):
+    def prepare(level, kk, storage=ibim.C128_PAIR):
         pb.set_level(level)
         pb.geometry(5)
         pb.tube(ibim.params(kk, 2 * h, 2 * h, dirichlet=1))
-        pb.assemble(ibim.C128_PAIR)
+        pb.assemble(storage)
         pb.rhs(ibim.RHS_POINT_SOURCE, x0=x0)
 
     def field(level, kk, tol_refined=True):
-        prepare(level, kk)
+        # the finite-difference fields: FP64 refinement to the FP64 floor around
+        # BiCG on the complex64 copy of the pairs, residuals on their complex128
+        # copy (the FP64 path's operator, so the same J)
+        prepare(level, kk, ibim.MIXED_PAIR)
         if tol_refined:
             pb.solve(ibim.SOLVE_REFINED, tol=1e-15, inner_tol=1e-5, max_outer=10)
         re, im = pb.evaluate(recv, ibim.EVAL_DOUBLE)

@@ -61,8 +61,12 @@ class Pipeline:
         pb.rhs(ib.RHS_POINT_SOURCE, x0=X0)
 
     def field(self, level, k):
-        """u at the receivers with a solve accurate to the FP64 floor."""
+        """u at the receivers with a solve accurate to the FP64 floor (for pair
+        storage: refinement around the complex64 copy of the same pairs, the
+        residuals on the complex128 copy, i.e. on the operator of the gradient)."""
         self.prepare(level, k)
+        if self.storage == self.ib.C128_PAIR:
+            self.pb.assemble(self.ib.MIXED_PAIR)
         info, _ = self.pb.solve(method=self.ib.SOLVE_REFINED, tol=1e-15, inner_tol=1e-5, max_outer=10)
         assert info.rel_residual < 1e-13, info.rel_residual
         re, im = self.pb.evaluate(self.recv, self.mode)
