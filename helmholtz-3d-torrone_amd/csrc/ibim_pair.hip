@@ -1019,171 +1019,262 @@ void launch_pair_batch_any(Context* ctx, int slot0, const MT* F, const double2* 
 // largest entries); the products add FP32 rounding that depends on the vector,
 // which the mixed solvers' FP64 residuals absorb.
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float fv2 __attribute__((ext_vector_type(2)));
 
 __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
 }
 
-// Gauss's three products in FP32, as Acc3 does in FP64. D = A B + C for a
-// 16x4 A, 4x16 B: lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and
-// D[4 (l / 16) + v][l % 16] in component v.
-struct Acc3f {
-    f4 k1 = {0, 0, 0, 0}, k2 = {0, 0, 0, 0}, k3 = {0, 0, 0, 0};
-    __device__ __forceinline__ void add(float ar, float ai, float br, float bi) {
-        k1 = mfma32(ar + ai, br, k1);
-        k2 = mfma32(ar, bi - br, k2);
-        k3 = mfma32(ai, br + bi, k3);
-    }
-    // chunk sums into the FP64 accumulators, then restart from zero
-    __device__ __forceinline__ void flush(Acc3& acc) {
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            acc.k1[v] += (double)k1[v];
-            acc.k2[v] += (double)k2[v];
-            acc.k3[v] += (double)k3[v];
-        }
-        k1 = f4{0, 0, 0, 0};
-        k2 = f4{0, 0, 0, 0};
-        k3 = f4{0, 0, 0, 0};
-    }
-};
+// D = A B + C for a 16x4 A, 4x16 B: lane l holds A[l % 16][l / 16],
+// B[l / 16][l % 16] and D[4 (l / 16) + v][l % 16] in component v. Complex
+// products take Gauss's three real products, as Acc3 does in FP64.
+// Raw buffer loads (vector offset + scalar offset + immediate) from a
+// wave-uniform base: one address register per lane for a whole chunk, and
+// reads at or past `bytes` return zero.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ fv2 ld8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(fv2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ f4 ld16(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+}
 
-// Point data of a staged column in FP32: m (= w n for Dirichlet, n for
-// Neumann) and w; the position stays FP64 for the separation.
-struct alignas(16) PairPointF {
-    float mx, my, mz, w;
-};
-
-template <int Kern, bool WithY, bool WithZ, int Waves>
-__global__ void __launch_bounds__(64 * Waves) k_pair_batch32(const float2* __restrict__ F,
-                                                             const int* __restrict__ blk_start,
-                                                             const PairPoint* __restrict__ pts, int count,
-                                                             ibim_params prm, const double2* __restrict__ P,
-                                                             const double2* __restrict__ Q, double2* __restrict__ Y,
-                                                             double2* __restrict__ Z, const int* __restrict__ stop) {
+// The walk of k_pair_batch in chunks of kBatchK columns, software-pipelined:
+// while a chunk's 16 k-steps run on the matrix cores, the next chunk's pair
+// values are in flight into registers and its P, Q and column points into
+// registers for the other half of a double-buffered LDS stage (one barrier per
+// chunk); inside a chunk each k-step's LDS operands are read one k-step ahead.
+// Lane (l % 16, l / 16) takes columns 16 (l / 16) + s at k-step s, so along a
+// row block's own tiles every lane reads 16 consecutive values (8 16-byte
+// loads) and along a column strip 16 lanes read 128 contiguous bytes. P and Q
+// are staged as FP32 Gauss triples (pr, pi - pr, pr + pi) and (qr, qi - qr,
+// -(qr + qi)), so a k-step spends no vector instruction on the vector side of
+// the three real products.
+template <int Kern, bool WithY, bool WithZ>
+__global__ void __launch_bounds__(256, 2) k_pair_batch32(const float2* __restrict__ F,
+                                                         const int* __restrict__ blk_start,
+                                                         const PairPoint* __restrict__ pts, int count,
+                                                         ibim_params prm, const double2* __restrict__ P,
+                                                         const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                         double2* __restrict__ Z, const int* __restrict__ stop) {
     constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
-    constexpr int R = kBatchRHS;
-    constexpr int kBatchRows = 16 * Waves, kThreads = 64 * Waves;
+    constexpr int R = kBatchRHS, KC = kBatchK, KS = kBatchK / 4;
+    constexpr int kStrip = kPR / KC, kOwn = kPC / KC;  // chunks per strip of a block, per own tile
+    static_assert(R == 16 && KS == 16 && kStrip == 8 && kOwn == 16, "chunk indexing below");
+    static_assert(KC * R == 4 * 256 && KC * 8 == 2 * 256, "staging: 4 vector and 2 point values per thread");
     if (stop != nullptr && *stop) return;
-    __shared__ float2 ps[WithY ? kBatchK : 1][R];
-    __shared__ float2 qs[WithZ ? kBatchK : 1][R];
-    __shared__ double3 sx[kBatchK];
-    __shared__ PairPointF sf[kBatchK];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __shared__ f4 ps[WithY ? 2 : 1][WithY ? KC : 1][R];
+    __shared__ f4 qs[WithZ ? 2 : 1][WithZ ? KC : 1][R];
+    __shared__ double spd[2][KC][8];  // column points: x, y, z as staged (the rest unused)
+    __shared__ f4 spf[2][KC][2];      // (mx, my, mz, s) of each column, then unused slots
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    const int rowW = blockIdx.x * kBatchRows;  // < count
+    const int rowW = blockIdx.x * 64;  // < count
     const int I = rowW / kPR;
     const int i0 = rowW + 16 * w;
     const int i = i0 + li;  // < count + kPC: inside the padded point table
     const PairPoint Ri = pts[i];
-    const float rmx = (float)Ri.mx, rmy = (float)Ri.my, rmz = (float)Ri.mz, rw = (float)Ri.w;
-    const float k = (float)prm.k, eta = (float)prm.eta;
-    Acc3 y, z;
-    Acc3f y32, z32;
+    const float rmx = (float)Ri.mx, rmy = (float)Ri.my, rmz = (float)Ri.mz;
+    const float rs = Combined ? (float)(prm.eta * Ri.w) : (float)Ri.w;
+    const float k = (float)prm.k;
+    const int n1 = (I + 1) * kStrip;  // column-strip chunks (j < rows), then own-tile chunks
+    const int nq = n1 + (blk_start[I + 1] - blk_start[I]) * kOwn;
+    // lane offsets (bytes) of the two chunk kinds inside a chunk's F region
+    const int voff_strip = (16 * lk * kPC + li) * 8, voff_own = (li * kPC + 16 * lk) * 8;
 
-    auto stage = [&](int j0) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < kBatchK * R; e += kThreads) {
-            const int j = j0 + e / R;
+    // chunk q: its first column and this wave's offset into F; true for a
+    // column strip (F(j, i) down a tile column), false for an own tile (F(i, j))
+    auto where = [&](int q, size_t* off, int* j0) -> bool {
+        if (q < n1) {
+            const int J = q / kStrip, rj0 = (q % kStrip) * KC;
+            const int t = (rowW - J * kPR) / kPC;
+            *off = ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)rj0 * kPC + (size_t)(i0 - J * kPR - t * kPC);
+            *j0 = J * kPR + rj0;
+            return true;
+        }
+        const int qq = q - n1, t = qq / kOwn, cb = (qq % kOwn) * KC;
+        *off = ((size_t)blk_start[I] + t) * kPR * kPC + (size_t)(i0 - I * kPR) * kPC + cb;
+        *j0 = I * kPR + t * kPC + cb;
+        return false;
+    };
+    // this lane's 16 pair values of chunk q (the tile positions k_pair_batch
+    // reads, permuted inside the chunk); a chunk's region spans < 1 MB of F
+    auto load = [&](int q, fv2 (&f)[KS]) {
+        size_t off;
+        int j0;
+        const bool strip = where(q, &off, &j0);
+        const __amdgpu_buffer_rsrc_t r = buffer_of(F + off, 1u << 20);
+        if (strip) {
+#pragma unroll
+            for (int s = 0; s < KS; ++s) f[s] = ld8(r, voff_strip, s * kPC * 8);
+        } else {
+#pragma unroll
+            for (int s = 0; s < KS / 2; ++s) {
+                const f4 v = ld16(r, voff_own + 16 * s, 0);
+                f[2 * s] = fv2{v.x, v.y};
+                f[2 * s + 1] = fv2{v.z, v.w};
+            }
+        }
+    };
+    // staging of chunk q through registers: P, Q rows j0 .. j0 + 63 (zero from
+    // count on, by the buffer's size) and the 8 doubles of each column's PairPoint
+    f4 pv[4], qv[4];
+    fv2 pt[2];
+    auto fetch = [&](int q) {
+        size_t off;
+        int j0;
+        (void)where(q, &off, &j0);
+        const unsigned bytes = j0 < count ? (unsigned)(count - j0) * R * 16u : 0u;
+        const __amdgpu_buffer_rsrc_t rp = buffer_of(P + (size_t)j0 * R, WithY ? bytes : 0u);
+        const __amdgpu_buffer_rsrc_t rq = buffer_of(Q + (size_t)j0 * R, WithZ ? bytes : 0u);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            if (WithY) pv[u] = ld16(rp, tid * 16, u * 4096);
+            if (WithZ) qv[u] = ld16(rq, tid * 16, u * 4096);
+        }
+        const __amdgpu_buffer_rsrc_t rt = buffer_of(pts + j0, KC * sizeof(PairPoint));  // j0 + 63 < count + kPC
+#pragma unroll
+        for (int u = 0; u < 2; ++u) pt[u] = ld8(rt, tid * 8, u * 2048);
+    };
+    auto put = [&](int b) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int e = tid + 256 * u;
             if (WithY) {
-                const double2 v = j < count ? P[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
-                ps[e / R][e % R] = make_float2((float)v.x, (float)v.y);
+                const f4 v = pv[u];
+                const float pr = (float)__builtin_bit_cast(double, fv2{v.x, v.y});
+                const float pi = (float)__builtin_bit_cast(double, fv2{v.z, v.w});
+                ps[b][e / R][e % R] = f4{pr, pi - pr, pr + pi, 0.0f};
             }
             if (WithZ) {
-                const double2 v = j < count ? Q[(size_t)j0 * R + e] : make_double2(0.0, 0.0);
-                qs[e / R][e % R] = make_float2((float)v.x, (float)v.y);
+                const f4 v = qv[u];
+                const float qr = (float)__builtin_bit_cast(double, fv2{v.x, v.y});
+                const float qi = (float)__builtin_bit_cast(double, fv2{v.z, v.w});
+                qs[b][e / R][e % R] = f4{qr, qi - qr, -(qr + qi), 0.0f};
             }
         }
-        if (threadIdx.x < kBatchK) {
-            const PairPoint C = pts[j0 + threadIdx.x];  // j0 + 63 < count + kPC
-            sx[threadIdx.x] = make_double3(C.x, C.y, C.z);
-            sf[threadIdx.x] = PairPointF{(float)C.mx, (float)C.my, (float)C.mz, (float)C.w};
+        // PairPoint field order x, y, z, mx, my, mz, c, w: every field as a
+        // double, and as a float in slot (mx, my, mz, s, c, x, y, z)
+        const int fld = tid & 7;
+        const int slot = fld == 7 ? 3 : fld == 6 ? 4 : fld >= 3 ? fld - 3 : fld + 5;
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            const int c = (tid + 256 * u) >> 3;
+            const double v = __builtin_bit_cast(double, pt[u]);
+            spd[b][c][fld] = v;
+            reinterpret_cast<float*>(spf[b][c])[slot] = (float)(Combined && fld == 7 ? prm.eta * v : v);
         }
-        __syncthreads();
-    };
-    auto apply = [&](const float2 f, int jj) {
-        const double3 cx = sx[jj];
-        const PairPointF C = sf[jj];
-        const float d1 = (float)(Ri.x - cx.x), d2 = (float)(Ri.y - cx.y), d3 = (float)(Ri.z - cx.z);
-        float al, be;  // alpha_ij, beta_ij from the separation
-        if (Dirichlet) {
-            al = fmaf(d1, C.mx, fmaf(d2, C.my, d3 * C.mz));
-            be = -fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
-        } else {
-            al = C.w * fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
-            be = rw * -fmaf(d1, C.mx, fmaf(d2, C.my, d3 * C.mz));
-        }
-        float ar, ai, br, bi;
-        if (Combined) {
-            const float r2 = fmaxf(fmaf(d1, d1, fmaf(d2, d2, d3 * d3)), 1e-30f);
-            const float u = __builtin_amdgcn_rsqf(r2);
-            const float u2 = u * u, ku = k * u;
-            const float xr = -u2 * al, xi = fmaf(ku, al, -eta * C.w);
-            const float yr = -u2 * be, yi = fmaf(ku, be, -eta * rw);
-            ar = fmaf(f.x, xr, -f.y * xi);
-            ai = fmaf(f.x, xi, f.y * xr);
-            br = fmaf(f.x, yr, -f.y * yi);
-            bi = fmaf(f.x, yi, f.y * yr);
-        } else {
-            ar = f.x * al;
-            ai = f.y * al;
-            br = f.x * be;
-            bi = f.y * be;
-        }
-        if (WithY) {
-            const float2 p = ps[jj][li];
-            y32.add(ar, ai, p.x, p.y);  // A_ij p_j
-        }
-        if (WithZ) {
-            const float2 q = qs[jj][li];
-            z32.add(br, -bi, q.x, q.y);  // conj(A_ji) q_j
-        }
-    };
-    auto flush = [&]() {
-        if (WithY) y32.flush(y);
-        if (WithZ) z32.flush(z);
     };
 
-    // j < rows: the column strips of tiles (J, t_J), J <= I, as k_pair_batch
-    for (int J = 0; J <= I; ++J) {
-        const int cJ = rowW - J * kPR;
-        const int t = cJ / kPC;
-        const float2* T = F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
-        for (int rj0 = 0; rj0 < kPR; rj0 += kBatchK) {
-            stage(J * kPR + rj0);
-#pragma unroll 8
-            for (int s = 0; s < kBatchK / 4; ++s) {
-                const int jj = 4 * s + lk;
-                apply(T[(size_t)(rj0 + jj) * kPC], jj);
+    double yre[4] = {0, 0, 0, 0}, yim[4] = {0, 0, 0, 0}, zre[4] = {0, 0, 0, 0}, zim[4] = {0, 0, 0, 0};
+    // the LDS operands of one k-step: the column's position and factors, its P
+    // and Q triples
+    struct Col {
+        double x, y, z;
+        f4 cf, p, q;
+    };
+    auto read_col = [&](int b, int jj) {
+        Col c;
+        c.x = spd[b][jj][0];
+        c.y = spd[b][jj][1];
+        c.z = spd[b][jj][2];
+        c.cf = spf[b][jj][0];
+        if (WithY) c.p = ps[b][jj][li];
+        if (WithZ) c.q = qs[b][jj][li];
+        return c;
+    };
+    // the 16 k-steps of a staged chunk, FP32 sums, then into the FP64 totals
+    auto compute = [&](const fv2 (&f)[KS], int b) {
+        f4 y1 = {0, 0, 0, 0}, y2 = y1, y3 = y1, z1 = y1, z2 = y1, z3 = y1;
+        Col nx = read_col(b, 16 * lk);
+#pragma unroll
+        for (int s = 0; s < KS; ++s) {
+            const Col c = nx;
+            if (s + 1 < KS) nx = read_col(b, 16 * lk + s + 1);
+            const float d1 = (float)(Ri.x - c.x), d2 = (float)(Ri.y - c.y), d3 = (float)(Ri.z - c.z);
+            float al, be;  // alpha_ij, beta_ij from the separation
+            if (Dirichlet) {
+                al = fmaf(d1, c.cf.x, fmaf(d2, c.cf.y, d3 * c.cf.z));
+                be = -fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
+            } else {
+                al = c.cf.w * fmaf(d1, rmx, fmaf(d2, rmy, d3 * rmz));
+                be = rs * -fmaf(d1, c.cf.x, fmaf(d2, c.cf.y, d3 * c.cf.z));
             }
-            flush();
-        }
-    }
-    // j >= rows: row i of the row block's tiles
-    const int nt = blk_start[I + 1] - blk_start[I];
-    const float2* Trow = F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
-    for (int t = 0; t < nt; ++t) {
-        const int c0 = I * kPR + t * kPC;
-        const float2* Tr = Trow + (size_t)t * kPR * kPC;
-        for (int cb = 0; cb < kPC; cb += kBatchK) {
-            stage(c0 + cb);
-#pragma unroll 8
-            for (int s = 0; s < kBatchK / 4; ++s) {
-                const int jj = 4 * s + lk;
-                apply(Tr[cb + jj], jj);
+            const float fx = f[s].x, fy = f[s].y;
+            float ar, ai, br, bi;  // A_ij, A_ji
+            if (Combined) {
+                const float r2 = fmaxf(fmaf(d1, d1, fmaf(d2, d2, d3 * d3)), 1e-30f);
+                const float u = __builtin_amdgcn_rsqf(r2);
+                const float nu2 = -u * u, ku = k * u;
+                const float xr = nu2 * al, xi = fmaf(ku, al, -c.cf.w);
+                const float yr = nu2 * be, yi = fmaf(ku, be, -rs);
+                ar = fmaf(fx, xr, -fy * xi);
+                ai = fmaf(fx, xi, fy * xr);
+                br = fmaf(fx, yr, -fy * yi);
+                bi = fmaf(fx, yi, fy * yr);
+            } else {
+                ar = fx * al;
+                ai = fy * al;
+                br = fx * be;
+                bi = fy * be;
             }
-            flush();
+            if (WithY) {  // A_ij p_j
+                y1 = mfma32(ar + ai, c.p.x, y1);
+                y2 = mfma32(ar, c.p.y, y2);
+                y3 = mfma32(ai, c.p.z, y3);
+            }
+            if (WithZ) {  // conj(A_ji) q_j = (br - i bi) q_j
+                z1 = mfma32(br - bi, c.q.x, z1);
+                z2 = mfma32(br, c.q.y, z2);
+                z3 = mfma32(bi, c.q.z, z3);
+            }
         }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            if (WithY) {
+                yre[v] += (double)y1[v] - (double)y3[v];
+                yim[v] += (double)y1[v] + (double)y2[v];
+            }
+            if (WithZ) {
+                zre[v] += (double)z1[v] - (double)z3[v];
+                zim[v] += (double)z1[v] + (double)z2[v];
+            }
+        }
+    };
+    // one chunk: next chunk's loads issued first (staging before the pair
+    // values, so put() waits for the staging only), this chunk's k-steps, the
+    // next stage written, one barrier
+    auto step = [&](int q, const fv2 (&cur)[KS], fv2 (&nxt)[KS]) {
+        const bool more = q + 1 < nq;
+        if (more) {
+            fetch(q + 1);
+            load(q + 1, nxt);
+        }
+        compute(cur, q & 1);
+        if (more) put((q + 1) & 1);
+        __syncthreads();
+    };
+
+    fv2 fa[KS], fb[KS];
+    fetch(0);
+    load(0, fa);
+    put(0);
+    __syncthreads();
+    for (int q = 0; q < nq; q += 2) {
+        step(q, fa, fb);
+        if (q + 1 < nq) step(q + 1, fb, fa);
     }
     // the FP32 instruction's D layout: lane (li, lk) holds rows 4 lk + v of
-    // column li (the FP64 one holds rows 4 v + lk); the FP64 chunk sums
-    // follow it
+    // column li (the FP64 one holds rows 4 v + lk)
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
         const int r = i0 + 4 * lk + v;
         if (r < count) {
-            if (WithY) Y[(size_t)r * R + li] = make_double2(y.re(v), y.im(v));
-            if (WithZ) Z[(size_t)r * R + li] = make_double2(z.re(v), z.im(v));
+            if (WithY) Y[(size_t)r * R + li] = make_double2(yre[v], yim[v]);
+            if (WithZ) Z[(size_t)r * R + li] = make_double2(zre[v], zim[v]);
         }
     }
 }
@@ -1217,7 +1308,7 @@ void launch_pair_batch32(Context* ctx, const float2* F, const double2* P, const 
                          const int* stop) {
     const int count = ctx->count;
     TimedScope ts(ctx, "batch_pair32");
-    hipLaunchKernelGGL((k_pair_batch32<Kern, WithY, WithZ, 4>), dim3((unsigned)((count + 63) / 64)), dim3(256), 0,
+    hipLaunchKernelGGL((k_pair_batch32<Kern, WithY, WithZ>), dim3((unsigned)((count + 63) / 64)), dim3(256), 0,
                        ctx->stream, F, ctx->pair_blk.as<int>(), ctx->pair_pts.as<PairPoint>(), count, ctx->prm, P, Q,
                        Y, Z, stop);
     double nr = 0.0, ni = 0.0;
