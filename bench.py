@@ -216,14 +216,13 @@ def run_batch(args):
     """Many right-hand sides and surfaces: the torus (128^3, k = 10) with 64 plane
     waves on a Fibonacci sphere, plus 8 spheres / ellipsoids with seed-random
     radii in [0.6, 1.2] on 48^3-96^3 grids with 16 plane waves each; sound-soft
-    data, tol 1e-7. FP64 path: batched BiCG on complex128 pair storage. Mixed
-    path: batched BiCG on complex64 pairs with FP64 residual replacement over
-    the complex128 copy of the same pairs (IBIM_MIXED_PAIR). 16 right-hand
-    sides per pass, A P and A^H Q fused into one pass over the pairs on the FP64
-    matrix cores (dense storage where the pairs would not fit). The torus runs
-    alone; the 8 smaller surfaces (2,6xx-25,0xx points, too few rows to fill the
-    GPU one at a time) run concurrently, one context and stream each."""
-    from concurrent.futures import ThreadPoolExecutor
+    data, tol 1e-7. All 9 surfaces (12 batches of 16 right-hand sides) are
+    solved together by ibim_solve_group: every BiCG iteration applies all of
+    their operators in one persistent launch (a queue of 64-row items, the
+    torus first). FP64 path: batched BiCG on complex128 pair storage, products
+    on the FP64 matrix cores. Mixed path: batched BiCG on complex64 pairs with
+    the products on the FP32 matrix cores, FP64 residual replacement over the
+    complex128 copy of the same pairs (IBIM_MIXED_PAIR)."""
     tol, k = 1e-7, 10.0
     rng = np.random.default_rng(0)
     torus = ("torus128", ibim.shape(ibim.SHAPE_TORUS, a=1.0, b=0.4), 128, 64)
@@ -238,79 +237,85 @@ def run_batch(args):
             shp = ibim.shape(ibim.SHAPE_ELLIPSOID, a=float(r[0]), b=float(r[1]), c=float(r[2]))
             name = "ellipsoid %.2f/%.2f/%.2f" % tuple(r)
         small.append((name, shp, grids[i], 16))
+    surfaces = [torus] + small
     dirs = fibonacci_directions(64)
 
-    def solve(surface, path, report):
-        name, shp, n, m = surface
-        h = 4.0 / n
-        pb = ibim.Problem()
-        if report:
-            progress(pb)
-        pb.set_level_shape(shp, n)
-        pb.sync()
+    def run(path):
+        pbs = []
+        for name, shp, n, m in surfaces:
+            pb = ibim.Problem()
+            pb.set_level_shape(shp, n)
+            pb.sync()
+            pbs.append(pb)
+        progress(pbs[0])
         t0 = time.perf_counter()
-        pb.geometry(5)
-        count = pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1))
+        counts = []
+        for (name, shp, n, m), pb in zip(surfaces, pbs):
+            h = 4.0 / n
+            pb.geometry(5)
+            counts.append(pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1)))
+            if path == "fp64":
+                pb.assemble(ibim.C128_PAIR)
+            else:
+                pb.assemble(ibim.MIXED_PAIR)
+                pb.set_fp32_products(args.batch_fp32_products)
+            pb.set_nrhs(m)
+            for c in range(m):
+                pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+        pbs[0].reset_timers()
+        pbs[0].set_profiling(True)
+        t1 = time.perf_counter()
         if path == "fp64":
-            pb.assemble(ibim.C128_PAIR if pair_bytes(count, 16) <= STORE_MAX_BYTES else ibim.C128)
+            infos = ibim.solve_group(pbs, ibim.SOLVE_BICG, maxit=3000, tol=tol)
         else:
-            store = mixed_storage(count)
-            pb.assemble(store if store is not None else ibim.C64)
-        pb.set_nrhs(m)
-        for c in range(m):
-            pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
-        if path == "fp64":
-            info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=tol, batch=0)
-        else:
-            info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=tol, inner_tol=BATCH_REPLACE, batch=0)
-        x = []
-        for c in range(m):
-            d = pb.density(c)
-            x.append(d[0] + 1j * d[1])
-        pb.sync()
+            infos = ibim.solve_group(pbs, ibim.SOLVE_RELIABLE, maxit=3000, tol=tol, inner_tol=args.batch_replace)
+        dens = []
+        for (name, shp, n, m), pb in zip(surfaces, pbs):
+            x = []
+            for c in range(m):
+                d = pb.density(c)
+                x.append(d[0] + 1j * d[1])
+            dens.append(np.stack(x))
+        for pb in pbs:
+            pb.sync()
         el = time.perf_counter() - t0
-        pb.close()
-        return el, info, np.stack(x), count
+        solve_s = time.perf_counter() - t1
+        pbs[0].set_profiling(False)
+        ms, launches = pbs[0].kernel_time("batch_group")
+        for pb in pbs:
+            pb.close()
+        print("[bench] batch %s: %.2f s (solve %.2f s), %d grouped passes of %.1f ms" %
+              (path, el, solve_s, launches, ms / max(launches, 1)), file=sys.stderr, flush=True)
+        return el, infos, dens, counts, dict(solve_s=solve_s, launches=launches, pass_ms=ms / max(launches, 1))
 
-    totals = {"fp64": 0.0, "mixed": 0.0}
-    res = {}  # (surface name, path) -> result
-    for path in ("fp64", "mixed"):
-        res[(torus[0], path)] = solve(torus, path, True)
-        totals[path] += res[(torus[0], path)][0]
-        print("[bench] %s %s: %.2f s" % (torus[0], path, res[(torus[0], path)][0]), file=sys.stderr, flush=True)
-        t0 = time.perf_counter()
-        with ThreadPoolExecutor(max_workers=len(small)) as pool:
-            outs = list(pool.map(lambda s: solve(s, path, False), small))
-        group = time.perf_counter() - t0
-        totals[path] += group
-        for s, o in zip(small, outs):
-            res[(s[0], path)] = o
-        print("[bench] 8 smaller surfaces %s, concurrently: %.2f s" % (path, group), file=sys.stderr, flush=True)
+    res = {path: run(path) for path in ("fp64", "mixed")}
     rows, n_rhs, worst_err = [], 0, 0.0
-    for surface in [torus] + small:
-        name, _, n, m = surface
-        f, mx = res[(name, "fp64")], res[(name, "mixed")]
-        err = float(max(np.linalg.norm(mx[2][c] - f[2][c]) / np.linalg.norm(f[2][c]) for c in range(m)))
+    for s_i, (name, _, n, m) in enumerate(surfaces):
+        f, mx = res["fp64"], res["mixed"]
+        err = float(max(np.linalg.norm(mx[2][s_i][c] - f[2][s_i][c]) / np.linalg.norm(f[2][s_i][c])
+                        for c in range(m)))
         worst_err = max(worst_err, err)
         n_rhs += m
-        rows.append({"surface": name, "n": n, "tube_points": f[3], "rhs": m,
-                     "fp64_s": f[0], "fp64_iterations": f[1].iterations, "fp64_status": f[1].status,
-                     "mixed_s": mx[0], "mixed_iterations": mx[1].iterations, "mixed_status": mx[1].status,
-                     "mixed_rel_err_vs_fp64": err})
-        print("[bench] %s: N_t=%d, %d rhs, fp64 %.2f s, mixed %.2f s, rel err %.1e" %
-              (name, f[3], m, f[0], mx[0], err), file=sys.stderr, flush=True)
-    return ({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / totals["mixed"],
-          "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": totals["mixed"] * 1e3,
+        rows.append({"surface": name, "n": n, "tube_points": f[3][s_i], "rhs": m,
+                     "fp64_iterations": f[1][s_i].iterations, "fp64_status": f[1][s_i].status,
+                     "mixed_iterations": mx[1][s_i].iterations, "mixed_status": mx[1][s_i].status,
+                     "mixed_replacements": mx[1][s_i].outer, "mixed_rel_err_vs_fp64": err})
+        print("[bench] %s: N_t=%d, %d rhs, iterations fp64 %d / mixed %d, rel err %.1e" %
+              (name, f[3][s_i], m, f[1][s_i].iterations, mx[1][s_i].iterations, err), file=sys.stderr, flush=True)
+    total = {p: res[p][0] for p in res}
+    return ({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / total["mixed"],
+          "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "complex64 pair storage, fp64 matrix-core products, fp64 residual replacement over the "
-                   "complex128 copy of the pairs",
+          "dtype": "complex64 pair storage, products on the fp32 matrix cores summed in fp64 per 64 columns, fp64 "
+                   "residual replacement over the complex128 copy of the pairs",
           "data": "synthetic: torus + 8 seed-random spheres/ellipsoids, Fibonacci plane waves",
           "config": {"workload": "batch: torus 128^3 x 64 plane waves + 8 surfaces on 48^3-96^3 x 16 plane waves, "
                                  "k=10, eps=tau=2h, tol %g" % tol, "rhs": n_rhs, "parallelism": "single GPU",
-                     "schedule": "torus alone, then the 8 smaller surfaces concurrently (one stream each); "
-                                 "per-surface times overlap"},
-          "fp64_solves_per_s": n_rhs / totals["fp64"], "speedup_mixed_vs_fp64": totals["fp64"] / totals["mixed"],
-          "worst_rel_err_mixed_vs_fp64": worst_err, "surfaces": rows})
+                     "schedule": "all 9 surfaces (12 batches of 16 right-hand sides) in one grouped solve: one "
+                                 "persistent launch of every operator per BiCG iteration (ibim_solve_group)"},
+          "fp64_solves_per_s": n_rhs / total["fp64"], "speedup_mixed_vs_fp64": total["fp64"] / total["mixed"],
+          "worst_rel_err_mixed_vs_fp64": worst_err,
+          "grouped_pass": {p: res[p][4] for p in res}, "surfaces": rows})
 
 
 def run_adjoint(args):
@@ -575,6 +580,10 @@ def main():
                     help="star256 mixed path: 16-column passes on the FP64 matrix cores (default: FP32)")
     ap.add_argument("--star-replace", type=float, default=BATCH_REPLACE,
                     help="star256 mixed path: residual drop between FP64 residual replacements")
+    ap.add_argument("--batch-fp32-products", action="store_true",
+                    help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
+    ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
+                    help="batch mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     ap.add_argument("--matrix-free", action="store_true",

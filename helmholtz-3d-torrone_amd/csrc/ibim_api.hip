@@ -151,7 +151,7 @@ void ibim_destroy(ibim_ctx* ctx) {
     collect_timers(ctx);
     for (DevBuf* b : {&ctx->level, &ctx->gx, &ctx->gy, &ctx->gz, &ctx->mc, &ctx->gc, &ctx->node, &ctx->tube,
                       &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->mat64, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
-                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch,
+                      &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch, &ctx->group,
                       &ctx->pair_tiles, &ctx->pair_blk, &ctx->pair_pts, &ctx->near_off, &ctx->near_idx,
                       &ctx->near_cur})
         b->release();
@@ -506,6 +506,20 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
     if (opts->method == IBIM_SOLVE_RELIABLE)
         return batched ? run_reliable_batched(ctx, opts, info, hist) : run_reliable(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
+}
+
+int ibim_solve_group(ibim_ctx* const* ctxs, int nctx, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+    if (!ctxs || nctx <= 0 || !opts || !infos || opts->maxit <= 0) return IBIM_ERR_ARG;
+    for (int c = 0; c < nctx; ++c) {
+        if (!ctxs[c]) {
+            set_error("ibim_solve_group: null handle %d", c);
+            return IBIM_ERR_ARG;
+        }
+        memset(&infos[c], 0, sizeof infos[c]);
+    }
+    CHECK(ctxs[0]);
+    std::vector<Context*> v(ctxs, ctxs + nctx);
+    return run_group(v.data(), nctx, opts->method, opts, infos);
 }
 
 int ibim_evaluate(ibim_ctx* ctx, int mode, int col, int nrecv, const double* xyz, double* ure, double* uim) {

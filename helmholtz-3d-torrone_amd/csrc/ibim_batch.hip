@@ -511,17 +511,27 @@ int batch_apply_host(Context* ctx, int op, bool adjoint, int nvec, const double*
     return IBIM_OK;
 }
 
+// Blocks of the batched vector kernels (16 points each, grid-strided beyond).
+static unsigned batch_blocks(const Context* ctx) {
+    return (unsigned)std::min<size_t>(((size_t)ctx->count + 15) / 16, kMaxVB);
+}
+
 // Workspace of one batch: x, r, r~, p, p~, v, w, b and `extra` more
 // interleaved vectors, per-block partials, the state and the residual history.
-static int batch_workspace(Context* ctx, int maxit, int extra, BArgs* a, unsigned* nvb_out, double2** extra_out) {
+static size_t batch_workspace_bytes(const Context* ctx, int maxit, int extra) {
+    const size_t vec = ctx->ld * kR;
+    return sizeof(double2) * vec * (8 + extra) + sizeof(double) * (size_t)batch_blocks(ctx) * kR * 8 +
+           sizeof(BState) + 256 + sizeof(double) * (size_t)(maxit + 1) * kR;
+}
+
+// The same layout at `base` (batch_workspace_bytes(ctx, maxit, extra) bytes, 256-B aligned).
+static int batch_workspace_at(Context* ctx, void* base, int maxit, int extra, BArgs* a, unsigned* nvb_out,
+                              double2** extra_out) {
     const int count = ctx->count;
     const size_t vec = ctx->ld * kR;
-    const unsigned nvb = (unsigned)std::min<size_t>(((size_t)count + 15) / 16, kMaxVB);
+    const unsigned nvb = batch_blocks(ctx);
     const int nv = 8 + extra;
-    const size_t want = sizeof(double2) * vec * nv + sizeof(double) * (size_t)nvb * kR * 8 + sizeof(BState) + 256 +
-                        sizeof(double) * (size_t)(maxit + 1) * kR;
-    if (ctx->batch.ensure(want)) return IBIM_ERR_NOMEM;
-    double2* v0 = ctx->batch.as<double2>();
+    double2* v0 = static_cast<double2*>(base);
     *a = BArgs{};
     a->x = v0;
     a->r = v0 + vec;
@@ -541,6 +551,11 @@ static int batch_workspace(Context* ctx, int maxit, int extra, BArgs* a, unsigne
     // padding rows (count..ld) of every vector read as zero in the passes
     IBIM_HIP(hipMemsetAsync(v0, 0, sizeof(double2) * vec * nv, ctx->stream));
     return IBIM_OK;
+}
+
+static int batch_workspace(Context* ctx, int maxit, int extra, BArgs* a, unsigned* nvb_out, double2** extra_out) {
+    if (ctx->batch.ensure(batch_workspace_bytes(ctx, maxit, extra))) return IBIM_ERR_NOMEM;
+    return batch_workspace_at(ctx, ctx->batch.ptr, maxit, extra, a, nvb_out, extra_out);
 }
 
 struct ConvMask {
@@ -639,6 +654,70 @@ struct BReplace {
 // operator can stall above the FP64 one's level).
 constexpr int kMaxStallRestarts = 3;
 
+// Residual replacement at a host check (reliable batched BiCG): columns whose
+// recursive residual fell by rep->factor since their last replacement, that
+// claim convergence or stopped on stagnation get r = b - A64 x (one batched
+// FP64 pass for all of them); refuted claims restart their column.
+static int replacement_step(Context* ctx, const BArgs& a, unsigned nvb, BState* hs, BReplace* rep, double tol,
+                            std::vector<double>& sums) {
+    int rc = IBIM_OK;
+    ConvMask m{};
+    bool any = false;
+    for (int r = 0; r < kR; ++r) {
+        const bool claims = hs->done[r] > 0 && hs->stop[r] == 1 && !rep->verified[r];
+        const bool stalled = hs->done[r] > 0 && hs->stop[r] == 3 && !rep->verified[r] &&
+                             rep->stalls[r] < kMaxStallRestarts;
+        const bool drop = hs->done[r] == 0 && hs->err[r] < rep->next[r];
+        if (stalled) rep->stalls[r]++;
+        m.c[r] = claims || stalled || drop;
+        any = any || m.c[r];
+    }
+    // the pass computes every column: running columns that have come
+    // at least half a replacement interval (sqrt(factor)) down since
+    // their last replacement are replaced with the others, so the
+    // columns stay in step and share the passes
+    if (any)
+        for (int r = 0; r < kR; ++r) {
+            const double base = rep->true_err[r] >= 0.0 ? rep->true_err[r] : 1.0;
+            if (hs->done[r] == 0 && hs->err[r] < base * sqrt(rep->factor)) m.c[r] = 1;
+        }
+    if (any) {
+        // a.v is free between iterations: the true products of all columns
+        rc = batch_apply(ctx, rep->true_op, false, a.x, a.v, nullptr);
+        if (rc) return rc;
+        hipLaunchKernelGGL(kb_replace, dim3(nvb), dim3(kBT), 0, ctx->stream, a.b, a.v, a.r, a.count, m,
+                           a.part);
+        IBIM_HIP(hipGetLastError());
+        rc = column_sums(ctx, a.part, nvb, 2, sums);
+        if (rc) return rc;
+        rep->replacements++;
+        ConvMask restart{};
+        bool any_restart = false;
+        for (int r = 0; r < kR; ++r) {
+            if (!m.c[r]) continue;
+            const double terr = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / hs->normb[r];
+            rep->true_err[r] = terr;
+            rep->next[r] = terr * rep->factor;
+            if (hs->done[r] > 0) {  // a convergence claim or a checked stagnation stop
+                if (terr < tol) {
+                    rep->verified[r] = true;
+                } else {
+                    restart.c[r] = 1;
+                    any_restart = true;
+                }
+            }
+        }
+        if (any_restart) {
+            hipLaunchKernelGGL(kb_restart, dim3(nvb), dim3(kBT), 0, ctx->stream, a, restart);
+            hipLaunchKernelGGL(kb_restart_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, restart);
+            IBIM_HIP(hipGetLastError());
+            IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
+            IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        }
+    }
+    return IBIM_OK;
+}
+
 // BiCG for the 16 interleaved right-hand sides in a.b (padding rows zero);
 // the solutions are left in a.x and the final state in *hs. With `rep`, FP64
 // residual replacement by rep->true_op.
@@ -716,82 +795,22 @@ static int bicg_batch_core(Context* ctx, int op, const BArgs& a, unsigned nvb, i
             if (!hs->all_done) report_progress(ctx, "batched BiCG", it - 1, worst);
         }
         if (rep) {
-            ConvMask m{};
-            bool any = false;
-            for (int r = 0; r < kR; ++r) {
-                const bool claims = hs->done[r] > 0 && hs->stop[r] == 1 && !rep->verified[r];
-                const bool stalled = hs->done[r] > 0 && hs->stop[r] == 3 && !rep->verified[r] &&
-                                     rep->stalls[r] < kMaxStallRestarts;
-                const bool drop = hs->done[r] == 0 && hs->err[r] < rep->next[r];
-                if (stalled) rep->stalls[r]++;
-                m.c[r] = claims || stalled || drop;
-                any = any || m.c[r];
-            }
-            // the pass computes every column: running columns that have come
-            // at least half a replacement interval (sqrt(factor)) down since
-            // their last replacement are replaced with the others, so the
-            // columns stay in step and share the passes
-            if (any)
-                for (int r = 0; r < kR; ++r) {
-                    const double base = rep->true_err[r] >= 0.0 ? rep->true_err[r] : 1.0;
-                    if (hs->done[r] == 0 && hs->err[r] < base * sqrt(rep->factor)) m.c[r] = 1;
-                }
-            if (any) {
-                // a.v is free between iterations: the true products of all columns
-                int rc = batch_apply(ctx, rep->true_op, false, a.x, a.v, nullptr);
-                if (rc) return rc;
-                hipLaunchKernelGGL(kb_replace, dim3(nvb), dim3(kBT), 0, ctx->stream, a.b, a.v, a.r, a.count, m,
-                                   a.part);
-                IBIM_HIP(hipGetLastError());
-                rc = column_sums(ctx, a.part, nvb, 2, sums);
-                if (rc) return rc;
-                rep->replacements++;
-                ConvMask restart{};
-                bool any_restart = false;
-                for (int r = 0; r < kR; ++r) {
-                    if (!m.c[r]) continue;
-                    const double terr = sqrt(sums[(size_t)r * 2] + sums[(size_t)r * 2 + 1]) / hs->normb[r];
-                    rep->true_err[r] = terr;
-                    rep->next[r] = terr * rep->factor;
-                    if (hs->done[r] > 0) {  // a convergence claim or a checked stagnation stop
-                        if (terr < tol) {
-                            rep->verified[r] = true;
-                        } else {
-                            restart.c[r] = 1;
-                            any_restart = true;
-                        }
-                    }
-                }
-                if (any_restart) {
-                    hipLaunchKernelGGL(kb_restart, dim3(nvb), dim3(kBT), 0, ctx->stream, a, restart);
-                    hipLaunchKernelGGL(kb_restart_finish, dim3(1), dim3(kBT), 0, ctx->stream, a, (int)nvb, restart);
-                    IBIM_HIP(hipGetLastError());
-                    IBIM_HIP(hipMemcpyAsync(hs, a.st, sizeof *hs, hipMemcpyDeviceToHost, ctx->stream));
-                    IBIM_HIP(hipStreamSynchronize(ctx->stream));
-                }
-            }
+            int rc = replacement_step(ctx, a, nvb, hs, rep, tol, sums);
+            if (rc) return rc;
         }
         if (hs->all_done) break;
     }
     return IBIM_OK;
 }
 
-// BiCG on columns [c0, c0 + ncol) of ctx->rhs (ncol <= 16) with operator `op`;
-// densities into the same columns of ctx->density. Per-column iteration counts,
-// residuals and stop reasons go to the out arrays.
-static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
-                      double* errs, int* stops, double* hist, BReplace* rep = nullptr) {
+// After a batch's iterations: columns that stopped without a verified
+// convergence claim report the true FP64 residual of their final iterate (the
+// reliable solver), the solutions go to density columns c0 .. c0 + ncol - 1,
+// and each column's iterations, residual and stop reason to the out arrays.
+static int batch_finish(Context* ctx, const BArgs& a, unsigned nvb, const BState& hs, BReplace* rep, int c0,
+                        int ncol, int maxit, int* iters, double* errs, int* stops, double* hist) {
     const int count = ctx->count;
-    BArgs a;
-    unsigned nvb;
-    int rc = batch_workspace(ctx, maxit, 0, &a, &nvb, nullptr);
-    if (rc) return rc;
-    const unsigned pb = (unsigned)((ctx->ld * kR + 255) / 256);
-    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol,
-                       (double2*)a.b, ctx->ld);
-    BState hs{};
-    rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs, rep);
-    if (rc) return rc;
+    int rc = IBIM_OK;
     if (rep) {
         // columns that stopped without a verified convergence claim
         // (stagnation, NaN, the iteration cap) report the true FP64 residual
@@ -833,6 +852,25 @@ static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double 
             for (int i = 1; i <= iters[0]; ++i) hist[i - 1] = h[(size_t)i * kR];
     }
     return IBIM_OK;
+}
+
+// BiCG on columns [c0, c0 + ncol) of ctx->rhs (ncol <= 16) with operator `op`;
+// densities into the same columns of ctx->density. Per-column iteration counts,
+// residuals and stop reasons go to the out arrays.
+static int bicg_batch(Context* ctx, int op, int c0, int ncol, int maxit, double tol, int chunk, int* iters,
+                      double* errs, int* stops, double* hist, BReplace* rep = nullptr) {
+    const int count = ctx->count;
+    BArgs a;
+    unsigned nvb;
+    int rc = batch_workspace(ctx, maxit, 0, &a, &nvb, nullptr);
+    if (rc) return rc;
+    const unsigned pb = (unsigned)((ctx->ld * kR + 255) / 256);
+    hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol,
+                       (double2*)a.b, ctx->ld);
+    BState hs{};
+    rc = bicg_batch_core(ctx, op, a, nvb, maxit, tol, chunk, &hs, rep);
+    if (rc) return rc;
+    return batch_finish(ctx, a, nvb, hs, rep, c0, ncol, maxit, iters, errs, stops, hist);
 }
 
 // X += D; and R = B - Y with per-block partials of |R|^2 (2 per column), the
@@ -1033,6 +1071,195 @@ int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info*
     info->rel_residual = worst;
     info->status = status;
     ctx->last_iters = total;
+    return IBIM_OK;
+}
+
+// ------------------------------------------------------ several problems --
+// Batched BiCG (IBIM_SOLVE_BICG) or reliable batched BiCG (IBIM_SOLVE_RELIABLE)
+// of every right-hand side of several pair-stored problems at once. A member
+// is one problem with one batch of up to 16 of its columns; all members
+// iterate in step, and each iteration applies every running member's operator
+// in ONE persistent launch (pair_group_pass: a queue of 64-row items over all
+// members, largest problems first), then runs each member's near field and
+// BiCG vector kernels. Host checks (and residual replacements) come every
+// `chunk` iterations as in the one-problem solvers, so each member takes the
+// same steps, and gives the same bits, as its own batched solve with that chunk.
+// All work goes to the first problem's stream for the duration of the call.
+namespace {
+
+struct GroupRun {
+    Context* ctx = nullptr;
+    int c0 = 0, ncol = 0;
+    BArgs a{};
+    unsigned nvb = 0;
+    BState hs{};
+    BReplace rep{};
+};
+
+// Points every context's work at one stream for the call (their own streams
+// drained first), and back afterwards.
+struct SharedStream {
+    std::vector<Context*> ctxs;
+    std::vector<hipStream_t> own;
+    ~SharedStream() {
+        if (!ctxs.empty()) (void)hipStreamSynchronize(ctxs[0]->stream);
+        for (size_t c = 0; c < ctxs.size(); ++c) ctxs[c]->stream = own[c];
+    }
+};
+
+}  // namespace
+
+int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+    if (method != IBIM_SOLVE_BICG && method != IBIM_SOLVE_RELIABLE) {
+        set_error("ibim_solve_group: BiCG or reliable BiCG only");
+        return IBIM_ERR_ARG;
+    }
+    const bool mixed = method == IBIM_SOLVE_RELIABLE;
+    int source = 0;
+    if (int rc = pair_group_source(ctxs, n, mixed, &source)) return rc;
+    Context* lead = ctxs[0];
+    const int maxit = opts->maxit;
+    SharedStream shared;
+    for (int c = 0; c < n; ++c) IBIM_HIP(hipStreamSynchronize(ctxs[c]->stream));
+    for (int c = 0; c < n; ++c) {
+        shared.ctxs.push_back(ctxs[c]);
+        shared.own.push_back(ctxs[c]->stream);
+        ctxs[c]->stream = lead->stream;
+    }
+    // members, the largest problems first (their items start first)
+    std::vector<GroupRun> runs;
+    std::vector<int> order(n);
+    for (int c = 0; c < n; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ctxs[x]->count > ctxs[y]->count; });
+    for (int c : order)
+        for (int c0 = 0; c0 < ctxs[c]->nrhs; c0 += kR) {
+            GroupRun g;
+            g.ctx = ctxs[c];
+            g.c0 = c0;
+            g.ncol = std::min(kR, ctxs[c]->nrhs - c0);
+            runs.push_back(g);
+        }
+    const int nm = (int)runs.size();
+    // workspaces, then the queue counter and the member table
+    std::vector<size_t> off(nm + 1, 0);
+    for (int m = 0; m < nm; ++m)
+        off[m + 1] = off[m] + ((batch_workspace_bytes(runs[m].ctx, maxit, 0) + 255) & ~(size_t)255);
+    const size_t table = off[nm] + 256;
+    if (lead->group.ensure(table + sizeof(GroupMember) * nm)) return IBIM_ERR_NOMEM;
+    char* base = lead->group.as<char>();
+    int* queue = reinterpret_cast<int*>(base + off[nm]);
+    GroupMember* dev_members = reinterpret_cast<GroupMember*>(base + table);
+    std::vector<GroupMember> members(nm);
+    int items = 0;
+    for (int m = 0; m < nm; ++m) {
+        GroupRun& g = runs[m];
+        Context* ctx = g.ctx;
+        if (int rc = batch_workspace_at(ctx, base + off[m], maxit, 0, &g.a, &g.nvb, nullptr)) return rc;
+        const unsigned pb = (unsigned)((ctx->ld * kR + 255) / 256);
+        hipLaunchKernelGGL(kb_pack, dim3(pb), dim3(256), 0, lead->stream, ctx->rhs.as<double2>(), ctx->count, g.c0,
+                           g.ncol, (double2*)g.a.b, ctx->ld);
+        hipLaunchKernelGGL(kb_init, dim3(g.nvb), dim3(kBT), 0, lead->stream, g.a);
+        hipLaunchKernelGGL(kb_init_finish, dim3(1), dim3(kBT), 0, lead->stream, g.a, (int)g.nvb);
+        IBIM_HIP(hipGetLastError());
+        GroupMember& d = members[m];
+        d.F = ctx->mat.ptr;
+        d.blk = ctx->pair_blk.as<int>();
+        d.pts = ctx->pair_pts.ptr;
+        d.P = g.a.p;
+        d.Q = g.a.pt;
+        d.Y = g.a.v;
+        d.Z = g.a.w;
+        d.stop = &g.a.st->all_done;
+        d.prm = ctx->prm;
+        d.count = ctx->count;
+        d.item0 = items;
+        items += (ctx->count + 63) / 64;
+        if (mixed) {
+            g.rep.true_op = fp64_operator(ctx);
+            g.rep.factor = opts->inner_tol > 0 && opts->inner_tol < 1 ? opts->inner_tol : 1e-2;
+            for (int r = 0; r < kR; ++r) {
+                g.rep.next[r] = g.rep.factor;
+                g.rep.true_err[r] = -1.0;
+                g.rep.verified[r] = false;
+                g.rep.stalls[r] = 0;
+            }
+        }
+    }
+    IBIM_HIP(hipMemcpyAsync(dev_members, members.data(), sizeof(GroupMember) * nm, hipMemcpyHostToDevice,
+                            lead->stream));
+    for (GroupRun& g : runs) IBIM_HIP(hipMemcpyAsync(&g.hs, g.a.st, sizeof g.hs, hipMemcpyDeviceToHost, lead->stream));
+    IBIM_HIP(hipStreamSynchronize(lead->stream));
+    // host checks as in the one-problem solvers: every 2 iterations for the
+    // reliable solver unless asked; BiCG: 2 when the largest problem has more
+    // than 20,000 points, else 16
+    int chunk = opts->chunk;
+    if (chunk <= 0) chunk = mixed ? 2 : (runs[0].ctx->count > 20000 ? 2 : 16);
+    std::vector<double> sums;
+    int it = 1;
+    while (it <= maxit) {
+        bool live = false;
+        for (const GroupRun& g : runs) live = live || !g.hs.all_done;
+        if (live) {
+            const int last = std::min(it + chunk - 1, maxit);
+            for (; it <= last; ++it) {
+                if (int rc = pair_group_pass(lead, source, dev_members, nm, items, queue)) return rc;
+                for (GroupRun& g : runs) {
+                    if (g.hs.all_done) continue;  // stays done until a host check restarts it
+                    const BArgs& a = g.a;
+                    if (int rc = pair_batch_near(g.ctx, a.p, a.pt, a.v, a.w, &a.st->all_done)) return rc;
+                    hipLaunchKernelGGL(kb_dot, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+                    hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb);
+                    hipLaunchKernelGGL(kb_update_xr, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+                    hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb, opts->tol);
+                    hipLaunchKernelGGL(kb_update_p, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+                }
+                IBIM_HIP(hipGetLastError());
+            }
+            double worst = 0.0;
+            for (GroupRun& g : runs) {
+                if (g.hs.all_done) continue;
+                IBIM_HIP(hipMemcpyAsync(&g.hs, g.a.st, sizeof g.hs, hipMemcpyDeviceToHost, lead->stream));
+            }
+            IBIM_HIP(hipStreamSynchronize(lead->stream));
+            for (const GroupRun& g : runs)
+                for (int r = 0; r < kR; ++r)
+                    if (g.hs.done[r] == 0) worst = std::max(worst, g.hs.err[r]);
+            report_progress(lead, "grouped batched BiCG", it - 1, worst);
+        }
+        bool all = true;
+        for (GroupRun& g : runs) {
+            if (mixed)
+                if (int rc = replacement_step(g.ctx, g.a, g.nvb, &g.hs, &g.rep, opts->tol, sums)) return rc;
+            all = all && g.hs.all_done;
+        }
+        if (all) break;
+    }
+    // per problem: its columns' results, summed as the one-problem solvers do
+    for (int c = 0; c < n; ++c) {
+        Context* ctx = ctxs[c];
+        const int nr = ctx->nrhs;
+        std::vector<int> iters(nr), stops(nr);
+        std::vector<double> errs(nr);
+        int replacements = 0;
+        for (GroupRun& g : runs) {
+            if (g.ctx != ctx) continue;
+            if (int rc = batch_finish(ctx, g.a, g.nvb, g.hs, mixed ? &g.rep : nullptr, g.c0, g.ncol, maxit,
+                                      iters.data(), errs.data(), stops.data(), nullptr))
+                return rc;
+            replacements += g.rep.replacements;
+        }
+        ibim_solve_info& info = infos[c];
+        info = ibim_solve_info{};
+        double worst = 0.0;
+        for (int col = 0; col < nr; ++col) {
+            info.iterations += iters[col];
+            worst = std::max(worst, errs[col]);
+            if (stops[col] && stops[col] != 4) info.status = stops[col];
+        }
+        info.outer = mixed ? replacements : 1;
+        info.rel_residual = worst;
+        ctx->last_iters = info.iterations;
+    }
     return IBIM_OK;
 }
 

@@ -119,6 +119,7 @@ struct Context {
     DevBuf xfer;    // receivers and other small transfers
     DevBuf refine;  // refinement vectors
     DevBuf batch;   // batched (16 right-hand sides) BiCG workspace
+    DevBuf group;   // ibim_solve_group with this context first: every member's workspace
     int last_iters = 0;
 
     // progress reports from long solves (at host checks, at most every few seconds)
@@ -196,6 +197,40 @@ int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 // product; kernels return at once while *stop is set.
 int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const double2* Q, double2* Y, double2* Z,
                      const int* stop);
+// The near-field part of pair_batch_apply alone (added to Y and Z after the far
+// field; both products when Y and Z are given).
+int pair_batch_near(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z, const int* stop);
+
+// One member of a grouped 16-column pass: one pair-stored operator and one
+// batch of its right-hand sides (device-side table entry).
+struct GroupMember {
+    const void* F;       // pair storage factors (complex64 or complex128, the same for every member)
+    const int* blk;      // first tile of each row block
+    const void* pts;     // point table
+    const double2* P;    // 16-column interleaved vectors in
+    const double2* Q;
+    double2* Y;          // A P and A^H Q out (far field)
+    double2* Z;
+    const int* stop;     // the batch's all-done flag: its items are skipped once set
+    ibim_params prm;
+    int count;           // points of the operator
+    int item0;           // first work item (64-row block) of the member
+};
+// Factor sources a group pass can read: complex64 pairs with FP64 or FP32
+// products, complex128 pairs.
+enum GroupSource { GROUP_C64 = 0, GROUP_C64_FP32 = 1, GROUP_C128 = 2 };
+// Checks that the contexts can share one pass (pair storage, one device, one
+// pair kernel, one factor source; mixed: complex64 pairs) and returns the source.
+int pair_group_source(Context* const* ctxs, int n, bool mixed, int* source);
+// The far field of every member in one persistent launch on lead's stream:
+// workgroups take 64-row work items from the queue counter (reset first) in
+// item order; dev_members is a device copy of the table.
+int pair_group_pass(Context* lead, int source, const GroupMember* dev_members, int nmembers, int nitems,
+                    int* queue);
+// Batched BiCG (IBIM_SOLVE_BICG) or reliable BiCG (IBIM_SOLVE_RELIABLE) of the
+// right-hand sides of several contexts together (ibim_batch.hip).
+int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* opts, ibim_solve_info* infos);
+
 // The same products of the matrix-free operators (traditional tube): each
 // factor computed where the complex128 pair storage would read it -- FP64
 // (that storage's bits) or, with fast, the mixed-precision factors. Builds the

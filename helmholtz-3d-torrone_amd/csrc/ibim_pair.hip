@@ -666,25 +666,25 @@ static_assert(kPR % 64 == 0 && kPC % 64 == 0, "a workgroup's rows share their ti
 // Factors other than kStoredFactors: F unused; every visit evaluates F of the
 // pair it would have read (kExactFactors: the complex128 pair storage's bits,
 // the FP64 matrix-free operator; kFastFactors: the mixed-precision one).
+// The rows rowW .. rowW + 16 Waves - 1 of the pass (one workgroup's share; the
+// body of k_pair_batch and of the grouped persistent pass).
 template <class MT, int Kern, bool WithY, bool WithZ, int Waves, int Factors>
-__global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
-                                                           const PairPoint* __restrict__ pts, int count,
-                                                           ibim_params prm, const double2* __restrict__ P,
-                                                           const double2* __restrict__ Q, double2* __restrict__ Y,
-                                                           double2* __restrict__ Z, const int* __restrict__ stop) {
+__device__ __forceinline__ void pair_batch_rows(const MT* __restrict__ F, const int* __restrict__ blk_start,
+                                                const PairPoint* __restrict__ pts, int count, const ibim_params& prm,
+                                                const double2* __restrict__ P, const double2* __restrict__ Q,
+                                                double2* __restrict__ Y, double2* __restrict__ Z, int rowW) {
     constexpr bool Computed = Factors != kStoredFactors;
     constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
     constexpr int R = kBatchRHS;
-    constexpr int kBatchRows = 16 * Waves, kThreads = 64 * Waves;
-    if (stop != nullptr && *stop) return;
+    constexpr int kThreads = 64 * Waves;
     __shared__ double2 ps[WithY ? kBatchK : 1][R];
     __shared__ double2 qs[WithZ ? kBatchK : 1][R];
     __shared__ PairPoint sp[kBatchK];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int li = lane & 15, lk = lane >> 4;
-    const int rowW = blockIdx.x * kBatchRows;  // < count
+    // rowW < count
     const int I = rowW / kPR;
     const int i0 = rowW + 16 * w;
     const int i = i0 + li;  // < count + kPC: inside the padded point table
@@ -826,6 +826,17 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
             if (WithZ) Z[(size_t)r * R + li] = make_double2(z.re(v), z.im(v));
         }
     }
+}
+
+template <class MT, int Kern, bool WithY, bool WithZ, int Waves, int Factors>
+__global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict__ F, const int* __restrict__ blk_start,
+                                                           const PairPoint* __restrict__ pts, int count,
+                                                           ibim_params prm, const double2* __restrict__ P,
+                                                           const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                           double2* __restrict__ Z, const int* __restrict__ stop) {
+    if (stop != nullptr && *stop) return;
+    pair_batch_rows<MT, Kern, WithY, WithZ, Waves, Factors>(F, blk_start, pts, count, prm, P, Q, Y, Z,
+                                                            blockIdx.x * 16 * Waves);
 }
 
 // Near-field entries of the batched products (one thread per (row, RHS)),
@@ -1052,19 +1063,19 @@ __device__ __forceinline__ f4 ld16(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 // are staged as FP32 Gauss triples (pr, pi - pr, pr + pi) and (qr, qi - qr,
 // -(qr + qi)), so a k-step spends no vector instruction on the vector side of
 // the three real products.
+// Rows rowW .. rowW + 63 (one 256-thread workgroup's share; the body of
+// k_pair_batch32 and of the grouped persistent pass).
 template <int Kern, bool WithY, bool WithZ>
-__global__ void __launch_bounds__(256, 2) k_pair_batch32(const float2* __restrict__ F,
-                                                         const int* __restrict__ blk_start,
-                                                         const PairPoint* __restrict__ pts, int count,
-                                                         ibim_params prm, const double2* __restrict__ P,
-                                                         const double2* __restrict__ Q, double2* __restrict__ Y,
-                                                         double2* __restrict__ Z, const int* __restrict__ stop) {
+__device__ __forceinline__ void pair_batch32_rows(const float2* __restrict__ F, const int* __restrict__ blk_start,
+                                                  const PairPoint* __restrict__ pts, int count,
+                                                  const ibim_params& prm, const double2* __restrict__ P,
+                                                  const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                  double2* __restrict__ Z, int rowW) {
     constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     constexpr int R = kBatchRHS, KC = kBatchK, KS = kBatchK / 4;
     constexpr int kStrip = kPR / KC, kOwn = kPC / KC;  // chunks per strip of a block, per own tile
     static_assert(R == 16 && KS == 16 && kStrip == 8 && kOwn == 16, "chunk indexing below");
     static_assert(KC * R == 4 * 256 && KC * 8 == 2 * 256, "staging: 4 vector and 2 point values per thread");
-    if (stop != nullptr && *stop) return;
     __shared__ f4 ps[WithY ? 2 : 1][WithY ? KC : 1][R];
     __shared__ f4 qs[WithZ ? 2 : 1][WithZ ? KC : 1][R];
     __shared__ double spd[2][KC][8];  // column points: x, y, z as staged (the rest unused)
@@ -1072,7 +1083,7 @@ __global__ void __launch_bounds__(256, 2) k_pair_batch32(const float2* __restric
     const int tid = threadIdx.x, lane = tid & 63;
     const int w = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int li = lane & 15, lk = lane >> 4;
-    const int rowW = blockIdx.x * 64;  // < count
+    // rowW < count
     const int I = rowW / kPR;
     const int i0 = rowW + 16 * w;
     const int i = i0 + li;  // < count + kPC: inside the padded point table
@@ -1279,6 +1290,17 @@ __global__ void __launch_bounds__(256, 2) k_pair_batch32(const float2* __restric
     }
 }
 
+template <int Kern, bool WithY, bool WithZ>
+__global__ void __launch_bounds__(256, 2) k_pair_batch32(const float2* __restrict__ F,
+                                                         const int* __restrict__ blk_start,
+                                                         const PairPoint* __restrict__ pts, int count,
+                                                         ibim_params prm, const double2* __restrict__ P,
+                                                         const double2* __restrict__ Q, double2* __restrict__ Y,
+                                                         double2* __restrict__ Z, const int* __restrict__ stop) {
+    if (stop != nullptr && *stop) return;
+    pair_batch32_rows<Kern, WithY, WithZ>(F, blk_start, pts, count, prm, P, Q, Y, Z, blockIdx.x * 64);
+}
+
 // Calls fn(std::integral_constant<int, K>) with the context's pair kernel K.
 template <class Fn>
 void with_kernel(const Context* ctx, Fn&& fn) {
@@ -1328,6 +1350,61 @@ void launch_pair_batch32_any(Context* ctx, const float2* F, const double2* P, co
         launch_pair_batch32<Kern, true, false>(ctx, F, P, Q, Y, Z, stop);
     else
         launch_pair_batch32<Kern, false, true>(ctx, F, P, Q, Y, Z, stop);
+}
+
+// ------------------------------------------------ several operators, one launch --
+// The far-field 16-column passes (A P and A^H Q) of several pair-stored
+// operators in one persistent launch. A member is one operator with one batch
+// of its right-hand sides; its work items are its 64-row blocks, numbered member
+// after member (the caller puts the largest operators first). Each workgroup
+// takes the next item from the queue counter until none is left, so the
+// uneven members fill the GPU together and the longest items start first. An
+// item runs the rows of k_pair_batch (64 rows per workgroup) or k_pair_batch32,
+// so every member gets the bits of its own pass; items of a member whose stop
+// flag is set are skipped.
+__global__ void k_group_reset(int* queue) {
+    if (threadIdx.x == 0) *queue = 0;
+}
+
+template <class MT, int Kern, bool FP32>
+__global__ void __launch_bounds__(256, FP32 ? 2 : 1) k_pair_batch_group(const GroupMember* __restrict__ mem,
+                                                                        int nmem, int nitems,
+                                                                        int* __restrict__ queue) {
+    __shared__ int next;
+    for (;;) {
+        __syncthreads();  // the previous item's LDS reads and `next` are done
+        if (threadIdx.x == 0) next = atomicAdd(queue, 1);
+        __syncthreads();
+        const int item = __builtin_amdgcn_readfirstlane(next);
+        if (item >= nitems) return;
+        int m = 0;
+        while (m + 1 < nmem && mem[m + 1].item0 <= item) ++m;
+        const GroupMember& g = mem[m];
+        if (g.stop != nullptr && *g.stop) continue;
+        const ibim_params prm = g.prm;
+        const int rowW = (item - g.item0) * 64;
+        if constexpr (FP32)
+            pair_batch32_rows<Kern, true, true>(static_cast<const float2*>(g.F), g.blk,
+                                                static_cast<const PairPoint*>(g.pts), g.count, prm, g.P, g.Q, g.Y,
+                                                g.Z, rowW);
+        else
+            pair_batch_rows<MT, Kern, true, true, 4, kStoredFactors>(static_cast<const MT*>(g.F), g.blk,
+                                                                     static_cast<const PairPoint*>(g.pts), g.count,
+                                                                     prm, g.P, g.Q, g.Y, g.Z, rowW);
+    }
+}
+
+template <class MT, int Kern, bool FP32>
+int launch_group(Context* lead, const GroupMember* mem, int nmem, int nitems, int* queue) {
+    auto kern = k_pair_batch_group<MT, Kern, FP32>;
+    int per_cu = 0, cus = 0;
+    IBIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
+    IBIM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lead->device));
+    const int grid = std::max(1, std::min(nitems, std::max(per_cu, 1) * cus));
+    hipLaunchKernelGGL(k_group_reset, dim3(1), dim3(64), 0, lead->stream, queue);
+    TimedScope ts(lead, "batch_group");
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(256), 0, lead->stream, mem, nmem, nitems, queue);
+    return IBIM_OK;
 }
 
 int near_lists(Context* ctx, dim3 grid);
@@ -1560,6 +1637,69 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
         else
             launch_pair_batch_any<float2, Kern>(ctx, 0, ctx->mat.as<float2>(), P, Q, Y, Z, stop);
     });
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int pair_batch_near(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z, const int* stop) {
+    if (!ctx->pair_aux || !Y || !Z || !P || !Q) {
+        set_error("batched near field without the near lists or vectors it needs");
+        return IBIM_ERR_ARG;
+    }
+    double nr, ni;
+    near_term(ctx, &nr, &ni);
+    with_kernel(ctx, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        hipLaunchKernelGGL((k_pair_near_batch<Kern, true, true>),
+                           dim3((unsigned)(((size_t)ctx->count * kBatchRHS + 255) / 256)), dim3(256), 0,
+                           ctx->stream, ctx->tube.as<double>(), ctx->count, ctx->near_off.as<int>(),
+                           ctx->near_idx.as<int>(), P, Q, Y, Z, stop, nr, ni);
+    });
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+int pair_group_source(Context* const* ctxs, int n, bool mixed, int* source) {
+    const Context* a = ctxs[0];
+    for (int c = 0; c < n; ++c) {
+        const Context* b = ctxs[c];
+        if (!b->have_tube || !b->mat_pair || !b->pair_aux || b->count <= 0 || b->nrhs <= 0) {
+            set_error("ibim_solve_group: problem %d has no pair storage of its tube or no right-hand sides", c);
+            return IBIM_ERR_ARG;
+        }
+        if (b->device != a->device || pair_kernel(b) != pair_kernel(a) || b->mat_precision != a->mat_precision ||
+            b->fp32_products != a->fp32_products) {
+            set_error("ibim_solve_group: problem %d differs from problem 0 in device, kernel (Dirichlet / Neumann / "
+                      "combined), pair precision or FP32 products", c);
+            return IBIM_ERR_ARG;
+        }
+        for (int d = 0; d < c; ++d)
+            if (ctxs[d] == b) {
+                set_error("ibim_solve_group: problem %d appears twice", c);
+                return IBIM_ERR_ARG;
+            }
+    }
+    if (mixed && (a->mat_precision != IBIM_C64 || !a->have_tube)) {
+        set_error("ibim_solve_group: the reliable solver runs on complex64 pairs (IBIM_C64_PAIR, IBIM_MIXED_PAIR)");
+        return IBIM_ERR_ARG;
+    }
+    *source = a->mat_precision == IBIM_C128 ? GROUP_C128 : a->fp32_products ? GROUP_C64_FP32 : GROUP_C64;
+    return IBIM_OK;
+}
+
+int pair_group_pass(Context* lead, int source, const GroupMember* dev_members, int nmembers, int nitems,
+                    int* queue) {
+    int rc = IBIM_OK;
+    with_kernel(lead, [&](auto K) {
+        constexpr int Kern = decltype(K)::value;
+        if (source == GROUP_C128)
+            rc = launch_group<double2, Kern, false>(lead, dev_members, nmembers, nitems, queue);
+        else if (source == GROUP_C64_FP32)
+            rc = launch_group<float2, Kern, true>(lead, dev_members, nmembers, nitems, queue);
+        else
+            rc = launch_group<float2, Kern, false>(lead, dev_members, nmembers, nitems, queue);
+    });
+    if (rc) return rc;
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

@@ -111,6 +111,7 @@ def lib():
             "ibim_set_nrhs": ([vp, I], I), "ibim_rhs": ([vp, I, P, D, P, I], I),
             "ibim_set_rhs_host": ([vp, I, P, P], I), "ibim_get_rhs": ([vp, I, P, P], I),
             "ibim_solve": ([vp, C.POINTER(SolveOpts), C.POINTER(SolveInfo), P], I),
+            "ibim_solve_group": ([C.POINTER(vp), I, C.POINTER(SolveOpts), C.POINTER(SolveInfo)], I),
             "ibim_get_density": ([vp, I, P, P], I), "ibim_set_density_host": ([vp, I, P, P], I),
             "ibim_evaluate": ([vp, I, I, I, P, P, P], I),
             "ibim_set_receivers": ([vp, I, P, P, P], I),
@@ -388,6 +389,18 @@ class Problem:
 # ---------------------------------------------------------------------------
 # The reference driver's text files.
 # ---------------------------------------------------------------------------
+def solve_group(problems, method=SOLVE_BICG, maxit=1000, tol=1e-7, inner_tol=1e-4, chunk=0):
+    """Every right-hand side of several problems solved together
+    (ibim_solve_group: one persistent launch per iteration for all of their
+    operators). Returns one SolveInfo per problem."""
+    n = len(problems)
+    hs = (C.c_void_p * n)(*[p.h for p in problems])
+    infos = (SolveInfo * n)()
+    o = SolveOpts(method, maxit, tol, inner_tol, 20, chunk, 0)
+    _check(lib().ibim_solve_group(hs, n, C.byref(o), infos), "ibim_solve_group")
+    return list(infos)
+
+
 def reference_names(n, k):
     """File names the reference driver uses for grid size n and wavenumber k
     (large3Dexample2.c:407, 607-608, 3001-3002)."""

@@ -244,6 +244,21 @@ int ibim_set_rhs_host(ibim_ctx *ctx, int col, const double *bre, const double *b
 int ibim_get_rhs(ibim_ctx *ctx, int col, double *bre, double *bim);
 int ibim_solve(ibim_ctx *ctx, const ibim_solve_opts *opts, ibim_solve_info *info, double *hist);
 int ibim_get_density(ibim_ctx *ctx, int col, double *re, double *im);
+/* Every right-hand side of nctx problems solved together, 16 columns per
+ * batch, by opts->method IBIM_SOLVE_BICG or IBIM_SOLVE_RELIABLE: each BiCG
+ * iteration applies the operators of all running batches in ONE persistent
+ * launch whose workgroups take 64-row blocks from a queue (largest problems
+ * first), then runs each batch's near field and vector updates. The problems
+ * need pair storage of their tubes (IBIM_C128_PAIR, IBIM_C64_PAIR or
+ * IBIM_MIXED_PAIR; the reliable solver: complex64 pairs), one device, the same
+ * kernel (Dirichlet, Neumann or combined) and ibim_set_fp32_products setting.
+ * Each problem's densities and infos[i] are what its own batched ibim_solve
+ * with the same opts gives (host checks every opts->chunk iterations; 0: 2
+ * for the reliable solver, and for BiCG 2 when the largest problem has more
+ * than 20,000 points, else 16). The work runs on ctxs[0]'s stream; no other
+ * thread may use the handles meanwhile. No counterpart in the reference (one
+ * surface and one right-hand side per run). */
+int ibim_solve_group(ibim_ctx *const *ctxs, int nctx, const ibim_solve_opts *opts, ibim_solve_info *infos);
 int ibim_set_density_host(ibim_ctx *ctx, int col, const double *re, const double *im);
 
 /* ------------------------------------------------------------ evaluation --- */

@@ -278,3 +278,78 @@ def test_contexts_solve_concurrently_from_host_threads(gpu):
         assert i0 == i1
         for c in range(16):
             assert np.array_equal(x0[c][0], x1[c][0]) and np.array_equal(x0[c][1], x1[c][1])
+
+
+def sphere_problem(gpu, n, store, fp32, dirichlet=1, rhs=16):
+    pb = gpu.Problem()
+    pb.set_level(shapes.sphere(n))
+    pb.geometry(5)
+    h = 4.0 / n
+    pb.tube(gpu.params(4.0, 3 * h, 2 * h, dirichlet=dirichlet))
+    pb.assemble(getattr(gpu, STORES[store]))
+    if fp32:
+        pb.set_fp32_products(True)
+    fill_rhs(gpu, pb, rhs)
+    return pb
+
+
+@pytest.mark.parametrize("store,method,fp32", [("c128_pair", "SOLVE_BICG", False),
+                                               ("c64_pair", "SOLVE_BICG", True),
+                                               ("mixed_pair", "SOLVE_RELIABLE", False),
+                                               ("mixed_pair", "SOLVE_RELIABLE", True),
+                                               ("c64_pair", "SOLVE_RELIABLE", True)])
+def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
+    """Several problems of uneven sizes solved together (one persistent launch
+    of every operator per iteration, a problem with two batches among them):
+    each problem's densities, iteration count and status are those of its own
+    batched solve, bit for bit; the operators run only in the grouped launch."""
+    m = getattr(gpu, method)
+    sizes, nrhs = (20, 28, 24), (16, 19, 5)
+    kw = dict(maxit=500, tol=1e-10, inner_tol=1e-2, chunk=2)
+    own = []
+    for n, r in zip(sizes, nrhs):
+        pb = sphere_problem(gpu, n, store, fp32, rhs=r)
+        info, _ = pb.solve(m, batch=0, **kw)
+        own.append((info, [pb.density(c) for c in range(r)]))
+        pb.close()
+    pbs = [sphere_problem(gpu, n, store, fp32, rhs=r) for n, r in zip(sizes, nrhs)]
+    for pb in pbs:
+        pb.reset_timers()
+        pb.set_profiling(True)
+    infos = gpu.solve_group(pbs, m, **kw)
+    for pb in pbs:
+        pb.sync()
+    launches = pbs[0].kernel_time("batch_group")[1]
+    for pb, info, (info0, x0), r in zip(pbs, infos, own, nrhs):
+        assert info.status == 0 and info.status == info0.status
+        assert info.iterations == info0.iterations and info.outer == info0.outer
+        assert info.rel_residual == info0.rel_residual
+        for c in range(r):
+            x = pb.density(c)
+            assert np.array_equal(x[0], x0[c][0]) and np.array_equal(x[1], x0[c][1]), c
+        # the single-problem passes did not run (the replacements' FP64 passes aside)
+        assert pb.kernel_time("batch_pair")[1] == 0 and pb.kernel_time("batch_pair32")[1] == 0
+        pb.set_profiling(False)
+    longest = max(i.iterations for i in infos)  # bounds every column's own count
+    assert 0 < launches <= longest + 2
+    for pb in pbs:
+        pb.close()
+
+
+def test_group_solve_refuses_what_cannot_share_a_launch(gpu):
+    a = sphere_problem(gpu, 20, "c128_pair", False)
+    b = sphere_problem(gpu, 24, "c128_pair", False, dirichlet=0)
+    with pytest.raises(gpu.IbimError):
+        gpu.solve_group([a, b], gpu.SOLVE_BICG)  # Dirichlet and Neumann kernels
+    c = sphere_problem(gpu, 24, "c64_pair", False)
+    with pytest.raises(gpu.IbimError):
+        gpu.solve_group([a, c], gpu.SOLVE_BICG)  # complex128 and complex64 pairs
+    with pytest.raises(gpu.IbimError):
+        gpu.solve_group([a], gpu.SOLVE_RELIABLE)  # the reliable solver needs complex64 pairs
+    with pytest.raises(gpu.IbimError):
+        gpu.solve_group([a, a], gpu.SOLVE_BICG)  # one problem twice
+    d = sphere_problem(gpu, 24, "c128", False)
+    with pytest.raises(gpu.IbimError):
+        gpu.solve_group([a, d], gpu.SOLVE_BICG)  # dense storage
+    for pb in (a, b, c, d):
+        pb.close()
