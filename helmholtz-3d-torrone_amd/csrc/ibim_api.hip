@@ -213,6 +213,15 @@ int ibim_set_level_shape(ibim_ctx* ctx, int n, double lo, double h, const ibim_s
     return launch_sdf(ctx, shape);
 }
 
+int ibim_redistance(ibim_ctx* ctx, double band) {
+    CHECK(ctx);
+    if (!ctx->n || !(band >= 1.0 && band <= 1e6)) {
+        set_error("ibim_redistance: needs a level set and a band of at least one grid spacing");
+        return IBIM_ERR_ARG;
+    }
+    return launch_redistance(ctx, band);
+}
+
 int ibim_geometry(ibim_ctx* ctx, int weno_order) {
     CHECK(ctx);
     if (!ctx->n) {

@@ -180,6 +180,8 @@ struct TimedScope {
 void report_progress(Context* ctx, const char* what, int iteration, double residual);
 
 int launch_sdf(Context* ctx, const ibim_shape* shape);
+// Level set -> signed distance to its zero set within band h (ibim_sdf.hip).
+int launch_redistance(Context* ctx, double band);
 int launch_gradient(Context* ctx, int order);
 int launch_curvatures(Context* ctx);
 int launch_tube(Context* ctx, int* count_out);

@@ -10,6 +10,7 @@
 // to Gauss-Newton steps when the Hessian is not positive definite. The sign
 // comes from the star-shaped / quadric inside test.
 #include <cmath>
+#include <utility>
 #include <vector>
 
 #include "ibim_internal.hpp"
@@ -200,7 +201,148 @@ __global__ void __launch_bounds__(kThreads) k_sdf(ibim_shape sh, int n, double l
     level[id] = d;
 }
 
+// ------------------------------------------------------------ redistancing --
+// A level set that is not a distance function (|grad| != 1) is replaced by the
+// signed distance to its own zero level set. The zero set is that of the
+// Catmull-Rom tricubic interpolant of the grid values (C1, exact for
+// quadratics: O(h^3) from a smooth level set). For a node x the closest point y
+// comes from projections y <- y - phi(y) grad / |grad|^2 onto the zero set, then
+// Chopp's iteration (SIAM J. Sci. Comput. 23 (2001) 230-244, section 3), which
+// adds to each projection the component of x - y tangent to the zero set, until
+// x - y is normal to it; d = sign(phi(x)) |x - y|.
+
+// Catmull-Rom weights of the 4 nodes around t in [0, 1] and their derivatives.
+__device__ __forceinline__ void cr_weights(double t, double (&w)[4], double (&dw)[4]) {
+    const double t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5 * (-t + 2.0 * t2 - t3);
+    w[1] = 0.5 * (2.0 - 5.0 * t2 + 3.0 * t3);
+    w[2] = 0.5 * (t + 4.0 * t2 - 3.0 * t3);
+    w[3] = 0.5 * (t3 - t2);
+    dw[0] = 0.5 * (-1.0 + 4.0 * t - 3.0 * t2);
+    dw[1] = 0.5 * (-10.0 * t + 9.0 * t2);
+    dw[2] = 0.5 * (1.0 + 8.0 * t - 9.0 * t2);
+    dw[3] = 0.5 * (3.0 * t2 - 2.0 * t);
+}
+
+// The interpolant and its gradient at y (x, y, z); the grid is raster (i, j, k)
+// = (y, x, z) as the level set, stencils clamped at the faces.
+__device__ double tricubic(const double* __restrict__ f, int S, double lo, double h, const double (&y)[3],
+                           double (&g)[3]) {
+    double w[3][4], dw[3][4];
+    int c[3];
+    for (int q = 0; q < 3; ++q) {
+        const double s = (y[q] - lo) / h;
+        const int ci = (int)fmin(fmax(floor(s), 0.0), (double)(S - 2));
+        c[q] = ci;
+        cr_weights(s - ci, w[q], dw[q]);
+    }
+    double v = 0.0, gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int b = 0; b < 4; ++b) {
+        const int iy = min(max(c[1] - 1 + b, 0), S - 1);
+        for (int a = 0; a < 4; ++a) {
+            const int jx = min(max(c[0] - 1 + a, 0), S - 1);
+            const double* row = f + ((size_t)iy * S + jx) * S;
+            double r = 0.0, rz = 0.0;
+            for (int e = 0; e < 4; ++e) {
+                const double val = row[min(max(c[2] - 1 + e, 0), S - 1)];
+                r += w[2][e] * val;
+                rz += dw[2][e] * val;
+            }
+            v += w[0][a] * w[1][b] * r;
+            gx += dw[0][a] * w[1][b] * r;
+            gy += w[0][a] * dw[1][b] * r;
+            gz += w[0][a] * w[1][b] * rz;
+        }
+    }
+    g[0] = gx / h;
+    g[1] = gy / h;
+    g[2] = gz / h;
+    return v;
+}
+
+__global__ void __launch_bounds__(kThreads) k_redistance(const double* __restrict__ f, int n, double lo, double h,
+                                                         double band, double* __restrict__ out) {
+    const int S = n + 1;
+    const size_t total = (size_t)S * S * S;
+    const size_t id = (size_t)blockIdx.x * kThreads + threadIdx.x;
+    if (id >= total) return;
+    const int k = (int)(id % S);
+    const size_t t = id / S;
+    const int j = (int)(t % S), i = (int)(t / S);
+    const double x[3] = {lo + h * j, lo + h * i, lo + h * k};
+    const double f0 = f[id];
+    const double sgn = f0 > 0.0 ? 1.0 : (f0 < 0.0 ? -1.0 : 0.0);
+    const double far = band * h;
+    double g[3];
+    (void)tricubic(f, S, lo, h, x, g);
+    const double gn = sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    // first-order distance estimate; nodes clearly beyond the band keep +-band h
+    if (!(gn > 0.0) || fabs(f0) > 1.5 * far * gn) {
+        out[id] = sgn * far;
+        return;
+    }
+    double y[3] = {x[0], x[1], x[2]};
+    bool ok = true;
+    for (int it = 0; it < 30; ++it) {  // onto the zero set
+        const double p = tricubic(f, S, lo, h, y, g);
+        const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        if (!(gg > 0.0)) {
+            ok = false;
+            break;
+        }
+        const double c = -p / gg;
+        y[0] += c * g[0];
+        y[1] += c * g[1];
+        y[2] += c * g[2];
+        if (fabs(c) * sqrt(gg) < 1e-12 * h) break;
+    }
+    for (int it = 0; ok && it < 100; ++it) {  // and to the foot of the normal through x
+        const double p = tricubic(f, S, lo, h, y, g);
+        const double gg = g[0] * g[0] + g[1] * g[1] + g[2] * g[2];
+        if (!(gg > 0.0)) {
+            ok = false;
+            break;
+        }
+        const double v[3] = {x[0] - y[0], x[1] - y[1], x[2] - y[2]};
+        const double vg = (v[0] * g[0] + v[1] * g[1] + v[2] * g[2]) / gg, c = -p / gg;
+        double step2 = 0.0;
+        for (int q = 0; q < 3; ++q) {
+            const double st = c * g[q] + v[q] - vg * g[q];
+            y[q] += st;
+            step2 += st * st;
+        }
+        if (step2 < 1e-20 * h * h) break;
+    }
+    // a closest point counts only on the zero set and inside the grid (beyond
+    // its faces the clamped interpolant extrapolates and can have spurious zeros)
+    const double hi = lo + h * n;
+    for (int q = 0; q < 3; ++q) ok = ok && y[q] >= lo && y[q] <= hi;
+    if (ok) {
+        const double p = tricubic(f, S, lo, h, y, g);
+        ok = fabs(p) <= 1e-6 * h * sqrt(g[0] * g[0] + g[1] * g[1] + g[2] * g[2]);
+    }
+    const double dx = x[0] - y[0], dy = x[1] - y[1], dz = x[2] - y[2];
+    const double d = sqrt(dx * dx + dy * dy + dz * dz);
+    // a failed search: the first-order estimate, or +-band h from a node whose
+    // estimate already lies beyond half the band
+    const double est = fabs(f0) / gn;
+    out[id] = ok && d == d ? sgn * fmin(d, far) : sgn * (est > 0.5 * far ? far : est);
+}
+
 }  // namespace
+
+int launch_redistance(Context* ctx, double band) {
+    const size_t total = (size_t)(ctx->n + 1) * (ctx->n + 1) * (ctx->n + 1);
+    // the mean-curvature grid is the scratch output; geometry recomputes it
+    TimedScope ts(ctx, "redistance");
+    hipLaunchKernelGGL(k_redistance, dim3((unsigned)((total + kThreads - 1) / kThreads)), dim3(kThreads), 0,
+                       ctx->stream, ctx->level.as<double>(), ctx->n, ctx->lo, ctx->h, band, ctx->mc.as<double>());
+    IBIM_HIP(hipGetLastError());
+    std::swap(ctx->level, ctx->mc);
+    ctx->have_geometry = false;
+    ctx->weno_order = 0;
+    return IBIM_OK;
+}
 
 int launch_sdf(Context* ctx, const ibim_shape* sh) {
     const size_t total = (size_t)(ctx->n + 1) * (ctx->n + 1) * (ctx->n + 1);

@@ -99,7 +99,7 @@ def lib():
             "ibim_create": ([C.POINTER(vp)], I), "ibim_destroy": ([vp], None), "ibim_sync": ([vp], I),
             "ibim_set_level_host": ([vp, I, D, D, P], I), "ibim_set_level_device": ([vp, I, D, D, vp], I),
             "ibim_set_level_shape": ([vp, I, D, D, C.POINTER(Shape)], I),
-            "ibim_geometry": ([vp, I], I), "ibim_get_level": ([vp, P], I), "ibim_get_grid": ([vp, P, P, P, P, P], I),
+            "ibim_geometry": ([vp, I], I), "ibim_redistance": ([vp, D], I), "ibim_get_level": ([vp, P], I), "ibim_get_grid": ([vp, P, P, P, P, P], I),
             "ibim_set_grid_fields_host": ([vp, P, P, P, P, P], I),
             "ibim_tube": ([vp, C.POINTER(Params), I, IP], I), "ibim_get_tube": ([vp, IP, P], I),
             "ibim_tube_device": ([vp], vp),
@@ -205,6 +205,11 @@ class Problem:
         out = np.zeros((S, S, S))
         _check(lib().ibim_get_level(self.h, _p(out)), "ibim_get_level")
         return out
+
+    def redistance(self, band=8.0):
+        """Level set -> signed distance to its zero set within band grid spacings."""
+        _check(lib().ibim_redistance(self.h, band), "ibim_redistance")
+        return self
 
     def geometry(self, weno_order=5):
         _check(lib().ibim_geometry(self.h, weno_order), "ibim_geometry")

@@ -167,6 +167,17 @@ typedef struct ibim_shape {
     double a, b, c;
 } ibim_shape;
 int ibim_set_level_shape(ibim_ctx *ctx, int n, double lo, double h, const ibim_shape *shape);
+/* Replaces the grid's level set, which need not be a distance function
+ * (|grad| != 1), by the signed distance to its own zero level set (the zero
+ * set of its Catmull-Rom tricubic interpolant; closest points by projection
+ * and Chopp's iteration), keeping its sign; nodes farther than band grid
+ * spacings get +-band h. The IBIM formulas assume a distance function
+ * (x* = x - d grad d): run this before ibim_geometry on any other level set,
+ * with band above the tube half-width plus the 3-node WENO stencil. No
+ * counterpart in the reference, which reads a precomputed distance file
+ * (large3Dexample2.c:405-424) and leaves its Godunov Hamiltonian
+ * (finite_difference3D.c:5-27) unused. */
+int ibim_redistance(ibim_ctx *ctx, double band);
 /* WENO-3/5 Godunov gradient and curvatures on every node
  * (ASSIGN_GRADIENT distance_application3D.c:56-141; BothCurvs :35-54). */
 int ibim_geometry(ibim_ctx *ctx, int weno_order);
