@@ -65,6 +65,10 @@ BATCH_REPLACE = 1e-3  # the same for the batched solver (at 1e-1 its torus128 co
 NRECV = 500
 STORE_MAX_BYTES = 200e9  # store an operator in HBM up to this size (288 GB per GPU)
 MATRIX_FREE = False  # --matrix-free: both timed paths recompute the operator per sweep
+# the 256^3 star's FP64 path keeps no complex128 pairs by default: caching the
+# first row blocks' pairs (249 of 491 GB) made its pass slower, 492.6 against
+# 478.8 ms (reading complex128 pairs at both visits costs what computing them does)
+STAR_FP64_CACHE_BYTES = 0.0
 STAR_STORE_MAX_BYTES = 260e9  # the 256^3 star's complex64 pairs (~251 GB) alone beside its vectors
 # star256 BiCG tolerance: a decade below the 1e-6 asked of the densities (at
 # tol 1e-6 the reference's stagnation rule, |change| < 1e-3 tol, stopped BiCG
@@ -78,6 +82,18 @@ def pair_bytes(count, esz):
     values from the diagonal block to the last column."""
     tiles = sum((count - b * PAIR_ROWS + PAIR_COLS - 1) // PAIR_COLS for b in range((count + PAIR_ROWS - 1) // PAIR_ROWS))
     return tiles * PAIR_ROWS * PAIR_COLS * esz
+
+
+def cache_bytes(count, budget):
+    """Bytes of complex128 pair tiles the FP64 matrix-free cache keeps for a
+    budget: whole row blocks of 512 rows, 512 x 1024-value tiles, from the first."""
+    tile, total = 512 * 1024 * 16, 0
+    for blk in range((count + 511) // 512):
+        t = (count - 512 * blk + 1023) // 1024
+        if (total + t) * tile > budget:
+            break
+        total += t
+    return total * tile
 
 
 def level_of(w):
@@ -447,6 +463,12 @@ def run_star(args):
             except ibim.IbimError as e:
                 print("[bench] star256: complex64 pairs not stored (%s); matrix-free" % e, file=sys.stderr,
                       flush=True)
+        if path == "fp64" and args.star_fp64_cache > 0:
+            # the cache of complex128 pairs: allocated here, untimed (as the
+            # mixed path's complex64 pairs), filled again inside the timed solve
+            pb.set_free_cache(args.star_fp64_cache)
+            pb.apply_batch(np.zeros((count, 1)), ibim.OP_FREE64)
+            pb.set_free_cache(args.star_fp64_cache)
         pb.sync()
         pb.reset_timers()
         pb.set_profiling(True)
@@ -473,7 +495,8 @@ def run_star(args):
         d = [pb.density(c) for c in range(m)]
         dens[path] = [a + 1j * b for a, b in d]
         out[path] = dict(seconds=el, info=info, count=count, setup=t_setup, pass_ms=ms / max(nl, 1), passes=nl,
-                         assemble_ms=ms_asm, replacement_passes=n_rep)
+                         assemble_ms=ms_asm, replacement_passes=n_rep,
+                         cache_bytes=cache_bytes(count, args.star_fp64_cache) if path == "fp64" else 0)
         print("[bench] star256 %s: N_t=%d setup %.2f s, %d iterations (budget %d) %.1f s, residual %.2e, status %d, "
               "%d passes of %.1f ms" % (path, count, t_setup, info.iterations, its, el, info.rel_residual,
                                          info.status, nl, ms / max(nl, 1)), file=sys.stderr, flush=True)
@@ -514,7 +537,10 @@ def run_star(args):
                                      "double layer 1/2 I + K", its),
                      "tube_points": out["mixed"]["count"], "parallelism": "single GPU",
                      "mixed_operator": mixed_operator,
-                     "fp64_operator": "matrix-free (fused pair pass, factors computed)"},
+                     "fp64_operator": "matrix-free (fused pair pass, factors computed)%s" % (
+                         ", complex128 pairs of the first row blocks cached (%.0f GB of %.0f GB)" % (
+                             out["fp64"]["cache_bytes"] / 1e9, pair_bytes(out["fp64"]["count"], 16) / 1e9)
+                         if out.get("fp64", {}).get("cache_bytes") else "")},
           "fp64_rate": fp64_rate, "speedup_mixed_vs_fp64": value / fp64_rate,
           "mixed_rel_err_vs_fp64_density": rel_err,
           "seconds": {p: out[p]["seconds"] for p in out},
@@ -578,6 +604,8 @@ def main():
                     help="star256: time the mixed path alone (no FP64 path, no comparison)")
     ap.add_argument("--star-fp64-products", action="store_true",
                     help="star256 mixed path: 16-column passes on the FP64 matrix cores (default: FP32)")
+    ap.add_argument("--star-fp64-cache", type=float, default=STAR_FP64_CACHE_BYTES,
+                    help="star256 FP64 path: bytes of complex128 pairs the matrix-free operator keeps (0: none)")
     ap.add_argument("--star-replace", type=float, default=BATCH_REPLACE,
                     help="star256 mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--batch-fp32-products", action="store_true",

@@ -153,7 +153,7 @@ void ibim_destroy(ibim_ctx* ctx) {
                       &ctx->blockcnt, &ctx->err_flag, &ctx->mat, &ctx->mat64, &ctx->rhs, &ctx->density, &ctx->work, &ctx->partial,
                       &ctx->scal, &ctx->done, &ctx->xfer, &ctx->refine, &ctx->recv, &ctx->adj, &ctx->gadj, &ctx->batch, &ctx->group,
                       &ctx->pair_tiles, &ctx->pair_blk, &ctx->pair_pts, &ctx->near_off, &ctx->near_idx,
-                      &ctx->near_cur})
+                      &ctx->near_cur, &ctx->free_cache})
         b->release();
     release_staging(ctx);
     (void)hipStreamDestroy(ctx->stream);
@@ -182,7 +182,7 @@ static int set_grid(ibim_ctx* ctx, int n, double lo, double h) {
     ctx->count = 0;
     ctx->have_tube = false;
     ctx->mat_precision = -1;
-    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = ctx->free_cache_valid = false;
     return IBIM_OK;
 }
 
@@ -293,7 +293,7 @@ int ibim_tube(ibim_ctx* ctx, const ibim_params* prm, int kind, int* count) {
     ctx->prm = *prm;
     ctx->kind = kind;
     ctx->mat_precision = -1;
-    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = ctx->free_cache_valid = false;
     ctx->count = 0;  // a failed build leaves no stale tube behind
     ctx->have_tube = false;
     int c = 0;
@@ -363,7 +363,7 @@ int ibim_set_matrix_host(ibim_ctx* ctx, int count, const double* re, const doubl
     ctx->mat_precision = -1;  // not a valid operator until the upload completes
     if (int rc = upload_rows(ctx, src, precision)) return rc;
     ctx->mat_precision = precision;
-    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = ctx->free_cache_valid = false;
     return ibim_set_nrhs(ctx, ctx->nrhs > 0 ? ctx->nrhs : 1);
 }
 
@@ -595,6 +595,16 @@ int ibim_set_progress(ibim_ctx* ctx, ibim_progress_fn fn, void* user) {
 int ibim_set_profiling(ibim_ctx* ctx, int on) {
     CHECK(ctx);
     ctx->profile = on != 0;
+    return IBIM_OK;
+}
+
+int ibim_set_free_cache(ibim_ctx* ctx, double max_bytes) {
+    CHECK(ctx);
+    if (!(max_bytes >= 0.0) || max_bytes > 1e15) return IBIM_ERR_ARG;
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));  // no pass may still read the old cache
+    ctx->free_cache_max = (size_t)max_bytes;
+    ctx->free_cache_valid = false;
+    if (ctx->free_cache_max == 0) ctx->free_cache.release();
     return IBIM_OK;
 }
 

@@ -116,7 +116,7 @@ int assemble_fd(ibim_ctx* ctx, double sign, double diag, double thr2, int const_
                        ctx->mat.as<double2>(), ctx->ld);
     IBIM_HIP(hipGetLastError());
     ctx->mat_precision = IBIM_C128;
-    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = ctx->free_cache_valid = false;
     return IBIM_OK;
 }
 
@@ -176,7 +176,7 @@ int dropin_compbicg(int size, double* density, double* densityimg, double** K, d
     ctx->have_tube = false;  // a host operator without points
     ctx->ld = ((size_t)size + 63) / 64 * 64;
     ctx->mat_precision = -1;
-    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = false;
+    ctx->mat_pair = ctx->mat_pair64 = ctx->pair_aux = ctx->free_cache_valid = false;
     if (ctx->mat.ensure(sizeof(double2) * ctx->ld * (size_t)size)) return IBIM_ERR_NOMEM;
     HostRows src;
     src.re_rows = K;

@@ -95,6 +95,14 @@ struct Context {
     // mixed) and product kind, 0 until timed; reset with every assembly
     int batch_waves[15] = {};
 
+    // the FP64 matrix-free operator's cache (ibim_set_free_cache): complex128
+    // pairs of the first free_cached_blocks row blocks (free_cached_tiles
+    // tiles), at most free_cache_max bytes; valid for the current tube
+    size_t free_cache_max = 0;
+    DevBuf free_cache;
+    int free_cached_blocks = 0, free_cached_tiles = 0;
+    bool free_cache_valid = false;
+
     // 16-column passes over complex64 pairs multiply on the FP32 matrix cores
     // (ibim_set_fp32_products)
     bool fp32_products = false;

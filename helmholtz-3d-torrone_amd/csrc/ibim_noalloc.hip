@@ -80,6 +80,12 @@ extern "C" int ibim_apply_noalloc(ibim_ctx* ctx, int op, const void* p_dev, cons
                       "pair storage builds them)");
             return IBIM_ERR_ARG;
         }
+        // and so is the FP64 operator's cache (ibim_set_free_cache)
+        if (kop == OP_FREE64 && free_pair_op(ctx, kop) && ctx->free_cache_max > 0 && !ctx->free_cache_valid) {
+            set_error("ibim_apply_noalloc: the FP64 matrix-free cache is not built (one ibim_apply with OP_FREE64 "
+                      "builds it)");
+            return IBIM_ERR_ARG;
+        }
     }
     const size_t need = matvec_scratch_bytes(ctx, kop);
     if (need > kScratchBytes) {

@@ -359,7 +359,9 @@ __device__ __forceinline__ void combined_entries(double gr, double gi, double u,
 // storage's bits: the FP64 matrix-free operator) or with
 // grad_phi_coeffs_fast (the mixed-precision matrix-free operator); one
 // evaluation per pair either way.
-enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2 };
+// kCachedFactors: the FP64 matrix-free operator with the complex128 pairs of
+// its first row blocks kept in HBM (read where cached, computed elsewhere).
+enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2, kCachedFactors = 3 };
 
 // One tile of the pair sweep (see the file comment for the planes written).
 // Matrix-free sweeps are held to 2 waves per SIMD: the FP64 factor's registers
@@ -672,8 +674,10 @@ template <class MT, int Kern, bool WithY, bool WithZ, int Waves, int Factors>
 __device__ __forceinline__ void pair_batch_rows(const MT* __restrict__ F, const int* __restrict__ blk_start,
                                                 const PairPoint* __restrict__ pts, int count, const ibim_params& prm,
                                                 const double2* __restrict__ P, const double2* __restrict__ Q,
-                                                double2* __restrict__ Y, double2* __restrict__ Z, int rowW) {
+                                                double2* __restrict__ Y, double2* __restrict__ Z, int rowW,
+                                                int cached = 0) {
     constexpr bool Computed = Factors != kStoredFactors;
+    constexpr bool Exact = Factors == kExactFactors || Factors == kCachedFactors;
     constexpr bool Dirichlet = Kern != kNeumann, Combined = Kern == kCombined;
     static_assert(!Computed || sizeof(MT) == sizeof(double2), "computed factors are complex128");
     constexpr bool Precise = sizeof(MT) == sizeof(double2);
@@ -718,7 +722,7 @@ __device__ __forceinline__ void pair_batch_rows(const MT* __restrict__ F, const 
             const double d1 = Ri.x - C.x, d2 = Ri.y - C.y, d3 = Ri.z - C.z;
             const double rho2 = d1 * d1 + d2 * d2 + d3 * d3;
             double c1, c2;
-            if (Factors == kExactFactors) {
+            if (Exact) {
                 const double rho = sqrt(rho2);
                 if (rho > prm.tau) {
                     if (Combined) {
@@ -779,14 +783,15 @@ __device__ __forceinline__ void pair_batch_rows(const MT* __restrict__ F, const 
     for (int J = 0; J <= I; ++J) {
         const int cJ = rowW - J * kPR;  // column of rowW relative to row block J
         const int t = cJ / kPC;
-        const MT* T = Computed ? nullptr
-                               : F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
+        const bool load = !Computed || (Factors == kCachedFactors && J < cached);  // wave-uniform
+        const MT* T = !load ? nullptr
+                            : F + ((size_t)blk_start[J] + t) * kPR * kPC + (size_t)(i0 - J * kPR - t * kPC) + li;
         for (int rj0 = 0; rj0 < kPR; rj0 += kBatchK) {
             stage(J * kPR + rj0);
 #pragma unroll 8
             for (int s = 0; s < kBatchK / 4; ++s) {
                 const int jj = 4 * s + lk;
-                if (Computed) {
+                if (!load) {
                     double u;
                     const MT f = factor(jj, J * kPR + rj0 + jj < i && i < count, &u);
                     apply(f, u, jj);
@@ -798,16 +803,17 @@ __device__ __forceinline__ void pair_batch_rows(const MT* __restrict__ F, const 
     }
     // j >= rows: row i of the row block's tiles
     const int nt = blk_start[I + 1] - blk_start[I];
-    const MT* Trow = Computed ? nullptr : F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
+    const bool load_own = !Computed || (Factors == kCachedFactors && I < cached);
+    const MT* Trow = !load_own ? nullptr : F + (size_t)blk_start[I] * kPR * kPC + (size_t)(i - I * kPR) * kPC;
     for (int t = 0; t < nt; ++t) {
         const int c0 = I * kPR + t * kPC;
-        const MT* Tr = Computed ? nullptr : Trow + (size_t)t * kPR * kPC;
+        const MT* Tr = !load_own ? nullptr : Trow + (size_t)t * kPR * kPC;
         for (int cb = 0; cb < kPC; cb += kBatchK) {
             stage(c0 + cb);
 #pragma unroll 8
             for (int s = 0; s < kBatchK / 4; ++s) {
                 const int jj = 4 * s + lk;
-                if (Computed) {
+                if (!load_own) {
                     const int j = c0 + cb + jj;
                     double u;
                     const MT f = factor(jj, i < j && j < count, &u);
@@ -833,10 +839,11 @@ __global__ void __launch_bounds__(64 * Waves) k_pair_batch(const MT* __restrict_
                                                            const PairPoint* __restrict__ pts, int count,
                                                            ibim_params prm, const double2* __restrict__ P,
                                                            const double2* __restrict__ Q, double2* __restrict__ Y,
-                                                           double2* __restrict__ Z, const int* __restrict__ stop) {
+                                                           double2* __restrict__ Z, const int* __restrict__ stop,
+                                                           int cached) {
     if (stop != nullptr && *stop) return;
     pair_batch_rows<MT, Kern, WithY, WithZ, Waves, Factors>(F, blk_start, pts, count, prm, P, Q, Y, Z,
-                                                            blockIdx.x * 16 * Waves);
+                                                            blockIdx.x * 16 * Waves, cached);
 }
 
 // Near-field entries of the batched products (one thread per (row, RHS)),
@@ -918,15 +925,16 @@ void launch_pair_batch_far(Context* ctx, int waves, const MT* F, const double2* 
     const dim3 grid((unsigned)((count + rows - 1) / rows)), block((unsigned)(64 * waves));
     const int* blk = ctx->pair_blk.as<int>();
     const PairPoint* pts = ctx->pair_pts.as<PairPoint>();
+    const int cached = Factors == kCachedFactors ? ctx->free_cached_blocks : 0;
     if (waves == 1)
         hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 1, Factors>), grid, block, 0, ctx->stream, F,
-                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop, cached);
     else if (waves == 2)
         hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 2, Factors>), grid, block, 0, ctx->stream, F,
-                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop, cached);
     else
         hipLaunchKernelGGL((k_pair_batch<MT, Kern, WithY, WithZ, 4, Factors>), grid, block, 0, ctx->stream, F,
-                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop);
+                           blk, pts, count, ctx->prm, P, Q, Y, Z, stop, cached);
 }
 
 // The workgroup shape of the far pass: IBIM_BATCH_WAVES (1, 2, 4) if set, else
@@ -993,7 +1001,7 @@ void launch_pair_batch(Context* ctx, int slot, const MT* F, const double2* P, co
                        double2* Z, const int* stop) {
     const int count = ctx->count;
     const int waves = tuned_waves<MT, Kern, WithY, WithZ, Factors>(ctx, slot, F, P, Q, Y, Z, stop);
-    TimedScope ts(ctx, Factors == kExactFactors  ? "batch_free64"
+    TimedScope ts(ctx, Factors == kExactFactors || Factors == kCachedFactors ? "batch_free64"
                        : Factors == kFastFactors ? "batch_free32"
                        : slot >= 6               ? "batch_pair64"
                                                  : "batch_pair");
@@ -1498,7 +1506,7 @@ int launch_assemble_pair(Context* ctx, int precision, bool with64) {
     const size_t esz = precision == IBIM_C128 ? sizeof(double2) : sizeof(float2);
     ctx->mat_precision = -1;
     ctx->mat_pair = ctx->mat_pair64 = false;
-    ctx->pair_aux = false;
+    ctx->pair_aux = ctx->free_cache_valid = false;
     ctx->ld = ((size_t)count + 63) / 64 * 64;
     std::vector<int2> tiles;
     if (int rc = pair_layout(ctx, tiles)) return rc;
@@ -1586,9 +1594,42 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
     return IBIM_OK;
 }
 
+// The FP64 matrix-free operator's cache (ibim_set_free_cache): the complex128
+// pairs of the first row blocks whose tiles fit in free_cache_max bytes, written
+// by the pair assembly's kernel (so they are the bits the sweeps would compute),
+// as the reference keeps the operator rows that fit and recomputes the rest
+// (FASTSIZEBOUND, large3Dexample2.c:27,3905-3926). Built once per tube.
+int ensure_free_cache(Context* ctx) {
+    if (ctx->free_cache_valid) return IBIM_OK;
+    const int count = ctx->count;
+    const size_t tile = sizeof(double2) * kPR * kPC;
+    int blocks = 0, tiles = 0;
+    for (int I = 0; I * kPR < count; ++I) {
+        const int T = (count - I * kPR + kPC - 1) / kPC;
+        if ((size_t)(tiles + T) * tile > ctx->free_cache_max) break;
+        tiles += T;
+        blocks = I + 1;
+    }
+    ctx->free_cached_blocks = blocks;
+    ctx->free_cached_tiles = tiles;
+    if (tiles > 0) {
+        if (ctx->free_cache.ensure((size_t)tiles * tile)) return IBIM_ERR_NOMEM;
+        // near counts go to the build cursors, scratch once the lists exist
+        TimedScope ts(ctx, "assemble");
+        hipLaunchKernelGGL(k_pair_assemble<double2>, dim3((unsigned)tiles, kPR / kPairAsmRows), dim3(kPT), 0,
+                           ctx->stream, ctx->tube.as<double>(), count, ctx->pair_tiles.as<int2>(), ctx->prm,
+                           ctx->free_cache.as<double2>(), nullptr, ctx->near_cur.as<int>());
+        IBIM_HIP(hipGetLastError());
+    }
+    ctx->free_cache_valid = true;
+    return IBIM_OK;
+}
+
 int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q, void* ypart, void* zpart,
                            const int* done) {
     if (int rc = prepare_pair_free(ctx)) return rc;
+    if (!fast && ctx->free_cache_max > 0)
+        if (int rc = ensure_free_cache(ctx)) return rc;
     const int count = ctx->count;
     const size_t ld = ctx->ld;
     const int ncol = (count + kPC - 1) / kPC, nrow = (count + kPR - 1) / kPR;
@@ -1606,10 +1647,19 @@ int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q
                 hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kFastFactors>), dim3((unsigned)ctx->pair_ntiles),
                                    dim3(kPT), 0, ctx->stream, nullptr, ctx->pair_tiles.as<int2>(),
                                    ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
-            else
-                hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kExactFactors>), dim3((unsigned)ctx->pair_ntiles),
-                                   dim3(kPT), 0, ctx->stream, nullptr, ctx->pair_tiles.as<int2>(),
-                                   ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+            else {
+                // cached tiles (the first ones) read their factors, the others compute them
+                const int nc = ctx->free_cache_max > 0 ? ctx->free_cached_tiles : 0;
+                if (nc > 0)
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kStoredFactors>), dim3((unsigned)nc), dim3(kPT),
+                                       0, ctx->stream, ctx->free_cache.as<double2>(), ctx->pair_tiles.as<int2>(),
+                                       ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+                if (ctx->pair_ntiles > nc)
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kExactFactors>),
+                                       dim3((unsigned)(ctx->pair_ntiles - nc)), dim3(kPT), 0, ctx->stream, nullptr,
+                                       ctx->pair_tiles.as<int2>() + nc, ctx->pair_pts.as<PairPoint>(), count, ld,
+                                       ncol, P, Q, Y, Z, done, ctx->prm);
+            }
         }
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;
@@ -1714,10 +1764,16 @@ int pair_free_batch_apply(Context* ctx, bool fast, const double2* P, const doubl
         return IBIM_ERR_ARG;
     }
     if (int rc = prepare_pair_free(ctx)) return rc;
+    const bool cache = !fast && ctx->free_cache_max > 0;
+    if (cache)
+        if (int rc = ensure_free_cache(ctx)) return rc;
     with_kernel(ctx, [&](auto K) {
         constexpr int Kern = decltype(K)::value;
         if (fast)
             launch_pair_batch_any<double2, Kern, kFastFactors>(ctx, 12, nullptr, P, Q, Y, Z, stop);
+        else if (cache && ctx->free_cached_blocks > 0)
+            launch_pair_batch_any<double2, Kern, kCachedFactors>(ctx, 9, ctx->free_cache.as<double2>(), P, Q, Y, Z,
+                                                                 stop);
         else
             launch_pair_batch_any<double2, Kern, kExactFactors>(ctx, 9, nullptr, P, Q, Y, Z, stop);
     });

@@ -118,7 +118,7 @@ def lib():
             "ibim_gradient": ([vp, I, I, C.POINTER(SolveOpts), P, P, P], I),
             "ibim_set_profiling": ([vp, I], I),
             "ibim_set_graphs": ([vp, I], I),
-            "ibim_set_fp32_products": ([vp, I], I),
+            "ibim_set_fp32_products": ([vp, I], I), "ibim_set_free_cache": ([vp, D], I),
             "ibim_set_progress": ([vp, PROGRESS_FN, vp], I), "ibim_kernel_time": ([vp, C.c_char_p, P, C.POINTER(L)], I),
             "ibim_reset_timers": ([vp], I),
             "ibim_read_distance_file": ([C.c_char_p, I, P], I), "ibim_write_distance_file": ([C.c_char_p, I, P], I),
@@ -373,6 +373,11 @@ class Problem:
     def set_fp32_products(self, on=True):
         """16-column passes over complex64 pairs on the FP32 matrix cores."""
         _check(lib().ibim_set_fp32_products(self.h, 1 if on else 0), "ibim_set_fp32_products")
+
+    def set_free_cache(self, max_bytes):
+        """Keep up to max_bytes of the FP64 matrix-free operator's complex128
+        pairs (its first row blocks) in HBM; 0 frees them."""
+        _check(lib().ibim_set_free_cache(self.h, float(max_bytes)), "ibim_set_free_cache")
 
     def set_graphs(self, on=True):
         """Replay one captured chunk of BiCG iterations as a HIP graph."""

@@ -311,6 +311,15 @@ int ibim_set_graphs(ibim_ctx *ctx, int on);
  * which the mixed solvers' FP64 residuals correct. No counterpart in the
  * reference. */
 int ibim_set_fp32_products(ibim_ctx *ctx, int on);
+/* The FP64 matrix-free operator (a traditional tube with no operator stored:
+ * OP_FREE64, BiCG without an assembly, the mixed solvers' FP64 residuals)
+ * keeps the complex128 pairs of its first row blocks, up to max_bytes of HBM,
+ * and computes the others' factors in every sweep, as the reference keeps the
+ * operator rows that fit in its cache and recomputes the rest (FASTSIZEBOUND,
+ * large3Dexample2.c:27,3905-3926). The cached pairs are the bits the sweeps
+ * compute, so results do not change. Built on first use after each tube; 0
+ * (the default) frees it. */
+int ibim_set_free_cache(ibim_ctx *ctx, double max_bytes);
 /* Total milliseconds and launch count of a kernel class ("matvec", "assemble", ...). */
 int ibim_kernel_time(ibim_ctx *ctx, const char *name, double *ms_total, long *launches);
 int ibim_reset_timers(ibim_ctx *ctx);

@@ -353,3 +353,61 @@ def test_group_solve_refuses_what_cannot_share_a_launch(gpu):
         gpu.solve_group([a, d], gpu.SOLVE_BICG)  # dense storage
     for pb in (a, b, c, d):
         pb.close()
+
+
+@pytest.mark.parametrize("combined", [0, 1])
+def test_matrix_free_cache_keeps_the_bits(gpu, combined):
+    """ibim_set_free_cache: the FP64 matrix-free operator keeps the complex128
+    pairs of its first row blocks (here 2 of 8) and computes the rest, giving
+    the uncached operator's bits in single-vector and 16-column products and in
+    batched BiCG; the cache is built once per tube, again after a tube change,
+    and released by a zero budget."""
+    h = 4.0 / N
+
+    def make():
+        pb = gpu.Problem()
+        pb.set_level(shapes.sphere(N))
+        pb.geometry(5)
+        pb.tube(gpu.params(4.0, 3 * h, 2 * h, dirichlet=1, combined=combined, eta=4.0))
+        return pb
+
+    plain, cached = make(), make()
+    assert cached.count > 7 * 512  # 8 row blocks of 512 rows
+    cached.set_free_cache(10 * 512 * 1024 * 16)  # 10 tiles: the 8 of row blocks 0 and 1
+    cached.set_profiling(True)
+    rng = np.random.default_rng(5)
+    p = rng.standard_normal(plain.count) + 1j * rng.standard_normal(plain.count)
+    q = rng.standard_normal(plain.count) + 1j * rng.standard_normal(plain.count)
+    P = rng.standard_normal((plain.count, 16)) + 1j * rng.standard_normal((plain.count, 16))
+    for _ in range(2):
+        y0, z0 = plain.apply(p, q, gpu.OP_FREE64)
+        y1, z1 = cached.apply(p, q, gpu.OP_FREE64)
+        assert np.array_equal(y0, y1) and np.array_equal(z0, z1)
+        for adjoint in (False, True):
+            assert np.array_equal(plain.apply_batch(P, gpu.OP_FREE64, adjoint),
+                                  cached.apply_batch(P, gpu.OP_FREE64, adjoint)), adjoint
+    cached.sync()
+    assert cached.kernel_time("assemble")[1] == 1  # the cache fill, once
+    for pb in (plain, cached):
+        fill_rhs(gpu, pb, 16)
+    ia, _ = plain.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+    ib, _ = cached.solve(gpu.SOLVE_BICG, maxit=500, tol=1e-12, batch=0)
+    assert ia.status == 0 and (ia.iterations, ia.status, ia.rel_residual) == (ib.iterations, ib.status,
+                                                                                ib.rel_residual)
+    for c in range(16):
+        a, b = plain.density(c), cached.density(c)
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1]), c
+    # a new tube (wider near field): the cache is rebuilt for it
+    for pb in (plain, cached):
+        pb.tube(gpu.params(4.0, 3 * h, 2.5 * h, dirichlet=1, combined=combined, eta=4.0))
+    y0, z0 = plain.apply(p, q, gpu.OP_FREE64)
+    y1, z1 = cached.apply(p, q, gpu.OP_FREE64)
+    assert np.array_equal(y0, y1) and np.array_equal(z0, z1)
+    cached.sync()
+    assert cached.kernel_time("assemble")[1] == 2
+    cached.set_free_cache(0)
+    y1, z1 = cached.apply(p, q, gpu.OP_FREE64)
+    assert np.array_equal(y0, y1) and np.array_equal(z0, z1)
+    cached.set_profiling(False)
+    plain.close()
+    cached.close()
