@@ -48,10 +48,10 @@ WORKLOADS = {
     # point source at (1, 0, 0.05): exterior Dirichlet problem, double layer.
     "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05), tol=1e-8),
     # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10;
-    # 11 iterations: replacements every 1e-3 of residual drop (3 instead of 4 at 1e-2,
-    # 9.8 vs 10.5 ms per solve, the same iterations)
+    # 9 iterations: replacements every 1e-5 of residual drop (2, and the FP64 path's
+    # 9 iterations: 8.9 ms per solve; at 1e-3 3 replacements and 11 iterations, 10.6 ms)
     "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-10,
-                     replace=1e-3),
+                     replace=1e-5),
     # small case for quick checks
     "sphere32": dict(shape="sphere", n=32, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-8),
 }
@@ -62,9 +62,14 @@ TOL = 1e-8          # default relative residual target of both timed paths
 REF_TOL = 1e-11     # the FP64 solve the densities are compared against
 REPLACE = 1e-2      # residual drop between FP64 residual replacements (torus128: 127 iterations, 4 replacements)
 BATCH_REPLACE = 1e-3  # the same for the batched solver (at 1e-1 its torus128 columns stagnated)
+# star256 mixed path (FP32 products): 4 replacement passes at 1e-4 (9.13 s) against 5 at
+# 1e-3 (9.70 s); at 1e-5 the FP32 operator's recursive residual drifted from the true one
+# and the columns stalled near 1e-5
+STAR_REPLACE = 1e-4
 NRECV = 500
 STORE_MAX_BYTES = 200e9  # store an operator in HBM up to this size (288 GB per GPU)
 MATRIX_FREE = False  # --matrix-free: both timed paths recompute the operator per sweep
+REPLACE_OVERRIDE = None  # --replace: the mixed path's replacement drop for every single-vector workload
 # the 256^3 star's FP64 path keeps no complex128 pairs by default: caching the
 # first row blocks' pairs (249 of 491 GB) made its pass slower, 492.6 against
 # 478.8 ms (reading complex128 pairs at both visits costs what computing them does)
@@ -134,8 +139,8 @@ def step(pb, w, recv, mixed):
         pb.assemble(mixed_storage(count))
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if mixed:
-        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"], inner_tol=w.get("replace", REPLACE),
-                           chunk=1)
+        info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"],
+                           inner_tol=REPLACE_OVERRIDE or w.get("replace", REPLACE), chunk=1)
     else:
         info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
     u = pb.evaluate(recv, ibim.EVAL_DOUBLE)
@@ -606,19 +611,22 @@ def main():
                     help="star256 mixed path: 16-column passes on the FP64 matrix cores (default: FP32)")
     ap.add_argument("--star-fp64-cache", type=float, default=STAR_FP64_CACHE_BYTES,
                     help="star256 FP64 path: bytes of complex128 pairs the matrix-free operator keeps (0: none)")
-    ap.add_argument("--star-replace", type=float, default=BATCH_REPLACE,
+    ap.add_argument("--star-replace", type=float, default=STAR_REPLACE,
                     help="star256 mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--batch-fp32-products", action="store_true",
                     help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
     ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
                     help="batch mixed path: residual drop between FP64 residual replacements")
+    ap.add_argument("--replace", type=float, default=None,
+                    help="single-vector workloads: residual drop between the mixed path's FP64 replacements")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     ap.add_argument("--matrix-free", action="store_true",
                     help="store no operator: both paths recompute the radial factors per pair and sweep")
     args = ap.parse_args()
-    global MATRIX_FREE
+    global MATRIX_FREE, REPLACE_OVERRIDE
     MATRIX_FREE = args.matrix_free
+    REPLACE_OVERRIDE = args.replace
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
