@@ -284,7 +284,10 @@ def run_batch(args):
             for c in range(m):
                 pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
         pbs[0].reset_timers()
-        pbs[0].set_profiling(True)
+        if args.batch_graphs:
+            pbs[0].set_graphs(True)  # replay captured chunks (graphs are off while profiling)
+        else:
+            pbs[0].set_profiling(True)
         t1 = time.perf_counter()
         if path == "fp64":
             infos = ibim.solve_group(pbs, ibim.SOLVE_BICG, maxit=3000, tol=tol)
@@ -615,6 +618,8 @@ def main():
                     help="star256 mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--batch-fp32-products", action="store_true",
                     help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
+    ap.add_argument("--batch-graphs", action="store_true",
+                    help="batch workload: replay the grouped iterations as HIP graphs (no per-pass timing)")
     ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
                     help="batch mixed path: residual drop between FP64 residual replacements")
     ap.add_argument("--replace", type=float, default=None,

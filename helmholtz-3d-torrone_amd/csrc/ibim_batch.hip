@@ -1195,26 +1195,73 @@ int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* op
     int chunk = opts->chunk;
     if (chunk <= 0) chunk = mixed ? 2 : (runs[0].ctx->count > 20000 ? 2 : 16);
     std::vector<double> sums;
+    // one iteration of every batch still running at the last host check: the
+    // same launches every time (the kernels read the iteration and stop flags
+    // from the device states), so a chunk of them can be captured and replayed
+    auto launch_iteration = [&]() -> int {
+        if (int rc = pair_group_pass(lead, source, dev_members, nm, items, queue)) return rc;
+        for (GroupRun& g : runs) {
+            if (g.hs.all_done) continue;  // stays done until a host check restarts it
+            const BArgs& a = g.a;
+            if (int rc = pair_batch_near(g.ctx, a.p, a.pt, a.v, a.w, &a.st->all_done)) return rc;
+            hipLaunchKernelGGL(kb_dot, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+            hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb);
+            hipLaunchKernelGGL(kb_update_xr, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+            hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb, opts->tol);
+            hipLaunchKernelGGL(kb_update_p, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+        }
+        return IBIM_OK;
+    };
+    // HIP graph replay (ibim_set_graphs on the first problem, not while
+    // profiling): full chunks after the first are captured once per set of
+    // running batches and replayed; a host check that changes the set (a batch
+    // finished or was restarted) makes the next chunk capture again
+    struct GraphExec {
+        hipGraphExec_t g = nullptr;
+        ~GraphExec() {
+            if (g) (void)hipGraphExecDestroy(g);
+        }
+    } chunk_graph;
+    std::vector<char> graph_live;  // the running batches the captured chunk launches
+    const bool graphs = lead->use_graphs && !lead->profile;
     int it = 1;
     while (it <= maxit) {
         bool live = false;
-        for (const GroupRun& g : runs) live = live || !g.hs.all_done;
+        std::vector<char> live_set(runs.size());
+        for (size_t m = 0; m < runs.size(); ++m) {
+            live_set[m] = !runs[m].hs.all_done;
+            live = live || live_set[m];
+        }
         if (live) {
             const int last = std::min(it + chunk - 1, maxit);
-            for (; it <= last; ++it) {
-                if (int rc = pair_group_pass(lead, source, dev_members, nm, items, queue)) return rc;
-                for (GroupRun& g : runs) {
-                    if (g.hs.all_done) continue;  // stays done until a host check restarts it
-                    const BArgs& a = g.a;
-                    if (int rc = pair_batch_near(g.ctx, a.p, a.pt, a.v, a.w, &a.st->all_done)) return rc;
-                    hipLaunchKernelGGL(kb_dot, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
-                    hipLaunchKernelGGL(kb_alpha, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb);
-                    hipLaunchKernelGGL(kb_update_xr, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
-                    hipLaunchKernelGGL(kb_beta, dim3(1), dim3(kBT), 0, lead->stream, a, (int)g.nvb, opts->tol);
-                    hipLaunchKernelGGL(kb_update_p, dim3(g.nvb), dim3(kBT), 0, lead->stream, a);
+            if (graphs && it > 1 && last - it + 1 == chunk) {
+                if (chunk_graph.g && graph_live != live_set) {
+                    IBIM_HIP(hipGraphExecDestroy(chunk_graph.g));
+                    chunk_graph.g = nullptr;
                 }
-                IBIM_HIP(hipGetLastError());
+                if (!chunk_graph.g) {
+                    hipGraph_t g = nullptr;
+                    IBIM_HIP(hipStreamBeginCapture(lead->stream, hipStreamCaptureModeThreadLocal));
+                    int rc = IBIM_OK;
+                    for (int c = 0; c < chunk && !rc; ++c) rc = launch_iteration();
+                    const hipError_t e = hipStreamEndCapture(lead->stream, &g);
+                    if (rc) {
+                        if (g) (void)hipGraphDestroy(g);
+                        return rc;
+                    }
+                    IBIM_HIP(e);
+                    const hipError_t ei = hipGraphInstantiate(&chunk_graph.g, g, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(g);
+                    IBIM_HIP(ei);
+                    graph_live = live_set;
+                }
+                IBIM_HIP(hipGraphLaunch(chunk_graph.g, lead->stream));
+                it = last + 1;
+            } else {
+                for (; it <= last; ++it)
+                    if (int rc = launch_iteration()) return rc;
             }
+            IBIM_HIP(hipGetLastError());
             double worst = 0.0;
             for (GroupRun& g : runs) {
                 if (g.hs.all_done) continue;

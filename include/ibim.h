@@ -267,7 +267,8 @@ int ibim_get_density(ibim_ctx *ctx, int col, double *re, double *im);
  * with the same opts gives (host checks every opts->chunk iterations; 0: 2
  * for the reliable solver, and for BiCG 2 when the largest problem has more
  * than 20,000 points, else 16). The work runs on ctxs[0]'s stream; no other
- * thread may use the handles meanwhile. No counterpart in the reference (one
+ * thread may use the handles meanwhile. With ibim_set_graphs on ctxs[0] the
+ * iterations replay captured HIP graphs (the same bits). No counterpart in the reference (one
  * surface and one right-hand side per run). */
 int ibim_solve_group(ibim_ctx *const *ctxs, int nctx, const ibim_solve_opts *opts, ibim_solve_info *infos);
 int ibim_set_density_host(ibim_ctx *ctx, int col, const double *re, const double *im);

@@ -129,3 +129,41 @@ def test_batched_graph_replay_gives_the_eager_bits(gpu, store, method, chunk):
         other = gpu.OP_FREE64 if store == "matrix-free" else gpu.OP_STORED
         pb.apply_batch(np.ones((pb.count, 3)), other)  # other device work between the solves
     pb.set_graphs(False)
+
+
+@pytest.mark.parametrize("store,method,fp32", [("C128_PAIR", "bicg", False), ("MIXED_PAIR", "reliable", True)])
+@pytest.mark.parametrize("chunk", [2, 3])
+def test_grouped_graph_replay_gives_the_eager_bits(gpu, store, method, fp32, chunk):
+    """ibim_solve_group with graphs on its first problem: full chunks after the
+    first replay a captured graph of every running batch's iterations, captured
+    again whenever a host check changes which batches run (here problems of
+    3,7xx, 2,6xx and 1,3xx points, one with two batches, finish at different
+    iterations). Twice in a row, with other device work between: the eager bits."""
+    sizes, nrhs = (28, 24, 18), (19, 16, 5)
+    pbs = []
+    for n, r in zip(sizes, nrhs):
+        pb = problem(gpu, n=n)
+        pb.assemble(getattr(gpu, store))
+        if fp32:
+            pb.set_fp32_products(True)
+        fill_rhs(gpu, pb, r)
+        pbs.append(pb)
+    m = gpu.SOLVE_BICG if method == "bicg" else gpu.SOLVE_RELIABLE
+
+    def run_group():
+        infos = gpu.solve_group(pbs, m, maxit=500, tol=1e-11, inner_tol=1e-2, chunk=chunk)
+        return ([(i.iterations, i.outer, i.status) for i in infos],
+                [[pb.density(c) for c in range(r)] for pb, r in zip(pbs, nrhs)])
+
+    eager = run_group()
+    assert all(st == 0 for _, _, st in eager[0])
+    pbs[0].set_graphs(True)
+    for rep in range(2):
+        got = run_group()
+        assert got[0] == eager[0], (rep, got[0], eager[0])
+        for dg, de in zip(got[1], eager[1]):
+            for a, b in zip(dg, de):
+                assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+        pbs[1].apply_batch(np.ones((pbs[1].count, 3)), gpu.OP_STORED)  # other device work between the solves
+    for pb in pbs:
+        pb.close()
