@@ -61,7 +61,9 @@ EXTRA = ("batch", "adjoint96", "star256")
 TOL = 1e-8          # default relative residual target of both timed paths
 REF_TOL = 1e-11     # the FP64 solve the densities are compared against
 REPLACE = 1e-2      # residual drop between FP64 residual replacements (torus128: 127 iterations, 4 replacements)
-BATCH_REPLACE = 1e-3  # the same for the batched solver (at 1e-1 its torus128 columns stagnated)
+# the same for the batched solvers (at 1e-1 the torus128 columns stagnated): batch workload
+# 15.71 s at 1e-4, 16.68 s at 1e-3, 17.02 s at 1e-2, 15.85 s at 1e-5
+BATCH_REPLACE = 1e-4
 # star256 mixed path (FP32 products): 4 replacement passes at 1e-4 (9.13 s) against 5 at
 # 1e-3 (9.70 s); at 1e-5 the FP32 operator's recursive residual drifted from the true one
 # and the columns stalled near 1e-5
