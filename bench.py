@@ -50,8 +50,10 @@ WORKLOADS = {
     # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10;
     # 9 iterations: replacements every 1e-5 of residual drop (2, and the FP64 path's
     # 9 iterations: 8.9 ms per solve; at 1e-3 3 replacements and 11 iterations, 10.6 ms)
+    # The mixed path keeps reliable BiCG here: 9.07 ms against 9.46 ms for GMRES
+    # refinement, whose host check per iteration weighs on a 10-iteration solve.
     "sphere64": dict(shape="sphere", n=64, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-10,
-                     replace=1e-5),
+                     replace=1e-5, mixed_solver="reliable"),
     # small case for quick checks
     "sphere32": dict(shape="sphere", n=32, k=2.0, eps_h=3.0, tau_h=2.0, x0=(0.1, 0.2, 0.3), tol=1e-8),
 }
@@ -132,7 +134,13 @@ def mixed_storage(count):
     return ibim.C64_PAIR if mixed_stores(count) else None
 
 
-def step(pb, w, recv, mixed):
+def step(pb, w, recv, path):
+    """One solve of a single-right-hand-side workload. path: "fp64" (the
+    reference's BiCG on complex128 pairs), "fp64_gmres" (FP64 GMRES on the same
+    pairs), "mixed" (FP64 refinement around GMRES on complex64 pairs; with
+    nothing stored, reliable BiCG on the mixed matrix-free operator) or
+    "mixed_reliable" (BiCG on complex64 pairs with FP64 residual replacement)."""
+    mixed = path.startswith("mixed")
     pb.geometry(5)
     count = pb.tube(params_of(w), ibim.TUBE_TRADITIONAL)
     if not mixed and not MATRIX_FREE:
@@ -140,26 +148,29 @@ def step(pb, w, recv, mixed):
     elif mixed and mixed_storage(count) is not None:
         pb.assemble(mixed_storage(count))
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
-    if mixed:
+    if path == "fp64":
+        info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
+    elif path == "fp64_gmres" or (path == "mixed" and mixed_storage(count) is not None and
+                                  w.get("mixed_solver", "gmres") == "gmres"):
+        info, _ = pb.solve(ibim.SOLVE_GMRES, maxit=3000, tol=w["tol"])
+    else:
         info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"],
                            inner_tol=REPLACE_OVERRIDE or w.get("replace", REPLACE), chunk=1)
-    else:
-        info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
     u = pb.evaluate(recv, ibim.EVAL_DOUBLE)
     x = pb.density()
     return info, x, u
 
 
-def timed(pb, w, recv, mixed, steps, warmup):
+def timed(pb, w, recv, path, steps, warmup):
     for _ in range(warmup):
-        step(pb, w, recv, mixed)
+        step(pb, w, recv, path)
     pb.sync()
     pb.reset_timers()
     pb.set_profiling(True)
     pb.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        info, x, u = step(pb, w, recv, mixed)
+        info, x, u = step(pb, w, recv, path)
     pb.sync()
     t = time.perf_counter() - t0
     pb.set_profiling(False)
@@ -619,6 +630,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="torus128", choices=sorted(WORKLOADS) + list(EXTRA))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="single-vector workloads: skip the reliable-BiCG and FP64-GMRES records beside the two paths")
     ap.add_argument("--no-configs", action="store_true",
                     help="default run: only the torus128 line, without the other workloads' records")
     ap.add_argument("--no-star", action="store_true", help="default run: leave out the star256 record")
@@ -694,19 +707,34 @@ def run_standard(args, workload):
     print("[bench] count=%d reference BiCG (dense complex128): %d iterations, status %d" %
           (count, info_ref.iterations, info_ref.status), file=sys.stderr, flush=True)
 
-    t64, info64, x64, u64 = timed(p64, w, recv, False, args.steps, args.warmup)
+    t64, info64, x64, u64 = timed(p64, w, recv, "fp64", args.steps, args.warmup)
     ms_mv, n_mv = p64.kernel_time("sweep_free64" if MATRIX_FREE else "matvec")
     ms_asm, n_asm = p64.kernel_time("assemble")
     print("[bench] fp64 path: %.3f s/solve, %d iterations" % (t64, info64.iterations), file=sys.stderr, flush=True)
     p64.close()
 
     pm = make_problem(level, w)
-    tm, infom, xm, um = timed(pm, w, recv, True, args.steps, args.warmup)
+    tm, infom, xm, um = timed(pm, w, recv, "mixed", args.steps, args.warmup)
     stored = mixed_stores(count)
     ms_dom, n_dom = pm.kernel_time("matvec" if stored else "sweep_free32")
     ms_rep, n_rep = pm.kernel_time("matvec64" if mixed_storage(count) == ibim.MIXED_PAIR else "sweep_free64")
-    print("[bench] mixed path: %.3f s/solve, %d iterations, %d FP64 residual replacements" %
-          (tm, infom.iterations, infom.outer), file=sys.stderr, flush=True)
+    gmres = stored and w.get("mixed_solver", "gmres") == "gmres"
+    solver = "FP64 refinement around GMRES" if gmres else "BiCG with FP64 residual replacement"
+    print("[bench] mixed path (%s): %.3f s/solve, %d iterations, %d FP64 residuals" %
+          (solver, tm, infom.iterations, infom.outer), file=sys.stderr, flush=True)
+    # beside them, fewer steps: the former mixed solver (reliable BiCG on the same
+    # complex64 pairs) and FP64 GMRES on the complex128 pairs, which separate the
+    # precision's gain from the solver's
+    extra = {}
+    if stored and not args.no_extra:
+        xs, ws = min(args.steps, 5), min(args.warmup, 1)
+        for name in ("mixed_reliable", "fp64_gmres"):
+            pe = make_problem(level, w)
+            te, infoe, xe, _ = timed(pe, w, recv, name, xs, ws)
+            extra[name] = (te, infoe, xe)
+            pe.close()
+            print("[bench] %s: %.3f s/solve, %d iterations" % (name, te, infoe.iterations), file=sys.stderr,
+                  flush=True)
 
     def rel(a, b):
         return float(np.linalg.norm(a - b) / np.linalg.norm(b))
@@ -719,8 +747,17 @@ def run_standard(args, workload):
     npairs = float(count) * (count - 1) / 2
     vec_bytes = 4 * 16 * count  # p, p~ in; A p, A^H p~ out (complex128)
     avg_dom = ms_dom / max(n_dom, 1) * 1e-3
-    if stored:
-        # dominant kernel: the fused pair sweep over the complex64 radial factors
+    if stored and gmres:
+        # dominant kernel: GMRES's pair sweep over the complex64 radial factors,
+        # A p alone (p in, A p out: 32 B per point)
+        dom_bytes = npairs * 8 + 32 * count
+        achieved = dom_bytes / avg_dom / 1e9 if n_dom else 0.0
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic("k_pair_sweep<float2,A_p>", workload),
+                "kernel": "k_pair_sweep<float2, A p only>: 8 B/pair x %d*%d/2 pairs + 32 B/point vectors per launch"
+                          % (count, count - 1)}
+    elif stored:
+        # dominant kernel: BiCG's fused pair sweep over the complex64 radial factors
         dom_bytes = npairs * 8 + vec_bytes
         achieved = dom_bytes / avg_dom / 1e9 if n_dom else 0.0
         roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -756,8 +793,8 @@ def run_standard(args, workload):
         "vs_baseline": None,
         "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors + fp64 residual replacement"
                   if MATRIX_FREE else
-                  "complex64 radial factors per pair (fp64 point factors, products and sums) + fp64 residual "
-                  "replacement"),
+                  "complex64 radial factors per pair (fp64 point factors, products and sums), %s" % (
+                      "GMRES, fp64 iterative refinement" if gmres else "BiCG, fp64 residual replacement")),
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
         "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g%s" % (
             workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"],
@@ -769,7 +806,8 @@ def run_standard(args, workload):
             "fp64_operator": ("FP64 radial factors recomputed per pair" if MATRIX_FREE else
                               "complex128 pair storage" if pair_bytes(count, 16) <= STORE_MAX_BYTES else
                               "dense complex128")},
-        "mixed": {"solves_per_s": 1.0 / tm, "iterations": infom.iterations, "replacements": infom.outer,
+        "mixed": {"solver": solver, "solves_per_s": 1.0 / tm, "iterations": infom.iterations,
+                  "fp64_residuals": infom.outer,
                   "rel_err_vs_fp64": err_mixed, "field_rel_err_vs_analytic": field_err,
                   "sweep_ms": avg_dom * 1e3, "replacement_sweep_ms": ms_rep / max(n_rep, 1)},
         "fp64": {"solves_per_s": 1.0 / t64, "iterations": info64.iterations, "rel_err_vs_fp64": err_fp64,
@@ -781,6 +819,11 @@ def run_standard(args, workload):
         "speedup_mixed_vs_fp64": t64 / tm,
         "roofline": roof,
     }
+    for name, (te, infoe, xe) in extra.items():
+        line[name] = {"solves_per_s": 1.0 / te, "iterations": infoe.iterations,
+                      "rel_err_vs_fp64": rel(xe[0] + 1j * xe[1], x_ref)}
+    if "fp64_gmres" in extra:
+        line["speedup_mixed_vs_fp64_gmres"] = extra["fp64_gmres"][0] / tm
     if not args.no_cpu:
         print("[bench] cpu baseline (oracle, bounded sample)", file=sys.stderr, flush=True)
         line["cpu_baseline"] = cpu_baseline(w, count, info64.iterations)
@@ -821,7 +864,8 @@ def other_configs(args):
 def compact(name, r):
     """The figures of a workload record that the default run reports."""
     if name in ("sphere64", "torus128_matrix_free"):
-        return {"workload": r["config"]["workload"], "tube_points": r["config"]["tube_points"],
+        d = {"workload": r["config"]["workload"], "tube_points": r["config"]["tube_points"],
+                "mixed_solver": r["mixed"]["solver"],
                 "solves_per_s": r["mixed"]["solves_per_s"], "fp64_solves_per_s": r["fp64"]["solves_per_s"],
                 "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
                 "rel_err_vs_fp64": {"mixed": r["mixed"]["rel_err_vs_fp64"], "fp64": r["fp64"]["rel_err_vs_fp64"]},
@@ -829,6 +873,10 @@ def compact(name, r):
                 "iterations": {"mixed": r["mixed"]["iterations"], "fp64": r["fp64"]["iterations"]},
                 "mixed_operator": r["config"]["mixed_operator"], "fp64_operator": r["config"]["fp64_operator"],
                 "roofline": r["roofline"]}
+        for k in ("mixed_reliable", "fp64_gmres"):
+            if k in r:
+                d[k + "_solves_per_s"] = r[k]["solves_per_s"]
+        return d
     if name == "batch":
         return {"workload": r["config"]["workload"], "rhs": r["config"]["rhs"], "solves_per_s": r["value"],
                 "fp64_solves_per_s": r["fp64_solves_per_s"], "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],

@@ -388,7 +388,10 @@ int ibim_apply(ibim_ctx* ctx, int op, const double* pre, const double* pim, cons
     }
     IBIM_HIP(hipMemcpyAsync(d, t.data(), sizeof(double) * 4 * n, hipMemcpyHostToDevice, ctx->stream));
     const int kinds[3] = {OP_DENSE, OP_FREE64, OP_FREE32};
-    int rc = matvec_op(ctx, kinds[op], d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
+    // no A^H outputs: the product A p alone (pair storage sweeps skip A^H q)
+    const bool y_only = !zre && !zim;
+    int rc = y_only ? matvec_op_y(ctx, kinds[op], d, d + 2 * (size_t)n)
+                    : matvec_op(ctx, kinds[op], d, d + n, d + 2 * (size_t)n, d + 3 * (size_t)n);
     if (rc) return rc;
     IBIM_HIP(hipMemcpyAsync(t.data(), d + 2 * (size_t)n, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
     IBIM_HIP(hipStreamSynchronize(ctx->stream));
@@ -514,6 +517,7 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
                    : run_refined(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_RELIABLE)
         return batched ? run_reliable_batched(ctx, opts, info, hist) : run_reliable(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_GMRES) return run_gmres(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
 }
 

@@ -473,11 +473,11 @@ unsigned vec_blocks(int count) {
 
 // One sweep y = A p, z = A^H q of operator `op` into the partial planes.
 static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
-                     const int* done) {
-    if (op == OP_PAIR64) return sweep_pair_launch(ctx, true, p, q, ypart, zpart, done);
+                     const int* done, bool with_z = true) {
+    if (op == OP_PAIR64) return sweep_pair_launch(ctx, true, p, q, ypart, zpart, done, with_z);
     if (free_pair_op(ctx, op)) return sweep_pair_free_launch(ctx, op == OP_FREE32, p, q, ypart, zpart, done);
     if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
-    if (ctx->mat_pair) return sweep_pair_launch(ctx, false, p, q, ypart, zpart, done);
+    if (ctx->mat_pair) return sweep_pair_launch(ctx, false, p, q, ypart, zpart, done, with_z);
     const int count = ctx->count;
     dim3 grid((unsigned)((count + kMvCols - 1) / kMvCols), (unsigned)((count + kMvRows - 1) / kMvRows));
     TimedScope ts(ctx, "matvec");
@@ -755,6 +755,28 @@ int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* 
     a.pt = (double2*)q_dev;
     IBIM_HIP(hipMemsetAsync(a.done, 0, sizeof(int), ctx->stream));
     hipLaunchKernelGGL(k_bicg_dot, dim3(ws.nvb), dim3(kVecThreads), 0, ctx->stream, a);
+    IBIM_HIP(hipGetLastError());
+    return IBIM_OK;
+}
+
+namespace {
+
+__global__ void __launch_bounds__(kVecThreads) k_sum_y(VecArgs a) {
+    for (int j = blockIdx.x * kVecThreads + threadIdx.x; j < a.count; j += gridDim.x * kVecThreads)
+        a.v[j] = sweep_y(a, j);
+}
+
+}  // namespace
+
+int matvec_op_y(Context* ctx, int op, const void* p_dev, void* y_dev) {
+    Workspace ws;
+    int rc = prepare(ctx, 1, &ws);
+    if (rc) return rc;
+    VecArgs a = with_layout(ws.a, ctx, op);
+    rc = sweep_any(ctx, op, (const double2*)p_dev, (const double2*)p_dev, a.ypart, a.zpart, nullptr, false);
+    if (rc) return rc;
+    a.v = (double2*)y_dev;
+    hipLaunchKernelGGL(k_sum_y, dim3(ws.nvb), dim3(kVecThreads), 0, ctx->stream, a);
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
 }

@@ -198,8 +198,9 @@ int launch_assemble(Context* ctx, int precision);
 // with64: keep a complex128 copy of the factors (the FP64 operator of the mixed solvers).
 int launch_assemble_pair(Context* ctx, int precision, bool with64);
 // fp64_copy: sweep the complex128 copy (OP_PAIR64) instead of the primary factors.
+// with_z = false: the v planes (A p) alone.
 int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
-                      const int* done);
+                      const int* done, bool with_z = true);
 // Dense rows of the pair operator into out_dev (nrows x count complex FP64).
 int pair_rows(Context* ctx, int row0, int nrows, double2* out_dev);
 // Y = A P and / or Z = A^H Q for 16 RHS-interleaved vectors over the pair
@@ -271,6 +272,8 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
 int run_bicg(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_reliable(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
+// FP64 refinement around GMRES on the stored operator, column by column (ibim_gmres.hip).
+int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_bicg_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
 int run_refined_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info);
 int run_reliable_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist);
@@ -303,6 +306,9 @@ constexpr int kSweepCols = 1024;
 constexpr int kSweepRows = 512;
 int sweep_free_launch(Context* ctx, int op, const void* p, const void* q, void* ypart, void* zpart, const int* done);
 int matvec_op(Context* ctx, int op, const void* p_dev, const void* q_dev, void* y_dev, void* z_dev);
+// y = A p alone (pair storage: a sweep without the A^H products; other
+// operators: the full sweep, A^H p discarded).
+int matvec_op_y(Context* ctx, int op, const void* p_dev, void* y_dev);
 // The same product with its workspace (partial planes, block partials, a flag)
 // in caller-provided device memory of matvec_scratch_bytes(ctx, op) bytes:
 // allocates nothing.

@@ -366,7 +366,8 @@ enum PairFactors { kStoredFactors = 0, kExactFactors = 1, kFastFactors = 2, kCac
 // One tile of the pair sweep (see the file comment for the planes written).
 // Matrix-free sweeps are held to 2 waves per SIMD: the FP64 factor's registers
 // otherwise leave the Neumann variant at one (14.8 vs 10.1 ms on 64,676 points).
-template <class MT, int Kern, int Factors = kStoredFactors>
+// WithZ = false: v = A p alone (q, the w planes and the A^H products skipped).
+template <class MT, int Kern, int Factors = kStoredFactors, bool WithZ = true>
 __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
     k_pair_sweep(const MT* __restrict__ F, const int2* __restrict__ tiles,
                                                     const PairPoint* __restrict__ pts, int count, size_t ld,
@@ -402,7 +403,7 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
         cw[m] = P.w;
         // F = 0 outside the operator, so clamped loads contribute nothing
         const int jc = j < count ? j : count - 1;
-        const double2 pv = p[jc], qv = q[jc];
+        const double2 pv = p[jc], qv = WithZ ? q[jc] : make_double2(0.0, 0.0);
         pre[m] = pv.x;
         pim[m] = pv.y;
         qre[m] = qv.x;
@@ -441,7 +442,7 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
             const int i = r0 + rr + u;  // < count + kPR: inside the padded point table
             const PairPoint R = pts[i];
             const int ic = i < count ? i : count - 1;
-            const double2 pi = p[ic], qi = q[ic];
+            const double2 pi = p[ic], qi = WithZ ? q[ic] : make_double2(0.0, 0.0);
             double svr = 0.0, svi = 0.0, swr = 0.0, swi = 0.0;
 #pragma unroll
             for (int m = 0; m < kPCpt; ++m) {
@@ -500,12 +501,14 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
                 }
                 svr = fma(ar, pre[m], fma(-ai, pim[m], svr));  // A_ij p_j
                 svi = fma(ar, pim[m], fma(ai, pre[m], svi));
-                swr = fma(br, qre[m], fma(bi, qim[m], swr));  // conj(A_ji) q_j
-                swi = fma(br, qim[m], fma(-bi, qre[m], swi));
                 vre[m] = fma(br, pi.x, fma(-bi, pi.y, vre[m]));  // A_ji p_i
                 vim[m] = fma(br, pi.y, fma(bi, pi.x, vim[m]));
-                wre[m] = fma(ar, qi.x, fma(ai, qi.y, wre[m]));  // conj(A_ij) q_i
-                wim[m] = fma(ar, qi.y, fma(-ai, qi.x, wim[m]));
+                if (WithZ) {
+                    swr = fma(br, qre[m], fma(bi, qim[m], swr));  // conj(A_ji) q_j
+                    swi = fma(br, qim[m], fma(-bi, qre[m], swi));
+                    wre[m] = fma(ar, qi.x, fma(ai, qi.y, wre[m]));  // conj(A_ij) q_i
+                    wim[m] = fma(ar, qi.y, fma(-ai, qi.x, wim[m]));
+                }
             }
             rows[4 * u] = svr;
             rows[4 * u + 1] = svi;
@@ -523,7 +526,7 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
 #pragma unroll
         for (int k = 0; k < 4; ++k) s[k] = (sh[k][0][row] + sh[k][1][row]) + (sh[k][2][row] + sh[k][3][row]);
         ypart[(size_t)tl.y * ld + i] = make_double2(s[0], s[1]);
-        zpart[(size_t)tl.y * ld + i] = make_double2(s[2], s[3]);
+        if (WithZ) zpart[(size_t)tl.y * ld + i] = make_double2(s[2], s[3]);
     }
     const size_t cp = (size_t)(ncol + tl.x) * ld;
 #pragma unroll
@@ -531,7 +534,7 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
         const int j = c0 + t + kPT * m;
         if (j < count) {
             ypart[cp + j] = make_double2(vre[m], vim[m]);
-            zpart[cp + j] = make_double2(wre[m], wim[m]);
+            if (WithZ) zpart[cp + j] = make_double2(wre[m], wim[m]);
         }
     }
 }
@@ -1554,7 +1557,7 @@ int prepare_pair_free(Context* ctx) {
 }
 
 int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q, void* ypart, void* zpart,
-                      const int* done) {
+                      const int* done, bool with_z) {
     if (!ctx->mat_pair || (fp64_copy && !ctx->mat_pair64)) {
         set_error("pair sweep without the pair storage it needs");
         return IBIM_ERR_ARG;
@@ -1575,14 +1578,24 @@ int sweep_pair_launch(Context* ctx, bool fp64_copy, const void* p, const void* q
         constexpr int Kern = decltype(K)::value;
         {
             TimedScope ts(ctx, fp64_copy ? "matvec64" : "matvec");
-            if (wide)
-                hipLaunchKernelGGL((k_pair_sweep<double2, Kern>), dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0,
-                                   ctx->stream, (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>(), tiles, pts, count,
-                                   ld, ncol, P, Q, Y, Z, done, ctx->prm);
-            else
-                hipLaunchKernelGGL((k_pair_sweep<float2, Kern>), dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0,
-                                   ctx->stream, ctx->mat.as<float2>(), tiles, pts, count, ld, ncol, P, Q, Y, Z, done,
-                                   ctx->prm);
+            const dim3 grid((unsigned)ctx->pair_ntiles);
+            if (wide) {
+                const double2* F = (fp64_copy ? ctx->mat64 : ctx->mat).as<double2>();
+                if (with_z)
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern>), grid, dim3(kPT), 0, ctx->stream, F, tiles, pts,
+                                       count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+                else
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kStoredFactors, false>), grid, dim3(kPT), 0,
+                                       ctx->stream, F, tiles, pts, count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+            } else {
+                const float2* F = ctx->mat.as<float2>();
+                if (with_z)
+                    hipLaunchKernelGGL((k_pair_sweep<float2, Kern>), grid, dim3(kPT), 0, ctx->stream, F, tiles, pts,
+                                       count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+                else
+                    hipLaunchKernelGGL((k_pair_sweep<float2, Kern, kStoredFactors, false>), grid, dim3(kPT), 0,
+                                       ctx->stream, F, tiles, pts, count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+            }
         }
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;

@@ -39,7 +39,7 @@ C128_PAIR, C64_PAIR = 2, 3
 MIXED_PAIR = 4
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
 EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO, EVAL_COMBINED = 0, 1, 2, 3
-SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE = 0, 1, 2
+SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE, SOLVE_GMRES = 0, 1, 2, 3
 OP_STORED, OP_FREE64, OP_FREE32 = 0, 1, 2
 
 
@@ -269,6 +269,15 @@ class Problem:
         _check(lib().ibim_apply(self.h, op, *[_p(a) for a in arrs], *[_p(a) for a in out]), "ibim_apply")
         return out[0] + 1j * out[1], out[2] + 1j * out[3]
 
+    def apply_y(self, p, op=0):
+        """y = A p alone (no A^H product)."""
+        p = np.asarray(p, dtype=np.complex128)
+        pre, pim = np.ascontiguousarray(p.real), np.ascontiguousarray(p.imag)
+        yre, yim = np.zeros(self.count), np.zeros(self.count)
+        _check(lib().ibim_apply(self.h, op, _p(pre), _p(pim), None, None, _p(yre), _p(yim), None, None),
+               "ibim_apply")
+        return yre + 1j * yim
+
     def apply_noalloc(self, p_dev, q_dev, y_dev, z_dev, op=0):
         """ibim_apply_noalloc: y = A p, z = A^H q on device buffers (addresses of
         count complex128 values on the handle's device); allocates nothing."""
@@ -316,7 +325,7 @@ class Problem:
         info = SolveInfo()
         hist = np.zeros(max(maxit, max_outer))
         _check(lib().ibim_solve(self.h, C.byref(o), C.byref(info), _p(hist)), "ibim_solve")
-        n = info.iterations if method in (SOLVE_BICG, SOLVE_RELIABLE) and self.nrhs == 1 else info.outer
+        n = info.iterations if method in (SOLVE_BICG, SOLVE_RELIABLE, SOLVE_GMRES) and self.nrhs == 1 else info.outer
         return info, hist[:n]
 
     def density(self, col=0):

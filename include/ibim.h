@@ -115,6 +115,11 @@ typedef struct ibim_params {
 #define IBIM_SOLVE_RELIABLE 2 /* complex-FP32 BiCG with FP64 residual replacement: every */
                               /* inner_tol drop of the residual (default 1e-2) and at    */
                               /* convergence r = b - A64 x; no restarts                  */
+#define IBIM_SOLVE_GMRES 3    /* FP64 iterative refinement around GMRES on the stored    */
+                              /* operator (complex64 storage: the mixed-precision path;  */
+                              /* complex128: FP64 GMRES); one product A p per iteration, */
+                              /* a restart every `chunk` products (0: 256). No            */
+                              /* counterpart in the reference (BiCG only)                 */
 
 typedef struct ibim_solve_opts {
     int method;        /* IBIM_SOLVE_*                                        */
@@ -215,7 +220,9 @@ int ibim_regcount(ibim_ctx *ctx);
  * of an FP64 phase (IBIM_OP_FREE32, traditional kernel only; the combined-field
  * kernel falls back to FP64). On the traditional tube both recompute one
  * radial factor per unordered pair, IBIM_OP_FREE64 with the operations of the
- * complex128 pair storage (the same bits). */
+ * complex128 pair storage (the same bits). With zre and zim both NULL only
+ * y = A p is formed (pair storage: a sweep without the A^H products, y the
+ * same bits). */
 #define IBIM_OP_STORED 0
 #define IBIM_OP_FREE64 1
 #define IBIM_OP_FREE32 2
