@@ -464,11 +464,13 @@ def run_star(args):
     here; BiCG does not converge on it at k = 40).
     FP64 path: batched BiCG on the FP64 matrix-free operator, the fused pair
     pass computing each factor where complex128 pairs (which would need
-    ~490 GB) would be read. Mixed path: batched BiCG on complex64 pair storage
-    (~246 GB, assembled inside the timed region into memory allocated before
-    it) with its 16-column passes on the FP32 matrix cores
-    (--star-fp64-products: on the FP64 ones), FP64 residual replacement by the
-    FP64 matrix-free operator. Reports solves/s for both paths when every
+    ~490 GB) would be read. Mixed path: complex64 pair storage (~246 GB,
+    assembled inside the timed region into memory allocated before it), its
+    16-column passes on the FP32 matrix cores (--star-fp64-products: on the
+    FP64 ones); FP64 iterative refinement around batched GMRES, one pass
+    Y = A P per iteration, the FP64 residuals by the FP64 matrix-free operator
+    (--star-solver reliable: batched BiCG, A P and A^H Q per pass, with FP64
+    residual replacement). Reports solves/s for both paths when every
     column converged (else batched iterations/s and the residuals reached) and
     the mixed densities' largest relative difference from the FP64 ones."""
     n, k, m, its = 256, args.star_k, 16, args.star_iterations
@@ -517,7 +519,12 @@ def run_star(args):
             if not args.star_fp64_products:
                 pb.set_fp32_products(True)
                 out["mixed_operator"] += ", FP32 matrix-core products"
-            info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=STAR_TOL, inner_tol=args.star_replace, batch=0)
+            if args.star_solver == "gmres":
+                info, _ = pb.solve(ibim.SOLVE_GMRES, maxit=its, tol=STAR_TOL, batch=0)
+            else:
+                info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=STAR_TOL, inner_tol=args.star_replace,
+                                   batch=0)
+            out["mixed_solver"] = args.star_solver
         else:
             info, _ = pb.solve(ibim.SOLVE_BICG, maxit=its, tol=STAR_TOL, batch=0)
         pb.sync()
@@ -539,12 +546,14 @@ def run_star(args):
                                          info.status, nl, ms / max(nl, 1)), file=sys.stderr, flush=True)
         pb.close()
     mixed_operator = out.pop("mixed_operator")
+    mixed_solver = out.pop("mixed_solver")
     if args.star_mixed_only:
         r = out["mixed"]
         return {"workload": "star256 mixed path only (k=%g)" % k, "mixed_operator": mixed_operator,
+                "mixed_solver": mixed_solver,
                 "seconds": r["seconds"], "iterations_summed_over_columns": r["info"].iterations,
                 "residual": r["info"].rel_residual, "status": r["info"].status, "pass_ms": r["pass_ms"],
-                "passes": r["passes"], "replacement_passes": r["replacement_passes"]}
+                "passes": r["passes"], "fp64_residual_passes": r["replacement_passes"]}
     converged = {p: out[p]["info"].status == 0 for p in out}
     rel_err = float(max(np.linalg.norm(dens["mixed"][c] - dens["fp64"][c]) / np.linalg.norm(dens["fp64"][c])
                         for c in range(m)))
@@ -559,21 +568,24 @@ def run_star(args):
         ms = 1e3 / rate["mixed"]
         fp64_rate = rate["fp64"]
     npairs = float(out["mixed"]["count"]) * (out["mixed"]["count"] - 1) / 2
-    # complex products per pass: A P and A^H Q, 16 columns, every pair visited
-    # from both of its rows (2 N^2 entries), 8 real flops per complex multiply-add
-    pass_flops = 2.0 * 2 * npairs * 16 * 8
+    # complex products per pass: A P and A^H Q (BiCG; GMRES: A P alone), 16
+    # columns, every pair visited from both of its rows (2 N^2 entries), 8 real
+    # flops per complex multiply-add
+    pass_flops = {p: (1 if p == "mixed" and mixed_solver == "gmres" else 2) * 2.0 * npairs * 16 * 8 for p in out}
     return ({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "mixed: complex64 pair storage (else fp32 sine / cosine of an fp64 phase), entries and products "
-                   "on the fp32 matrix cores summed in fp64 per 64 columns, fp64 residual replacement; fp64: fp64 "
-                   "matrix-free factors, fp64 matrix-core products",
+                   "on the fp32 matrix cores summed in fp64 per 64 columns, %s; fp64: fp64 "
+                   "matrix-free factors, fp64 matrix-core products, the reference's BiCG" % (
+                       "GMRES with fp64 iterative refinement" if mixed_solver == "gmres" else
+                       "BiCG with fp64 residual replacement"),
           "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
           "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, tol 1e-7, "
                                  "%s, %d-iteration budget" % (
                                      k, "combined field 1/2 I + K - i eta S, eta = k" if combined else
                                      "double layer 1/2 I + K", its),
                      "tube_points": out["mixed"]["count"], "parallelism": "single GPU",
-                     "mixed_operator": mixed_operator,
+                     "mixed_operator": mixed_operator, "mixed_solver": mixed_solver,
                      "fp64_operator": "matrix-free (fused pair pass, factors computed)%s" % (
                          ", complex128 pairs of the first row blocks cached (%.0f GB of %.0f GB)" % (
                              out["fp64"]["cache_bytes"] / 1e9, pair_bytes(out["fp64"]["count"], 16) / 1e9)
@@ -584,9 +596,10 @@ def run_star(args):
           "iterations_summed_over_columns": {p: out[p]["info"].iterations for p in out},
           "residual": {p: out[p]["info"].rel_residual for p in out}, "converged": converged,
           "pass_ms": {p: out[p]["pass_ms"] for p in out},
-          "pass_tflops": {p: pass_flops / (out[p]["pass_ms"] * 1e-3) / 1e12 for p in out if out[p]["pass_ms"] > 0},
+          "pass_tflops": {p: pass_flops[p] / (out[p]["pass_ms"] * 1e-3) / 1e12 for p in out
+                          if out[p]["pass_ms"] > 0},
           "mixed_assemble_ms": out["mixed"]["assemble_ms"],
-          "mixed_replacement_passes": out["mixed"]["replacement_passes"],
+          "mixed_fp64_residual_passes": out["mixed"]["replacement_passes"],
           "setup_s": out["mixed"]["setup"]})
 
 
@@ -646,7 +659,11 @@ def main():
     ap.add_argument("--star-fp64-cache", type=float, default=STAR_FP64_CACHE_BYTES,
                     help="star256 FP64 path: bytes of complex128 pairs the matrix-free operator keeps (0: none)")
     ap.add_argument("--star-replace", type=float, default=STAR_REPLACE,
-                    help="star256 mixed path: residual drop between FP64 residual replacements")
+                    help="star256 mixed path with --star-solver reliable: residual drop between FP64 residual "
+                         "replacements")
+    ap.add_argument("--star-solver", choices=("gmres", "reliable"), default="gmres",
+                    help="star256 mixed path: FP64 refinement around batched GMRES (one product A P per "
+                         "iteration) or batched BiCG with FP64 residual replacement")
     ap.add_argument("--batch-fp32-products", action="store_true",
                     help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
     ap.add_argument("--batch-graphs", action="store_true",
@@ -895,6 +912,7 @@ def compact(name, r):
     d["workload"] = r["config"]["workload"]
     d["tube_points"] = r["config"]["tube_points"]
     d["mixed_operator"] = r["config"]["mixed_operator"]
+    d["mixed_solver"] = r["config"]["mixed_solver"]
     return d
 
 

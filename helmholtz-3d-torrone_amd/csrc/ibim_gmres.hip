@@ -1,5 +1,5 @@
 // ibim_gmres.hip -- IBIM_SOLVE_GMRES: FP64 iterative refinement around GMRES
-// on the stored operator.
+// on the stored operator, for one right-hand side at a time or for 16 at once.
 //
 // GMRES (Saad and Schultz, SIAM J. Sci. Stat. Comput. 7, 1986) builds an
 // orthonormal Krylov basis -- here by classical Gram-Schmidt applied twice
@@ -15,31 +15,55 @@
 // to tol ||b||; x += d; r = b - A64 x with the FP64 operator } until
 // ||r|| / ||b|| < tol. On complex64 storage (IBIM_C64_PAIR, IBIM_MIXED_PAIR,
 // IBIM_C64) that is the mixed-precision solver (entries rounded to complex64,
-// products and vectors FP64, residuals FP64); on complex128 storage the FP64
-// operator is the stored one and a single GMRES run is the FP64 solver.
+// residuals FP64); on complex128 storage the FP64 operator is the stored one
+// and a single GMRES run is the FP64 solver. With nothing stored, GMRES runs on
+// the mixed matrix-free operator (OP_FREE32) and the residuals on the FP64
+// matrix-free one.
+//
+// Several right-hand sides (ibim_set_nrhs, opts->batch == 0) run 16 columns
+// per batch: the vectors are right-hand-side interleaved (point j, column r at
+// j * 16 + r), one 16-column pass of ibim_batch.hip / ibim_pair.hip applies
+// the operator to all of them (Y = A P only; on complex64 pairs with
+// ibim_set_fp32_products, on the FP32 matrix cores), and each column keeps its
+// own Hessenberg matrix, rotations and stop rule. A column that has reached
+// its target gets zero basis vectors from then on (its products are exact
+// zeros), so it neither moves nor disturbs the others. The Gram-Schmidt
+// kernels are the single-vector ones with a column index: with one column
+// they do the same operations in the same order.
 #include <algorithm>
 #include <cmath>
 #include <complex>
 #include <vector>
 
 #include "ibim_internal.hpp"
+#include "ibim_mfma.hpp"
 
 namespace ibim {
 
 namespace {
 
 constexpr int kGT = 256;          // threads per block of the vector kernels
-constexpr int kMaxBasis = 256;    // Krylov vectors per GMRES cycle (then restart)
+constexpr int kMaxBasis = 256;    // Krylov vectors per GMRES cycle (then restart), one column
+constexpr int kMaxBasisB = 128;   // the same for 16-column batches (vectors 16x as long)
+constexpr int kDotChunks = 16;    // blocks per basis vector in the dot products
 
-// part[i][c] = sum over chunk c of conj(V_i[e]) w[e]: block (c, i) takes
-// every kDotChunks-th group of kGT elements; a fixed-order tree over the block.
-constexpr int kDotChunks = 16;
-__global__ void __launch_bounds__(kGT) k_gm_dots(const double2* __restrict__ V, size_t ld, int count,
+template <int C>
+constexpr int max_basis() {
+    return C == 1 ? kMaxBasis : kMaxBasisB;
+}
+
+// part[(i kDotChunks + c) C + col] = sum over chunk c of conj(V_i[e]) w[e] over
+// the elements e of column col (e % C == col): block (c, i) takes every
+// kDotChunks-th group of kGT elements; a fixed-order tree over the block down
+// to one sum per column (kGT is a multiple of C, so thread t only ever sees
+// column t % C).
+template <int C>
+__global__ void __launch_bounds__(kGT) k_gm_dots(const double2* __restrict__ V, size_t vs, int n,
                                                  const double2* __restrict__ w, double2* __restrict__ part) {
     __shared__ double sr[kGT], si[kGT];
-    const double2* v = V + (size_t)blockIdx.y * ld;
+    const double2* v = V + (size_t)blockIdx.y * vs;
     double ar = 0.0, ai = 0.0;
-    for (int e = blockIdx.x * kGT + threadIdx.x; e < count; e += kDotChunks * kGT) {
+    for (int e = blockIdx.x * kGT + threadIdx.x; e < n; e += kDotChunks * kGT) {
         const double2 a = v[e], b = w[e];
         ar += a.x * b.x + a.y * b.y;
         ai += a.x * b.y - a.y * b.x;
@@ -47,40 +71,46 @@ __global__ void __launch_bounds__(kGT) k_gm_dots(const double2* __restrict__ V, 
     sr[threadIdx.x] = ar;
     si[threadIdx.x] = ai;
     __syncthreads();
-    for (int s = kGT / 2; s > 0; s >>= 1) {
+    for (int s = kGT / 2; s >= C; s >>= 1) {
         if (threadIdx.x < s) {
             sr[threadIdx.x] += sr[threadIdx.x + s];
             si[threadIdx.x] += si[threadIdx.x + s];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) part[(size_t)blockIdx.y * kDotChunks + blockIdx.x] = make_double2(sr[0], si[0]);
+    if (threadIdx.x < C)
+        part[((size_t)blockIdx.y * kDotChunks + blockIdx.x) * C + threadIdx.x] =
+            make_double2(sr[threadIdx.x], si[threadIdx.x]);
 }
 
-// out[i] = sum_c part[i][c] in order (one thread per i)
+// out[i C + col] = sum_c part[(i kDotChunks + c) C + col] in order (one thread per entry)
+template <int C>
 __global__ void k_gm_dots_sum(const double2* __restrict__ part, int n, double2* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n * C) return;
+    const int i = t / C, col = t % C;
     double sr = 0.0, si = 0.0;
     for (int c = 0; c < kDotChunks; ++c) {
-        sr += part[(size_t)i * kDotChunks + c].x;
-        si += part[(size_t)i * kDotChunks + c].y;
+        sr += part[((size_t)i * kDotChunks + c) * C + col].x;
+        si += part[((size_t)i * kDotChunks + c) * C + col].y;
     }
-    out[i] = make_double2(sr, si);
+    out[t] = make_double2(sr, si);
 }
 
-// w[e] += sign sum_{i < n} V_i[e] h[i]  (n <= kMaxBasis + 1)
-__global__ void __launch_bounds__(kGT) k_gm_combine(const double2* __restrict__ V, size_t ld, int count,
-                                                    const double2* __restrict__ h, int n, double sign,
+// w[e] += sign sum_{i < nv} V_i[e] h[i C + e % C]  (nv <= max_basis<C>() + 1)
+template <int C>
+__global__ void __launch_bounds__(kGT) k_gm_combine(const double2* __restrict__ V, size_t vs, int n,
+                                                    const double2* __restrict__ h, int nv, double sign,
                                                     double2* __restrict__ w) {
-    __shared__ double2 hs[kMaxBasis + 1];
-    for (int i = threadIdx.x; i < n; i += kGT) hs[i] = h[i];
+    __shared__ double2 hs[(max_basis<C>() + 1) * C];
+    for (int i = threadIdx.x; i < nv * C; i += kGT) hs[i] = h[i];
     __syncthreads();
     const int e = blockIdx.x * kGT + threadIdx.x;
-    if (e >= count) return;
+    if (e >= n) return;
+    const int col = e % C;
     double cr = 0.0, ci = 0.0;
-    for (int i = 0; i < n; ++i) {
-        const double2 v = V[(size_t)i * ld + e], c = hs[i];
+    for (int i = 0; i < nv; ++i) {
+        const double2 v = V[(size_t)i * vs + e], c = hs[i * C + col];
         cr += v.x * c.x - v.y * c.y;
         ci += v.x * c.y + v.y * c.x;
     }
@@ -88,176 +118,294 @@ __global__ void __launch_bounds__(kGT) k_gm_combine(const double2* __restrict__ 
     w[e] = make_double2(o.x + sign * cr, o.y + sign * ci);
 }
 
-// v = s w (s real); out = a - b
-__global__ void __launch_bounds__(kGT) k_gm_scale(const double2* __restrict__ w, double s, int count,
+// One real factor per column.
+template <int C>
+struct Scales {
+    double s[C];
+};
+
+// v[e] = s[e % C] w[e]
+template <int C>
+__global__ void __launch_bounds__(kGT) k_gm_scale(const double2* __restrict__ w, Scales<C> s, int n,
                                                   double2* __restrict__ v) {
     const int e = blockIdx.x * kGT + threadIdx.x;
-    if (e < count) v[e] = make_double2(s * w[e].x, s * w[e].y);
+    if (e < n) {
+        const double f = s.s[e % C];
+        v[e] = make_double2(f * w[e].x, f * w[e].y);
+    }
 }
 
+// out = a - b; x += d
 __global__ void __launch_bounds__(kGT) k_gm_sub(const double2* __restrict__ a, const double2* __restrict__ b,
-                                                int count, double2* __restrict__ out) {
+                                                int n, double2* __restrict__ out) {
     const int e = blockIdx.x * kGT + threadIdx.x;
-    if (e < count) out[e] = make_double2(a[e].x - b[e].x, a[e].y - b[e].y);
+    if (e < n) out[e] = make_double2(a[e].x - b[e].x, a[e].y - b[e].y);
+}
+
+__global__ void __launch_bounds__(kGT) k_gm_add(double2* __restrict__ x, const double2* __restrict__ d, int n) {
+    const int e = blockIdx.x * kGT + threadIdx.x;
+    if (e < n) x[e] = make_double2(x[e].x + d[e].x, x[e].y + d[e].y);
+}
+
+// Columns c0 .. c0 + ncol - 1 of the column-major right-hand sides into a
+// 16-column interleaved block (other columns zero), and back.
+__global__ void k_gm_pack(const double2* __restrict__ rhs, int count, int c0, int ncol, double2* __restrict__ B) {
+    const size_t e = (size_t)blockIdx.x * kGT + threadIdx.x;
+    if (e >= (size_t)count * kBatchRHS) return;
+    const int j = (int)(e / kBatchRHS), r = (int)(e % kBatchRHS);
+    B[e] = r < ncol ? rhs[(size_t)(c0 + r) * count + j] : make_double2(0.0, 0.0);
+}
+
+__global__ void k_gm_unpack(const double2* __restrict__ X, int count, int c0, int ncol, double2* __restrict__ out) {
+    const size_t e = (size_t)blockIdx.x * kGT + threadIdx.x;
+    if (e >= (size_t)count * kBatchRHS) return;
+    const int j = (int)(e / kBatchRHS), r = (int)(e % kBatchRHS);
+    if (r < ncol) out[(size_t)(c0 + r) * count + j] = X[e];
 }
 
 typedef std::complex<double> cplx;
 
+// Device workspace of one solve. Vectors hold C interleaved columns of ld rows
+// (vs = ld C elements; padding rows stay zero).
 struct GmresSpace {
-    double2 *V, *w, *z, *d, *r, *y, *coef;  // basis (kMaxBasis + 1 vectors), work vectors, coefficients
-    double2* part;                          // dot-product partials, (kMaxBasis + 1) x kDotChunks
+    double2 *V, *w, *d, *r;  // basis (max_basis + 1 vectors), A v, the correction, GMRES's residual
+    double2 *res, *b, *x;    // refinement: residual, right-hand side (batched: packed), solution
+    double2 *y, *coef;       // combination coefficients, Gram-Schmidt coefficients and norms
+    double2* part;           // dot-product partials
+    size_t vs;               // elements per vector
+    int n;                   // elements in use (count C)
 };
 
-// out[i] = V_i^H w for i < n (device)
-void dots(Context* ctx, const GmresSpace& g, const double2* V, int n, const double2* w, double2* out) {
-    hipLaunchKernelGGL(k_gm_dots, dim3(kDotChunks, (unsigned)n), dim3(kGT), 0, ctx->stream, V, ctx->ld, ctx->count,
-                       w, g.part);
-    hipLaunchKernelGGL(k_gm_dots_sum, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, ctx->stream, g.part, n, out);
+template <int C>
+size_t space_bytes(const Context* ctx) {
+    const int mb = max_basis<C>();
+    const size_t vs = ctx->ld * C;
+    return sizeof(double2) * (vs * ((size_t)mb + 1 + 6) + (size_t)C * (2 * (mb + 1) + 1) + (size_t)C * mb +
+                              (size_t)C * (mb + 1) * kDotChunks) +
+           512;
 }
 
-// ||v|| by the dot kernels (host value; synchronizes)
-int norm_of(Context* ctx, const GmresSpace& g, const double2* v, double* out) {
-    dots(ctx, g, v, 1, v, g.coef);
-    double2 h;
-    IBIM_HIP(hipMemcpyAsync(&h, g.coef, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
-    IBIM_HIP(hipStreamSynchronize(ctx->stream));
-    *out = sqrt(h.x);
+template <int C>
+int make_space(Context* ctx, GmresSpace* g) {
+    const int mb = max_basis<C>();
+    if (ctx->refine.ensure(space_bytes<C>(ctx))) return IBIM_ERR_NOMEM;
+    g->vs = ctx->ld * C;
+    g->n = ctx->count * C;
+    g->V = ctx->refine.as<double2>();
+    g->w = g->V + ((size_t)mb + 1) * g->vs;
+    g->d = g->w + g->vs;
+    g->r = g->d + g->vs;
+    g->res = g->r + g->vs;
+    g->b = g->res + g->vs;
+    g->x = g->b + g->vs;
+    g->coef = g->x + g->vs;
+    g->y = g->coef + (size_t)C * (2 * (mb + 1) + 1);
+    g->part = g->y + (size_t)C * mb;
+    // padding rows of every vector read as zero in the operator passes
+    IBIM_HIP(hipMemsetAsync(g->V, 0, sizeof(double2) * g->vs * ((size_t)mb + 1 + 6), ctx->stream));
     return IBIM_OK;
 }
 
-// GMRES(m), m <= kMaxBasis, for A d = rhs with operator `op`, from d = 0, until
-// the residual estimate is below target (absolute) or `budget` products are used.
-// Returns the products applied and the last estimate.
-int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, double target, int budget, int m,
-          int* products, double* estimate, double* hist, int* hist_at, int hist_len, double normb) {
-    const int count = ctx->count;
-    const size_t ld = ctx->ld;
-    const unsigned nb = (unsigned)((count + kGT - 1) / kGT);
-    IBIM_HIP(hipMemsetAsync(g.d, 0, sizeof(double2) * ld, ctx->stream));
-    *products = 0;
-    std::vector<cplx> H((size_t)(kMaxBasis + 1) * kMaxBasis), cs(kMaxBasis), sn(kMaxBasis), gv(kMaxBasis + 1);
-    std::vector<double2> hh(2 * (kMaxBasis + 1) + 1);
-    auto Hc = [&](int i, int j) -> cplx& { return H[(size_t)j * (kMaxBasis + 1) + i]; };
+// out[i C + col] = V_i^H w per column, i < nv (device)
+template <int C>
+void dots(Context* ctx, const GmresSpace& g, const double2* V, int nv, const double2* w, double2* out) {
+    hipLaunchKernelGGL(k_gm_dots<C>, dim3(kDotChunks, (unsigned)nv), dim3(kGT), 0, ctx->stream, V, g.vs, g.n, w,
+                       g.part);
+    hipLaunchKernelGGL(k_gm_dots_sum<C>, dim3((unsigned)((nv * C + 63) / 64)), dim3(64), 0, ctx->stream, g.part, nv,
+                       out);
+}
+
+// ||v|| per column (host values; synchronizes)
+template <int C>
+int norms_of(Context* ctx, const GmresSpace& g, const double2* v, double* out) {
+    dots<C>(ctx, g, v, 1, v, g.coef);
+    double2 h[C];
+    IBIM_HIP(hipMemcpyAsync(h, g.coef, sizeof h, hipMemcpyDeviceToHost, ctx->stream));
+    IBIM_HIP(hipStreamSynchronize(ctx->stream));
+    for (int c = 0; c < C; ++c) out[c] = sqrt(h[c].x);
+    return IBIM_OK;
+}
+
+// The operator on C-column vectors: the single-vector Y-only sweep, or the
+// 16-column pass (Y = A P alone).
+template <int C>
+int apply_op(Context* ctx, int op, const double2* p, double2* y) {
+    if constexpr (C == 1)
+        return matvec_op_y(ctx, op, p, y);
+    else
+        return batch_apply(ctx, op, false, p, y, nullptr);
+}
+
+// GMRES(m), m <= max_basis<C>(), for A d = rhs per column with operator `op`,
+// from d = 0, until each column's residual estimate is below its target
+// (absolute; +inf leaves a column out) or `budget` products are used. Adds the
+// products each column took part in to col_products and returns the passes
+// applied and the last estimates. hist (column 0) gets estimate / normb.
+template <int C>
+int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const double* target, int budget, int m,
+          int* passes, int* col_products, double* estimate, double* hist, int* hist_at, int hist_len, double normb) {
+    constexpr int MB = max_basis<C>();
+    const int n = g.n;
+    const unsigned nb = (unsigned)((n + kGT - 1) / kGT);
+    IBIM_HIP(hipMemsetAsync(g.d, 0, sizeof(double2) * g.vs, ctx->stream));
+    *passes = 0;
+    // per column: Hessenberg (column-major, MB + 1 rows), rotations, rotated residual
+    std::vector<cplx> H((size_t)C * (MB + 1) * MB), cs((size_t)C * MB), sn((size_t)C * MB), gv((size_t)C * (MB + 1));
+    std::vector<double2> hh((size_t)C * (2 * (MB + 1) + 1));
+    auto Hc = [&](int c, int i, int j) -> cplx& { return H[((size_t)c * MB + j) * (MB + 1) + i]; };
+    bool active[C];
+    int ncols[C];
+    double hn[C];
     bool first = true;
     for (;;) {
         // r0 = rhs - A d (the first cycle: rhs)
         if (first) {
-            IBIM_HIP(hipMemcpyAsync(g.r, rhs, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
+            IBIM_HIP(hipMemcpyAsync(g.r, rhs, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
         } else {
-            if (*products >= budget) return IBIM_OK;
-            if (int rc = matvec_op_y(ctx, op, g.d, g.w)) return rc;
-            ++*products;
-            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, rhs, g.w, count, g.r);
+            if (*passes >= budget) return IBIM_OK;
+            if (int rc = apply_op<C>(ctx, op, g.d, g.w)) return rc;
+            ++*passes;
+            for (int c = 0; c < C; ++c) col_products[c] += active[c];
+            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, rhs, g.w, n, g.r);
         }
         first = false;
-        double beta;
-        if (int rc = norm_of(ctx, g, g.r, &beta)) return rc;
-        *estimate = beta;
-        if (!(beta > target)) return IBIM_OK;
-        hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kGT), 0, ctx->stream, g.r, 1.0 / beta, count, g.V);
-        std::fill(gv.begin(), gv.end(), cplx(0.0));
-        gv[0] = beta;
+        double beta[C];
+        if (int rc = norms_of<C>(ctx, g, g.r, beta)) return rc;
+        Scales<C> sc;
+        bool any = false;
+        for (int c = 0; c < C; ++c) {
+            estimate[c] = beta[c];
+            active[c] = beta[c] > target[c];
+            sc.s[c] = active[c] ? 1.0 / beta[c] : 0.0;
+            any = any || active[c];
+            ncols[c] = 0;
+            std::fill(gv.begin() + (size_t)c * (MB + 1), gv.begin() + (size_t)(c + 1) * (MB + 1), cplx(0.0));
+            gv[(size_t)c * (MB + 1)] = beta[c];
+        }
+        if (!any) return IBIM_OK;
+        hipLaunchKernelGGL(k_gm_scale<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.r, sc, n, g.V);
         int j = 0;
-        bool stop = false;
-        for (; j < m && *products < budget; ++j) {
-            double2* vj = g.V + (size_t)j * ld;
-            if (int rc = matvec_op_y(ctx, op, vj, g.w)) return rc;
-            ++*products;
+        for (; j < m && *passes < budget; ++j) {
+            double2* vj = g.V + (size_t)j * g.vs;
+            if (int rc = apply_op<C>(ctx, op, vj, g.w)) return rc;
+            ++*passes;
+            for (int c = 0; c < C; ++c) col_products[c] += active[c];
             // CGS2: h = V^H w, w -= V h, twice
             for (int pass = 0; pass < 2; ++pass) {
-                double2* coef = g.coef + (size_t)pass * (kMaxBasis + 1);
-                dots(ctx, g, g.V, j + 1, g.w, coef);
-                hipLaunchKernelGGL(k_gm_combine, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, ld, count, coef, j + 1,
+                double2* coef = g.coef + (size_t)pass * (MB + 1) * C;
+                dots<C>(ctx, g, g.V, j + 1, g.w, coef);
+                hipLaunchKernelGGL(k_gm_combine<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, g.vs, n, coef, j + 1,
                                    -1.0, g.w);
             }
-            double2* nrm = g.coef + 2 * (size_t)(kMaxBasis + 1);
-            dots(ctx, g, g.w, 1, g.w, nrm);
+            double2* nrm = g.coef + 2 * (size_t)(MB + 1) * C;
+            dots<C>(ctx, g, g.w, 1, g.w, nrm);
             IBIM_HIP(hipGetLastError());
-            IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * (2 * (kMaxBasis + 1) + 1),
-                                    hipMemcpyDeviceToHost, ctx->stream));
+            IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * hh.size(), hipMemcpyDeviceToHost,
+                                    ctx->stream));
             IBIM_HIP(hipStreamSynchronize(ctx->stream));
-            for (int i = 0; i <= j; ++i)
-                Hc(i, j) = cplx(hh[i].x + hh[kMaxBasis + 1 + i].x, hh[i].y + hh[kMaxBasis + 1 + i].y);
-            const double hn = sqrt(hh[2 * (kMaxBasis + 1)].x);
-            Hc(j + 1, j) = hn;
-            // previous rotations, then the one that zeroes H(j + 1, j)
-            for (int i = 0; i < j; ++i) {
-                const cplx t = cs[i] * Hc(i, j) + sn[i] * Hc(i + 1, j);
-                Hc(i + 1, j) = -std::conj(sn[i]) * Hc(i, j) + std::conj(cs[i]) * Hc(i + 1, j);
-                Hc(i, j) = t;
+            any = false;
+            for (int c = 0; c < C; ++c) {
+                if (!active[c]) {
+                    sc.s[c] = 0.0;
+                    continue;
+                }
+                cplx* gc = gv.data() + (size_t)c * (MB + 1);
+                cplx* csc = cs.data() + (size_t)c * MB;
+                cplx* snc = sn.data() + (size_t)c * MB;
+                for (int i = 0; i <= j; ++i) {
+                    const double2 a = hh[(size_t)i * C + c], b = hh[((size_t)(MB + 1) + i) * C + c];
+                    Hc(c, i, j) = cplx(a.x + b.x, a.y + b.y);
+                }
+                hn[c] = sqrt(hh[2 * (size_t)(MB + 1) * C + c].x);
+                Hc(c, j + 1, j) = hn[c];
+                // previous rotations, then the one that zeroes H(j + 1, j)
+                for (int i = 0; i < j; ++i) {
+                    const cplx t = csc[i] * Hc(c, i, j) + snc[i] * Hc(c, i + 1, j);
+                    Hc(c, i + 1, j) = -std::conj(snc[i]) * Hc(c, i, j) + std::conj(csc[i]) * Hc(c, i + 1, j);
+                    Hc(c, i, j) = t;
+                }
+                const cplx a = Hc(c, j, j), bb = Hc(c, j + 1, j);
+                const double rr = sqrt(std::norm(a) + std::norm(bb));
+                csc[j] = rr > 0.0 ? std::conj(a) / rr : cplx(1.0);
+                snc[j] = rr > 0.0 ? std::conj(bb) / rr : cplx(0.0);
+                Hc(c, j, j) = rr;
+                Hc(c, j + 1, j) = 0.0;
+                gc[j + 1] = -std::conj(snc[j]) * gc[j];
+                gc[j] = csc[j] * gc[j];
+                estimate[c] = std::abs(gc[j + 1]);
+                if (c == 0 && hist && *hist_at < hist_len) hist[(*hist_at)++] = estimate[c] / normb;
+                if (!(estimate[c] > target[c]) || !(hn[c] > 0.0) || estimate[c] != estimate[c]) {
+                    active[c] = false;  // done with j + 1 basis vectors
+                    ncols[c] = j + 1;
+                    sc.s[c] = 0.0;
+                } else {
+                    sc.s[c] = 1.0 / hn[c];
+                    any = true;
+                }
             }
-            const cplx a = Hc(j, j), bb = Hc(j + 1, j);
-            const double rr = sqrt(std::norm(a) + std::norm(bb));
-            cs[j] = rr > 0.0 ? std::conj(a) / rr : cplx(1.0);
-            sn[j] = rr > 0.0 ? std::conj(bb) / rr : cplx(0.0);
-            Hc(j, j) = rr;
-            Hc(j + 1, j) = 0.0;
-            gv[j + 1] = -std::conj(sn[j]) * gv[j];
-            gv[j] = cs[j] * gv[j];
-            *estimate = std::abs(gv[j + 1]);
-            if (hist && *hist_at < hist_len) hist[(*hist_at)++] = *estimate / normb;
-            if (!(*estimate > target) || !(hn > 0.0) || *estimate != *estimate) {
-                stop = true;
+            if (!any) {
                 ++j;
                 break;
             }
             if (j + 1 < m)
-                hipLaunchKernelGGL(k_gm_scale, dim3(nb), dim3(kGT), 0, ctx->stream, g.w, 1.0 / hn, count,
-                                   g.V + (size_t)(j + 1) * ld);
+                hipLaunchKernelGGL(k_gm_scale<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.w, sc, n,
+                                   g.V + (size_t)(j + 1) * g.vs);
         }
-        // y = R^{-1} g over the j columns, d += V y
-        const int n = j;
-        std::vector<cplx> yv(n);
-        for (int i = n - 1; i >= 0; --i) {
-            cplx s = gv[i];
-            for (int c = i + 1; c < n; ++c) s -= Hc(i, c) * yv[c];
-            yv[i] = Hc(i, i) != cplx(0.0) ? s / Hc(i, i) : cplx(0.0);
+        // y = R^{-1} g over each column's basis vectors, d += V y
+        bool again = false;
+        int nmax = 0;
+        for (int c = 0; c < C; ++c) {
+            if (active[c]) {
+                ncols[c] = j;
+                again = true;
+            }
+            nmax = std::max(nmax, ncols[c]);
         }
-        std::vector<double2> yd(n);
-        for (int i = 0; i < n; ++i) yd[i] = make_double2(yv[i].real(), yv[i].imag());
-        if (n > 0) {
-            IBIM_HIP(hipMemcpyAsync(g.y, yd.data(), sizeof(double2) * n, hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(k_gm_combine, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, ld, count, g.y, n, 1.0, g.d);
+        std::vector<double2> yd((size_t)nmax * C, make_double2(0.0, 0.0));
+        for (int c = 0; c < C; ++c) {
+            const int nc = ncols[c];
+            const cplx* gc = gv.data() + (size_t)c * (MB + 1);
+            std::vector<cplx> yv(nc);
+            for (int i = nc - 1; i >= 0; --i) {
+                cplx s = gc[i];
+                for (int k = i + 1; k < nc; ++k) s -= Hc(c, i, k) * yv[k];
+                yv[i] = Hc(c, i, i) != cplx(0.0) ? s / Hc(c, i, i) : cplx(0.0);
+            }
+            for (int i = 0; i < nc; ++i) yd[(size_t)i * C + c] = make_double2(yv[i].real(), yv[i].imag());
+        }
+        if (nmax > 0) {
+            IBIM_HIP(hipMemcpyAsync(g.y, yd.data(), sizeof(double2) * yd.size(), hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL(k_gm_combine<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, g.vs, n, g.y, nmax, 1.0,
+                               g.d);
             IBIM_HIP(hipGetLastError());
             IBIM_HIP(hipStreamSynchronize(ctx->stream));  // yd goes out of scope
         }
-        if (stop || *products >= budget) return IBIM_OK;
+        // a restart for the columns that ran out of basis vectors
+        if (!again || *passes >= budget) return IBIM_OK;
     }
 }
 
-__global__ void __launch_bounds__(kGT) k_gm_add(double2* __restrict__ x, const double2* __restrict__ d, int count) {
-    const int e = blockIdx.x * kGT + threadIdx.x;
-    if (e < count) x[e] = make_double2(x[e].x + d[e].x, x[e].y + d[e].y);
-}
-
-}  // namespace
-
-int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+// The operators of a refinement: GMRES on the stored one, residuals with the
+// FP64 one (complex128 storage: itself); with nothing stored, GMRES on the
+// mixed matrix-free operator and residuals with the FP64 matrix-free one.
+void refinement_ops(const Context* ctx, int* inner, int* outer) {
     if (ctx->mat_precision < 0) {
-        set_error("IBIM_SOLVE_GMRES runs on a stored operator (ibim_assemble)");
-        return IBIM_ERR_ARG;
+        *inner = OP_FREE32;
+        *outer = OP_FREE64;
+        return;
     }
+    *inner = OP_DENSE;
+    *outer = ctx->mat_precision == IBIM_C128 ? OP_DENSE : fp64_operator(ctx);
+}
+
+// One right-hand side at a time.
+int gmres_columns(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    int inner, outer_op;
+    refinement_ops(ctx, &inner, &outer_op);
     const int count = ctx->count;
-    const size_t ld = ctx->ld;
     const unsigned nb = (unsigned)((count + kGT - 1) / kGT);
-    const int inner = OP_DENSE;
-    const int outer_op = ctx->mat_precision == IBIM_C128 ? OP_DENSE : fp64_operator(ctx);
-    const size_t nvec = (size_t)(kMaxBasis + 1) + 5;
-    if (ctx->refine.ensure(sizeof(double2) * ld * nvec + sizeof(double2) * (2 * (kMaxBasis + 1) + 1) +
-                           sizeof(double2) * kMaxBasis + sizeof(double2) * (kMaxBasis + 1) * kDotChunks + 512))
-        return IBIM_ERR_NOMEM;
     GmresSpace g;
-    g.V = ctx->refine.as<double2>();
-    g.w = g.V + (size_t)(kMaxBasis + 1) * ld;
-    g.z = g.w + ld;
-    g.d = g.z + ld;
-    g.r = g.d + ld;
-    double2* res = g.r + ld;  // b - A64 x
-    g.coef = res + ld;
-    g.y = g.coef + 2 * (kMaxBasis + 1) + 1;
-    g.part = g.y + kMaxBasis;
-    // padding rows of the basis read as zero in the sweeps
-    IBIM_HIP(hipMemsetAsync(g.V, 0, sizeof(double2) * ld * nvec, ctx->stream));
+    if (int rc = make_space<1>(ctx, &g)) return rc;
     const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
     // the basis length: opts->chunk when set (a restart every chunk products), else kMaxBasis
     const int m = opts->chunk > 0 ? std::min(opts->chunk, kMaxBasis) : kMaxBasis;
@@ -267,26 +415,27 @@ int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, 
         const double2* b = ctx->rhs.as<double2>() + (size_t)c * count;
         double2* x = ctx->density.as<double2>() + (size_t)c * count;
         IBIM_HIP(hipMemsetAsync(x, 0, sizeof(double2) * count, ctx->stream));
-        IBIM_HIP(hipMemcpyAsync(res, b, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(g.res, b, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
         double normb;
-        if (int rc = norm_of(ctx, g, res, &normb)) return rc;
+        if (int rc = norms_of<1>(ctx, g, g.res, &normb)) return rc;
         if (normb == 0.0) continue;  // zero data: zero density, no iterations
         int col_status = 1, used = 0, outer = 0;
         double relres = 1.0;
         for (outer = 1; outer <= max_outer && used < opts->maxit; ++outer) {
-            int products = 0;
+            int products = 0, passes = 0;
             double est = 0.0;
             // the correction only has to bring the residual to tol ||b||
-            if (int rc = gmres(ctx, g, inner, res, opts->tol * normb, opts->maxit - used, m, &products, &est,
-                               c == 0 ? hist : nullptr, &hist_at, opts->maxit, normb))
+            const double target = opts->tol * normb;
+            if (int rc = gmres<1>(ctx, g, inner, g.res, &target, opts->maxit - used, m, &passes, &products, &est,
+                                  c == 0 ? hist : nullptr, &hist_at, opts->maxit, normb))
                 return rc;
-            used += products;
+            used += passes;
             hipLaunchKernelGGL(k_gm_add, dim3(nb), dim3(kGT), 0, ctx->stream, x, g.d, count);
             if (int rc = matvec_op_y(ctx, outer_op, x, g.w)) return rc;
-            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, b, g.w, count, res);
+            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, b, g.w, count, g.res);
             IBIM_HIP(hipGetLastError());
             double rn;
-            if (int rc = norm_of(ctx, g, res, &rn)) return rc;
+            if (int rc = norms_of<1>(ctx, g, g.res, &rn)) return rc;
             relres = rn / normb;
             if (relres != relres) {
                 col_status = 2;
@@ -309,6 +458,97 @@ int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, 
     info->status = status;
     ctx->last_iters = total;
     return IBIM_OK;
+}
+
+// 16 right-hand sides per batch: the same refinement per column, every
+// product of a batch one 16-column pass. Iterations are summed over the
+// columns (each column counts the passes it took part in), as the batched
+// BiCG counts them.
+int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    constexpr int C = kBatchRHS;
+    int inner, outer_op;
+    refinement_ops(ctx, &inner, &outer_op);
+    const int count = ctx->count;
+    GmresSpace g;
+    if (int rc = make_space<C>(ctx, &g)) return rc;
+    const unsigned nb = (unsigned)((g.n + kGT - 1) / kGT);
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    const int m = opts->chunk > 0 ? std::min(opts->chunk, kMaxBasisB) : kMaxBasisB;
+    int total = 0, outer_max = 0, hist_at = 0, status = 4;
+    double worst = 0.0;
+    for (int c0 = 0; c0 < ctx->nrhs; c0 += C) {
+        const int ncol = std::min(C, ctx->nrhs - c0);
+        hipLaunchKernelGGL(k_gm_pack, dim3(nb), dim3(kGT), 0, ctx->stream, ctx->rhs.as<double2>(), count, c0, ncol,
+                           g.b);
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipMemsetAsync(g.x, 0, sizeof(double2) * g.vs, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(g.res, g.b, sizeof(double2) * g.n, hipMemcpyDeviceToDevice, ctx->stream));
+        double normb[C], relres[C], target[C];
+        int col_status[C], col_used[C] = {};
+        if (int rc = norms_of<C>(ctx, g, g.res, normb)) return rc;
+        bool any = false;
+        for (int c = 0; c < C; ++c) {
+            const bool live = c < ncol && normb[c] > 0.0;
+            relres[c] = live ? 1.0 : 0.0;
+            col_status[c] = live ? 1 : 4;  // zero data: zero density, no iterations
+            target[c] = live ? opts->tol * normb[c] : INFINITY;
+            any = any || live;
+        }
+        int used = 0, outer = 0;
+        for (outer = 1; any && outer <= max_outer && used < opts->maxit; ++outer) {
+            int passes = 0;
+            double est[C];
+            if (int rc = gmres<C>(ctx, g, inner, g.res, target, opts->maxit - used, m, &passes, col_used, est,
+                                  c0 == 0 ? hist : nullptr, &hist_at, opts->maxit, normb[0]))
+                return rc;
+            used += passes;
+            hipLaunchKernelGGL(k_gm_add, dim3(nb), dim3(kGT), 0, ctx->stream, g.x, g.d, g.n);
+            if (int rc = apply_op<C>(ctx, outer_op, g.x, g.w)) return rc;
+            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, g.b, g.w, g.n, g.res);
+            IBIM_HIP(hipGetLastError());
+            double rn[C];
+            if (int rc = norms_of<C>(ctx, g, g.res, rn)) return rc;
+            any = false;
+            for (int c = 0; c < C; ++c) {
+                if (col_status[c] != 1) continue;
+                relres[c] = rn[c] / normb[c];
+                if (relres[c] != relres[c])
+                    col_status[c] = 2;
+                else if (relres[c] < opts->tol)
+                    col_status[c] = 0;
+                if (col_status[c] != 1) target[c] = INFINITY;  // out of the next corrections
+                any = any || col_status[c] == 1;
+            }
+        }
+        hipLaunchKernelGGL(k_gm_unpack, dim3(nb), dim3(kGT), 0, ctx->stream, g.x, count, c0, ncol,
+                           ctx->density.as<double2>());
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        outer_max = std::max(outer_max, std::min(outer, max_outer));
+        for (int c = 0; c < ncol; ++c) {
+            total += col_used[c];
+            if (col_status[c] == 4) continue;
+            worst = std::max(worst, relres[c]);
+            if (status == 4 || col_status[c] != 0) status = col_status[c];
+        }
+        report_progress(ctx, "GMRES (16 columns)", used, worst);
+    }
+    info->iterations = total;
+    info->outer = outer_max;
+    info->rel_residual = worst;
+    info->status = status;
+    ctx->last_iters = total;
+    return IBIM_OK;
+}
+
+}  // namespace
+
+int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
+    // 16-column passes where the batched products exist: every operator of
+    // the traditional tube, or dense complex128 storage refined against itself
+    const bool batched = ctx->nrhs > 1 && opts->batch == 0 &&
+                         (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C128);
+    return batched ? gmres_batched(ctx, opts, info, hist) : gmres_columns(ctx, opts, info, hist);
 }
 
 }  // namespace ibim

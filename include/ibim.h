@@ -117,9 +117,11 @@ typedef struct ibim_params {
                               /* convergence r = b - A64 x; no restarts                  */
 #define IBIM_SOLVE_GMRES 3    /* FP64 iterative refinement around GMRES on the stored    */
                               /* operator (complex64 storage: the mixed-precision path;  */
-                              /* complex128: FP64 GMRES); one product A p per iteration, */
-                              /* a restart every `chunk` products (0: 256). No            */
-                              /* counterpart in the reference (BiCG only)                 */
+                              /* complex128: FP64 GMRES; nothing stored: GMRES on the    */
+                              /* mixed matrix-free operator); one product A p per        */
+                              /* iteration, a restart every `chunk` products (0: 256;    */
+                              /* 16-column batches, batch = 0 with several right-hand    */
+                              /* sides: 128). No counterpart in the reference (BiCG only) */
 
 typedef struct ibim_solve_opts {
     int method;        /* IBIM_SOLVE_*                                        */

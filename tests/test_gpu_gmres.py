@@ -106,3 +106,113 @@ def test_product_without_the_adjoint_is_the_same_bits(gpu, store, dirichlet, com
     y, _ = pb.apply(p, q, gpu.OP_STORED)
     assert np.array_equal(pb.apply_y(p, gpu.OP_STORED), y)
     pb.close()
+
+
+def test_gmres_batched_columns_match_single_columns(gpu):
+    """20 right-hand sides (a batch of 16 and a ragged one of 4, one column
+    zero) in 16-column passes against the same refinement one column at a time:
+    the same solutions to the tolerance, zero data gives a zero density and no
+    iterations, and two batched solves give the same bits."""
+    pb = problem(gpu, k=6.0)
+    m = 20
+    set_columns(gpu, pb, m)
+    pb.set_rhs(np.zeros(pb.count), np.zeros(pb.count), col=5)
+    pb.assemble(gpu.MIXED_PAIR)
+    single, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-10, batch=1)
+    xs = densities(pb, m)
+    batched, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-10)
+    xb = densities(pb, m)
+    print("GMRES products: one column at a time", single.iterations, "16-column passes", batched.iterations)
+    assert single.status == 0 and batched.status == 0, (single.status, batched.status)
+    assert batched.rel_residual < 1e-10
+    assert not xb[5].any()
+    for c in range(m):
+        if c != 5:
+            assert relnorm(xb[c], xs[c]) < 1e-8, (c, relnorm(xb[c], xs[c]))
+    # products counted per column: the batched passes add only rounding differences
+    assert batched.iterations <= 1.2 * single.iterations + 2 * m, (batched.iterations, single.iterations)
+    again, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-10)
+    assert (again.iterations, again.outer) == (batched.iterations, batched.outer)
+    for c, x in enumerate(densities(pb, m)):
+        assert np.array_equal(x, xb[c]), c
+    pb.close()
+
+
+@pytest.mark.parametrize("combined", [0, 1])
+def test_gmres_batched_on_fp32_matrix_cores(gpu, combined):
+    """16 plane waves on complex64 pairs with the products on the FP32 matrix
+    cores (ibim_set_fp32_products; the large-scale workload's mixed path): the
+    refinement's FP64 residuals bring every column to the FP64 solution, and
+    the FP32 rounding of the products, which depends on the vector, costs GMRES
+    few products against the FP64-product pass over the same pairs."""
+    n, k = 28, 6.0
+    h = 4.0 / n
+    pb = gpu.Problem()
+    pb.set_level_shape(gpu.shape(gpu.SHAPE_STAR, a=1.0, b=0.2, m=4, n=3, variant=1), n)
+    pb.geometry(5)
+    pb.tube(gpu.params(k, 2 * h, 2 * h, dirichlet=1, combined=combined, eta=k))
+    m = 16
+    pb.set_nrhs(m)
+    rng = np.random.default_rng(41 + combined)
+    dirs = rng.standard_normal((m, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    for c in range(m):
+        pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+    pb.assemble(gpu.C128_PAIR)
+    ref_info, _ = pb.solve(gpu.SOLVE_BICG, maxit=3000, tol=1e-12, batch=0)
+    assert ref_info.status == 0
+    ref = densities(pb, m)
+    pb.assemble(gpu.C64_PAIR)
+    info64, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-9)
+    pb.set_fp32_products(True)
+    info32, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-9)
+    x32 = densities(pb, m)
+    pb.set_fp32_products(False)
+    print("GMRES products (summed over columns): FP64 products", info64.iterations, info64.outer,
+          "FP32 products", info32.iterations, info32.outer)
+    assert info64.status == 0 and info32.status == 0 and info32.rel_residual < 1e-9
+    for c in range(m):
+        assert relnorm(x32[c], ref[c]) < 1e-7, (c, relnorm(x32[c], ref[c]))
+    assert info32.iterations <= 1.5 * info64.iterations, (info32.iterations, info64.iterations)
+    pb.close()
+
+
+@pytest.mark.parametrize("store", ["C128", "C64"])
+def test_gmres_on_the_combo_tube(gpu, store):
+    """The Neumann combo kernel's tube (no pair form, no 16-column FP64
+    matrix-free pass): dense storage, two right-hand sides, against FP64 BiCG."""
+    pb = gpu.Problem().set_level(shapes.sphere(24)).geometry(5)
+    h = 4.0 / 24
+    pb.tube(gpu.params(2.0, 3 * h, 2 * h, delta_in=0.3 * h, dirichlet=1), gpu.TUBE_COMBO)
+    set_columns(gpu, pb, 2)
+    pb.assemble(gpu.C128)
+    ref_info, _ = pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-13, batch=1)
+    assert ref_info.status == 0
+    ref = densities(pb, 2)
+    pb.assemble(getattr(gpu, store))
+    info, _ = pb.solve(gpu.SOLVE_GMRES, maxit=2000, tol=1e-10)
+    assert info.status == 0 and info.rel_residual < 1e-10, (info.status, info.rel_residual)
+    for c, x in enumerate(densities(pb, 2)):
+        assert relnorm(x, ref[c]) < 1e-8, (c, relnorm(x, ref[c]))
+    pb.close()
+
+
+@pytest.mark.parametrize("nrhs", [1, 3])
+def test_gmres_matrix_free(gpu, nrhs):
+    """Nothing stored: GMRES on the mixed matrix-free operator, the FP64
+    residuals from the FP64 matrix-free one (16-column passes for several
+    right-hand sides), against FP64 BiCG on complex128 pairs."""
+    ref_pb = problem(gpu, k=6.0)
+    set_columns(gpu, ref_pb, nrhs)
+    ref_pb.assemble(gpu.C128_PAIR)
+    ref_info, _ = ref_pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-13, batch=1)
+    assert ref_info.status == 0
+    ref = densities(ref_pb, nrhs)
+    ref_pb.close()
+    pb = problem(gpu, k=6.0)
+    set_columns(gpu, pb, nrhs)
+    info, _ = pb.solve(gpu.SOLVE_GMRES, maxit=2000, tol=1e-10)
+    assert info.status == 0 and info.rel_residual < 1e-10, (info.status, info.rel_residual)
+    for c, x in enumerate(densities(pb, nrhs)):
+        assert relnorm(x, ref[c]) < 1e-8, (c, relnorm(x, ref[c]))
+    pb.close()
