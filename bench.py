@@ -254,9 +254,11 @@ def run_batch(args):
     solved together by ibim_solve_group: every BiCG iteration applies all of
     their operators in one persistent launch (a queue of 64-row items, the
     torus first). FP64 path: batched BiCG on complex128 pair storage, products
-    on the FP64 matrix cores. Mixed path: batched BiCG on complex64 pairs with
-    the products on the FP32 matrix cores, FP64 residual replacement over the
-    complex128 copy of the same pairs (IBIM_MIXED_PAIR)."""
+    on the FP64 matrix cores. Mixed path (IBIM_MIXED_PAIR): complex64 pairs
+    (products on the FP64 matrix cores; --batch-fp32-products: FP32), FP64
+    iterative refinement around grouped GMRES, one pass Y = A P per iteration,
+    its FP64 residuals over the complex128 copy of the same pairs
+    (--batch-solver reliable: grouped BiCG with FP64 residual replacement)."""
     tol, k = 1e-7, 10.0
     rng = np.random.default_rng(0)
     torus = ("torus128", ibim.shape(ibim.SHAPE_TORUS, a=1.0, b=0.4), 128, 64)
@@ -304,6 +306,8 @@ def run_batch(args):
         t1 = time.perf_counter()
         if path == "fp64":
             infos = ibim.solve_group(pbs, ibim.SOLVE_BICG, maxit=3000, tol=tol)
+        elif args.batch_solver == "gmres":
+            infos = ibim.solve_group(pbs, ibim.SOLVE_GMRES, maxit=3000, tol=tol)
         else:
             infos = ibim.solve_group(pbs, ibim.SOLVE_RELIABLE, maxit=3000, tol=tol, inner_tol=args.batch_replace)
         dens = []
@@ -336,20 +340,23 @@ def run_batch(args):
         rows.append({"surface": name, "n": n, "tube_points": f[3][s_i], "rhs": m,
                      "fp64_iterations": f[1][s_i].iterations, "fp64_status": f[1][s_i].status,
                      "mixed_iterations": mx[1][s_i].iterations, "mixed_status": mx[1][s_i].status,
-                     "mixed_replacements": mx[1][s_i].outer, "mixed_rel_err_vs_fp64": err})
+                     "mixed_fp64_residual_steps": mx[1][s_i].outer, "mixed_rel_err_vs_fp64": err})
         print("[bench] %s: N_t=%d, %d rhs, iterations fp64 %d / mixed %d, rel err %.1e" %
               (name, f[3][s_i], m, f[1][s_i].iterations, mx[1][s_i].iterations, err), file=sys.stderr, flush=True)
     total = {p: res[p][0] for p in res}
     return ({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / total["mixed"],
           "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-          "dtype": "complex64 pair storage, products on the fp32 matrix cores summed in fp64 per 64 columns, fp64 "
-                   "residual replacement over the complex128 copy of the pairs",
+          "dtype": "complex64 pair storage, products on the %s matrix cores, %s over the complex128 copy of the "
+                   "pairs" % ("fp32" if args.batch_fp32_products else "fp64",
+                              "GMRES with fp64 iterative refinement" if args.batch_solver == "gmres" else
+                              "BiCG with fp64 residual replacement"),
           "data": "synthetic: torus + 8 seed-random spheres/ellipsoids, Fibonacci plane waves",
           "config": {"workload": "batch: torus 128^3 x 64 plane waves + 8 surfaces on 48^3-96^3 x 16 plane waves, "
                                  "k=10, eps=tau=2h, tol %g" % tol, "rhs": n_rhs, "parallelism": "single GPU",
                      "schedule": "all 9 surfaces (12 batches of 16 right-hand sides) in one grouped solve: one "
-                                 "persistent launch of every operator per BiCG iteration (ibim_solve_group)"},
+                                 "persistent launch of every operator per iteration (ibim_solve_group)",
+                     "mixed_solver": args.batch_solver},
           "fp64_solves_per_s": n_rhs / total["fp64"], "speedup_mixed_vs_fp64": total["fp64"] / total["mixed"],
           "worst_rel_err_mixed_vs_fp64": worst_err,
           "grouped_pass": {p: res[p][4] for p in res}, "surfaces": rows})
@@ -669,7 +676,11 @@ def main():
     ap.add_argument("--batch-graphs", action="store_true",
                     help="batch workload: replay the grouped iterations as HIP graphs (no per-pass timing)")
     ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
-                    help="batch mixed path: residual drop between FP64 residual replacements")
+                    help="batch mixed path with --batch-solver reliable: residual drop between FP64 residual "
+                         "replacements")
+    ap.add_argument("--batch-solver", choices=("gmres", "reliable"), default="gmres",
+                    help="batch mixed path: FP64 refinement around grouped GMRES (one pass Y = A P per iteration) "
+                         "or grouped BiCG with FP64 residual replacement")
     ap.add_argument("--replace", type=float, default=None,
                     help="single-vector workloads: residual drop between the mixed path's FP64 replacements")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -896,6 +907,7 @@ def compact(name, r):
         return d
     if name == "batch":
         return {"workload": r["config"]["workload"], "rhs": r["config"]["rhs"], "solves_per_s": r["value"],
+                "mixed_solver": r["config"]["mixed_solver"],
                 "fp64_solves_per_s": r["fp64_solves_per_s"], "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
                 "worst_rel_err_mixed_vs_fp64": r["worst_rel_err_mixed_vs_fp64"],
                 "schedule": r["config"]["schedule"]}

@@ -1096,22 +1096,12 @@ struct GroupRun {
     BReplace rep{};
 };
 
-// Points every context's work at one stream for the call (their own streams
-// drained first), and back afterwards.
-struct SharedStream {
-    std::vector<Context*> ctxs;
-    std::vector<hipStream_t> own;
-    ~SharedStream() {
-        if (!ctxs.empty()) (void)hipStreamSynchronize(ctxs[0]->stream);
-        for (size_t c = 0; c < ctxs.size(); ++c) ctxs[c]->stream = own[c];
-    }
-};
-
 }  // namespace
 
 int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+    if (method == IBIM_SOLVE_GMRES) return run_group_gmres(ctxs, n, opts, infos);
     if (method != IBIM_SOLVE_BICG && method != IBIM_SOLVE_RELIABLE) {
-        set_error("ibim_solve_group: BiCG or reliable BiCG only");
+        set_error("ibim_solve_group: BiCG, reliable BiCG or GMRES only");
         return IBIM_ERR_ARG;
     }
     const bool mixed = method == IBIM_SOLVE_RELIABLE;
@@ -1120,12 +1110,7 @@ int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* op
     Context* lead = ctxs[0];
     const int maxit = opts->maxit;
     SharedStream shared;
-    for (int c = 0; c < n; ++c) IBIM_HIP(hipStreamSynchronize(ctxs[c]->stream));
-    for (int c = 0; c < n; ++c) {
-        shared.ctxs.push_back(ctxs[c]);
-        shared.own.push_back(ctxs[c]->stream);
-        ctxs[c]->stream = lead->stream;
-    }
+    if (int rc = shared.share(ctxs, n)) return rc;
     // members, the largest problems first (their items start first)
     std::vector<GroupRun> runs;
     std::vector<int> order(n);

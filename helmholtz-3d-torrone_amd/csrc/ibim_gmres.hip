@@ -185,13 +185,13 @@ size_t space_bytes(const Context* ctx) {
            512;
 }
 
+// The workspace at `base` (space_bytes<C>(ctx) bytes, 256-B aligned).
 template <int C>
-int make_space(Context* ctx, GmresSpace* g) {
+int space_at(Context* ctx, void* base, GmresSpace* g) {
     const int mb = max_basis<C>();
-    if (ctx->refine.ensure(space_bytes<C>(ctx))) return IBIM_ERR_NOMEM;
     g->vs = ctx->ld * C;
     g->n = ctx->count * C;
-    g->V = ctx->refine.as<double2>();
+    g->V = static_cast<double2*>(base);
     g->w = g->V + ((size_t)mb + 1) * g->vs;
     g->d = g->w + g->vs;
     g->r = g->d + g->vs;
@@ -204,6 +204,12 @@ int make_space(Context* ctx, GmresSpace* g) {
     // padding rows of every vector read as zero in the operator passes
     IBIM_HIP(hipMemsetAsync(g->V, 0, sizeof(double2) * g->vs * ((size_t)mb + 1 + 6), ctx->stream));
     return IBIM_OK;
+}
+
+template <int C>
+int make_space(Context* ctx, GmresSpace* g) {
+    if (ctx->refine.ensure(space_bytes<C>(ctx))) return IBIM_ERR_NOMEM;
+    return space_at<C>(ctx, ctx->refine.ptr, g);
 }
 
 // out[i C + col] = V_i^H w per column, i < nv (device)
@@ -236,122 +242,164 @@ int apply_op(Context* ctx, int op, const double2* p, double2* y) {
         return batch_apply(ctx, op, false, p, y, nullptr);
 }
 
-// GMRES(m), m <= max_basis<C>(), for A d = rhs per column with operator `op`,
-// from d = 0, until each column's residual estimate is below its target
-// (absolute; +inf leaves a column out) or `budget` products are used. Adds the
-// products each column took part in to col_products and returns the passes
-// applied and the last estimates. hist (column 0) gets estimate / normb.
+// GMRES(m), m <= max_basis<C>(), for A d = rhs per column, from d = 0, until
+// each column's residual estimate is below its target (absolute; +inf leaves a
+// column out) or `budget` products are used. Resumable, so that several runs
+// can share one operator pass: the caller applies the operator to in() into
+// out(), then calls launch_post() (device work and the copy of its results
+// to the host), synchronizes the stream, and calls host_post(); until done().
 template <int C>
-int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const double* target, int budget, int m,
-          int* passes, int* col_products, double* estimate, double* hist, int* hist_at, int hist_len, double normb) {
-    constexpr int MB = max_basis<C>();
-    const int n = g.n;
-    const unsigned nb = (unsigned)((n + kGT - 1) / kGT);
-    IBIM_HIP(hipMemsetAsync(g.d, 0, sizeof(double2) * g.vs, ctx->stream));
-    *passes = 0;
+struct GmresRun {
+    enum Phase { kIterate, kRestart, kDone };
+    static constexpr int MB = max_basis<C>();
+    Context* ctx = nullptr;
+    GmresSpace g{};
+    const double2* rhs = nullptr;
+    double target[C] = {};
+    int budget = 0, m = 0;
+    int passes = 0;               // operator applications
+    int* col_products = nullptr;  // += the products each column took part in
+    double estimate[C] = {};
+    double* hist = nullptr;  // column 0's estimate / normb per product
+    int* hist_at = nullptr;
+    int hist_len = 0;
+    double normb = 1.0;
+    Phase phase = kDone;
+    int j = 0;
+    bool active[C] = {};
+    int ncols[C] = {};
     // per column: Hessenberg (column-major, MB + 1 rows), rotations, rotated residual
-    std::vector<cplx> H((size_t)C * (MB + 1) * MB), cs((size_t)C * MB), sn((size_t)C * MB), gv((size_t)C * (MB + 1));
-    std::vector<double2> hh((size_t)C * (2 * (MB + 1) + 1));
-    auto Hc = [&](int c, int i, int j) -> cplx& { return H[((size_t)c * MB + j) * (MB + 1) + i]; };
-    bool active[C];
-    int ncols[C];
-    double hn[C];
-    bool first = true;
-    for (;;) {
-        // r0 = rhs - A d (the first cycle: rhs)
-        if (first) {
-            IBIM_HIP(hipMemcpyAsync(g.r, rhs, sizeof(double2) * n, hipMemcpyDeviceToDevice, ctx->stream));
-        } else {
-            if (*passes >= budget) return IBIM_OK;
-            if (int rc = apply_op<C>(ctx, op, g.d, g.w)) return rc;
-            ++*passes;
-            for (int c = 0; c < C; ++c) col_products[c] += active[c];
-            hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, rhs, g.w, n, g.r);
+    std::vector<cplx> H, cs, sn, gv;
+    std::vector<double2> hh, yd;  // host copies of the coefficients; the combination
+
+    cplx& Hc(int c, int i, int k) { return H[((size_t)c * MB + k) * (MB + 1) + i]; }
+    bool done() const { return phase == kDone; }
+    const double2* in() const { return phase == kIterate ? g.V + (size_t)j * g.vs : g.d; }
+    double2* out() const { return g.w; }
+    unsigned blocks() const { return (unsigned)((g.n + kGT - 1) / kGT); }
+
+    // d = 0, r = rhs, the first cycle's norms (synchronizes)
+    int start() {
+        H.assign((size_t)C * (MB + 1) * MB, cplx(0.0));
+        cs.assign((size_t)C * MB, cplx(0.0));
+        sn.assign((size_t)C * MB, cplx(0.0));
+        gv.assign((size_t)C * (MB + 1), cplx(0.0));
+        hh.assign((size_t)C * (2 * (MB + 1) + 1), make_double2(0.0, 0.0));
+        passes = 0;
+        for (int c = 0; c < C; ++c) active[c] = false;
+        IBIM_HIP(hipMemsetAsync(g.d, 0, sizeof(double2) * g.vs, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(g.r, rhs, sizeof(double2) * g.n, hipMemcpyDeviceToDevice, ctx->stream));
+        dots<C>(ctx, g, g.r, 1, g.r, g.coef);
+        IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * C, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        return cycle_begin();
+    }
+
+    // after out() = A in(): the device side of the step and its host copy
+    int launch_post() {
+        ++passes;
+        for (int c = 0; c < C; ++c) col_products[c] += active[c];
+        if (phase == kRestart) {
+            // r0 = rhs - A d
+            hipLaunchKernelGGL(k_gm_sub, dim3(blocks()), dim3(kGT), 0, ctx->stream, rhs, g.w, g.n, g.r);
+            dots<C>(ctx, g, g.r, 1, g.r, g.coef);
+            IBIM_HIP(hipGetLastError());
+            IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * C, hipMemcpyDeviceToHost, ctx->stream));
+            return IBIM_OK;
         }
-        first = false;
-        double beta[C];
-        if (int rc = norms_of<C>(ctx, g, g.r, beta)) return rc;
+        // CGS2: h = V^H w, w -= V h, twice; then ||w||
+        for (int pass = 0; pass < 2; ++pass) {
+            double2* coef = g.coef + (size_t)pass * (MB + 1) * C;
+            dots<C>(ctx, g, g.V, j + 1, g.w, coef);
+            hipLaunchKernelGGL(k_gm_combine<C>, dim3(blocks()), dim3(kGT), 0, ctx->stream, g.V, g.vs, g.n, coef,
+                               j + 1, -1.0, g.w);
+        }
+        dots<C>(ctx, g, g.w, 1, g.w, g.coef + 2 * (size_t)(MB + 1) * C);
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * hh.size(), hipMemcpyDeviceToHost, ctx->stream));
+        return IBIM_OK;
+    }
+
+    // after the stream has caught up with launch_post
+    int host_post() {
+        if (phase == kRestart) return cycle_begin();
         Scales<C> sc;
         bool any = false;
         for (int c = 0; c < C; ++c) {
-            estimate[c] = beta[c];
-            active[c] = beta[c] > target[c];
-            sc.s[c] = active[c] ? 1.0 / beta[c] : 0.0;
+            if (!active[c]) {
+                sc.s[c] = 0.0;
+                continue;
+            }
+            cplx* gc = gv.data() + (size_t)c * (MB + 1);
+            cplx* csc = cs.data() + (size_t)c * MB;
+            cplx* snc = sn.data() + (size_t)c * MB;
+            for (int i = 0; i <= j; ++i) {
+                const double2 a = hh[(size_t)i * C + c], b = hh[((size_t)(MB + 1) + i) * C + c];
+                Hc(c, i, j) = cplx(a.x + b.x, a.y + b.y);
+            }
+            const double hn = sqrt(hh[2 * (size_t)(MB + 1) * C + c].x);
+            Hc(c, j + 1, j) = hn;
+            // previous rotations, then the one that zeroes H(j + 1, j)
+            for (int i = 0; i < j; ++i) {
+                const cplx t = csc[i] * Hc(c, i, j) + snc[i] * Hc(c, i + 1, j);
+                Hc(c, i + 1, j) = -std::conj(snc[i]) * Hc(c, i, j) + std::conj(csc[i]) * Hc(c, i + 1, j);
+                Hc(c, i, j) = t;
+            }
+            const cplx a = Hc(c, j, j), bb = Hc(c, j + 1, j);
+            const double rr = sqrt(std::norm(a) + std::norm(bb));
+            csc[j] = rr > 0.0 ? std::conj(a) / rr : cplx(1.0);
+            snc[j] = rr > 0.0 ? std::conj(bb) / rr : cplx(0.0);
+            Hc(c, j, j) = rr;
+            Hc(c, j + 1, j) = 0.0;
+            gc[j + 1] = -std::conj(snc[j]) * gc[j];
+            gc[j] = csc[j] * gc[j];
+            estimate[c] = std::abs(gc[j + 1]);
+            if (c == 0 && hist && *hist_at < hist_len) hist[(*hist_at)++] = estimate[c] / normb;
+            if (!(estimate[c] > target[c]) || !(hn > 0.0) || estimate[c] != estimate[c]) {
+                active[c] = false;  // done with j + 1 basis vectors
+                ncols[c] = j + 1;
+                sc.s[c] = 0.0;
+            } else {
+                sc.s[c] = 1.0 / hn;
+                any = true;
+            }
+        }
+        ++j;
+        if (!any || j >= m || passes >= budget) return cycle_end();
+        hipLaunchKernelGGL(k_gm_scale<C>, dim3(blocks()), dim3(kGT), 0, ctx->stream, g.w, sc, g.n,
+                           g.V + (size_t)j * g.vs);
+        IBIM_HIP(hipGetLastError());
+        return IBIM_OK;
+    }
+
+    // a cycle from the residual r (its norms in hh): v0 = r / ||r|| per column
+    int cycle_begin() {
+        Scales<C> sc;
+        bool any = false;
+        for (int c = 0; c < C; ++c) {
+            const double beta = sqrt(hh[c].x);
+            estimate[c] = beta;
+            active[c] = beta > target[c];
+            sc.s[c] = active[c] ? 1.0 / beta : 0.0;
             any = any || active[c];
             ncols[c] = 0;
             std::fill(gv.begin() + (size_t)c * (MB + 1), gv.begin() + (size_t)(c + 1) * (MB + 1), cplx(0.0));
-            gv[(size_t)c * (MB + 1)] = beta[c];
+            gv[(size_t)c * (MB + 1)] = beta;
         }
-        if (!any) return IBIM_OK;
-        hipLaunchKernelGGL(k_gm_scale<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.r, sc, n, g.V);
-        int j = 0;
-        for (; j < m && *passes < budget; ++j) {
-            double2* vj = g.V + (size_t)j * g.vs;
-            if (int rc = apply_op<C>(ctx, op, vj, g.w)) return rc;
-            ++*passes;
-            for (int c = 0; c < C; ++c) col_products[c] += active[c];
-            // CGS2: h = V^H w, w -= V h, twice
-            for (int pass = 0; pass < 2; ++pass) {
-                double2* coef = g.coef + (size_t)pass * (MB + 1) * C;
-                dots<C>(ctx, g, g.V, j + 1, g.w, coef);
-                hipLaunchKernelGGL(k_gm_combine<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, g.vs, n, coef, j + 1,
-                                   -1.0, g.w);
-            }
-            double2* nrm = g.coef + 2 * (size_t)(MB + 1) * C;
-            dots<C>(ctx, g, g.w, 1, g.w, nrm);
-            IBIM_HIP(hipGetLastError());
-            IBIM_HIP(hipMemcpyAsync(hh.data(), g.coef, sizeof(double2) * hh.size(), hipMemcpyDeviceToHost,
-                                    ctx->stream));
-            IBIM_HIP(hipStreamSynchronize(ctx->stream));
-            any = false;
-            for (int c = 0; c < C; ++c) {
-                if (!active[c]) {
-                    sc.s[c] = 0.0;
-                    continue;
-                }
-                cplx* gc = gv.data() + (size_t)c * (MB + 1);
-                cplx* csc = cs.data() + (size_t)c * MB;
-                cplx* snc = sn.data() + (size_t)c * MB;
-                for (int i = 0; i <= j; ++i) {
-                    const double2 a = hh[(size_t)i * C + c], b = hh[((size_t)(MB + 1) + i) * C + c];
-                    Hc(c, i, j) = cplx(a.x + b.x, a.y + b.y);
-                }
-                hn[c] = sqrt(hh[2 * (size_t)(MB + 1) * C + c].x);
-                Hc(c, j + 1, j) = hn[c];
-                // previous rotations, then the one that zeroes H(j + 1, j)
-                for (int i = 0; i < j; ++i) {
-                    const cplx t = csc[i] * Hc(c, i, j) + snc[i] * Hc(c, i + 1, j);
-                    Hc(c, i + 1, j) = -std::conj(snc[i]) * Hc(c, i, j) + std::conj(csc[i]) * Hc(c, i + 1, j);
-                    Hc(c, i, j) = t;
-                }
-                const cplx a = Hc(c, j, j), bb = Hc(c, j + 1, j);
-                const double rr = sqrt(std::norm(a) + std::norm(bb));
-                csc[j] = rr > 0.0 ? std::conj(a) / rr : cplx(1.0);
-                snc[j] = rr > 0.0 ? std::conj(bb) / rr : cplx(0.0);
-                Hc(c, j, j) = rr;
-                Hc(c, j + 1, j) = 0.0;
-                gc[j + 1] = -std::conj(snc[j]) * gc[j];
-                gc[j] = csc[j] * gc[j];
-                estimate[c] = std::abs(gc[j + 1]);
-                if (c == 0 && hist && *hist_at < hist_len) hist[(*hist_at)++] = estimate[c] / normb;
-                if (!(estimate[c] > target[c]) || !(hn[c] > 0.0) || estimate[c] != estimate[c]) {
-                    active[c] = false;  // done with j + 1 basis vectors
-                    ncols[c] = j + 1;
-                    sc.s[c] = 0.0;
-                } else {
-                    sc.s[c] = 1.0 / hn[c];
-                    any = true;
-                }
-            }
-            if (!any) {
-                ++j;
-                break;
-            }
-            if (j + 1 < m)
-                hipLaunchKernelGGL(k_gm_scale<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.w, sc, n,
-                                   g.V + (size_t)(j + 1) * g.vs);
+        j = 0;
+        if (!any) {
+            phase = kDone;
+            return IBIM_OK;
         }
-        // y = R^{-1} g over each column's basis vectors, d += V y
+        hipLaunchKernelGGL(k_gm_scale<C>, dim3(blocks()), dim3(kGT), 0, ctx->stream, g.r, sc, g.n, g.V);
+        IBIM_HIP(hipGetLastError());
+        phase = kIterate;
+        return IBIM_OK;
+    }
+
+    // y = R^{-1} g over each column's basis vectors, d += V y; then a restart
+    // for the columns that ran out of basis vectors, budget permitting
+    int cycle_end() {
         bool again = false;
         int nmax = 0;
         for (int c = 0; c < C; ++c) {
@@ -361,7 +409,7 @@ int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const d
             }
             nmax = std::max(nmax, ncols[c]);
         }
-        std::vector<double2> yd((size_t)nmax * C, make_double2(0.0, 0.0));
+        yd.assign((size_t)nmax * C, make_double2(0.0, 0.0));
         for (int c = 0; c < C; ++c) {
             const int nc = ncols[c];
             const cplx* gc = gv.data() + (size_t)c * (MB + 1);
@@ -375,14 +423,43 @@ int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const d
         }
         if (nmax > 0) {
             IBIM_HIP(hipMemcpyAsync(g.y, yd.data(), sizeof(double2) * yd.size(), hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL(k_gm_combine<C>, dim3(nb), dim3(kGT), 0, ctx->stream, g.V, g.vs, n, g.y, nmax, 1.0,
-                               g.d);
+            hipLaunchKernelGGL(k_gm_combine<C>, dim3(blocks()), dim3(kGT), 0, ctx->stream, g.V, g.vs, g.n, g.y, nmax,
+                               1.0, g.d);
             IBIM_HIP(hipGetLastError());
-            IBIM_HIP(hipStreamSynchronize(ctx->stream));  // yd goes out of scope
         }
-        // a restart for the columns that ran out of basis vectors
-        if (!again || *passes >= budget) return IBIM_OK;
+        phase = again && passes < budget ? kRestart : kDone;
+        if (phase == kDone) IBIM_HIP(hipStreamSynchronize(ctx->stream));  // yd stays untouched until d is formed
+        return IBIM_OK;
     }
+};
+
+// One GMRES solve on one context (see GmresRun). Adds each column's products
+// to col_products; returns the passes applied and the last estimates.
+template <int C>
+int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const double* target, int budget, int m,
+          int* passes, int* col_products, double* estimate, double* hist, int* hist_at, int hist_len, double normb) {
+    GmresRun<C> run;
+    run.ctx = ctx;
+    run.g = g;
+    run.rhs = rhs;
+    for (int c = 0; c < C; ++c) run.target[c] = target[c];
+    run.budget = budget;
+    run.m = m;
+    run.col_products = col_products;
+    run.hist = hist;
+    run.hist_at = hist_at;
+    run.hist_len = hist_len;
+    run.normb = normb;
+    int rc = run.start();
+    while (!rc && !run.done()) {
+        rc = apply_op<C>(ctx, op, run.in(), run.out());
+        if (!rc) rc = run.launch_post();
+        if (!rc) IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        if (!rc) rc = run.host_post();
+    }
+    *passes = run.passes;
+    for (int c = 0; c < C; ++c) estimate[c] = run.estimate[c];
+    return rc;
 }
 
 // The operators of a refinement: GMRES on the stored one, residuals with the
@@ -494,8 +571,9 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
             target[c] = live ? opts->tol * normb[c] : INFINITY;
             any = any || live;
         }
-        int used = 0, outer = 0;
-        for (outer = 1; any && outer <= max_outer && used < opts->maxit; ++outer) {
+        int used = 0, steps = 0;
+        while (any && steps < max_outer && used < opts->maxit) {
+            ++steps;
             int passes = 0;
             double est[C];
             if (int rc = gmres<C>(ctx, g, inner, g.res, target, opts->maxit - used, m, &passes, col_used, est,
@@ -524,7 +602,7 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
                            ctx->density.as<double2>());
         IBIM_HIP(hipGetLastError());
         IBIM_HIP(hipStreamSynchronize(ctx->stream));
-        outer_max = std::max(outer_max, std::min(outer, max_outer));
+        outer_max = std::max(outer_max, steps);
         for (int c = 0; c < ncol; ++c) {
             total += col_used[c];
             if (col_status[c] == 4) continue;
@@ -541,7 +619,189 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
     return IBIM_OK;
 }
 
+// One batch of up to 16 columns of one problem in a grouped solve.
+struct GroupBatch {
+    Context* ctx = nullptr;
+    int c0 = 0, ncol = 0;
+    int inner = OP_DENSE, outer = OP_DENSE;
+    GmresSpace g{};
+    GmresRun<kBatchRHS> run;
+    double normb[kBatchRHS] = {}, relres[kBatchRHS] = {}, target[kBatchRHS] = {};
+    int status[kBatchRHS] = {}, used[kBatchRHS] = {};
+    int passes = 0, outer_steps = 0;
+    bool running = false;
+};
+
 }  // namespace
+
+// Several pair-stored problems at once (ibim_solve_group): every batch of 16
+// columns of every problem runs the refinement of gmres_batched, and the
+// GMRES products of all batches still iterating are ONE persistent launch per
+// iteration (pair_group_pass, Y = A P alone; largest problems first), followed
+// by each batch's near field and Gram-Schmidt work and one host
+// synchronization for all. The refinement steps are taken together: a batch
+// whose GMRES has finished waits for the others before its FP64 residual.
+// Each batch takes the same steps, and gives the same bits, as its own
+// batched solve.
+int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+    constexpr int C = kBatchRHS;
+    int source = 0;
+    if (int rc = pair_group_source(ctxs, n, false, &source)) return rc;
+    Context* lead = ctxs[0];
+    SharedStream shared;
+    if (int rc = shared.share(ctxs, n)) return rc;
+    std::vector<int> order(n);
+    for (int c = 0; c < n; ++c) order[c] = c;
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ctxs[x]->count > ctxs[y]->count; });
+    std::vector<GroupBatch> bs;
+    for (int c : order)
+        for (int c0 = 0; c0 < ctxs[c]->nrhs; c0 += C) {
+            GroupBatch b;
+            b.ctx = ctxs[c];
+            b.c0 = c0;
+            b.ncol = std::min(C, ctxs[c]->nrhs - c0);
+            bs.push_back(std::move(b));
+        }
+    const int nb = (int)bs.size();
+    // workspaces, then the queue counter and the member table
+    std::vector<size_t> off(nb + 1, 0);
+    for (int m = 0; m < nb; ++m) off[m + 1] = off[m] + ((space_bytes<C>(bs[m].ctx) + 255) & ~(size_t)255);
+    const size_t table = off[nb] + 256;
+    if (lead->group.ensure(table + sizeof(GroupMember) * nb)) return IBIM_ERR_NOMEM;
+    char* base = lead->group.as<char>();
+    int* queue = reinterpret_cast<int*>(base + off[nb]);
+    GroupMember* dev_members = reinterpret_cast<GroupMember*>(base + table);
+    const int maxit = opts->maxit;
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    const int mlen = opts->chunk > 0 ? std::min(opts->chunk, kMaxBasisB) : kMaxBasisB;
+    for (int m = 0; m < nb; ++m) {
+        GroupBatch& b = bs[m];
+        Context* ctx = b.ctx;
+        refinement_ops(ctx, &b.inner, &b.outer);
+        if (int rc = space_at<C>(ctx, base + off[m], &b.g)) return rc;
+        const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
+        hipLaunchKernelGGL(k_gm_pack, dim3(blocks), dim3(kGT), 0, ctx->stream, ctx->rhs.as<double2>(), ctx->count,
+                           b.c0, b.ncol, b.g.b);
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipMemcpyAsync(b.g.res, b.g.b, sizeof(double2) * b.g.n, hipMemcpyDeviceToDevice, ctx->stream));
+        if (int rc = norms_of<C>(ctx, b.g, b.g.res, b.normb)) return rc;
+        for (int c = 0; c < C; ++c) {
+            const bool live = c < b.ncol && b.normb[c] > 0.0;
+            b.relres[c] = live ? 1.0 : 0.0;
+            b.status[c] = live ? 1 : 4;
+            b.target[c] = live ? opts->tol * b.normb[c] : INFINITY;
+            b.running = b.running || live;
+        }
+    }
+    std::vector<GroupMember> tab;
+    for (int outer = 1; outer <= max_outer; ++outer) {
+        // a GMRES solve for every batch with running columns and budget left
+        std::vector<int> live, stepped;
+        for (int m = 0; m < nb; ++m) {
+            GroupBatch& b = bs[m];
+            if (!b.running || b.passes >= maxit) continue;
+            GmresRun<C>& r = b.run;
+            r.ctx = b.ctx;
+            r.g = b.g;
+            r.rhs = b.g.res;
+            for (int c = 0; c < C; ++c) r.target[c] = b.target[c];
+            r.budget = maxit - b.passes;
+            r.m = mlen;
+            r.col_products = b.used;
+            if (int rc = r.start()) return rc;
+            stepped.push_back(m);
+            if (!r.done()) live.push_back(m);
+        }
+        if (stepped.empty()) break;
+        while (!live.empty()) {
+            tab.clear();
+            int items = 0;
+            for (int m : live) {
+                const GroupBatch& b = bs[m];
+                const Context* ctx = b.ctx;
+                GroupMember d{};
+                d.F = ctx->mat.ptr;
+                d.blk = ctx->pair_blk.as<int>();
+                d.pts = ctx->pair_pts.ptr;
+                d.P = b.run.in();
+                d.Y = b.run.out();
+                d.prm = ctx->prm;
+                d.count = ctx->count;
+                d.item0 = items;
+                items += (ctx->count + 63) / 64;
+                tab.push_back(d);
+            }
+            IBIM_HIP(hipMemcpyAsync(dev_members, tab.data(), sizeof(GroupMember) * tab.size(), hipMemcpyHostToDevice,
+                                    lead->stream));
+            if (int rc = pair_group_pass(lead, source, dev_members, (int)tab.size(), items, queue, false)) return rc;
+            for (int m : live) {
+                GroupBatch& b = bs[m];
+                if (int rc = pair_batch_near(b.ctx, b.run.in(), nullptr, b.run.out(), nullptr, nullptr)) return rc;
+                if (int rc = b.run.launch_post()) return rc;
+            }
+            IBIM_HIP(hipStreamSynchronize(lead->stream));  // also: tab may change
+            std::vector<int> still;
+            for (int m : live) {
+                if (int rc = bs[m].run.host_post()) return rc;
+                if (!bs[m].run.done()) still.push_back(m);
+            }
+            live.swap(still);
+        }
+        // the FP64 residual of every batch that took a step
+        double worst = 0.0;
+        for (int m : stepped) {
+            GroupBatch& b = bs[m];
+            Context* ctx = b.ctx;
+            const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
+            b.passes += b.run.passes;
+            b.outer_steps = outer;
+            hipLaunchKernelGGL(k_gm_add, dim3(blocks), dim3(kGT), 0, ctx->stream, b.g.x, b.g.d, b.g.n);
+            if (int rc = apply_op<C>(ctx, b.outer, b.g.x, b.g.w)) return rc;
+            hipLaunchKernelGGL(k_gm_sub, dim3(blocks), dim3(kGT), 0, ctx->stream, b.g.b, b.g.w, b.g.n, b.g.res);
+            IBIM_HIP(hipGetLastError());
+            double rn[C];
+            if (int rc = norms_of<C>(ctx, b.g, b.g.res, rn)) return rc;
+            b.running = false;
+            for (int c = 0; c < C; ++c) {
+                if (b.status[c] != 1) continue;
+                b.relres[c] = rn[c] / b.normb[c];
+                if (b.relres[c] != b.relres[c])
+                    b.status[c] = 2;
+                else if (b.relres[c] < opts->tol)
+                    b.status[c] = 0;
+                if (b.status[c] != 1) b.target[c] = INFINITY;
+                b.running = b.running || b.status[c] == 1;
+                if (b.status[c] == 1) worst = std::max(worst, b.relres[c]);
+            }
+        }
+        report_progress(lead, "grouped GMRES, refinement", outer, worst);
+    }
+    // per problem: its columns' results, summed as the one-problem solver does
+    for (int c = 0; c < n; ++c) {
+        Context* ctx = ctxs[c];
+        ibim_solve_info& info = infos[c];
+        info = ibim_solve_info{};
+        int status = 4;
+        for (GroupBatch& b : bs) {
+            if (b.ctx != ctx) continue;
+            const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
+            hipLaunchKernelGGL(k_gm_unpack, dim3(blocks), dim3(kGT), 0, ctx->stream, b.g.x, ctx->count, b.c0, b.ncol,
+                               ctx->density.as<double2>());
+            IBIM_HIP(hipGetLastError());
+            info.outer = std::max(info.outer, b.outer_steps);
+            for (int col = 0; col < b.ncol; ++col) {
+                info.iterations += b.used[col];
+                if (b.status[col] == 4) continue;
+                info.rel_residual = std::max(info.rel_residual, b.relres[col]);
+                if (status == 4 || b.status[col] != 0) status = b.status[col];
+            }
+        }
+        info.status = status;
+        ctx->last_iters = info.iterations;
+    }
+    IBIM_HIP(hipStreamSynchronize(lead->stream));
+    return IBIM_OK;
+}
 
 int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
     // 16-column passes where the batched products exist: every operator of

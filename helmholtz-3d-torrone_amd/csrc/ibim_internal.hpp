@@ -235,9 +235,32 @@ enum GroupSource { GROUP_C64 = 0, GROUP_C64_FP32 = 1, GROUP_C128 = 2 };
 int pair_group_source(Context* const* ctxs, int n, bool mixed, int* source);
 // The far field of every member in one persistent launch on lead's stream:
 // workgroups take 64-row work items from the queue counter (reset first) in
-// item order; dev_members is a device copy of the table.
+// item order; dev_members is a device copy of the table. with_z false: Y = A P
+// alone (the members' Q and Z unused).
 int pair_group_pass(Context* lead, int source, const GroupMember* dev_members, int nmembers, int nitems,
-                    int* queue);
+                    int* queue, bool with_z = true);
+// Points every context's work at the first one's stream for a grouped solve
+// (their own streams drained first), and back when it goes out of scope.
+struct SharedStream {
+    std::vector<Context*> ctxs;
+    std::vector<hipStream_t> own;
+    int share(Context* const* c, int n) {
+        for (int i = 0; i < n; ++i) IBIM_HIP(hipStreamSynchronize(c[i]->stream));
+        for (int i = 0; i < n; ++i) {
+            ctxs.push_back(c[i]);
+            own.push_back(c[i]->stream);
+            c[i]->stream = c[0]->stream;
+        }
+        return IBIM_OK;
+    }
+    ~SharedStream() {
+        if (!ctxs.empty()) (void)hipStreamSynchronize(ctxs[0]->stream);
+        for (size_t c = 0; c < ctxs.size(); ++c) ctxs[c]->stream = own[c];
+    }
+};
+// Grouped FP64 refinement around GMRES (ibim_solve_group with IBIM_SOLVE_GMRES,
+// ibim_gmres.hip).
+int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos);
 // Batched BiCG (IBIM_SOLVE_BICG) or reliable BiCG (IBIM_SOLVE_RELIABLE) of the
 // right-hand sides of several contexts together (ibim_batch.hip).
 int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* opts, ibim_solve_info* infos);

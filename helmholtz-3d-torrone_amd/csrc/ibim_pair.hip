@@ -1377,7 +1377,7 @@ __global__ void k_group_reset(int* queue) {
     if (threadIdx.x == 0) *queue = 0;
 }
 
-template <class MT, int Kern, bool FP32>
+template <class MT, int Kern, bool FP32, bool WithZ>
 __global__ void __launch_bounds__(256, FP32 ? 2 : 1) k_pair_batch_group(const GroupMember* __restrict__ mem,
                                                                         int nmem, int nitems,
                                                                         int* __restrict__ queue) {
@@ -1395,19 +1395,19 @@ __global__ void __launch_bounds__(256, FP32 ? 2 : 1) k_pair_batch_group(const Gr
         const ibim_params prm = g.prm;
         const int rowW = (item - g.item0) * 64;
         if constexpr (FP32)
-            pair_batch32_rows<Kern, true, true>(static_cast<const float2*>(g.F), g.blk,
-                                                static_cast<const PairPoint*>(g.pts), g.count, prm, g.P, g.Q, g.Y,
-                                                g.Z, rowW);
+            pair_batch32_rows<Kern, true, WithZ>(static_cast<const float2*>(g.F), g.blk,
+                                                 static_cast<const PairPoint*>(g.pts), g.count, prm, g.P, g.Q, g.Y,
+                                                 g.Z, rowW);
         else
-            pair_batch_rows<MT, Kern, true, true, 4, kStoredFactors>(static_cast<const MT*>(g.F), g.blk,
-                                                                     static_cast<const PairPoint*>(g.pts), g.count,
-                                                                     prm, g.P, g.Q, g.Y, g.Z, rowW);
+            pair_batch_rows<MT, Kern, true, WithZ, 4, kStoredFactors>(static_cast<const MT*>(g.F), g.blk,
+                                                                      static_cast<const PairPoint*>(g.pts), g.count,
+                                                                      prm, g.P, g.Q, g.Y, g.Z, rowW);
     }
 }
 
-template <class MT, int Kern, bool FP32>
+template <class MT, int Kern, bool FP32, bool WithZ>
 int launch_group(Context* lead, const GroupMember* mem, int nmem, int nitems, int* queue) {
-    auto kern = k_pair_batch_group<MT, Kern, FP32>;
+    auto kern = k_pair_batch_group<MT, Kern, FP32, WithZ>;
     int per_cu = 0, cus = 0;
     IBIM_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 256, 0));
     IBIM_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, lead->device));
@@ -1705,18 +1705,24 @@ int pair_batch_apply(Context* ctx, bool fp64_copy, const double2* P, const doubl
 }
 
 int pair_batch_near(Context* ctx, const double2* P, const double2* Q, double2* Y, double2* Z, const int* stop) {
-    if (!ctx->pair_aux || !Y || !Z || !P || !Q) {
+    // both products, or A P alone (Q and Z null)
+    if (!ctx->pair_aux || !Y || !P || (!Q != !Z)) {
         set_error("batched near field without the near lists or vectors it needs");
         return IBIM_ERR_ARG;
     }
     double nr, ni;
     near_term(ctx, &nr, &ni);
+    const dim3 grid((unsigned)(((size_t)ctx->count * kBatchRHS + 255) / 256));
     with_kernel(ctx, [&](auto K) {
         constexpr int Kern = decltype(K)::value;
-        hipLaunchKernelGGL((k_pair_near_batch<Kern, true, true>),
-                           dim3((unsigned)(((size_t)ctx->count * kBatchRHS + 255) / 256)), dim3(256), 0,
-                           ctx->stream, ctx->tube.as<double>(), ctx->count, ctx->near_off.as<int>(),
-                           ctx->near_idx.as<int>(), P, Q, Y, Z, stop, nr, ni);
+        if (Z)
+            hipLaunchKernelGGL((k_pair_near_batch<Kern, true, true>), grid, dim3(256), 0, ctx->stream,
+                               ctx->tube.as<double>(), ctx->count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(),
+                               P, Q, Y, Z, stop, nr, ni);
+        else
+            hipLaunchKernelGGL((k_pair_near_batch<Kern, true, false>), grid, dim3(256), 0, ctx->stream,
+                               ctx->tube.as<double>(), ctx->count, ctx->near_off.as<int>(), ctx->near_idx.as<int>(),
+                               P, Q, Y, Z, stop, nr, ni);
     });
     IBIM_HIP(hipGetLastError());
     return IBIM_OK;
@@ -1751,16 +1757,23 @@ int pair_group_source(Context* const* ctxs, int n, bool mixed, int* source) {
 }
 
 int pair_group_pass(Context* lead, int source, const GroupMember* dev_members, int nmembers, int nitems,
-                    int* queue) {
+                    int* queue, bool with_z) {
     int rc = IBIM_OK;
     with_kernel(lead, [&](auto K) {
         constexpr int Kern = decltype(K)::value;
-        if (source == GROUP_C128)
-            rc = launch_group<double2, Kern, false>(lead, dev_members, nmembers, nitems, queue);
-        else if (source == GROUP_C64_FP32)
-            rc = launch_group<float2, Kern, true>(lead, dev_members, nmembers, nitems, queue);
+        auto go = [&](auto Z) {
+            constexpr bool WithZ = decltype(Z)::value;
+            if (source == GROUP_C128)
+                rc = launch_group<double2, Kern, false, WithZ>(lead, dev_members, nmembers, nitems, queue);
+            else if (source == GROUP_C64_FP32)
+                rc = launch_group<float2, Kern, true, WithZ>(lead, dev_members, nmembers, nitems, queue);
+            else
+                rc = launch_group<float2, Kern, false, WithZ>(lead, dev_members, nmembers, nitems, queue);
+        };
+        if (with_z)
+            go(std::true_type{});
         else
-            rc = launch_group<float2, Kern, false>(lead, dev_members, nmembers, nitems, queue);
+            go(std::false_type{});
     });
     if (rc) return rc;
     IBIM_HIP(hipGetLastError());

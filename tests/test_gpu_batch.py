@@ -297,7 +297,10 @@ def sphere_problem(gpu, n, store, fp32, dirichlet=1, rhs=16):
                                                ("c64_pair", "SOLVE_BICG", True),
                                                ("mixed_pair", "SOLVE_RELIABLE", False),
                                                ("mixed_pair", "SOLVE_RELIABLE", True),
-                                               ("c64_pair", "SOLVE_RELIABLE", True)])
+                                               ("c64_pair", "SOLVE_RELIABLE", True),
+                                               ("c128_pair", "SOLVE_GMRES", False),
+                                               ("mixed_pair", "SOLVE_GMRES", False),
+                                               ("c64_pair", "SOLVE_GMRES", True)])
 def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
     """Several problems of uneven sizes solved together (one persistent launch
     of every operator per iteration, a problem with two batches among them):
@@ -305,7 +308,8 @@ def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
     batched solve, bit for bit; the operators run only in the grouped launch."""
     m = getattr(gpu, method)
     sizes, nrhs = (20, 28, 24), (16, 19, 5)
-    kw = dict(maxit=500, tol=1e-10, inner_tol=1e-2, chunk=2)
+    # chunk: BiCG's host-check interval; GMRES: its basis length (0: the default)
+    kw = dict(maxit=500, tol=1e-10, inner_tol=1e-2, chunk=0 if method == "SOLVE_GMRES" else 2)
     own = []
     for n, r in zip(sizes, nrhs):
         pb = sphere_problem(gpu, n, store, fp32, rhs=r)
@@ -327,8 +331,11 @@ def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
         for c in range(r):
             x = pb.density(c)
             assert np.array_equal(x[0], x0[c][0]) and np.array_equal(x[1], x0[c][1]), c
-        # the single-problem passes did not run (the replacements' FP64 passes aside)
-        assert pb.kernel_time("batch_pair")[1] == 0 and pb.kernel_time("batch_pair32")[1] == 0
+        # the single-problem passes did not run (the FP64 passes of the replacements
+        # and refinement residuals aside: on complex128 pairs GMRES takes them
+        # from the stored pairs themselves, one per refinement step and batch)
+        own_passes = (r + 15) // 16 * info.outer if (method, store) == ("SOLVE_GMRES", "c128_pair") else 0
+        assert pb.kernel_time("batch_pair")[1] <= own_passes and pb.kernel_time("batch_pair32")[1] == 0
         pb.set_profiling(False)
     longest = max(i.iterations for i in infos)  # bounds every column's own count
     assert 0 < launches <= longest + 2
