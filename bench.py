@@ -138,8 +138,9 @@ def step(pb, w, recv, path):
     """One solve of a single-right-hand-side workload. path: "fp64" (the
     reference's BiCG on complex128 pairs), "fp64_gmres" (FP64 GMRES on the same
     pairs), "mixed" (FP64 refinement around GMRES on complex64 pairs; with
-    nothing stored, reliable BiCG on the mixed matrix-free operator) or
-    "mixed_reliable" (BiCG on complex64 pairs with FP64 residual replacement)."""
+    nothing stored, on the mixed matrix-free operator with FP64 matrix-free
+    residuals) or "mixed_reliable" (BiCG on the same operator with FP64
+    residual replacement)."""
     mixed = path.startswith("mixed")
     pb.geometry(5)
     count = pb.tube(params_of(w), ibim.TUBE_TRADITIONAL)
@@ -150,8 +151,7 @@ def step(pb, w, recv, path):
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if path == "fp64":
         info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
-    elif path == "fp64_gmres" or (path == "mixed" and mixed_storage(count) is not None and
-                                  w.get("mixed_solver", "gmres") == "gmres"):
+    elif path == "fp64_gmres" or (path == "mixed" and w.get("mixed_solver", "gmres") == "gmres"):
         info, _ = pb.solve(ibim.SOLVE_GMRES, maxit=3000, tol=w["tol"])
     else:
         info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"],
@@ -746,7 +746,7 @@ def run_standard(args, workload):
     stored = mixed_stores(count)
     ms_dom, n_dom = pm.kernel_time("matvec" if stored else "sweep_free32")
     ms_rep, n_rep = pm.kernel_time("matvec64" if mixed_storage(count) == ibim.MIXED_PAIR else "sweep_free64")
-    gmres = stored and w.get("mixed_solver", "gmres") == "gmres"
+    gmres = w.get("mixed_solver", "gmres") == "gmres"
     solver = "FP64 refinement around GMRES" if gmres else "BiCG with FP64 residual replacement"
     print("[bench] mixed path (%s): %.3f s/solve, %d iterations, %d FP64 residuals" %
           (solver, tm, infom.iterations, infom.outer), file=sys.stderr, flush=True)
@@ -754,9 +754,10 @@ def run_standard(args, workload):
     # complex64 pairs) and FP64 GMRES on the complex128 pairs, which separate the
     # precision's gain from the solver's
     extra = {}
-    if stored and not args.no_extra:
+    if not args.no_extra:
         xs, ws = min(args.steps, 5), min(args.warmup, 1)
-        for name in ("mixed_reliable", "fp64_gmres"):
+        # FP64 GMRES needs a stored complex128 operator
+        for name in ("mixed_reliable", "fp64_gmres") if stored else ("mixed_reliable",):
             pe = make_problem(level, w)
             te, infoe, xe, _ = timed(pe, w, recv, name, xs, ws)
             extra[name] = (te, infoe, xe)
@@ -797,12 +798,14 @@ def run_standard(args, workload):
         # one evaluation per pair (C_EVAL) + point factors alpha, beta (12) + four
         # complex multiply-adds with their scalings (36), FP64 on the vector
         # units, whose peak equals the FP64 matrix-core figure on this GPU
-        flops = npairs * (C_EVAL + 48)
+        # (GMRES: A p alone, two complex multiply-adds with their scalings, 18)
+        per_pair = C_EVAL + (30 if gmres else 48)
+        flops = npairs * per_pair
         achieved = flops / avg_dom / 1e12 if n_dom else 0.0
         roof = {"bound": "mfma", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": None,
-                "kernel": "k_pair_sweep<double2, fast factors>: %d flops/pair x %d*%d/2 pairs per launch" %
-                          (C_EVAL + 48, count, count - 1)}
+                "kernel": "k_pair_sweep<double2, fast factors%s>: %d flops/pair x %d*%d/2 pairs per launch" %
+                          (", A p only" if gmres else "", per_pair, count, count - 1)}
     # the FP64 path's fused sweep (HBM-bound): one pass over the complex128 pairs
     avg_mv = ms_mv / max(n_mv, 1) * 1e-3
     mv_gbs = (npairs * 16 + vec_bytes) / avg_mv / 1e9 if n_mv and not MATRIX_FREE else None
@@ -819,7 +822,8 @@ def run_standard(args, workload):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors + fp64 residual replacement"
+        "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors, %s" % (
+                      "GMRES, fp64 iterative refinement" if gmres else "BiCG, fp64 residual replacement")
                   if MATRIX_FREE else
                   "complex64 radial factors per pair (fp64 point factors, products and sums), %s" % (
                       "GMRES, fp64 iterative refinement" if gmres else "BiCG, fp64 residual replacement")),

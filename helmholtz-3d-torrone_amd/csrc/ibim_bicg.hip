@@ -475,7 +475,7 @@ unsigned vec_blocks(int count) {
 static int sweep_any(Context* ctx, int op, const double2* p, const double2* q, double2* ypart, double2* zpart,
                      const int* done, bool with_z = true) {
     if (op == OP_PAIR64) return sweep_pair_launch(ctx, true, p, q, ypart, zpart, done, with_z);
-    if (free_pair_op(ctx, op)) return sweep_pair_free_launch(ctx, op == OP_FREE32, p, q, ypart, zpart, done);
+    if (free_pair_op(ctx, op)) return sweep_pair_free_launch(ctx, op == OP_FREE32, p, q, ypart, zpart, done, with_z);
     if (op != OP_DENSE) return sweep_free_launch(ctx, op, p, q, ypart, zpart, done);
     if (ctx->mat_pair) return sweep_pair_launch(ctx, false, p, q, ypart, zpart, done, with_z);
     const int count = ctx->count;

@@ -275,9 +275,9 @@ int pair_free_batch_apply(Context* ctx, bool fast, const double2* P, const doubl
 // The single-vector sweep of the same operator into the pair planes (one
 // factor evaluation per pair; the complex128 pair sweep's bits), or with fast
 // the mixed-precision matrix-free operator: FP32 sine and cosine of an FP64
-// phase in the factors, FP64 everywhere else.
+// phase in the factors, FP64 everywhere else. with_z false: A p alone.
 int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q, void* ypart, void* zpart,
-                           const int* done);
+                           const int* done, bool with_z = true);
 // Point table and near lists of the matrix-free pair operator (built once per
 // tube; host synchronisation, so before any stream capture).
 int prepare_pair_free(Context* ctx);

@@ -1639,7 +1639,7 @@ int ensure_free_cache(Context* ctx) {
 }
 
 int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q, void* ypart, void* zpart,
-                           const int* done) {
+                           const int* done, bool with_z) {
     if (int rc = prepare_pair_free(ctx)) return rc;
     if (!fast && ctx->free_cache_max > 0)
         if (int rc = ensure_free_cache(ctx)) return rc;
@@ -1654,26 +1654,34 @@ int sweep_pair_free_launch(Context* ctx, bool fast, const void* p, const void* q
     near_term(ctx, &nr, &ni);
     with_kernel(ctx, [&](auto K) {
         constexpr int Kern = decltype(K)::value;
-        {
+        // both products, or A p alone (with_z false: the same A p bits)
+        auto sweeps = [&](auto WZ) {
+            constexpr bool WithZ = decltype(WZ)::value;
             TimedScope ts(ctx, fast ? "sweep_free32" : "sweep_free64");
             if (fast)
-                hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kFastFactors>), dim3((unsigned)ctx->pair_ntiles),
-                                   dim3(kPT), 0, ctx->stream, nullptr, ctx->pair_tiles.as<int2>(),
-                                   ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+                hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kFastFactors, WithZ>),
+                                   dim3((unsigned)ctx->pair_ntiles), dim3(kPT), 0, ctx->stream, nullptr,
+                                   ctx->pair_tiles.as<int2>(), ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q,
+                                   Y, Z, done, ctx->prm);
             else {
                 // cached tiles (the first ones) read their factors, the others compute them
                 const int nc = ctx->free_cache_max > 0 ? ctx->free_cached_tiles : 0;
                 if (nc > 0)
-                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kStoredFactors>), dim3((unsigned)nc), dim3(kPT),
-                                       0, ctx->stream, ctx->free_cache.as<double2>(), ctx->pair_tiles.as<int2>(),
-                                       ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P, Q, Y, Z, done, ctx->prm);
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kStoredFactors, WithZ>), dim3((unsigned)nc),
+                                       dim3(kPT), 0, ctx->stream, ctx->free_cache.as<double2>(),
+                                       ctx->pair_tiles.as<int2>(), ctx->pair_pts.as<PairPoint>(), count, ld, ncol, P,
+                                       Q, Y, Z, done, ctx->prm);
                 if (ctx->pair_ntiles > nc)
-                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kExactFactors>),
+                    hipLaunchKernelGGL((k_pair_sweep<double2, Kern, kExactFactors, WithZ>),
                                        dim3((unsigned)(ctx->pair_ntiles - nc)), dim3(kPT), 0, ctx->stream, nullptr,
                                        ctx->pair_tiles.as<int2>() + nc, ctx->pair_pts.as<PairPoint>(), count, ld,
                                        ncol, P, Q, Y, Z, done, ctx->prm);
             }
-        }
+        };
+        if (with_z)
+            sweeps(std::true_type{});
+        else
+            sweeps(std::false_type{});
         TimedScope ts(ctx, "near");
         const size_t np = (size_t)(ncol + nrow) * ld;
         hipLaunchKernelGGL((k_pair_near<Kern>), dim3((unsigned)((count + 255) / 256)), dim3(256), 0, ctx->stream,

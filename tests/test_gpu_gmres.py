@@ -216,3 +216,17 @@ def test_gmres_matrix_free(gpu, nrhs):
     for c, x in enumerate(densities(pb, nrhs)):
         assert relnorm(x, ref[c]) < 1e-8, (c, relnorm(x, ref[c]))
     pb.close()
+
+
+@pytest.mark.parametrize("op", ["OP_FREE64", "OP_FREE32"])
+@pytest.mark.parametrize("dirichlet,combined", [(1, 0), (0, 0), (1, 1)])
+def test_matrix_free_product_without_the_adjoint_is_the_same_bits(gpu, op, dirichlet, combined):
+    """The matrix-free pair sweeps without their A^H half (the products GMRES
+    applies with nothing stored) give the full sweep's y = A p bit for bit."""
+    pb = problem(gpu, dirichlet=dirichlet, combined=combined)
+    rng = np.random.default_rng(13)
+    p = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    q = rng.standard_normal(pb.count) + 1j * rng.standard_normal(pb.count)
+    y, _ = pb.apply(p, q, getattr(gpu, op))
+    assert np.array_equal(pb.apply_y(p, getattr(gpu, op)), y)
+    pb.close()
