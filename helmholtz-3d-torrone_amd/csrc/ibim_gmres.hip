@@ -475,6 +475,20 @@ void refinement_ops(const Context* ctx, int* inner, int* outer) {
     *outer = ctx->mat_precision == IBIM_C128 ? OP_DENSE : fp64_operator(ctx);
 }
 
+// Each GMRES run of a refinement aims at kInnerMargin tol ||b|| when GMRES runs
+// on a stored operator and the FP64 residual has to be computed (the FP64
+// matrix-free operator): the FP64 residual of a correction computed on the
+// lower-precision operator lands somewhat above GMRES's own estimate, and one
+// FP64 pass then costs several GMRES products (star surface on 256^3, 16 plane
+// waves, FP32 products: 357 against 127 ms; true residual 1.09e-7 after a run
+// to tol = 1e-7, so a second run and a second FP64 pass, 5.4 s in all; 6.6e-8
+// after a run to 0.5 tol, 5.2 s). Where the FP64 pass is cheap (stored
+// complex128 copy, or both operators computed) the second step costs less than
+// the margin's extra products (torus 128^3, matrix-free: 0.785 against
+// 0.754 s), so GMRES aims at tol itself.
+constexpr double kInnerMargin = 0.5;
+double inner_margin(int inner, int outer) { return inner == OP_DENSE && outer == OP_FREE64 ? kInnerMargin : 1.0; }
+
 // One right-hand side at a time.
 int gmres_columns(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
     int inner, outer_op;
@@ -502,7 +516,7 @@ int gmres_columns(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
             int products = 0, passes = 0;
             double est = 0.0;
             // the correction only has to bring the residual to tol ||b||
-            const double target = opts->tol * normb;
+            const double target = inner_margin(inner, outer_op) * opts->tol * normb;
             if (int rc = gmres<1>(ctx, g, inner, g.res, &target, opts->maxit - used, m, &passes, &products, &est,
                                   c == 0 ? hist : nullptr, &hist_at, opts->maxit, normb))
                 return rc;
@@ -568,7 +582,7 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
             const bool live = c < ncol && normb[c] > 0.0;
             relres[c] = live ? 1.0 : 0.0;
             col_status[c] = live ? 1 : 4;  // zero data: zero density, no iterations
-            target[c] = live ? opts->tol * normb[c] : INFINITY;
+            target[c] = live ? inner_margin(inner, outer_op) * opts->tol * normb[c] : INFINITY;
             any = any || live;
         }
         int used = 0, steps = 0;
@@ -689,7 +703,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
             const bool live = c < b.ncol && b.normb[c] > 0.0;
             b.relres[c] = live ? 1.0 : 0.0;
             b.status[c] = live ? 1 : 4;
-            b.target[c] = live ? opts->tol * b.normb[c] : INFINITY;
+            b.target[c] = live ? inner_margin(b.inner, b.outer) * opts->tol * b.normb[c] : INFINITY;
             b.running = b.running || live;
         }
     }
