@@ -45,54 +45,59 @@ namespace {
 constexpr int kGT = 256;          // threads per block of the vector kernels
 constexpr int kMaxBasis = 256;    // Krylov vectors per GMRES cycle (then restart), one column
 constexpr int kMaxBasisB = 128;   // the same for 16-column batches (vectors 16x as long)
-constexpr int kDotChunks = 16;    // blocks per basis vector in the dot products
 
 template <int C>
 constexpr int max_basis() {
     return C == 1 ? kMaxBasis : kMaxBasisB;
 }
-
-// part[(i kDotChunks + c) C + col] = sum over chunk c of conj(V_i[e]) w[e] over
-// the elements e of column col (e % C == col): block (c, i) takes every
-// kDotChunks-th group of kGT elements; a fixed-order tree over the block down
-// to one sum per column (kGT is a multiple of C, so thread t only ever sees
-// column t % C).
+// element chunks per basis vector in the dot products (one wave each)
 template <int C>
-__global__ void __launch_bounds__(kGT) k_gm_dots(const double2* __restrict__ V, size_t vs, int n,
+constexpr int dot_chunks() {
+    return C == 1 ? 32 : 256;
+}
+
+// part[(i D + c) C + col] = sum over chunk c of conj(V_i[e]) w[e] over the
+// elements e of column col (e % C == col), D = dot_chunks<C>(): one wave per
+// (chunk, basis vector) -- four basis vectors per block -- whose lanes stride
+// over the chunk (chunk lengths are multiples of 64, so lane l only ever sees
+// column l % C), then a fixed butterfly over the lanes of each column.
+template <int C>
+__global__ void __launch_bounds__(kGT) k_gm_dots(const double2* __restrict__ V, size_t vs, int n, int nv,
                                                  const double2* __restrict__ w, double2* __restrict__ part) {
-    __shared__ double sr[kGT], si[kGT];
-    const double2* v = V + (size_t)blockIdx.y * vs;
+    constexpr int D = dot_chunks<C>();
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.y * (kGT / 64) + (threadIdx.x >> 6);
+    if (i >= nv) return;  // a whole wave; no barrier below
+    const int c = blockIdx.x;
+    const int len = ((n + D - 1) / D + 63) / 64 * 64;
+    const int e1 = min(n, (c + 1) * len);
+    const double2* v = V + (size_t)i * vs;
     double ar = 0.0, ai = 0.0;
-    for (int e = blockIdx.x * kGT + threadIdx.x; e < n; e += kDotChunks * kGT) {
+#pragma unroll 4
+    for (int e = c * len + lane; e < e1; e += 64) {
         const double2 a = v[e], b = w[e];
         ar += a.x * b.x + a.y * b.y;
         ai += a.x * b.y - a.y * b.x;
     }
-    sr[threadIdx.x] = ar;
-    si[threadIdx.x] = ai;
-    __syncthreads();
-    for (int s = kGT / 2; s >= C; s >>= 1) {
-        if (threadIdx.x < s) {
-            sr[threadIdx.x] += sr[threadIdx.x + s];
-            si[threadIdx.x] += si[threadIdx.x + s];
-        }
-        __syncthreads();
+#pragma unroll
+    for (int off = 32; off >= C; off >>= 1) {
+        ar += __shfl_xor(ar, off);
+        ai += __shfl_xor(ai, off);
     }
-    if (threadIdx.x < C)
-        part[((size_t)blockIdx.y * kDotChunks + blockIdx.x) * C + threadIdx.x] =
-            make_double2(sr[threadIdx.x], si[threadIdx.x]);
+    if (lane < C) part[((size_t)i * D + c) * C + lane] = make_double2(ar, ai);
 }
 
-// out[i C + col] = sum_c part[(i kDotChunks + c) C + col] in order (one thread per entry)
+// out[i C + col] = sum_c part[(i D + c) C + col] in order (one thread per entry)
 template <int C>
 __global__ void k_gm_dots_sum(const double2* __restrict__ part, int n, double2* __restrict__ out) {
+    constexpr int D = dot_chunks<C>();
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n * C) return;
     const int i = t / C, col = t % C;
     double sr = 0.0, si = 0.0;
-    for (int c = 0; c < kDotChunks; ++c) {
-        sr += part[((size_t)i * kDotChunks + c) * C + col].x;
-        si += part[((size_t)i * kDotChunks + c) * C + col].y;
+    for (int c = 0; c < D; ++c) {
+        sr += part[((size_t)i * D + c) * C + col].x;
+        si += part[((size_t)i * D + c) * C + col].y;
     }
     out[t] = make_double2(sr, si);
 }
@@ -181,7 +186,7 @@ size_t space_bytes(const Context* ctx) {
     const int mb = max_basis<C>();
     const size_t vs = ctx->ld * C;
     return sizeof(double2) * (vs * ((size_t)mb + 1 + 6) + (size_t)C * (2 * (mb + 1) + 1) + (size_t)C * mb +
-                              (size_t)C * (mb + 1) * kDotChunks) +
+                              (size_t)C * (mb + 1) * dot_chunks<C>()) +
            512;
 }
 
@@ -215,8 +220,8 @@ int make_space(Context* ctx, GmresSpace* g) {
 // out[i C + col] = V_i^H w per column, i < nv (device)
 template <int C>
 void dots(Context* ctx, const GmresSpace& g, const double2* V, int nv, const double2* w, double2* out) {
-    hipLaunchKernelGGL(k_gm_dots<C>, dim3(kDotChunks, (unsigned)nv), dim3(kGT), 0, ctx->stream, V, g.vs, g.n, w,
-                       g.part);
+    hipLaunchKernelGGL(k_gm_dots<C>, dim3(dot_chunks<C>(), (unsigned)((nv + 3) / 4)), dim3(kGT), 0, ctx->stream, V,
+                       g.vs, g.n, nv, w, g.part);
     hipLaunchKernelGGL(k_gm_dots_sum<C>, dim3((unsigned)((nv * C + 63) / 64)), dim3(64), 0, ctx->stream, g.part, nv,
                        out);
 }
