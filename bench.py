@@ -290,7 +290,7 @@ def run_batch(args):
             h = 4.0 / n
             pb.geometry(5)
             counts.append(pb.tube(ibim.params(k, 2 * h, 2 * h, dirichlet=1)))
-            if path == "fp64":
+            if path.startswith("fp64"):
                 pb.assemble(ibim.C128_PAIR)
             else:
                 pb.assemble(ibim.MIXED_PAIR)
@@ -306,8 +306,11 @@ def run_batch(args):
         t1 = time.perf_counter()
         if path == "fp64":
             infos = ibim.solve_group(pbs, ibim.SOLVE_BICG, maxit=3000, tol=tol)
-        elif args.batch_solver == "gmres":
-            infos = ibim.solve_group(pbs, ibim.SOLVE_GMRES, maxit=3000, tol=tol)
+        elif path == "fp64_block":
+            infos = ibim.solve_group(pbs, ibim.SOLVE_BLOCK_GMRES, maxit=3000, tol=tol)
+        elif args.batch_solver in ("gmres", "block"):
+            infos = ibim.solve_group(pbs, ibim.SOLVE_GMRES if args.batch_solver == "gmres" else
+                                     ibim.SOLVE_BLOCK_GMRES, maxit=3000, tol=tol)
         else:
             infos = ibim.solve_group(pbs, ibim.SOLVE_RELIABLE, maxit=3000, tol=tol, inner_tol=args.batch_replace)
         dens = []
@@ -329,7 +332,9 @@ def run_batch(args):
               (path, el, solve_s, launches, ms / max(launches, 1)), file=sys.stderr, flush=True)
         return el, infos, dens, counts, dict(solve_s=solve_s, launches=launches, pass_ms=ms / max(launches, 1))
 
-    res = {path: run(path) for path in ("fp64", "mixed")}
+    # beside the two paths (not with --no-extra): block GMRES over the complex128
+    # pairs, which separates the solver's gain from the precision's
+    res = {path: run(path) for path in ("fp64", "mixed") + (() if args.no_extra else ("fp64_block",))}
     rows, n_rhs, worst_err = [], 0, 0.0
     for s_i, (name, _, n, m) in enumerate(surfaces):
         f, mx = res["fp64"], res["mixed"]
@@ -344,13 +349,23 @@ def run_batch(args):
         print("[bench] %s: N_t=%d, %d rhs, iterations fp64 %d / mixed %d, rel err %.1e" %
               (name, f[3][s_i], m, f[1][s_i].iterations, mx[1][s_i].iterations, err), file=sys.stderr, flush=True)
     total = {p: res[p][0] for p in res}
+    extra = {}
+    if "fp64_block" in res:
+        b, f = res["fp64_block"], res["fp64"]
+        extra["fp64_block_gmres"] = {
+            "solves_per_s": n_rhs / total["fp64_block"], "speedup_mixed_vs_it": total["fp64_block"] / total["mixed"],
+            "grouped_passes": b[4]["launches"],
+            "worst_rel_err_vs_fp64": float(max(np.linalg.norm(b[2][s_i][c] - f[2][s_i][c]) /
+                                               np.linalg.norm(f[2][s_i][c])
+                                               for s_i, (_, _, _, m) in enumerate(surfaces) for c in range(m)))}
     return ({"metric": "solves/s (batched right-hand sides, all surfaces)", "value": n_rhs / total["mixed"],
           "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "complex64 pair storage, products on the %s matrix cores, %s over the complex128 copy of the "
                    "pairs" % ("fp32" if args.batch_fp32_products else "fp64",
-                              "GMRES with fp64 iterative refinement" if args.batch_solver == "gmres" else
-                              "BiCG with fp64 residual replacement"),
+                              {"gmres": "GMRES with fp64 iterative refinement",
+                               "block": "block GMRES with fp64 iterative refinement"}.get(
+                                  args.batch_solver, "BiCG with fp64 residual replacement")),
           "data": "synthetic: torus + 8 seed-random spheres/ellipsoids, Fibonacci plane waves",
           "config": {"workload": "batch: torus 128^3 x 64 plane waves + 8 surfaces on 48^3-96^3 x 16 plane waves, "
                                  "k=10, eps=tau=2h, tol %g" % tol, "rhs": n_rhs, "parallelism": "single GPU",
@@ -359,7 +374,7 @@ def run_batch(args):
                      "mixed_solver": args.batch_solver},
           "fp64_solves_per_s": n_rhs / total["fp64"], "speedup_mixed_vs_fp64": total["fp64"] / total["mixed"],
           "worst_rel_err_mixed_vs_fp64": worst_err,
-          "grouped_pass": {p: res[p][4] for p in res}, "surfaces": rows})
+          "grouped_pass": {p: res[p][4] for p in res}, "surfaces": rows, **extra})
 
 
 def run_adjoint(args):
@@ -526,8 +541,9 @@ def run_star(args):
             if not args.star_fp64_products:
                 pb.set_fp32_products(True)
                 out["mixed_operator"] += ", FP32 matrix-core products"
-            if args.star_solver == "gmres":
-                info, _ = pb.solve(ibim.SOLVE_GMRES, maxit=its, tol=STAR_TOL, batch=0)
+            if args.star_solver in ("gmres", "block"):
+                info, _ = pb.solve(ibim.SOLVE_GMRES if args.star_solver == "gmres" else ibim.SOLVE_BLOCK_GMRES,
+                                   maxit=its, tol=STAR_TOL, batch=0)
             else:
                 info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=its, tol=STAR_TOL, inner_tol=args.star_replace,
                                    batch=0)
@@ -578,14 +594,16 @@ def run_star(args):
     # complex products per pass: A P and A^H Q (BiCG; GMRES: A P alone), 16
     # columns, every pair visited from both of its rows (2 N^2 entries), 8 real
     # flops per complex multiply-add
-    pass_flops = {p: (1 if p == "mixed" and mixed_solver == "gmres" else 2) * 2.0 * npairs * 16 * 8 for p in out}
+    pass_flops = {p: (1 if p == "mixed" and mixed_solver in ("gmres", "block") else 2) * 2.0 * npairs * 16 * 8
+                  for p in out}
     return ({"metric": metric, "value": value, "unit": unit, "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": ms,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "mixed: complex64 pair storage (else fp32 sine / cosine of an fp64 phase), entries and products "
                    "on the fp32 matrix cores summed in fp64 per 64 columns, %s; fp64: fp64 "
                    "matrix-free factors, fp64 matrix-core products, the reference's BiCG" % (
-                       "GMRES with fp64 iterative refinement" if mixed_solver == "gmres" else
-                       "BiCG with fp64 residual replacement"),
+                       {"gmres": "GMRES with fp64 iterative refinement",
+                        "block": "block GMRES with fp64 iterative refinement"}.get(
+                           mixed_solver, "BiCG with fp64 residual replacement")),
           "data": "synthetic: star signed distance 256^3, 16 Fibonacci plane waves",
           "config": {"workload": "star256: r=1+0.2 cos4t sin3p on 256^3, k=%g, eps=tau=2h, 16 plane waves, tol 1e-7, "
                                  "%s, %d-iteration budget" % (
@@ -668,9 +686,10 @@ def main():
     ap.add_argument("--star-replace", type=float, default=STAR_REPLACE,
                     help="star256 mixed path with --star-solver reliable: residual drop between FP64 residual "
                          "replacements")
-    ap.add_argument("--star-solver", choices=("gmres", "reliable"), default="gmres",
-                    help="star256 mixed path: FP64 refinement around batched GMRES (one product A P per "
-                         "iteration) or batched BiCG with FP64 residual replacement")
+    ap.add_argument("--star-solver", choices=("block", "gmres", "reliable"), default="block",
+                    help="star256 mixed path: FP64 refinement around block GMRES (one Krylov space for the 16 "
+                         "columns) or GMRES per column (one pass A P per iteration), or batched BiCG with FP64 "
+                         "residual replacement")
     ap.add_argument("--batch-fp32-products", action="store_true",
                     help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
     ap.add_argument("--batch-graphs", action="store_true",
@@ -678,9 +697,10 @@ def main():
     ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
                     help="batch mixed path with --batch-solver reliable: residual drop between FP64 residual "
                          "replacements")
-    ap.add_argument("--batch-solver", choices=("gmres", "reliable"), default="gmres",
-                    help="batch mixed path: FP64 refinement around grouped GMRES (one pass Y = A P per iteration) "
-                         "or grouped BiCG with FP64 residual replacement")
+    ap.add_argument("--batch-solver", choices=("block", "gmres", "reliable"), default="block",
+                    help="batch mixed path: FP64 refinement around grouped block GMRES (one Krylov space per "
+                         "batch of 16) or GMRES per column (one pass Y = A P per iteration), or grouped BiCG "
+                         "with FP64 residual replacement")
     ap.add_argument("--replace", type=float, default=None,
                     help="single-vector workloads: residual drop between the mixed path's FP64 replacements")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -914,7 +934,9 @@ def compact(name, r):
                 "mixed_solver": r["config"]["mixed_solver"],
                 "fp64_solves_per_s": r["fp64_solves_per_s"], "speedup_mixed_vs_fp64": r["speedup_mixed_vs_fp64"],
                 "worst_rel_err_mixed_vs_fp64": r["worst_rel_err_mixed_vs_fp64"],
-                "schedule": r["config"]["schedule"]}
+                "schedule": r["config"]["schedule"],
+                **({"fp64_block_gmres_solves_per_s": r["fp64_block_gmres"]["solves_per_s"]}
+                   if "fp64_block_gmres" in r else {})}
     if name == "adjoint96":
         keys = ("value", "unit", "forward_s", "gradient_s", "forward_iterations", "fd_entries_resolved",
                 "fd_worst_rel_err", "fd_step_1e-6", "dk_rel_err")

@@ -517,7 +517,8 @@ int ibim_solve(ibim_ctx* ctx, const ibim_solve_opts* opts, ibim_solve_info* info
                    : run_refined(ctx, opts, info, hist);
     if (opts->method == IBIM_SOLVE_RELIABLE)
         return batched ? run_reliable_batched(ctx, opts, info, hist) : run_reliable(ctx, opts, info, hist);
-    if (opts->method == IBIM_SOLVE_GMRES) return run_gmres(ctx, opts, info, hist);
+    if (opts->method == IBIM_SOLVE_GMRES || opts->method == IBIM_SOLVE_BLOCK_GMRES)
+        return run_gmres(ctx, opts, info, hist);
     return IBIM_ERR_ARG;
 }
 

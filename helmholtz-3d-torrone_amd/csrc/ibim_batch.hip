@@ -1099,9 +1099,9 @@ struct GroupRun {
 }  // namespace
 
 int run_group(Context* const* ctxs, int n, int method, const ibim_solve_opts* opts, ibim_solve_info* infos) {
-    if (method == IBIM_SOLVE_GMRES) return run_group_gmres(ctxs, n, opts, infos);
+    if (method == IBIM_SOLVE_GMRES || method == IBIM_SOLVE_BLOCK_GMRES) return run_group_gmres(ctxs, n, opts, infos);
     if (method != IBIM_SOLVE_BICG && method != IBIM_SOLVE_RELIABLE) {
-        set_error("ibim_solve_group: BiCG, reliable BiCG or GMRES only");
+        set_error("ibim_solve_group: BiCG, reliable BiCG, GMRES or block GMRES only");
         return IBIM_ERR_ARG;
     }
     const bool mixed = method == IBIM_SOLVE_RELIABLE;

@@ -33,6 +33,7 @@
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <type_traits>
 #include <vector>
 
 #include "ibim_internal.hpp"
@@ -168,6 +169,79 @@ __global__ void k_gm_unpack(const double2* __restrict__ X, int count, int c0, in
     if (r < ncol) out[(size_t)(c0 + r) * count + j] = X[e];
 }
 
+// ---------------------------------------------------------- block GMRES --
+// Block products of 16-column blocks (point j, column a at j * 16 + a) on the
+// FP64 matrix cores: one v_mfma_f64_16x16x4f64 k-step takes 4 points, and
+// lane l loads element 64 s + l of both blocks -- A[a][k] = conj(V[4 s + k][a])
+// and B[k][b] = W[4 s + k][b] sit at the same index -- so the D tile holds
+// V^H W (Acc3: three real products per complex one).
+constexpr int kBlockChunks = 64;  // waves per basis block in the block products
+constexpr int kMaxBlocks = 64;    // blocks per block-GMRES cycle (then restart)
+
+// part[(i kBlockChunks + c) 256 + 16 a + b] = sum over chunk c of conj(V_i[j][a]) W[j][b]
+__global__ void __launch_bounds__(kGT) k_bg_dots(const double2* __restrict__ V, size_t vs, int count, int nv,
+                                                 const double2* __restrict__ W, double2* __restrict__ part) {
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.y * (kGT / 64) + (threadIdx.x >> 6);
+    if (i >= nv) return;  // a whole wave; no barrier below
+    const int c = blockIdx.x;
+    const int steps = (count + 3) / 4;
+    const int per = (steps + kBlockChunks - 1) / kBlockChunks;
+    const int s1 = min(steps, (c + 1) * per);
+    const double2* v = V + (size_t)i * vs;
+    const size_t n = (size_t)count * kBatchRHS;
+    Acc3 acc;
+    for (int s = c * per; s < s1; ++s) {
+        const size_t e = (size_t)s * 64 + lane;
+        double2 a = make_double2(0.0, 0.0), b = a;
+        if (e < n) {
+            a = v[e];
+            b = W[e];
+        }
+        acc.add(a.x, -a.y, b.x, b.y);
+    }
+    double2* out = part + ((size_t)i * kBlockChunks + c) * 256;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) out[(4 * q + lane / 16) * 16 + lane % 16] = make_double2(acc.re(q), acc.im(q));
+}
+
+// out[i 256 + t] = sum_c part[(i kBlockChunks + c) 256 + t] in order
+__global__ void k_bg_dots_sum(const double2* __restrict__ part, int nv, double2* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nv * 256) return;
+    const int i = t / 256, r = t % 256;
+    double sr = 0.0, si = 0.0;
+    for (int c = 0; c < kBlockChunks; ++c) {
+        sr += part[((size_t)i * kBlockChunks + c) * 256 + r].x;
+        si += part[((size_t)i * kBlockChunks + c) * 256 + r].y;
+    }
+    out[t] = make_double2(sr, si);
+}
+
+// W[j][b] = (accumulate ? W[j][b] : 0) + sign sum_{i < nv} sum_a V_i[j][a] M_i[a][b]
+// (M_i: 16 x 16 blocks, row-major, M + 256 i)
+__global__ void __launch_bounds__(kGT) k_bg_gemm(const double2* __restrict__ V, size_t vs, int count,
+                                                 const double2* __restrict__ M, int nv, double sign, bool accumulate,
+                                                 double2* __restrict__ W) {
+    const size_t e = (size_t)blockIdx.x * kGT + threadIdx.x;
+    if (e >= (size_t)count * kBatchRHS) return;
+    const size_t row = e - e % kBatchRHS;
+    const int b = (int)(e % kBatchRHS);
+    double cr = 0.0, ci = 0.0;
+    for (int i = 0; i < nv; ++i) {
+        const double2* v = V + (size_t)i * vs + row;
+        const double2* m = M + (size_t)i * 256 + b;
+#pragma unroll
+        for (int a = 0; a < kBatchRHS; ++a) {
+            const double2 x = v[a], y = m[16 * a];
+            cr += x.x * y.x - x.y * y.y;
+            ci += x.x * y.y + x.y * y.x;
+        }
+    }
+    const double2 o = accumulate ? W[e] : make_double2(0.0, 0.0);
+    W[e] = make_double2(o.x + sign * cr, o.y + sign * ci);
+}
+
 typedef std::complex<double> cplx;
 
 // Device workspace of one solve. Vectors hold C interleaved columns of ld rows
@@ -276,6 +350,9 @@ struct GmresRun {
     // per column: Hessenberg (column-major, MB + 1 rows), rotations, rotated residual
     std::vector<cplx> H, cs, sn, gv;
     std::vector<double2> hh, yd;  // host copies of the coefficients; the combination
+
+    static size_t space_bytes(const Context* ctx) { return ibim::space_bytes<C>(ctx); }
+    static int space_at(Context* ctx, void* base, GmresSpace* s) { return ibim::space_at<C>(ctx, base, s); }
 
     cplx& Hc(int c, int i, int k) { return H[((size_t)c * MB + k) * (MB + 1) + i]; }
     bool done() const { return phase == kDone; }
@@ -438,6 +515,330 @@ struct GmresRun {
     }
 };
 
+// Block GMRES (Saad, Iterative Methods for Sparse Linear Systems, 2003, 6.12)
+// over one batch of 16 columns: ONE Krylov space for all of them, built from
+// the residual block R0 = V0 B0. Iteration j: W = A V_j (one 16-column pass),
+// block classical Gram-Schmidt applied twice against V_0 .. V_j (16 x 16
+// coefficient blocks on the matrix cores), then W = V_{j+1} R by Cholesky QR
+// applied twice (Fukaya et al., 2014; the 16 x 16 Cholesky on the host); the
+// block Hessenberg matrix is reduced by Givens rotations as it grows, and the
+// rotated B0 gives every column's residual. Each column of a batch searches
+// the whole block space (16 (j + 1) dimensions after j + 1 passes) where
+// per-column GMRES searches its own (j + 1): on problems whose columns share
+// slow modes -- near an interior resonance, where the double layer's
+// per-column GMRES needs ~140 products -- the block space finds them in a
+// fraction of the passes. Columns at their target before the cycle starts
+// stay out of it (zero basis columns); a Cholesky pivot below 1e-24 of the
+// block's largest (a basis block that has lost a direction) ends the cycle
+// and the next one restarts from the true residuals of the columns left.
+// The same resumable interface as GmresRun.
+struct BlockRun {
+    enum Phase { kIterate, kRestart, kDone };
+    static constexpr int P = kBatchRHS;
+    static constexpr int MB = kMaxBlocks;
+    Context* ctx = nullptr;
+    GmresSpace g{};
+    const double2* rhs = nullptr;
+    double target[P] = {};
+    int budget = 0, m = 0;
+    int passes = 0;
+    int* col_products = nullptr;
+    double estimate[P] = {};
+    Phase phase = kDone;
+    int j = 0;            // basis blocks of the cycle with their product applied
+    int p = 0;            // columns in the cycle
+    int idx[P] = {};      // their slots
+    bool active[P] = {};  // slot in the cycle
+    // block Hessenberg ((MB + 1) p x MB p, column-major), rotated right-hand side
+    // ((MB + 1) p x p), Givens rotations of rows (r, r + 1) in order
+    std::vector<cplx> H, E;
+    struct Rot {
+        int r;
+        cplx c, s;
+    };
+    std::vector<Rot> rots;
+    std::vector<double2> hh;      // device results on the host
+    std::vector<double2> upm, upy;  // uploads: a 16 x 16 inverse, the combination blocks (kept until synchronized)
+    double2* mat = nullptr;       // device 16 x 16 upload slot
+    double2* coefs = nullptr;     // device: 2 (MB + 1) coefficient blocks, then the Gram block
+    int hrows = 0;
+
+    static size_t space_bytes(const Context* ctx) {
+        const size_t vs = ctx->ld * P;
+        const size_t part = std::max((size_t)(MB + 1) * kBlockChunks * 256, (size_t)dot_chunks<P>() * P);
+        return sizeof(double2) * (vs * ((size_t)MB + 1 + 6) + (2 * (size_t)(MB + 1) + 1) * 256 +
+                                  ((size_t)MB + 1) * 256 + part) +
+               512;
+    }
+    static int space_at(Context* ctx, void* base, GmresSpace* s) {
+        s->vs = ctx->ld * P;
+        s->n = ctx->count * P;
+        s->V = static_cast<double2*>(base);
+        s->w = s->V + ((size_t)MB + 1) * s->vs;
+        s->d = s->w + s->vs;
+        s->r = s->d + s->vs;
+        s->res = s->r + s->vs;
+        s->b = s->res + s->vs;
+        s->x = s->b + s->vs;
+        s->coef = s->x + s->vs;
+        s->y = s->coef + (2 * (size_t)(MB + 1) + 1) * 256;
+        s->part = s->y + ((size_t)MB + 1) * 256;
+        IBIM_HIP(hipMemsetAsync(s->V, 0, sizeof(double2) * s->vs * ((size_t)MB + 1 + 6), ctx->stream));
+        return IBIM_OK;
+    }
+
+    cplx& Hc(int r, int c) { return H[(size_t)c * hrows + r]; }
+    cplx& Ec(int r, int c) { return E[(size_t)c * hrows + r]; }
+    bool done() const { return phase == kDone; }
+    const double2* in() const { return phase == kIterate ? g.V + (size_t)j * g.vs : g.d; }
+    double2* out() const { return g.w; }
+    unsigned blocks() const { return (unsigned)((g.n + kGT - 1) / kGT); }
+
+    // out[i] = V_i^H W for i < nv (16 x 16 blocks, device)
+    void block_dots(const double2* V, int nv, const double2* W, double2* outp) {
+        hipLaunchKernelGGL(k_bg_dots, dim3(kBlockChunks, (unsigned)((nv + 3) / 4)), dim3(kGT), 0, ctx->stream, V,
+                           g.vs, ctx->count, nv, W, g.part);
+        hipLaunchKernelGGL(k_bg_dots_sum, dim3((unsigned)((nv * 256 + 255) / 256)), dim3(256), 0, ctx->stream, g.part,
+                           nv, outp);
+    }
+    // W = (accumulate ? W : 0) + sign sum_i V_i M_i
+    void block_gemm(const double2* V, const double2* M, int nv, double sign, bool accumulate, double2* W) {
+        hipLaunchKernelGGL(k_bg_gemm, dim3(blocks()), dim3(kGT), 0, ctx->stream, V, g.vs, ctx->count, M, nv, sign,
+                           accumulate, W);
+    }
+    // the Gram block of W (device) -> host (synchronizes)
+    int gram(const double2* W, cplx* G) {
+        double2* gd = coefs + 2 * (size_t)(MB + 1) * 256;
+        block_dots(W, 1, W, gd);
+        IBIM_HIP(hipGetLastError());
+        hh.resize(256);
+        IBIM_HIP(hipMemcpyAsync(hh.data(), gd, sizeof(double2) * 256, hipMemcpyDeviceToHost, ctx->stream));
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));
+        for (int t = 0; t < 256; ++t) G[t] = cplx(hh[t].x, hh[t].y);
+        return IBIM_OK;
+    }
+    // R (p x p upper, row-major) with G[idx][idx] = R^H R; false on a pivot
+    // below 1e-24 of the largest diagonal entry
+    bool cholesky(const cplx* G, cplx* R) const {
+        double dmax = 0.0;
+        for (int a = 0; a < p; ++a) dmax = std::max(dmax, G[idx[a] * 16 + idx[a]].real());
+        for (int t = 0; t < p * p; ++t) R[t] = 0.0;
+        for (int k = 0; k < p; ++k) {
+            double d = G[idx[k] * 16 + idx[k]].real();
+            for (int q = 0; q < k; ++q) d -= std::norm(R[q * p + k]);
+            if (!(d > 1e-24 * dmax)) return false;
+            const double rk = sqrt(d);
+            R[k * p + k] = rk;
+            for (int c = k + 1; c < p; ++c) {
+                cplx v = G[idx[k] * 16 + idx[c]];
+                for (int q = 0; q < k; ++q) v -= std::conj(R[q * p + k]) * R[q * p + c];
+                R[k * p + c] = v / rk;
+            }
+        }
+        return true;
+    }
+    // upload the inverse of R (p x p upper) as a 16 x 16 block over the slots
+    int upload_inverse(const cplx* R) {
+        std::vector<cplx> Ri((size_t)p * p, cplx(0.0));
+        for (int c = 0; c < p; ++c) {  // column c of R^{-1}: R x = e_c
+            for (int r = c; r >= 0; --r) {
+                cplx v = r == c ? cplx(1.0) : cplx(0.0);
+                for (int q = r + 1; q <= c; ++q) v -= R[r * p + q] * Ri[q * p + c];
+                Ri[r * p + c] = v / R[r * p + r];
+            }
+        }
+        // the previous upload from upm has completed: every call follows a gram() synchronization
+        upm.assign(256, make_double2(0.0, 0.0));
+        for (int a = 0; a < p; ++a)
+            for (int b = 0; b < p; ++b)
+                upm[idx[a] * 16 + idx[b]] = make_double2(Ri[a * p + b].real(), Ri[a * p + b].imag());
+        IBIM_HIP(hipMemcpyAsync(mat, upm.data(), sizeof(double2) * 256, hipMemcpyHostToDevice, ctx->stream));
+        return IBIM_OK;
+    }
+    // dst = src R^{-1} with R^H R = the Gram block of src, twice (Cholesky QR 2);
+    // Rout = R2 R1. False when a pivot fails. tmp: a free vector.
+    int orthonormalize(const double2* src, double2* dst, double2* tmp, cplx* Rout, bool* ok) {
+        std::vector<cplx> G(256), R1((size_t)p * p), R2((size_t)p * p);
+        if (int rc = gram(src, G.data())) return rc;
+        *ok = cholesky(G.data(), R1.data());
+        if (!*ok) return IBIM_OK;
+        if (int rc = upload_inverse(R1.data())) return rc;
+        block_gemm(src, mat, 1, 1.0, false, tmp);
+        if (int rc = gram(tmp, G.data())) return rc;  // also: the upload is done
+        *ok = cholesky(G.data(), R2.data());
+        if (!*ok) return IBIM_OK;
+        if (int rc = upload_inverse(R2.data())) return rc;
+        block_gemm(tmp, mat, 1, 1.0, false, dst);
+        IBIM_HIP(hipGetLastError());
+        for (int a = 0; a < p; ++a)
+            for (int b = 0; b < p; ++b) {
+                cplx v = 0.0;
+                for (int q = 0; q < p; ++q) v += R2[a * p + q] * R1[q * p + b];
+                Rout[a * p + b] = v;
+            }
+        return IBIM_OK;
+    }
+
+    int start() {
+        passes = 0;
+        mat = g.y + (size_t)MB * 256;
+        coefs = g.coef;
+        IBIM_HIP(hipMemsetAsync(g.d, 0, sizeof(double2) * g.vs, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(g.r, rhs, sizeof(double2) * g.n, hipMemcpyDeviceToDevice, ctx->stream));
+        return cycle_begin();
+    }
+
+    // a cycle from the residual block r: the columns above their targets, V_0 R0 = r
+    int cycle_begin() {
+        std::vector<cplx> G(256);
+        if (int rc = gram(g.r, G.data())) return rc;
+        p = 0;
+        for (int c = 0; c < P; ++c) {
+            estimate[c] = sqrt(std::max(G[c * 16 + c].real(), 0.0));
+            active[c] = estimate[c] > target[c];
+            if (active[c]) idx[p++] = c;
+        }
+        j = 0;
+        if (p == 0) {
+            phase = kDone;
+            return IBIM_OK;
+        }
+        std::vector<cplx> R0((size_t)p * p);
+        bool ok = false;
+        // V_0 in the slot of basis block 0; g.w is free until the first product
+        if (int rc = orthonormalize(g.r, g.V, g.w, R0.data(), &ok)) return rc;
+        if (!ok) {
+            // dependent residual columns: the cycle takes the largest one alone
+            int best = idx[0];
+            for (int a = 0; a < p; ++a)
+                if (estimate[idx[a]] > estimate[best]) best = idx[a];
+            for (int c = 0; c < P; ++c) active[c] = c == best;
+            p = 1;
+            idx[0] = best;
+            if (int rc = orthonormalize(g.r, g.V, g.w, R0.data(), &ok)) return rc;
+            if (!ok) {
+                phase = kDone;
+                return IBIM_OK;
+            }
+        }
+        hrows = (MB + 1) * p;
+        H.assign((size_t)hrows * MB * p, cplx(0.0));
+        E.assign((size_t)hrows * p, cplx(0.0));
+        rots.clear();
+        for (int a = 0; a < p; ++a)
+            for (int b = a; b < p; ++b) Ec(a, b) = R0[a * p + b];
+        phase = kIterate;
+        return IBIM_OK;
+    }
+
+    int launch_post() {
+        ++passes;
+        for (int c = 0; c < P; ++c) col_products[c] += active[c];
+        if (phase == kRestart) {
+            hipLaunchKernelGGL(k_gm_sub, dim3(blocks()), dim3(kGT), 0, ctx->stream, rhs, g.w, g.n, g.r);
+            IBIM_HIP(hipGetLastError());
+            return IBIM_OK;
+        }
+        // block CGS2: C = V^H W, W -= V C, twice
+        for (int pass = 0; pass < 2; ++pass) {
+            double2* c = coefs + (size_t)pass * (MB + 1) * 256;
+            block_dots(g.V, j + 1, g.w, c);
+            block_gemm(g.V, c, j + 1, -1.0, true, g.w);
+        }
+        IBIM_HIP(hipGetLastError());
+        hh.resize(2 * (size_t)(MB + 1) * 256);
+        for (int pass = 0; pass < 2; ++pass)
+            IBIM_HIP(hipMemcpyAsync(hh.data() + (size_t)pass * (MB + 1) * 256, coefs + (size_t)pass * (MB + 1) * 256,
+                                    sizeof(double2) * 256 * (j + 1), hipMemcpyDeviceToHost, ctx->stream));
+        return IBIM_OK;
+    }
+
+    // rotation zeroing H(r + 1, col) against H(r, col), applied from column col on
+    void rotate(int r, int col, int last) {
+        const cplx a = Hc(r, col), bb = Hc(r + 1, col);
+        const double rr = sqrt(std::norm(a) + std::norm(bb));
+        const cplx c = rr > 0.0 ? std::conj(a) / rr : cplx(1.0), sn = rr > 0.0 ? std::conj(bb) / rr : cplx(0.0);
+        rots.push_back({r, c, sn});
+        for (int q = col; q <= last; ++q) {
+            const cplx x = Hc(r, q), y = Hc(r + 1, q);
+            Hc(r, q) = c * x + sn * y;
+            Hc(r + 1, q) = -std::conj(sn) * x + std::conj(c) * y;
+        }
+        for (int q = 0; q < p; ++q) {
+            const cplx x = Ec(r, q), y = Ec(r + 1, q);
+            Ec(r, q) = c * x + sn * y;
+            Ec(r + 1, q) = -std::conj(sn) * x + std::conj(c) * y;
+        }
+    }
+
+    int host_post() {
+        if (phase == kRestart) return cycle_begin();
+        // block column j of H: the coefficient blocks, then W = V_{j+1} R
+        const int c0 = j * p;
+        for (int i = 0; i <= j; ++i)
+            for (int a = 0; a < p; ++a)
+                for (int b = 0; b < p; ++b) {
+                    const double2 x = hh[(size_t)i * 256 + idx[a] * 16 + idx[b]];
+                    const double2 y = hh[((size_t)(MB + 1) + i) * 256 + idx[a] * 16 + idx[b]];
+                    Hc(i * p + a, c0 + b) = cplx(x.x + y.x, x.y + y.y);
+                }
+        std::vector<cplx> R((size_t)p * p, cplx(0.0));
+        bool ok = false;
+        // V_{j+1} in basis slot j + 1 (g.r is free between restarts)
+        if (j + 1 <= MB)
+            if (int rc = orthonormalize(g.w, g.V + (size_t)(j + 1) * g.vs, g.r, R.data(), &ok)) return rc;
+        for (int a = 0; a < p; ++a)
+            for (int b = a; b < p; ++b) Hc((j + 1) * p + a, c0 + b) = ok ? R[a * p + b] : cplx(0.0);
+        // earlier rotations, then the new subdiagonal block eliminated column by column
+        for (const Rot& t : rots)
+            for (int q = c0; q < c0 + p; ++q) {
+                const cplx x = Hc(t.r, q), y = Hc(t.r + 1, q);
+                Hc(t.r, q) = t.c * x + t.s * y;
+                Hc(t.r + 1, q) = -std::conj(t.s) * x + std::conj(t.c) * y;
+            }
+        for (int b = 0; b < p; ++b) {
+            const int col = c0 + b;
+            for (int r = (j + 1) * p + b - 1; r >= col; --r) rotate(r, col, c0 + p - 1);
+        }
+        ++j;
+        bool any = false;
+        for (int a = 0; a < p; ++a) {
+            double s2 = 0.0;
+            for (int r = j * p; r < (j + 1) * p; ++r) s2 += std::norm(Ec(r, a));
+            estimate[idx[a]] = sqrt(s2);
+            any = any || estimate[idx[a]] > target[idx[a]] || estimate[idx[a]] != estimate[idx[a]];
+        }
+        if (!ok || !any || j >= m || passes >= budget) return cycle_end(any);
+        return IBIM_OK;
+    }
+
+    // Y = R^{-1} E over the j block columns, d += sum_i V_i Y_i; then a restart
+    // for the columns still above their targets, budget permitting
+    int cycle_end(bool again) {
+        const int nc = j * p;
+        std::vector<cplx> Y((size_t)nc * p, cplx(0.0));
+        for (int q = 0; q < p; ++q)
+            for (int r = nc - 1; r >= 0; --r) {
+                cplx v = Ec(r, q);
+                for (int t = r + 1; t < nc; ++t) v -= Hc(r, t) * Y[(size_t)q * nc + t];
+                Y[(size_t)q * nc + r] = Hc(r, r) != cplx(0.0) ? v / Hc(r, r) : cplx(0.0);
+            }
+        upy.assign((size_t)j * 256, make_double2(0.0, 0.0));
+        for (int i = 0; i < j; ++i)
+            for (int a = 0; a < p; ++a)
+                for (int q = 0; q < p; ++q) {
+                    const cplx v = Y[(size_t)q * nc + i * p + a];
+                    upy[(size_t)i * 256 + idx[a] * 16 + idx[q]] = make_double2(v.real(), v.imag());
+                }
+        IBIM_HIP(hipMemcpyAsync(g.y, upy.data(), sizeof(double2) * upy.size(), hipMemcpyHostToDevice, ctx->stream));
+        block_gemm(g.V, g.y, j, 1.0, true, g.d);
+        IBIM_HIP(hipGetLastError());
+        IBIM_HIP(hipStreamSynchronize(ctx->stream));  // upy stays untouched until d is formed
+        phase = again && passes < budget ? kRestart : kDone;
+        return IBIM_OK;
+    }
+};
+
 // One GMRES solve on one context (see GmresRun). Adds each column's products
 // to col_products; returns the passes applied and the last estimates.
 template <int C>
@@ -465,6 +866,31 @@ int gmres(Context* ctx, const GmresSpace& g, int op, const double2* rhs, const d
     *passes = run.passes;
     for (int c = 0; c < C; ++c) estimate[c] = run.estimate[c];
     return rc;
+}
+
+// Runs a resumable solve (GmresRun, BlockRun) to its end on one context.
+template <class Run>
+int solve_run(Run& run, int op) {
+    int rc = run.start();
+    while (!rc && !run.done()) {
+        rc = apply_op<kBatchRHS>(run.ctx, op, run.in(), run.out());
+        if (!rc) rc = run.launch_post();
+        if (!rc) IBIM_HIP(hipStreamSynchronize(run.ctx->stream));
+        if (!rc) rc = run.host_post();
+    }
+    return rc;
+}
+
+// Basis length of a batched run: opts->chunk when set, else the largest.
+template <class Run>
+int basis_of(const ibim_solve_opts* opts);
+template <>
+int basis_of<GmresRun<kBatchRHS>>(const ibim_solve_opts* opts) {
+    return opts->chunk > 0 ? std::min(opts->chunk, kMaxBasisB) : kMaxBasisB;
+}
+template <>
+int basis_of<BlockRun>(const ibim_solve_opts* opts) {
+    return opts->chunk > 0 ? std::min(opts->chunk, kMaxBlocks) : kMaxBlocks;
 }
 
 // The operators of a refinement: GMRES on the stored one, residuals with the
@@ -557,19 +983,22 @@ int gmres_columns(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
 }
 
 // 16 right-hand sides per batch: the same refinement per column, every
-// product of a batch one 16-column pass. Iterations are summed over the
-// columns (each column counts the passes it took part in), as the batched
-// BiCG counts them.
+// product of a batch one 16-column pass (Run: GmresRun<16>, a Krylov space per
+// column; BlockRun, one block Krylov space per batch). Iterations are summed
+// over the columns (each column counts the passes it took part in), as the
+// batched BiCG counts them.
+template <class Run>
 int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
     constexpr int C = kBatchRHS;
     int inner, outer_op;
     refinement_ops(ctx, &inner, &outer_op);
     const int count = ctx->count;
     GmresSpace g;
-    if (int rc = make_space<C>(ctx, &g)) return rc;
+    if (ctx->refine.ensure(Run::space_bytes(ctx))) return IBIM_ERR_NOMEM;
+    if (int rc = Run::space_at(ctx, ctx->refine.ptr, &g)) return rc;
     const unsigned nb = (unsigned)((g.n + kGT - 1) / kGT);
     const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
-    const int m = opts->chunk > 0 ? std::min(opts->chunk, kMaxBasisB) : kMaxBasisB;
+    const int m = basis_of<Run>(opts);
     int total = 0, outer_max = 0, hist_at = 0, status = 4;
     double worst = 0.0;
     for (int c0 = 0; c0 < ctx->nrhs; c0 += C) {
@@ -593,12 +1022,22 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
         int used = 0, steps = 0;
         while (any && steps < max_outer && used < opts->maxit) {
             ++steps;
-            int passes = 0;
-            double est[C];
-            if (int rc = gmres<C>(ctx, g, inner, g.res, target, opts->maxit - used, m, &passes, col_used, est,
-                                  c0 == 0 ? hist : nullptr, &hist_at, opts->maxit, normb[0]))
-                return rc;
-            used += passes;
+            Run run;
+            run.ctx = ctx;
+            run.g = g;
+            run.rhs = g.res;
+            for (int c = 0; c < C; ++c) run.target[c] = target[c];
+            run.budget = opts->maxit - used;
+            run.m = m;
+            run.col_products = col_used;
+            if constexpr (std::is_same<Run, GmresRun<C>>::value) {
+                run.hist = c0 == 0 ? hist : nullptr;
+                run.hist_at = &hist_at;
+                run.hist_len = opts->maxit;
+                run.normb = normb[0];
+            }
+            if (int rc = solve_run(run, inner)) return rc;
+            used += run.passes;
             hipLaunchKernelGGL(k_gm_add, dim3(nb), dim3(kGT), 0, ctx->stream, g.x, g.d, g.n);
             if (int rc = apply_op<C>(ctx, outer_op, g.x, g.w)) return rc;
             hipLaunchKernelGGL(k_gm_sub, dim3(nb), dim3(kGT), 0, ctx->stream, g.b, g.w, g.n, g.res);
@@ -628,7 +1067,7 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
             worst = std::max(worst, relres[c]);
             if (status == 4 || col_status[c] != 0) status = col_status[c];
         }
-        report_progress(ctx, "GMRES (16 columns)", used, worst);
+        report_progress(ctx, std::is_same<Run, BlockRun>::value ? "block GMRES" : "GMRES (16 columns)", used, worst);
     }
     info->iterations = total;
     info->outer = outer_max;
@@ -639,19 +1078,18 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
 }
 
 // One batch of up to 16 columns of one problem in a grouped solve.
+template <class Run>
 struct GroupBatch {
     Context* ctx = nullptr;
     int c0 = 0, ncol = 0;
     int inner = OP_DENSE, outer = OP_DENSE;
     GmresSpace g{};
-    GmresRun<kBatchRHS> run;
+    Run run;
     double normb[kBatchRHS] = {}, relres[kBatchRHS] = {}, target[kBatchRHS] = {};
     int status[kBatchRHS] = {}, used[kBatchRHS] = {};
     int passes = 0, outer_steps = 0;
     bool running = false;
 };
-
-}  // namespace
 
 // Several pair-stored problems at once (ibim_solve_group): every batch of 16
 // columns of every problem runs the refinement of gmres_batched, and the
@@ -662,7 +1100,8 @@ struct GroupBatch {
 // whose GMRES has finished waits for the others before its FP64 residual.
 // Each batch takes the same steps, and gives the same bits, as its own
 // batched solve.
-int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+template <class Run>
+int group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos) {
     constexpr int C = kBatchRHS;
     int source = 0;
     if (int rc = pair_group_source(ctxs, n, false, &source)) return rc;
@@ -672,10 +1111,10 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
     std::vector<int> order(n);
     for (int c = 0; c < n; ++c) order[c] = c;
     std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return ctxs[x]->count > ctxs[y]->count; });
-    std::vector<GroupBatch> bs;
+    std::vector<GroupBatch<Run>> bs;
     for (int c : order)
         for (int c0 = 0; c0 < ctxs[c]->nrhs; c0 += C) {
-            GroupBatch b;
+            GroupBatch<Run> b;
             b.ctx = ctxs[c];
             b.c0 = c0;
             b.ncol = std::min(C, ctxs[c]->nrhs - c0);
@@ -684,7 +1123,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
     const int nb = (int)bs.size();
     // workspaces, then the queue counter and the member table
     std::vector<size_t> off(nb + 1, 0);
-    for (int m = 0; m < nb; ++m) off[m + 1] = off[m] + ((space_bytes<C>(bs[m].ctx) + 255) & ~(size_t)255);
+    for (int m = 0; m < nb; ++m) off[m + 1] = off[m] + ((Run::space_bytes(bs[m].ctx) + 255) & ~(size_t)255);
     const size_t table = off[nb] + 256;
     if (lead->group.ensure(table + sizeof(GroupMember) * nb)) return IBIM_ERR_NOMEM;
     char* base = lead->group.as<char>();
@@ -692,12 +1131,12 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
     GroupMember* dev_members = reinterpret_cast<GroupMember*>(base + table);
     const int maxit = opts->maxit;
     const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
-    const int mlen = opts->chunk > 0 ? std::min(opts->chunk, kMaxBasisB) : kMaxBasisB;
+    const int mlen = basis_of<Run>(opts);
     for (int m = 0; m < nb; ++m) {
-        GroupBatch& b = bs[m];
+        GroupBatch<Run>& b = bs[m];
         Context* ctx = b.ctx;
         refinement_ops(ctx, &b.inner, &b.outer);
-        if (int rc = space_at<C>(ctx, base + off[m], &b.g)) return rc;
+        if (int rc = Run::space_at(ctx, base + off[m], &b.g)) return rc;
         const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
         hipLaunchKernelGGL(k_gm_pack, dim3(blocks), dim3(kGT), 0, ctx->stream, ctx->rhs.as<double2>(), ctx->count,
                            b.c0, b.ncol, b.g.b);
@@ -717,9 +1156,9 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
         // a GMRES solve for every batch with running columns and budget left
         std::vector<int> live, stepped;
         for (int m = 0; m < nb; ++m) {
-            GroupBatch& b = bs[m];
+            GroupBatch<Run>& b = bs[m];
             if (!b.running || b.passes >= maxit) continue;
-            GmresRun<C>& r = b.run;
+            Run& r = b.run;
             r.ctx = b.ctx;
             r.g = b.g;
             r.rhs = b.g.res;
@@ -736,7 +1175,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
             tab.clear();
             int items = 0;
             for (int m : live) {
-                const GroupBatch& b = bs[m];
+                const GroupBatch<Run>& b = bs[m];
                 const Context* ctx = b.ctx;
                 GroupMember d{};
                 d.F = ctx->mat.ptr;
@@ -754,7 +1193,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
                                     lead->stream));
             if (int rc = pair_group_pass(lead, source, dev_members, (int)tab.size(), items, queue, false)) return rc;
             for (int m : live) {
-                GroupBatch& b = bs[m];
+                GroupBatch<Run>& b = bs[m];
                 if (int rc = pair_batch_near(b.ctx, b.run.in(), nullptr, b.run.out(), nullptr, nullptr)) return rc;
                 if (int rc = b.run.launch_post()) return rc;
             }
@@ -769,7 +1208,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
         // the FP64 residual of every batch that took a step
         double worst = 0.0;
         for (int m : stepped) {
-            GroupBatch& b = bs[m];
+            GroupBatch<Run>& b = bs[m];
             Context* ctx = b.ctx;
             const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
             b.passes += b.run.passes;
@@ -793,7 +1232,9 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
                 if (b.status[c] == 1) worst = std::max(worst, b.relres[c]);
             }
         }
-        report_progress(lead, "grouped GMRES, refinement", outer, worst);
+        report_progress(lead, std::is_same<Run, BlockRun>::value ? "grouped block GMRES, refinement"
+                                                                 : "grouped GMRES, refinement",
+                        outer, worst);
     }
     // per problem: its columns' results, summed as the one-problem solver does
     for (int c = 0; c < n; ++c) {
@@ -801,7 +1242,7 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
         ibim_solve_info& info = infos[c];
         info = ibim_solve_info{};
         int status = 4;
-        for (GroupBatch& b : bs) {
+        for (GroupBatch<Run>& b : bs) {
             if (b.ctx != ctx) continue;
             const unsigned blocks = (unsigned)((b.g.n + kGT - 1) / kGT);
             hipLaunchKernelGGL(k_gm_unpack, dim3(blocks), dim3(kGT), 0, ctx->stream, b.g.x, ctx->count, b.c0, b.ncol,
@@ -822,12 +1263,21 @@ int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ib
     return IBIM_OK;
 }
 
+}  // namespace
+
+int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos) {
+    return opts->method == IBIM_SOLVE_BLOCK_GMRES ? group_gmres<BlockRun>(ctxs, n, opts, infos)
+                                                  : group_gmres<GmresRun<kBatchRHS>>(ctxs, n, opts, infos);
+}
+
 int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, double* hist) {
     // 16-column passes where the batched products exist: every operator of
     // the traditional tube, or dense complex128 storage refined against itself
     const bool batched = ctx->nrhs > 1 && opts->batch == 0 &&
                          (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C128);
-    return batched ? gmres_batched(ctx, opts, info, hist) : gmres_columns(ctx, opts, info, hist);
+    if (!batched) return gmres_columns(ctx, opts, info, hist);  // block GMRES of one column is GMRES
+    return opts->method == IBIM_SOLVE_BLOCK_GMRES ? gmres_batched<BlockRun>(ctx, opts, info, hist)
+                                                  : gmres_batched<GmresRun<kBatchRHS>>(ctx, opts, info, hist);
 }
 
 }  // namespace ibim

@@ -39,7 +39,7 @@ C128_PAIR, C64_PAIR = 2, 3
 MIXED_PAIR = 4
 RHS_POINT_SOURCE, RHS_PLANE_NEUMANN, RHS_PLANE_DIRICHLET = 0, 1, 2
 EVAL_SINGLE, EVAL_DOUBLE, EVAL_COMBO, EVAL_COMBINED = 0, 1, 2, 3
-SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE, SOLVE_GMRES = 0, 1, 2, 3
+SOLVE_BICG, SOLVE_REFINED, SOLVE_RELIABLE, SOLVE_GMRES, SOLVE_BLOCK_GMRES = 0, 1, 2, 3, 4
 OP_STORED, OP_FREE64, OP_FREE32 = 0, 1, 2
 
 
@@ -325,7 +325,7 @@ class Problem:
         info = SolveInfo()
         hist = np.zeros(max(maxit, max_outer))
         _check(lib().ibim_solve(self.h, C.byref(o), C.byref(info), _p(hist)), "ibim_solve")
-        n = info.iterations if method in (SOLVE_BICG, SOLVE_RELIABLE, SOLVE_GMRES) and self.nrhs == 1 else info.outer
+        n = info.iterations if method in (SOLVE_BICG, SOLVE_RELIABLE, SOLVE_GMRES, SOLVE_BLOCK_GMRES) and self.nrhs == 1 else info.outer
         return info, hist[:n]
 
     def density(self, col=0):

@@ -300,7 +300,9 @@ def sphere_problem(gpu, n, store, fp32, dirichlet=1, rhs=16):
                                                ("c64_pair", "SOLVE_RELIABLE", True),
                                                ("c128_pair", "SOLVE_GMRES", False),
                                                ("mixed_pair", "SOLVE_GMRES", False),
-                                               ("c64_pair", "SOLVE_GMRES", True)])
+                                               ("c64_pair", "SOLVE_GMRES", True),
+                                               ("mixed_pair", "SOLVE_BLOCK_GMRES", False),
+                                               ("c64_pair", "SOLVE_BLOCK_GMRES", True)])
 def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
     """Several problems of uneven sizes solved together (one persistent launch
     of every operator per iteration, a problem with two batches among them):
@@ -309,7 +311,7 @@ def test_group_solve_gives_each_problem_its_own_bits(gpu, store, method, fp32):
     m = getattr(gpu, method)
     sizes, nrhs = (20, 28, 24), (16, 19, 5)
     # chunk: BiCG's host-check interval; GMRES: its basis length (0: the default)
-    kw = dict(maxit=500, tol=1e-10, inner_tol=1e-2, chunk=0 if method == "SOLVE_GMRES" else 2)
+    kw = dict(maxit=500, tol=1e-10, inner_tol=1e-2, chunk=0 if "GMRES" in method else 2)
     own = []
     for n, r in zip(sizes, nrhs):
         pb = sphere_problem(gpu, n, store, fp32, rhs=r)

@@ -230,3 +230,73 @@ def test_matrix_free_product_without_the_adjoint_is_the_same_bits(gpu, op, diric
     y, _ = pb.apply(p, q, getattr(gpu, op))
     assert np.array_equal(pb.apply_y(p, getattr(gpu, op)), y)
     pb.close()
+
+
+@pytest.mark.parametrize("store", ["C128_PAIR", "MIXED_PAIR", "C64_PAIR", "C128", "C64"])
+@pytest.mark.parametrize("dirichlet,combined", [(1, 0), (0, 0), (1, 1)])
+def test_block_gmres_reaches_the_fp64_solution(gpu, store, dirichlet, combined):
+    """IBIM_SOLVE_BLOCK_GMRES, one Krylov space per batch of 16 columns: 20
+    right-hand sides (a batch of 16 and a ragged one of 4 with a zero column)
+    reach the FP64 solution on every storage and kernel; two solves give the
+    same bits."""
+    pb = problem(gpu, dirichlet=dirichlet, combined=combined)
+    m = 20
+    set_columns(gpu, pb, m)
+    pb.set_rhs(np.zeros(pb.count), np.zeros(pb.count), col=17)
+    pb.assemble(gpu.C128)
+    ref_info, _ = pb.solve(gpu.SOLVE_BICG, maxit=2000, tol=1e-13, batch=0)
+    assert ref_info.status == 0
+    ref = densities(pb, m)
+    pb.assemble(getattr(gpu, store))
+    info, _ = pb.solve(gpu.SOLVE_BLOCK_GMRES, maxit=2000, tol=1e-10)
+    assert info.status == 0 and info.rel_residual < 1e-10, (info.status, info.rel_residual)
+    got = densities(pb, m)
+    assert not got[17].any()
+    for c in range(m):
+        if c != 17:
+            assert relnorm(got[c], ref[c]) < 1e-8, (c, relnorm(got[c], ref[c]))
+    again, _ = pb.solve(gpu.SOLVE_BLOCK_GMRES, maxit=2000, tol=1e-10)
+    assert (again.iterations, again.outer) == (info.iterations, info.outer)
+    for c, x in enumerate(densities(pb, m)):
+        assert np.array_equal(x, got[c]), c
+    pb.close()
+
+
+def test_block_gmres_restarted_and_against_column_gmres(gpu):
+    """16 plane waves on the double layer near the sphere's first interior
+    resonances (k = 4.5): the block space needs fewer passes than GMRES column
+    by column needs products per column; a restart every 3 block passes still
+    reaches the same solution; FP32 matrix-core products too."""
+    pb = problem(gpu, k=4.5)
+    m = 16
+    pb.set_nrhs(m)
+    rng = np.random.default_rng(17)
+    dirs = rng.standard_normal((m, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    for c in range(m):
+        pb.rhs(gpu.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
+    pb.assemble(gpu.C64_PAIR)
+    col, _ = pb.solve(gpu.SOLVE_GMRES, maxit=3000, tol=1e-10)
+    x_col = densities(pb, m)
+    pb.reset_timers()
+    pb.set_profiling(True)
+    blk, _ = pb.solve(gpu.SOLVE_BLOCK_GMRES, maxit=3000, tol=1e-10)
+    passes = pb.kernel_time("batch_pair")[1]
+    pb.set_profiling(False)
+    x_blk = densities(pb, m)
+    short, _ = pb.solve(gpu.SOLVE_BLOCK_GMRES, maxit=3000, tol=1e-10, chunk=3)
+    x_short = densities(pb, m)
+    pb.set_fp32_products(True)
+    b32, _ = pb.solve(gpu.SOLVE_BLOCK_GMRES, maxit=3000, tol=1e-9)
+    x32 = densities(pb, m)
+    pb.set_fp32_products(False)
+    print("products per column: GMRES %.1f; block GMRES passes %d (%d column products), restarted %d, FP32 %d"
+          % (col.iterations / m, passes, blk.iterations, short.iterations, b32.iterations))
+    for info in (col, blk, short, b32):
+        assert info.status == 0, info.status
+    assert passes < col.iterations / m, (passes, col.iterations / m)
+    for c in range(m):
+        assert relnorm(x_blk[c], x_col[c]) < 1e-8, c
+        assert relnorm(x_short[c], x_col[c]) < 1e-8, c
+        assert relnorm(x32[c], x_col[c]) < 1e-7, c
+    pb.close()
