@@ -94,6 +94,10 @@ struct Context {
     // source (complex64, complex128, complex128 copy, computed FP64, computed
     // mixed) and product kind, 0 until timed; reset with every assembly
     int batch_waves[15] = {};
+    // untuned 16-column passes take 64 rows per workgroup without timing: set
+    // around passes a solve applies once or twice (refinement residuals), where
+    // timing the shapes would cost more than any shape could save
+    bool batch_untimed = false;
 
     // the FP64 matrix-free operator's cache (ibim_set_free_cache): complex128
     // pairs of the first free_cached_blocks row blocks (free_cached_tiles

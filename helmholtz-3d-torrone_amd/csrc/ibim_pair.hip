@@ -951,6 +951,7 @@ int tuned_waves(Context* ctx, int slot, const MT* F, const double2* P, const dou
         if (w == 1 || w == 2 || w == 4) return w;
     }
     if (ctx->batch_waves[slot]) return ctx->batch_waves[slot];
+    if (ctx->batch_untimed) return 4;
     // Operators of 2^17 points and more (the 256^3 surfaces: 0.3-0.5 s per
     // pass) take 64 rows per workgroup without timing: that shape won at
     // every size measured alone, and each timed shape costs a whole pass of
