@@ -515,16 +515,23 @@ __global__ void __launch_bounds__(kPT, Factors == kStoredFactors ? 1 : 2)
             rows[4 * u + 2] = swr;
             rows[4 * u + 3] = swi;
         }
-        const double tot = wave_sum8_transposed_x(rows, lane);
-        if ((lane & 7) == 0) sh[my_val & 3][wave][rr + (my_val >> 2)] = tot;
+        if constexpr (WithZ) {
+            const double tot = wave_sum8_transposed_x(rows, lane);
+            if ((lane & 7) == 0) sh[my_val & 3][wave][rr + (my_val >> 2)] = tot;
+        } else {
+            const double r4[4] = {rows[0], rows[1], rows[4], rows[5]};
+            const double tot = wave_sum4_transposed_x(r4);
+            if ((lane & 15) == 0) sh[(lane >> 4) & 1][wave][rr + (lane >> 5)] = tot;
+        }
     }
     __syncthreads();
     for (int row = t; row < kPR; row += kPT) {
         const int i = r0 + row;
         if (i >= count) break;
-        double s[4];
+        double s[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
-        for (int k = 0; k < 4; ++k) s[k] = (sh[k][0][row] + sh[k][1][row]) + (sh[k][2][row] + sh[k][3][row]);
+        for (int k = 0; k < (WithZ ? 4 : 2); ++k)
+            s[k] = (sh[k][0][row] + sh[k][1][row]) + (sh[k][2][row] + sh[k][3][row]);
         ypart[(size_t)tl.y * ld + i] = make_double2(s[0], s[1]);
         if (WithZ) zpart[(size_t)tl.y * ld + i] = make_double2(s[2], s[3]);
     }

@@ -81,6 +81,20 @@ __device__ __forceinline__ double wave_sum8_transposed_x(const double (&v)[8], i
     return z;
 }
 
+// Four per-lane values summed as wave_sum8_transposed_x sums values 0, 1, 4, 5
+// (the same pairings: xor 32, 16, 8, 4, 2, 1, so the same bits), for sweeps
+// without their second pair of sums. Lane l ends with value
+// ((l >> 5) & 1) * 2 + ((l >> 4) & 1); valid in lanes with (lane & 15) == 0.
+__device__ __forceinline__ double wave_sum4_transposed_x(const double (&v)[4]) {
+    const double w0 = swap32_sum(v[0], v[2]), w1 = swap32_sum(v[1], v[3]);
+    double z = swap16_sum(w0, w1);
+    z += dpp_f64<0x128>(z);  // row_ror:8 = xor 8
+    z += dpp_f64<0x104>(z);  // row_shl:4: lane l <- l + 4
+    z += dpp_f64<0x4E>(z);   // quad_perm [2,3,0,1]
+    z += dpp_f64<0xB1>(z);   // quad_perm [1,0,3,2]
+    return z;
+}
+
 // Streaming (non-temporal) loads of a stored operator: read once per sweep, it
 // should not displace the vectors and point data from the caches (the pair
 // sweeps ran 10 % faster with them on 64,676 points).
