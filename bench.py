@@ -41,12 +41,16 @@ import shapes  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_MATRIX_PEAK_TFLOPS = 157.3  # dense FP32 matrix cores
 C_EVAL = 26  # flops per entry evaluation (fixed convention)
 
 WORKLOADS = {
     # Torus (R=1, r=0.4) on 128^3 over [-2,2]^3, eps = tau = 2h, k = 10, interior
     # point source at (1, 0, 0.05): exterior Dirichlet problem, double layer.
-    "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05), tol=1e-8),
+    # mixed path: enlarged GMRES with the 16-column passes on the FP32 matrix cores
+    # (23 passes; GMRES column by column: 107 single-vector A p sweeps, 0.356 s against 0.253 s)
+    "torus128": dict(shape="torus", n=128, k=10.0, eps_h=2.0, tau_h=2.0, x0=(1.0, 0.0, 0.05), tol=1e-8,
+                     mixed_solver="enlarged32"),
     # Unit sphere on 64^3, eps = 3h, k = 2, source at (0.1, 0.2, 0.3), BiCG tol 1e-10;
     # 9 iterations: replacements every 1e-5 of residual drop (2, and the FP64 path's
     # 9 iterations: 8.9 ms per solve; at 1e-3 3 replacements and 11 iterations, 10.6 ms)
@@ -74,6 +78,7 @@ NRECV = 500
 STORE_MAX_BYTES = 200e9  # store an operator in HBM up to this size (288 GB per GPU)
 MATRIX_FREE = False  # --matrix-free: both timed paths recompute the operator per sweep
 REPLACE_OVERRIDE = None  # --replace: the mixed path's replacement drop for every single-vector workload
+SOLVER_OVERRIDE = None  # --mixed-solver: the mixed path's solver for every single-vector workload
 # the 256^3 star's FP64 path keeps no complex128 pairs by default: caching the
 # first row blocks' pairs (249 of 491 GB) made its pass slower, 492.6 against
 # 478.8 ms (reading complex128 pairs at both visits costs what computing them does)
@@ -134,6 +139,12 @@ def mixed_storage(count):
     return ibim.C64_PAIR if mixed_stores(count) else None
 
 
+def mixed_solver_of(w):
+    s = SOLVER_OVERRIDE or w.get("mixed_solver", "gmres")
+    # FP32 matrix-core products exist for stored complex64 pairs only
+    return "enlarged" if s == "enlarged32" and MATRIX_FREE else s
+
+
 def step(pb, w, recv, path):
     """One solve of a single-right-hand-side workload. path: "fp64" (the
     reference's BiCG on complex128 pairs), "fp64_gmres" (FP64 GMRES on the same
@@ -151,8 +162,11 @@ def step(pb, w, recv, path):
     pb.rhs(ibim.RHS_POINT_SOURCE, x0=w["x0"])
     if path == "fp64":
         info, _ = pb.solve(ibim.SOLVE_BICG, maxit=3000, tol=w["tol"], chunk=1)
-    elif path == "fp64_gmres" or (path == "mixed" and w.get("mixed_solver", "gmres") == "gmres"):
+    elif path in ("fp64_gmres", "mixed_gmres") or (path == "mixed" and mixed_solver_of(w) == "gmres"):
         info, _ = pb.solve(ibim.SOLVE_GMRES, maxit=3000, tol=w["tol"])
+    elif path == "mixed" and mixed_solver_of(w) in ("enlarged", "enlarged32"):
+        pb.set_fp32_products(mixed_solver_of(w) == "enlarged32")
+        info, _ = pb.solve(ibim.SOLVE_BLOCK_GMRES, maxit=3000, tol=w["tol"])
     else:
         info, _ = pb.solve(ibim.SOLVE_RELIABLE, maxit=3000, tol=w["tol"],
                            inner_tol=REPLACE_OVERRIDE or w.get("replace", REPLACE), chunk=1)
@@ -703,14 +717,17 @@ def main():
                          "with FP64 residual replacement")
     ap.add_argument("--replace", type=float, default=None,
                     help="single-vector workloads: residual drop between the mixed path's FP64 replacements")
+    ap.add_argument("--mixed-solver", choices=("enlarged", "enlarged32", "gmres", "reliable"), default=None,
+                    help="single-vector workloads: the mixed path's solver (default: the workload's own)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a 1 GiB device copy (reference for the HBM traffic counters)")
     ap.add_argument("--matrix-free", action="store_true",
                     help="store no operator: both paths recompute the radial factors per pair and sweep")
     args = ap.parse_args()
-    global MATRIX_FREE, REPLACE_OVERRIDE
+    global MATRIX_FREE, REPLACE_OVERRIDE, SOLVER_OVERRIDE
     MATRIX_FREE = args.matrix_free
     REPLACE_OVERRIDE = args.replace
+    SOLVER_OVERRIDE = args.mixed_solver
     if args.gpus != 1:
         print("bench.py measures one GPU: the library has no multi-GPU path (--gpus %d requested)" % args.gpus,
               file=sys.stderr)
@@ -764,10 +781,22 @@ def run_standard(args, workload):
     pm = make_problem(level, w)
     tm, infom, xm, um = timed(pm, w, recv, "mixed", args.steps, args.warmup)
     stored = mixed_stores(count)
-    ms_dom, n_dom = pm.kernel_time("matvec" if stored else "sweep_free32")
+    enlarged = mixed_solver_of(w) in ("enlarged", "enlarged32")
+    dom_timer = ({"enlarged32": "batch_pair32", "enlarged": "batch_pair" if stored else "batch_free32"}
+                 [mixed_solver_of(w)] if enlarged else "matvec" if stored else "sweep_free32")
+    ms_dom, n_dom = pm.kernel_time(dom_timer)
     ms_rep, n_rep = pm.kernel_time("matvec64" if mixed_storage(count) == ibim.MIXED_PAIR else "sweep_free64")
-    gmres = w.get("mixed_solver", "gmres") == "gmres"
-    solver = "FP64 refinement around GMRES" if gmres else "BiCG with FP64 residual replacement"
+    msolver = mixed_solver_of(w)
+    msolver_text = {
+        "enlarged32": "entries and products on the fp32 matrix cores (16-column passes, fp64 sums per 64 columns), "
+                      "enlarged GMRES, fp64 iterative refinement",
+        "enlarged": "fp64 products (16-column passes), enlarged GMRES, fp64 iterative refinement",
+        "gmres": "fp64 point factors, products and sums, GMRES, fp64 iterative refinement"}.get(
+        msolver, "fp64 point factors, products and sums, BiCG, fp64 residual replacement")
+    gmres = msolver in ("gmres", "enlarged", "enlarged32")
+    solver = {"gmres": "FP64 refinement around GMRES", "enlarged": "FP64 refinement around enlarged GMRES",
+              "enlarged32": "FP64 refinement around enlarged GMRES, FP32 matrix-core products"}.get(
+        msolver, "BiCG with FP64 residual replacement")
     print("[bench] mixed path (%s): %.3f s/solve, %d iterations, %d FP64 residuals" %
           (solver, tm, infom.iterations, infom.outer), file=sys.stderr, flush=True)
     # beside them, fewer steps: the former mixed solver (reliable BiCG on the same
@@ -776,8 +805,12 @@ def run_standard(args, workload):
     extra = {}
     if not args.no_extra:
         xs, ws = min(args.steps, 5), min(args.warmup, 1)
-        # FP64 GMRES needs a stored complex128 operator
-        for name in ("mixed_reliable", "fp64_gmres") if stored else ("mixed_reliable",):
+        # FP64 GMRES needs a stored complex128 operator; mixed_gmres: GMRES column by
+        # column on the mixed operator, when the mixed path is enlarged GMRES
+        names = ("mixed_reliable", "fp64_gmres") if stored else ("mixed_reliable",)
+        if enlarged:
+            names = ("mixed_gmres",) + names
+        for name in names:
             pe = make_problem(level, w)
             te, infoe, xe, _ = timed(pe, w, recv, name, xs, ws)
             extra[name] = (te, infoe, xe)
@@ -796,7 +829,19 @@ def run_standard(args, workload):
     npairs = float(count) * (count - 1) / 2
     vec_bytes = 4 * 16 * count  # p, p~ in; A p, A^H p~ out (complex128)
     avg_dom = ms_dom / max(n_dom, 1) * 1e-3
-    if stored and gmres:
+    if enlarged:
+        # dominant kernel: the 16-column pass Y = A P over the operator (every
+        # ordered entry times 16 columns, 8 flops per complex multiply-add) on the
+        # FP32 (stored complex64 pairs, FP32 products) or FP64 matrix cores
+        flops = float(count) * count * 16 * 8
+        peak = FP32_MATRIX_PEAK_TFLOPS if mixed_solver_of(w) == "enlarged32" else FP64_PEAK_TFLOPS
+        achieved = flops / avg_dom / 1e12 if n_dom else 0.0
+        kname = {"batch_pair32": "k_pair_batch32<1, A P>", "batch_pair": "k_pair_batch<float2, A P>",
+                 "batch_free32": "k_pair_batch<double2, fast factors, A P>"}[dom_timer]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
+                "traffic": measured_traffic(kname.replace(" ", ""), workload),
+                "kernel": "%s: 16 columns x %d^2 entries x 8 flops per launch (enlarged GMRES)" % (kname, count)}
+    elif stored and gmres:
         # dominant kernel: GMRES's pair sweep over the complex64 radial factors,
         # A p alone (p in, A p out: 32 B per point)
         dom_bytes = npairs * 8 + 32 * count
@@ -842,11 +887,9 @@ def run_standard(args, workload):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors, %s" % (
-                      "GMRES, fp64 iterative refinement" if gmres else "BiCG, fp64 residual replacement")
+        "dtype": ("fp64 with fp32 sine / cosine in the recomputed radial factors, %s" % msolver_text
                   if MATRIX_FREE else
-                  "complex64 radial factors per pair (fp64 point factors, products and sums), %s" % (
-                      "GMRES, fp64 iterative refinement" if gmres else "BiCG, fp64 residual replacement")),
+                  "complex64 radial factors per pair, %s" % msolver_text),
         "data": "synthetic: %s signed-distance level set, point-source Dirichlet data" % w["shape"],
         "config": {"workload": "%s: %s N=%d, k=%g, eps=%gh, tau=%gh, point source %s, %d receivers, tol %g%s" % (
             workload, w["shape"], w["n"], w["k"], w["eps_h"], w["tau_h"], list(w["x0"]), NRECV, w["tol"],
@@ -925,7 +968,7 @@ def compact(name, r):
                 "iterations": {"mixed": r["mixed"]["iterations"], "fp64": r["fp64"]["iterations"]},
                 "mixed_operator": r["config"]["mixed_operator"], "fp64_operator": r["config"]["fp64_operator"],
                 "roofline": r["roofline"]}
-        for k in ("mixed_reliable", "fp64_gmres"):
+        for k in ("mixed_gmres", "mixed_reliable", "fp64_gmres"):
             if k in r:
                 d[k + "_solves_per_s"] = r[k]["solves_per_s"]
         return d

@@ -175,34 +175,42 @@ __global__ void k_gm_unpack(const double2* __restrict__ X, int count, int c0, in
 // lane l loads element 64 s + l of both blocks -- A[a][k] = conj(V[4 s + k][a])
 // and B[k][b] = W[4 s + k][b] sit at the same index -- so the D tile holds
 // V^H W (Acc3: three real products per complex one).
-constexpr int kBlockChunks = 64;  // waves per basis block in the block products
+constexpr int kBlockChunks = 256;  // point chunks per basis block in the block products
 constexpr int kMaxBlocks = 64;    // blocks per block-GMRES cycle (then restart)
 
-// part[(i kBlockChunks + c) 256 + 16 a + b] = sum over chunk c of conj(V_i[j][a]) W[j][b]
+// part[(i kBlockChunks + c) 256 + 16 a + b] = sum over chunk c of conj(V_i[j][a]) W[j][b]:
+// a wave takes chunk c of four basis blocks, so each element of W is loaded
+// once for four products
+constexpr int kBlocksPerWave = 4;
 __global__ void __launch_bounds__(kGT) k_bg_dots(const double2* __restrict__ V, size_t vs, int count, int nv,
                                                  const double2* __restrict__ W, double2* __restrict__ part) {
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.y * (kGT / 64) + (threadIdx.x >> 6);
-    if (i >= nv) return;  // a whole wave; no barrier below
+    const int i0 = (blockIdx.y * (kGT / 64) + (threadIdx.x >> 6)) * kBlocksPerWave;
+    if (i0 >= nv) return;  // a whole wave; no barrier below
     const int c = blockIdx.x;
     const int steps = (count + 3) / 4;
     const int per = (steps + kBlockChunks - 1) / kBlockChunks;
     const int s1 = min(steps, (c + 1) * per);
-    const double2* v = V + (size_t)i * vs;
     const size_t n = (size_t)count * kBatchRHS;
-    Acc3 acc;
+    Acc3 acc[kBlocksPerWave];
     for (int s = c * per; s < s1; ++s) {
         const size_t e = (size_t)s * 64 + lane;
-        double2 a = make_double2(0.0, 0.0), b = a;
-        if (e < n) {
-            a = v[e];
-            b = W[e];
-        }
-        acc.add(a.x, -a.y, b.x, b.y);
-    }
-    double2* out = part + ((size_t)i * kBlockChunks + c) * 256;
+        const double2 b = e < n ? W[e] : make_double2(0.0, 0.0);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) out[(4 * q + lane / 16) * 16 + lane % 16] = make_double2(acc.re(q), acc.im(q));
+        for (int q = 0; q < kBlocksPerWave; ++q) {
+            if (i0 + q >= nv) break;  // wave-uniform
+            const double2 a = e < n ? V[(size_t)(i0 + q) * vs + e] : make_double2(0.0, 0.0);
+            acc[q].add(a.x, -a.y, b.x, b.y);
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < kBlocksPerWave; ++q) {
+        if (i0 + q >= nv) break;
+        double2* out = part + ((size_t)(i0 + q) * kBlockChunks + c) * 256;
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            out[(4 * v + lane / 16) * 16 + lane % 16] = make_double2(acc[q].re(v), acc[q].im(v));
+    }
 }
 
 // out[i 256 + t] = sum_c part[(i kBlockChunks + c) 256 + t] in order
@@ -219,27 +227,53 @@ __global__ void k_bg_dots_sum(const double2* __restrict__ part, int nv, double2*
 }
 
 // W[j][b] = (accumulate ? W[j][b] : 0) + sign sum_{i < nv} sum_a V_i[j][a] M_i[a][b]
-// (M_i: 16 x 16 blocks, row-major, M + 256 i)
+// (M_i: 16 x 16 blocks, row-major, M + 256 i) on the FP64 matrix cores: a wave
+// owns 16 points, A[p][k] = V_i[p][a0 + k], B[k][b] = M_i[a0 + k][b], four
+// k-steps per basis block
 __global__ void __launch_bounds__(kGT) k_bg_gemm(const double2* __restrict__ V, size_t vs, int count,
                                                  const double2* __restrict__ M, int nv, double sign, bool accumulate,
                                                  double2* __restrict__ W) {
-    const size_t e = (size_t)blockIdx.x * kGT + threadIdx.x;
-    if (e >= (size_t)count * kBatchRHS) return;
-    const size_t row = e - e % kBatchRHS;
-    const int b = (int)(e % kBatchRHS);
-    double cr = 0.0, ci = 0.0;
+    const int lane = threadIdx.x & 63;
+    const int e0 = (blockIdx.x * (kGT / 64) + (threadIdx.x >> 6)) * 16;
+    if (e0 >= count) return;  // a whole wave; no barrier below
+    const int pt = e0 + lane % 16;
+    Acc3 acc;
     for (int i = 0; i < nv; ++i) {
-        const double2* v = V + (size_t)i * vs + row;
-        const double2* m = M + (size_t)i * 256 + b;
+        const double2* v = V + (size_t)i * vs;
+        const double2* m = M + (size_t)i * 256;
 #pragma unroll
-        for (int a = 0; a < kBatchRHS; ++a) {
-            const double2 x = v[a], y = m[16 * a];
-            cr += x.x * y.x - x.y * y.y;
-            ci += x.x * y.y + x.y * y.x;
+        for (int a0 = 0; a0 < kBatchRHS; a0 += 4) {
+            const double2 x = pt < count ? v[(size_t)pt * kBatchRHS + a0 + lane / 16] : make_double2(0.0, 0.0);
+            const double2 y = m[(a0 + lane / 16) * 16 + lane % 16];
+            acc.add(x.x, x.y, y.x, y.y);
         }
     }
-    const double2 o = accumulate ? W[e] : make_double2(0.0, 0.0);
-    W[e] = make_double2(o.x + sign * cr, o.y + sign * ci);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int r = e0 + 4 * q + lane / 16;
+        if (r < count) {
+            const size_t e = (size_t)r * kBatchRHS + lane % 16;
+            const double2 o = accumulate ? W[e] : make_double2(0.0, 0.0);
+            W[e] = make_double2(o.x + sign * acc.re(q), o.y + sign * acc.im(q));
+        }
+    }
+}
+
+// B[j][c] = v[j] for the points j of piece c -- [c count / 16, (c + 1) count / 16):
+// tube points come in grid order, so a piece is one part of the surface --
+// and 0 elsewhere: the columns sum to v.
+__global__ void k_gm_split(const double2* __restrict__ v, int count, double2* __restrict__ B) {
+    const size_t e = (size_t)blockIdx.x * kGT + threadIdx.x;
+    if (e >= (size_t)count * kBatchRHS) return;
+    const int j = (int)(e / kBatchRHS), c = (int)(e % kBatchRHS);
+    const long long lo = (long long)c * count / kBatchRHS, hi = (long long)(c + 1) * count / kBatchRHS;
+    B[e] = j >= lo && j < hi ? v[j] : make_double2(0.0, 0.0);
+}
+
+// x[j] += D[j][0]
+__global__ void k_gm_add_col0(double2* __restrict__ x, const double2* __restrict__ D, int count) {
+    const int j = blockIdx.x * kGT + threadIdx.x;
+    if (j < count) x[j] = make_double2(x[j].x + D[(size_t)j * kBatchRHS].x, x[j].y + D[(size_t)j * kBatchRHS].y);
 }
 
 typedef std::complex<double> cplx;
@@ -545,6 +579,11 @@ struct BlockRun {
     int* col_products = nullptr;
     double estimate[P] = {};
     Phase phase = kDone;
+    // enlarged mode (one right-hand side split over the columns): the block
+    // space is built as above, but ONE residual -- the sum of the columns --
+    // is minimized over all of it; column 0 carries its estimate, target and
+    // correction, and a cycle ends the run (the refinement restarts it)
+    bool single = false;
     int j = 0;            // basis blocks of the cycle with their product applied
     int p = 0;            // columns in the cycle
     int idx[P] = {};      // their slots
@@ -596,14 +635,16 @@ struct BlockRun {
 
     // out[i] = V_i^H W for i < nv (16 x 16 blocks, device)
     void block_dots(const double2* V, int nv, const double2* W, double2* outp) {
-        hipLaunchKernelGGL(k_bg_dots, dim3(kBlockChunks, (unsigned)((nv + 3) / 4)), dim3(kGT), 0, ctx->stream, V,
-                           g.vs, ctx->count, nv, W, g.part);
+        const int per_block = (kGT / 64) * kBlocksPerWave;
+        hipLaunchKernelGGL(k_bg_dots, dim3(kBlockChunks, (unsigned)((nv + per_block - 1) / per_block)), dim3(kGT), 0,
+                           ctx->stream, V, g.vs, ctx->count, nv, W, g.part);
         hipLaunchKernelGGL(k_bg_dots_sum, dim3((unsigned)((nv * 256 + 255) / 256)), dim3(256), 0, ctx->stream, g.part,
                            nv, outp);
     }
     // W = (accumulate ? W : 0) + sign sum_i V_i M_i
     void block_gemm(const double2* V, const double2* M, int nv, double sign, bool accumulate, double2* W) {
-        hipLaunchKernelGGL(k_bg_gemm, dim3(blocks()), dim3(kGT), 0, ctx->stream, V, g.vs, ctx->count, M, nv, sign,
+        const unsigned tiles = (unsigned)((ctx->count + 16 * (kGT / 64) - 1) / (16 * (kGT / 64)));
+        hipLaunchKernelGGL(k_bg_gemm, dim3(tiles), dim3(kGT), 0, ctx->stream, V, g.vs, ctx->count, M, nv, sign,
                            accumulate, W);
     }
     // the Gram block of W (device) -> host (synchronizes)
@@ -693,13 +734,26 @@ struct BlockRun {
         std::vector<cplx> G(256);
         if (int rc = gram(g.r, G.data())) return rc;
         p = 0;
+        double total = 0.0;
         for (int c = 0; c < P; ++c) {
             estimate[c] = sqrt(std::max(G[c * 16 + c].real(), 0.0));
-            active[c] = estimate[c] > target[c];
+            // enlarged mode: every piece of the one residual above 1e-12 of the largest
+            active[c] = single ? estimate[c] > 0.0 : estimate[c] > target[c];
+        }
+        if (single) {
+            double big = 0.0;
+            for (int c = 0; c < P; ++c) big = std::max(big, estimate[c]);
+            for (int c = 0; c < P; ++c) active[c] = estimate[c] > 1e-12 * big;
+        }
+        for (int c = 0; c < P; ++c)
             if (active[c]) idx[p++] = c;
+        if (single) {  // the pieces have disjoint supports: their norms add
+            for (int c = 0; c < P; ++c) total += estimate[c] * estimate[c];
+            for (int c = 1; c < P; ++c) estimate[c] = 0.0;
+            estimate[0] = sqrt(total);
         }
         j = 0;
-        if (p == 0) {
+        if (p == 0 || (single && !(estimate[0] > target[0]))) {
             phase = kDone;
             return IBIM_OK;
         }
@@ -707,6 +761,10 @@ struct BlockRun {
         bool ok = false;
         // V_0 in the slot of basis block 0; g.w is free until the first product
         if (int rc = orthonormalize(g.r, g.V, g.w, R0.data(), &ok)) return rc;
+        if (!ok && single) {  // disjoint pieces: only a piece below 1e-12 of the largest can fail
+            phase = kDone;
+            return IBIM_OK;
+        }
         if (!ok) {
             // dependent residual columns: the cycle takes the largest one alone
             int best = idx[0];
@@ -723,17 +781,25 @@ struct BlockRun {
         }
         hrows = (MB + 1) * p;
         H.assign((size_t)hrows * MB * p, cplx(0.0));
-        E.assign((size_t)hrows * p, cplx(0.0));
+        E.assign((size_t)hrows * ecols(), cplx(0.0));
         rots.clear();
         for (int a = 0; a < p; ++a)
-            for (int b = a; b < p; ++b) Ec(a, b) = R0[a * p + b];
+            for (int b = a; b < p; ++b) {
+                if (single)
+                    Ec(a, 0) += R0[a * p + b];  // r = V_0 R0 (1, ..., 1)^T
+                else
+                    Ec(a, b) = R0[a * p + b];
+            }
         phase = kIterate;
         return IBIM_OK;
     }
 
     int launch_post() {
         ++passes;
-        for (int c = 0; c < P; ++c) col_products[c] += active[c];
+        if (single)
+            ++col_products[0];
+        else
+            for (int c = 0; c < P; ++c) col_products[c] += active[c];
         if (phase == kRestart) {
             hipLaunchKernelGGL(k_gm_sub, dim3(blocks()), dim3(kGT), 0, ctx->stream, rhs, g.w, g.n, g.r);
             IBIM_HIP(hipGetLastError());
@@ -764,12 +830,13 @@ struct BlockRun {
             Hc(r, q) = c * x + sn * y;
             Hc(r + 1, q) = -std::conj(sn) * x + std::conj(c) * y;
         }
-        for (int q = 0; q < p; ++q) {
+        for (int q = 0; q < ecols(); ++q) {
             const cplx x = Ec(r, q), y = Ec(r + 1, q);
             Ec(r, q) = c * x + sn * y;
             Ec(r + 1, q) = -std::conj(sn) * x + std::conj(c) * y;
         }
     }
+    int ecols() const { return single ? 1 : p; }
 
     int host_post() {
         if (phase == kRestart) return cycle_begin();
@@ -802,13 +869,14 @@ struct BlockRun {
         }
         ++j;
         bool any = false;
-        for (int a = 0; a < p; ++a) {
+        for (int q = 0; q < ecols(); ++q) {
+            const int c = single ? 0 : idx[q];
             double s2 = 0.0;
-            for (int r = j * p; r < (j + 1) * p; ++r) s2 += std::norm(Ec(r, a));
-            estimate[idx[a]] = sqrt(s2);
-            any = any || estimate[idx[a]] > target[idx[a]] || estimate[idx[a]] != estimate[idx[a]];
+            for (int r = j * p; r < (j + 1) * p; ++r) s2 += std::norm(Ec(r, q));
+            estimate[c] = sqrt(s2);
+            any = any || estimate[c] > target[c] || estimate[c] != estimate[c];
         }
-        if (!ok || !any || j >= m || passes >= budget) return cycle_end(any);
+        if (!ok || !any || j >= m || passes >= budget) return cycle_end(any && !single);
         return IBIM_OK;
     }
 
@@ -817,7 +885,7 @@ struct BlockRun {
     int cycle_end(bool again) {
         const int nc = j * p;
         std::vector<cplx> Y((size_t)nc * p, cplx(0.0));
-        for (int q = 0; q < p; ++q)
+        for (int q = 0; q < ecols(); ++q)
             for (int r = nc - 1; r >= 0; --r) {
                 cplx v = Ec(r, q);
                 for (int t = r + 1; t < nc; ++t) v -= Hc(r, t) * Y[(size_t)q * nc + t];
@@ -826,9 +894,10 @@ struct BlockRun {
         upy.assign((size_t)j * 256, make_double2(0.0, 0.0));
         for (int i = 0; i < j; ++i)
             for (int a = 0; a < p; ++a)
-                for (int q = 0; q < p; ++q) {
+                for (int q = 0; q < ecols(); ++q) {
                     const cplx v = Y[(size_t)q * nc + i * p + a];
-                    upy[(size_t)i * 256 + idx[a] * 16 + idx[q]] = make_double2(v.real(), v.imag());
+                    // enlarged mode: the correction is column 0 of d
+                    upy[(size_t)i * 256 + idx[a] * 16 + (single ? 0 : idx[q])] = make_double2(v.real(), v.imag());
                 }
         IBIM_HIP(hipMemcpyAsync(g.y, upy.data(), sizeof(double2) * upy.size(), hipMemcpyHostToDevice, ctx->stream));
         block_gemm(g.V, g.y, j, 1.0, true, g.d);
@@ -1087,6 +1156,97 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
     return IBIM_OK;
 }
 
+// Enlarged GMRES (Grigori, Moufawad and Nataf, SIAM J. Sci. Comput. 38, 2016)
+// for one right-hand side at a time, inside the same FP64 refinement: each
+// refinement step splits its residual over 16 pieces of the surface
+// (k_gm_split), builds one block Krylov space from them with the 16-column
+// passes (BlockRun, enlarged mode) and minimizes the one residual over all of
+// it. That space contains GMRES's own, so a pass never does worse than a GMRES
+// product, and it holds 16 directions per pass: on the double layer, whose
+// slow modes GMRES finds one product at a time, a fraction of the passes
+// (numpy model on the oracle's torus operator, 32^3 and 48^3, k = 10: 136
+// GMRES products, 18 passes).
+int gmres_enlarged(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info) {
+    int inner, outer_op;
+    refinement_ops(ctx, &inner, &outer_op);
+    const int count = ctx->count;
+    const size_t ld = ctx->ld;
+    const size_t bbytes = (BlockRun::space_bytes(ctx) + 255) & ~(size_t)255;
+    if (ctx->refine.ensure(bbytes + sizeof(double2) * ld * 2)) return IBIM_ERR_NOMEM;
+    GmresSpace g;
+    if (int rc = BlockRun::space_at(ctx, ctx->refine.ptr, &g)) return rc;
+    double2* res = reinterpret_cast<double2*>(ctx->refine.as<char>() + bbytes);  // b - A64 x
+    double2* ax = res + ld;
+    GmresSpace g1 = g;  // the single-vector norms: the one-column dot kernels
+    g1.vs = ld;
+    g1.n = count;
+    const unsigned nb1 = (unsigned)((count + kGT - 1) / kGT), nbb = (unsigned)((g.n + kGT - 1) / kGT);
+    const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
+    const int m = basis_of<BlockRun>(opts);
+    int total = 0, outer_max = 0, status = 4;
+    double worst = 0.0;
+    for (int c = 0; c < ctx->nrhs; ++c) {
+        const double2* b = ctx->rhs.as<double2>() + (size_t)c * count;
+        double2* x = ctx->density.as<double2>() + (size_t)c * count;
+        IBIM_HIP(hipMemsetAsync(x, 0, sizeof(double2) * count, ctx->stream));
+        IBIM_HIP(hipMemcpyAsync(res, b, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
+        double normb;
+        if (int rc = norms_of<1>(ctx, g1, res, &normb)) return rc;
+        if (normb == 0.0) continue;  // zero data: zero density, no iterations
+        int col_status = 1, used = 0, steps = 0;
+        double relres = 1.0;
+        while (steps < max_outer && used < opts->maxit) {
+            ++steps;
+            hipLaunchKernelGGL(k_gm_split, dim3(nbb), dim3(kGT), 0, ctx->stream, res, count, g.b);
+            IBIM_HIP(hipGetLastError());
+            BlockRun run;
+            run.single = true;
+            run.ctx = ctx;
+            run.g = g;
+            run.rhs = g.b;
+            for (int q = 0; q < kBatchRHS; ++q) run.target[q] = INFINITY;
+            run.target[0] = inner_margin(inner, outer_op) * opts->tol * normb;
+            run.budget = opts->maxit - used;
+            run.m = m;
+            int prods[kBatchRHS] = {};
+            run.col_products = prods;
+            // a few tens of passes per solve: the 64-row shape, untimed
+            const bool untimed = ctx->batch_untimed;
+            ctx->batch_untimed = true;
+            const int rc = solve_run(run, inner);
+            ctx->batch_untimed = untimed;
+            if (rc) return rc;
+            used += run.passes;
+            hipLaunchKernelGGL(k_gm_add_col0, dim3(nb1), dim3(kGT), 0, ctx->stream, x, g.d, count);
+            if (int rc = matvec_op_y(ctx, outer_op, x, ax)) return rc;
+            hipLaunchKernelGGL(k_gm_sub, dim3(nb1), dim3(kGT), 0, ctx->stream, b, ax, count, res);
+            IBIM_HIP(hipGetLastError());
+            double rn;
+            if (int rc = norms_of<1>(ctx, g1, res, &rn)) return rc;
+            relres = rn / normb;
+            if (relres != relres) {
+                col_status = 2;
+                break;
+            }
+            if (relres < opts->tol) {
+                col_status = 0;
+                break;
+            }
+        }
+        total += used;
+        outer_max = std::max(outer_max, steps);
+        worst = std::max(worst, relres);
+        if (status == 4 || col_status != 0) status = col_status;
+        report_progress(ctx, "enlarged GMRES", used, relres);
+    }
+    info->iterations = total;
+    info->outer = outer_max;
+    info->rel_residual = worst;
+    info->status = status;
+    ctx->last_iters = total;
+    return IBIM_OK;
+}
+
 // One batch of up to 16 columns of one problem in a grouped solve.
 template <class Run>
 struct GroupBatch {
@@ -1285,7 +1445,12 @@ int run_gmres(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info, 
     // the traditional tube, or dense complex128 storage refined against itself
     const bool batched = ctx->nrhs > 1 && opts->batch == 0 &&
                          (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C128);
-    if (!batched) return gmres_columns(ctx, opts, info, hist);  // block GMRES of one column is GMRES
+    if (!batched) {
+        // one column at a time: enlarged GMRES where the 16-column passes exist
+        const bool passes = ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C128;
+        return opts->method == IBIM_SOLVE_BLOCK_GMRES && passes ? gmres_enlarged(ctx, opts, info)
+                                                                : gmres_columns(ctx, opts, info, hist);
+    }
     return opts->method == IBIM_SOLVE_BLOCK_GMRES ? gmres_batched<BlockRun>(ctx, opts, info, hist)
                                                   : gmres_batched<GmresRun<kBatchRHS>>(ctx, opts, info, hist);
 }
