@@ -124,8 +124,10 @@ typedef struct ibim_params {
                               /* sides: 128). No counterpart in the reference (BiCG only) */
 #define IBIM_SOLVE_BLOCK_GMRES 4 /* the same refinement around block GMRES: one Krylov     */
                                  /* space per batch of 16 right-hand sides (batch = 0),   */
-                                 /* a restart every `chunk` passes (0: 64); one column:   */
-                                 /* GMRES. No counterpart in the reference                */
+                                 /* a restart every `chunk` passes (0: 64); one column at */
+                                 /* a time: enlarged GMRES (the residual split over 16    */
+                                 /* pieces of the surface, one block space). No           */
+                                 /* counterpart in the reference                          */
 
 typedef struct ibim_solve_opts {
     int method;        /* IBIM_SOLVE_*                                        */
