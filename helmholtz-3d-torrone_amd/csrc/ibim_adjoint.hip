@@ -505,7 +505,12 @@ int run_gradient(Context* ctx, int col, int mode, const ibim_solve_opts* opts, d
     // adjoint solve A^H lambda = E^H e
     const int op = fp64_operator(ctx);
     ibim_solve_info info{};
-    rc = bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true, false);
+    // IBIM_SOLVE_BLOCK_GMRES: enlarged GMRES on the 16-column A^H passes where
+    // they exist (pair storage or dense, traditional tube), else the reference's BiCG
+    const bool enlarged = opts->method == IBIM_SOLVE_BLOCK_GMRES &&
+                          (ctx->kind == IBIM_TUBE_TRADITIONAL || ctx->mat_precision == IBIM_C128);
+    rc = enlarged ? enlarged_solve_op(ctx, op, true, g, lam, opts->maxit, opts->tol, &info)
+                  : bicg_solve_op(ctx, op, g, lam, opts->maxit, opts->tol, opts->chunk, &info, nullptr, true, false);
     if (rc) return rc;
     // an unconverged lambda gives a wrong gradient: fail rather than return it
     // (status 4, a zero E^H e, has the exact zero lambda)

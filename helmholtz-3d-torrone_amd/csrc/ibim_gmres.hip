@@ -584,6 +584,7 @@ struct BlockRun {
     // is minimized over all of it; column 0 carries its estimate, target and
     // correction, and a cycle ends the run (the refinement restarts it)
     bool single = false;
+    bool adjoint = false;  // the passes apply A^H (the adjoint solve of a gradient)
     int j = 0;            // basis blocks of the cycle with their product applied
     int p = 0;            // columns in the cycle
     int idx[P] = {};      // their slots
@@ -952,7 +953,10 @@ template <class Run>
 int solve_run(Run& run, int op) {
     int rc = run.start();
     while (!rc && !run.done()) {
-        rc = apply_op<kBatchRHS>(run.ctx, op, run.in(), run.out());
+        if constexpr (std::is_same<Run, BlockRun>::value)
+            rc = batch_apply(run.ctx, op, run.adjoint, run.in(), run.out(), nullptr);
+        else
+            rc = apply_op<kBatchRHS>(run.ctx, op, run.in(), run.out());
         if (!rc) rc = run.launch_post();
         if (!rc) IBIM_HIP(hipStreamSynchronize(run.ctx->stream));
         if (!rc) rc = run.host_post();
@@ -1166,21 +1170,100 @@ int gmres_batched(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* in
 // slow modes GMRES finds one product at a time, a fraction of the passes
 // (numpy model on the oracle's torus operator, 32^3 and 48^3, k = 10: 136
 // GMRES products, 18 passes).
+struct EnlargedSpace {
+    GmresSpace g, g1;  // the block space; its single-vector view for the norms
+    double2 *res, *ax, *ay;
+};
+
+int enlarged_space(Context* ctx, EnlargedSpace* e) {
+    const size_t ld = ctx->ld;
+    const size_t bbytes = (BlockRun::space_bytes(ctx) + 255) & ~(size_t)255;
+    if (ctx->refine.ensure(bbytes + sizeof(double2) * ld * 3)) return IBIM_ERR_NOMEM;
+    if (int rc = BlockRun::space_at(ctx, ctx->refine.ptr, &e->g)) return rc;
+    e->res = reinterpret_cast<double2*>(ctx->refine.as<char>() + bbytes);  // b - op(A64) x
+    e->ax = e->res + ld;
+    e->ay = e->ax + ld;
+    e->g1 = e->g;
+    e->g1.vs = ld;
+    e->g1.n = ctx->count;
+    return IBIM_OK;
+}
+
+// One right-hand side b: x = 0, then refinement steps of enlarged GMRES with
+// `inner` (A^H when adjoint) and FP64 residuals with `outer`, until
+// ||b - op(A) x|| < tol ||b||. Reports the passes, steps, residual and status
+// (0 converged, 1 budget, 2 NaN, 4 zero data).
+int enlarged_core(Context* ctx, const EnlargedSpace& e, int inner, int outer_op, bool adjoint, const double2* b,
+                  double2* x, double tol, int maxit, int max_outer, int m, int* used_out, int* steps_out,
+                  double* relres_out, int* status_out) {
+    const int count = ctx->count;
+    const unsigned nb1 = (unsigned)((count + kGT - 1) / kGT), nbb = (unsigned)((e.g.n + kGT - 1) / kGT);
+    IBIM_HIP(hipMemsetAsync(x, 0, sizeof(double2) * count, ctx->stream));
+    IBIM_HIP(hipMemcpyAsync(e.res, b, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
+    double normb;
+    if (int rc = norms_of<1>(ctx, e.g1, e.res, &normb)) return rc;
+    *used_out = *steps_out = 0;
+    *relres_out = 0.0;
+    *status_out = 4;
+    if (normb == 0.0) return IBIM_OK;  // zero data: zero solution, no iterations
+    int col_status = 1, used = 0, steps = 0;
+    double relres = 1.0;
+    while (steps < max_outer && used < maxit) {
+        ++steps;
+        hipLaunchKernelGGL(k_gm_split, dim3(nbb), dim3(kGT), 0, ctx->stream, e.res, count, e.g.b);
+        IBIM_HIP(hipGetLastError());
+        BlockRun run;
+        run.single = true;
+        run.adjoint = adjoint;
+        run.ctx = ctx;
+        run.g = e.g;
+        run.rhs = e.g.b;
+        for (int q = 0; q < kBatchRHS; ++q) run.target[q] = INFINITY;
+        run.target[0] = inner_margin(inner, outer_op) * tol * normb;
+        run.budget = maxit - used;
+        run.m = m;
+        int prods[kBatchRHS] = {};
+        run.col_products = prods;
+        // a few tens of passes per solve: the 64-row shape, untimed
+        const bool untimed = ctx->batch_untimed;
+        ctx->batch_untimed = true;
+        const int rc = solve_run(run, inner);
+        ctx->batch_untimed = untimed;
+        if (rc) return rc;
+        used += run.passes;
+        hipLaunchKernelGGL(k_gm_add_col0, dim3(nb1), dim3(kGT), 0, ctx->stream, x, e.g.d, count);
+        if (adjoint) {  // A^H x: the fused sweep's second product
+            if (int rc2 = matvec_op(ctx, outer_op, x, x, e.ay, e.ax)) return rc2;
+        } else if (int rc2 = matvec_op_y(ctx, outer_op, x, e.ax)) {
+            return rc2;
+        }
+        hipLaunchKernelGGL(k_gm_sub, dim3(nb1), dim3(kGT), 0, ctx->stream, b, e.ax, count, e.res);
+        IBIM_HIP(hipGetLastError());
+        double rn;
+        if (int rc2 = norms_of<1>(ctx, e.g1, e.res, &rn)) return rc2;
+        relres = rn / normb;
+        if (relres != relres) {
+            col_status = 2;
+            break;
+        }
+        if (relres < tol) {
+            col_status = 0;
+            break;
+        }
+    }
+    *used_out = used;
+    *steps_out = steps;
+    *relres_out = relres;
+    *status_out = col_status;
+    return IBIM_OK;
+}
+
 int gmres_enlarged(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* info) {
     int inner, outer_op;
     refinement_ops(ctx, &inner, &outer_op);
     const int count = ctx->count;
-    const size_t ld = ctx->ld;
-    const size_t bbytes = (BlockRun::space_bytes(ctx) + 255) & ~(size_t)255;
-    if (ctx->refine.ensure(bbytes + sizeof(double2) * ld * 2)) return IBIM_ERR_NOMEM;
-    GmresSpace g;
-    if (int rc = BlockRun::space_at(ctx, ctx->refine.ptr, &g)) return rc;
-    double2* res = reinterpret_cast<double2*>(ctx->refine.as<char>() + bbytes);  // b - A64 x
-    double2* ax = res + ld;
-    GmresSpace g1 = g;  // the single-vector norms: the one-column dot kernels
-    g1.vs = ld;
-    g1.n = count;
-    const unsigned nb1 = (unsigned)((count + kGT - 1) / kGT), nbb = (unsigned)((g.n + kGT - 1) / kGT);
+    EnlargedSpace e;
+    if (int rc = enlarged_space(ctx, &e)) return rc;
     const int max_outer = opts->max_outer > 0 ? opts->max_outer : 20;
     const int m = basis_of<BlockRun>(opts);
     int total = 0, outer_max = 0, status = 4;
@@ -1188,51 +1271,12 @@ int gmres_enlarged(Context* ctx, const ibim_solve_opts* opts, ibim_solve_info* i
     for (int c = 0; c < ctx->nrhs; ++c) {
         const double2* b = ctx->rhs.as<double2>() + (size_t)c * count;
         double2* x = ctx->density.as<double2>() + (size_t)c * count;
-        IBIM_HIP(hipMemsetAsync(x, 0, sizeof(double2) * count, ctx->stream));
-        IBIM_HIP(hipMemcpyAsync(res, b, sizeof(double2) * count, hipMemcpyDeviceToDevice, ctx->stream));
-        double normb;
-        if (int rc = norms_of<1>(ctx, g1, res, &normb)) return rc;
-        if (normb == 0.0) continue;  // zero data: zero density, no iterations
-        int col_status = 1, used = 0, steps = 0;
-        double relres = 1.0;
-        while (steps < max_outer && used < opts->maxit) {
-            ++steps;
-            hipLaunchKernelGGL(k_gm_split, dim3(nbb), dim3(kGT), 0, ctx->stream, res, count, g.b);
-            IBIM_HIP(hipGetLastError());
-            BlockRun run;
-            run.single = true;
-            run.ctx = ctx;
-            run.g = g;
-            run.rhs = g.b;
-            for (int q = 0; q < kBatchRHS; ++q) run.target[q] = INFINITY;
-            run.target[0] = inner_margin(inner, outer_op) * opts->tol * normb;
-            run.budget = opts->maxit - used;
-            run.m = m;
-            int prods[kBatchRHS] = {};
-            run.col_products = prods;
-            // a few tens of passes per solve: the 64-row shape, untimed
-            const bool untimed = ctx->batch_untimed;
-            ctx->batch_untimed = true;
-            const int rc = solve_run(run, inner);
-            ctx->batch_untimed = untimed;
-            if (rc) return rc;
-            used += run.passes;
-            hipLaunchKernelGGL(k_gm_add_col0, dim3(nb1), dim3(kGT), 0, ctx->stream, x, g.d, count);
-            if (int rc = matvec_op_y(ctx, outer_op, x, ax)) return rc;
-            hipLaunchKernelGGL(k_gm_sub, dim3(nb1), dim3(kGT), 0, ctx->stream, b, ax, count, res);
-            IBIM_HIP(hipGetLastError());
-            double rn;
-            if (int rc = norms_of<1>(ctx, g1, res, &rn)) return rc;
-            relres = rn / normb;
-            if (relres != relres) {
-                col_status = 2;
-                break;
-            }
-            if (relres < opts->tol) {
-                col_status = 0;
-                break;
-            }
-        }
+        int used, steps, col_status;
+        double relres;
+        if (int rc = enlarged_core(ctx, e, inner, outer_op, false, b, x, opts->tol, opts->maxit, max_outer, m, &used,
+                                   &steps, &relres, &col_status))
+            return rc;
+        if (col_status == 4) continue;
         total += used;
         outer_max = std::max(outer_max, steps);
         worst = std::max(worst, relres);
@@ -1434,6 +1478,22 @@ int group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_s
 }
 
 }  // namespace
+
+int enlarged_solve_op(Context* ctx, int op, bool adjoint, const void* b_dev, void* x_dev, int maxit, double tol,
+                      ibim_solve_info* info) {
+    EnlargedSpace e;
+    if (int rc = enlarged_space(ctx, &e)) return rc;
+    int used, steps, status;
+    double relres;
+    if (int rc = enlarged_core(ctx, e, op, op, adjoint, static_cast<const double2*>(b_dev), static_cast<double2*>(x_dev),
+                               tol, maxit, 20, kMaxBlocks, &used, &steps, &relres, &status))
+        return rc;
+    info->iterations = used;
+    info->outer = steps;
+    info->rel_residual = relres;
+    info->status = status;
+    return IBIM_OK;
+}
 
 int run_group_gmres(Context* const* ctxs, int n, const ibim_solve_opts* opts, ibim_solve_info* infos) {
     return opts->method == IBIM_SOLVE_BLOCK_GMRES ? group_gmres<BlockRun>(ctxs, n, opts, infos)

@@ -348,6 +348,12 @@ int noalloc_scratch(int device, void** ptr);
 // ref_guards = false for correction / adjoint solves (only exact zeros are zero data).
 int bicg_solve_op(Context* ctx, int op, const void* b_dev, void* x_dev, int maxit, double tol, int chunk,
                   ibim_solve_info* info, double* hist, bool adjoint = false, bool ref_guards = true);
+// The same system, op(A) x = b (op(A) = A^H when adjoint), by enlarged GMRES on
+// the 16-column passes of operator `op` (ibim_gmres.hip): b split over 16
+// pieces of the surface, one block Krylov space, restarted with the true
+// residual until it is below tol ||b||.
+int enlarged_solve_op(Context* ctx, int op, bool adjoint, const void* b_dev, void* x_dev, int maxit, double tol,
+                      ibim_solve_info* info);
 
 }  // namespace ibim
 

@@ -356,10 +356,11 @@ class Problem:
         _check(lib().ibim_set_receivers(self.h, m, _p(xyz), _p(tre), _p(tim)), "ibim_set_receivers")
         self.nrecv = m
 
-    def gradient(self, mode=EVAL_DOUBLE, col=0, maxit=1000, tol=1e-12, chunk=0):
+    def gradient(self, mode=EVAL_DOUBLE, col=0, maxit=1000, tol=1e-12, chunk=0, method=SOLVE_BICG):
         """J and dJ/d(level-set values) (shape (n+1)^3, [y][x][z]) and dJ/dk for
-        the solved column `col` (adjoint BiCG to `tol`)."""
-        o = SolveOpts(SOLVE_BICG, maxit, tol, 1e-4, 20, chunk, 1)
+        the solved column `col` (adjoint solve to `tol`: BiCG, or with
+        SOLVE_BLOCK_GMRES enlarged GMRES on the 16-column A^H passes)."""
+        o = SolveOpts(method, maxit, tol, 1e-4, 20, chunk, 1)
         J, dk = np.zeros(1), np.zeros(1)
         S = self.n + 1
         g = np.zeros((S, S, S))

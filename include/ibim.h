@@ -296,8 +296,9 @@ int ibim_evaluate(ibim_ctx *ctx, int mode, int col, int nrecv, const double *xyz
 /* Misfit objective J = sum_r |u(x_r) - t_r|^2 of the field of density column col
  * (mode IBIM_EVAL_SINGLE or IBIM_EVAL_DOUBLE) at nrecv receivers with targets t,
  * and its gradient with respect to every level-set grid value and the
- * wavenumber k. The adjoint solves A^H lambda = E^H (u - t) with the same BiCG,
- * differentiates the operator, evaluator and right-hand side entries exactly on
+ * wavenumber k. The adjoint solves A^H lambda = E^H (u - t) with the same BiCG
+ * (opts->method IBIM_SOLVE_BLOCK_GMRES: enlarged GMRES on the 16-column A^H
+ * passes, traditional tube or dense complex128), differentiates the operator, evaluator and right-hand side entries exactly on
  * the fly (no stored derivative matrices), then back-propagates through the
  * quadrature weights, projection, trilinear sampling, WENO / Godunov gradient
  * and the limited curvature stencils (branch choices held fixed). Requires a

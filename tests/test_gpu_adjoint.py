@@ -281,3 +281,25 @@ def test_gradient_matches_differences_of_the_oracle_pipeline(gpu):
     assert resolved >= 7
     fk = (J_oracle(level, k + 1e-6) - J_oracle(level, k - 1e-6)) / 2e-6
     assert abs(fk - dk) <= 1e-5 * abs(dk), (fk, dk)
+
+
+def test_gradient_with_enlarged_adjoint_solve_matches_bicg(gpu):
+    """ibim_gradient with method IBIM_SOLVE_BLOCK_GMRES (enlarged GMRES on the
+    16-column A^H passes) gives the gradient of the reference's adjoint BiCG to
+    the solves' tolerance, on complex128 pairs and on the dense operator, for the
+    exterior Dirichlet and the interior Neumann problem; an iteration budget it
+    cannot meet is an error as well."""
+    for dirichlet, interior, mode, k in ((1, 0, 1, 3.0), (0, 1, 0, 2.5)):
+        for store in (gpu.C128_PAIR, gpu.C128):
+            P = Pipeline(gpu, dirichlet, interior, mode, k, storage=store)
+            target = P.field(P.star(0.10), k)
+            level = P.star(0.15)
+            J0, g0, k0 = P.gradient(level, k, target)
+            J1, g1, k1 = P.pb.gradient(mode=mode, tol=1e-13, maxit=2000, method=gpu.SOLVE_BLOCK_GMRES)
+            assert J1 == J0
+            rel = np.linalg.norm(g1 - g0) / np.linalg.norm(g0)
+            assert rel <= 1e-8, (dirichlet, store, rel)
+            assert abs(k1 - k0) <= 1e-8 * abs(k0), (dirichlet, store, k1, k0)
+            with pytest.raises(gpu.IbimError, match="adjoint"):
+                P.pb.gradient(mode=mode, tol=1e-13, maxit=2, method=gpu.SOLVE_BLOCK_GMRES)
+            P.pb.close()
