@@ -1076,9 +1076,11 @@ __device__ __forceinline__ f4 ld16(__amdgpu_buffer_rsrc_t r, int voff, int soff)
 // values are in flight into registers and its P, Q and column points into
 // registers for the other half of a double-buffered LDS stage (one barrier per
 // chunk); inside a chunk each k-step's LDS operands are read one k-step ahead.
-// Lane (l % 16, l / 16) takes columns 16 (l / 16) + s at k-step s, so along a
-// row block's own tiles every lane reads 16 consecutive values (8 16-byte
-// loads) and along a column strip 16 lanes read 128 contiguous bytes. P and Q
+// Lane (l % 16, l / 16) takes chunk column 8 (s / 2) + 2 (l / 16) + s % 2 at
+// k-step s: along a column strip 16 lanes read 128 contiguous bytes, and
+// along a row block's own tiles each 16-byte load reads 64 contiguous bytes
+// of each of 16 rows (the four lanes of a row side by side; two consecutive
+// loads fill the same 128-byte lines). P and Q
 // are staged as FP32 Gauss triples (pr, pi - pr, pr + pi) and (qr, qi - qr,
 // -(qr + qi)), so a k-step spends no vector instruction on the vector side of
 // the three real products.
@@ -1113,7 +1115,9 @@ __device__ __forceinline__ void pair_batch32_rows(const float2* __restrict__ F, 
     const int n1 = (I + 1) * kStrip;  // column-strip chunks (j < rows), then own-tile chunks
     const int nq = n1 + (blk_start[I + 1] - blk_start[I]) * kOwn;
     // lane offsets (bytes) of the two chunk kinds inside a chunk's F region
-    const int voff_strip = (16 * lk * kPC + li) * 8, voff_own = (li * kPC + 16 * lk) * 8;
+    const int voff_strip = (2 * lk * kPC + li) * 8, voff_own = (li * kPC + 2 * lk) * 8;
+    // the chunk column of this lane's k-step s
+    auto col_of = [&](int s) { return 8 * (s >> 1) + 2 * lk + (s & 1); };
 
     // chunk q: its first column and this wave's offset into F; true for a
     // column strip (F(j, i) down a tile column), false for an own tile (F(i, j))
@@ -1139,11 +1143,11 @@ __device__ __forceinline__ void pair_batch32_rows(const float2* __restrict__ F, 
         const __amdgpu_buffer_rsrc_t r = buffer_of(F + off, 1u << 20);
         if (strip) {
 #pragma unroll
-            for (int s = 0; s < KS; ++s) f[s] = ld8(r, voff_strip, s * kPC * 8);
+            for (int s = 0; s < KS; ++s) f[s] = ld8(r, voff_strip, (col_of(s) - 2 * lk) * kPC * 8);
         } else {
 #pragma unroll
             for (int s = 0; s < KS / 2; ++s) {
-                const f4 v = ld16(r, voff_own + 16 * s, 0);
+                const f4 v = ld16(r, voff_own + 64 * s, 0);
                 f[2 * s] = fv2{v.x, v.y};
                 f[2 * s + 1] = fv2{v.z, v.w};
             }
@@ -1219,11 +1223,15 @@ __device__ __forceinline__ void pair_batch32_rows(const float2* __restrict__ F, 
     // the 16 k-steps of a staged chunk, FP32 sums, then into the FP64 totals
     auto compute = [&](const fv2 (&f)[KS], int b) {
         f4 y1 = {0, 0, 0, 0}, y2 = y1, y3 = y1, z1 = y1, z2 = y1, z3 = y1;
-        Col nx = read_col(b, 16 * lk);
+        Col nx = read_col(b, col_of(0));
 #pragma unroll
         for (int s = 0; s < KS; ++s) {
             const Col c = nx;
-            if (s + 1 < KS) nx = read_col(b, 16 * lk + s + 1);
+            if (s + 1 < KS) nx = read_col(b, col_of(s + 1));
+            // the next k-step's LDS reads stay ahead of this one's work (the
+            // scheduler would otherwise sink them next to their use and wait
+            // out the LDS latency every k-step)
+            __builtin_amdgcn_sched_barrier(0);
             const float d1 = (float)(Ri.x - c.x), d2 = (float)(Ri.y - c.y), d3 = (float)(Ri.z - c.z);
             float al, be;  // alpha_ij, beta_ij from the separation
             if (Dirichlet) {
