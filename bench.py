@@ -268,11 +268,13 @@ def run_batch(args):
     solved together by ibim_solve_group: every BiCG iteration applies all of
     their operators in one persistent launch (a queue of 64-row items, the
     torus first). FP64 path: batched BiCG on complex128 pair storage, products
-    on the FP64 matrix cores. Mixed path (IBIM_MIXED_PAIR): complex64 pairs
-    (products on the FP64 matrix cores; --batch-fp32-products: FP32), FP64
-    iterative refinement around grouped GMRES, one pass Y = A P per iteration,
-    its FP64 residuals over the complex128 copy of the same pairs
-    (--batch-solver reliable: grouped BiCG with FP64 residual replacement)."""
+    on the FP64 matrix cores. Mixed path (IBIM_MIXED_PAIR): complex64 pairs,
+    FP64 iterative refinement around grouped block GMRES, one pass Y = A P per
+    iteration, its FP64 residuals over the complex128 copy of the same pairs
+    (--batch-solver gmres: GMRES per column; reliable: grouped BiCG with FP64
+    residual replacement). The passes run on the FP32 matrix cores for block
+    GMRES and on the FP64 ones for the per-column solvers (--batch-products
+    overrides)."""
     tol, k = 1e-7, 10.0
     rng = np.random.default_rng(0)
     torus = ("torus128", ibim.shape(ibim.SHAPE_TORUS, a=1.0, b=0.4), 128, 64)
@@ -308,7 +310,7 @@ def run_batch(args):
                 pb.assemble(ibim.C128_PAIR)
             else:
                 pb.assemble(ibim.MIXED_PAIR)
-                pb.set_fp32_products(args.batch_fp32_products)
+                pb.set_fp32_products(batch_fp32(args))
             pb.set_nrhs(m)
             for c in range(m):
                 pb.rhs(ibim.RHS_PLANE_DIRICHLET, amp=1.0, direction=dirs[c], col=c)
@@ -376,7 +378,7 @@ def run_batch(args):
           "unit": "solves/s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": total["mixed"] * 1e3,
           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
           "dtype": "complex64 pair storage, products on the %s matrix cores, %s over the complex128 copy of the "
-                   "pairs" % ("fp32" if args.batch_fp32_products else "fp64",
+                   "pairs" % ("fp32" if batch_fp32(args) else "fp64",
                               {"gmres": "GMRES with fp64 iterative refinement",
                                "block": "block GMRES with fp64 iterative refinement"}.get(
                                   args.batch_solver, "BiCG with fp64 residual replacement")),
@@ -655,6 +657,16 @@ def run_star(args):
           "setup_s": out["mixed"]["setup"]})
 
 
+def batch_fp32(args):
+    """Whether the batch workload's mixed passes run on the FP32 matrix cores:
+    block GMRES gains from them (one Krylov space absorbs the products' rounding
+    in a few more passes), GMRES and BiCG per column lose more iterations near
+    resonance than the cheaper passes save."""
+    if args.batch_products != "auto":
+        return args.batch_products == "fp32"
+    return args.batch_solver == "block"
+
+
 def measured_traffic(kernel, workload):
     """HBM bytes per launch of `kernel` from the committed counter summary
     (FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
@@ -717,8 +729,9 @@ def main():
                     help="star256 mixed path: FP64 refinement around block GMRES (one Krylov space for the 16 "
                          "columns) or GMRES per column (one pass A P per iteration), or batched BiCG with FP64 "
                          "residual replacement")
-    ap.add_argument("--batch-fp32-products", action="store_true",
-                    help="batch mixed path: 16-column passes on the FP32 matrix cores (default: FP64)")
+    ap.add_argument("--batch-products", choices=("auto", "fp32", "fp64"), default="auto",
+                    help="batch mixed path: matrix cores of the 16-column passes (auto: FP32 for block GMRES, "
+                         "FP64 for the per-column solvers)")
     ap.add_argument("--batch-graphs", action="store_true",
                     help="batch workload: replay the grouped iterations as HIP graphs (no per-pass timing)")
     ap.add_argument("--batch-replace", type=float, default=BATCH_REPLACE,
