@@ -1064,11 +1064,17 @@ __device__ __forceinline__ f4 mfma32(float a, float b, f4 c) {
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_of(const void* base, unsigned bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
 }
+// Aux is the cache policy: 0 for data other waves read again (P, Q, the
+// points), kStream (nt) for the pair values, read once per visit like the
+// single-vector sweeps' streaming loads.
+constexpr int kStream = 2;
+template <int Aux = 0>
 __device__ __forceinline__ fv2 ld8(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(fv2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+    return __builtin_bit_cast(fv2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, Aux));
 }
+template <int Aux = 0>
 __device__ __forceinline__ f4 ld16(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
-    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0));
+    return __builtin_bit_cast(f4, __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, Aux));
 }
 
 // The walk of k_pair_batch in chunks of kBatchK columns, software-pipelined:
@@ -1143,11 +1149,11 @@ __device__ __forceinline__ void pair_batch32_rows(const float2* __restrict__ F, 
         const __amdgpu_buffer_rsrc_t r = buffer_of(F + off, 1u << 20);
         if (strip) {
 #pragma unroll
-            for (int s = 0; s < KS; ++s) f[s] = ld8(r, voff_strip, (col_of(s) - 2 * lk) * kPC * 8);
+            for (int s = 0; s < KS; ++s) f[s] = ld8<kStream>(r, voff_strip, (col_of(s) - 2 * lk) * kPC * 8);
         } else {
 #pragma unroll
             for (int s = 0; s < KS / 2; ++s) {
-                const f4 v = ld16(r, voff_own + 64 * s, 0);
+                const f4 v = ld16<kStream>(r, voff_own + 64 * s, 0);
                 f[2 * s] = fv2{v.x, v.y};
                 f[2 * s + 1] = fv2{v.z, v.w};
             }
